@@ -1,0 +1,195 @@
+/*
+ * rbhip.h — C ABI of the MI355X (gfx950) RigidBodyIntegrator step library.
+ *
+ * This is the boundary between the host C++ plugin layer (a drop-in
+ * HipIntegrateRigidBodyStepKernel behind the reference's unchanged
+ * RigidBodyIntegrator / IntegrateRigidBodyStepKernel API) and the
+ * hand-written HIP kernels.  Plain pointers and sizes, int status codes, a
+ * thread-local last-error string, no exceptions across the ABI.
+ *
+ * Each entry point cites the reference interface it replaces (paths relative
+ * to the reference repository).
+ *
+ * Conventions
+ *   - Host arrays of vectors are xyz-interleaved FP64 in ORIGINAL atom order.
+ *   - Status: RBHIP_OK (0) or a negative RBHIP_E_* code; rbhip_last_error()
+ *     describes the last failure on the calling thread.
+ *   - One rbhip_context = one simulated system on one device: the platform's
+ *     atom arrays (positions, velocities, forces), its force field and the
+ *     rigid-body integrator state.  A context is driven by one host thread.
+ *   - Device memory holds atoms in an internal order (free atoms, then body
+ *     atoms grouped by body size class, each body's atoms contiguous); the
+ *     get/set entry points translate to and from the original order.
+ */
+#ifndef RBHIP_H_
+#define RBHIP_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBHIP_VERSION 1
+
+enum {
+    RBHIP_OK = 0,
+    RBHIP_E_ARG = -1,       /* invalid argument */
+    RBHIP_E_HIP = -2,       /* HIP runtime failure */
+    RBHIP_E_STATE = -3,     /* call not valid in the current state */
+    RBHIP_E_NOMEM = -4,     /* device allocation failed */
+    RBHIP_E_UNSUPPORTED = -5
+};
+
+enum { RBHIP_PRECISION_MIXED = 0, RBHIP_PRECISION_DOUBLE = 1 };
+
+/* indices of the per-kernel timing array filled by rbhip_time_kernels */
+enum { RBHIP_KERNEL_PART1 = 0, RBHIP_KERNEL_FORCES = 1, RBHIP_KERNEL_PART2 = 2, RBHIP_NUM_KERNELS = 3 };
+
+typedef struct rbhip_context rbhip_context;
+typedef struct rbhip_batch rbhip_batch;
+typedef void* rbhip_stream; /* a hipStream_t; NULL = the library's own stream */
+
+const char* rbhip_last_error(void);
+int rbhip_version(void);
+/* number of visible devices (0 without a GPU) */
+int rbhip_device_count(int* count);
+/* hipDeviceSynchronize on the current device */
+int rbhip_device_synchronize(void);
+
+/* ---------------------------------------------------------------------- */
+/* Context lifetime                                                        */
+/* ---------------------------------------------------------------------- */
+/* Creates the device context and runs the integrator's initialize:
+ *   RigidBodyIntegrator::initialize   (openmmapi/src/RigidBodyIntegrator.cpp:40-51)
+ *   RigidBodySystem::initialize       (openmmapi/src/RigidBodySystem.cpp:55-114)
+ *   IntegrateRigidBodyStepKernel::initialize (openmmapi/include/RigidBodyKernels.h:60)
+ * bodyIndices follow the reference convention (<= 0 free atom, equal positive
+ * labels one rigid body).  rotationMode: 0 exact, n > 0 NO-SQUISH with n
+ * substeps (RigidBodyIntegrator.h:58-63).  Errors mirror the reference's
+ * exceptions (negative mode, constraints on body atoms) plus this platform's
+ * documented limits (massless atoms, single-atom bodies). */
+int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, double stepSize,
+                 int rotationMode, int precision, int device, rbhip_context** out);
+int rbhip_destroy(rbhip_context* ctx);
+/* launch stream for all work of this context (hipStream_t; NULL restores the
+   context's own stream) */
+int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream);
+/* device bytes held by the context (for admission against an HBM budget) */
+int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes);
+
+/* ---------------------------------------------------------------------- */
+/* Integer bookkeeping (bit-exact contract with RigidBodySystem)           */
+/* ---------------------------------------------------------------------- */
+/* out5 = numFree, numBodies, numActualAtoms, numBodyAtoms, numDOF
+   (RigidBodySystem.h:36-40; numDOF valid after the first set_positions) */
+int rbhip_get_counts(rbhip_context* ctx, int* out5);
+/* RigidBodySystem::atomIndex: free atoms ascending, then body 1's atoms,
+   body 2's, ... (RigidBodySystem.cpp:84-105); out has numActualAtoms ints */
+int rbhip_get_atom_index(rbhip_context* ctx, int* out);
+/* cleanBodyIndices result (RigidBodySystem.cpp:28-49); out has numAtoms ints */
+int rbhip_get_body_index(rbhip_context* ctx, int* out);
+
+/* ---------------------------------------------------------------------- */
+/* Force field slot (ContextImpl::calcForcesAndEnergy,                      */
+/* ReferenceRigidBodyKernels.cpp:101 / CudaRigidBodyKernels.cpp:424)       */
+/* ---------------------------------------------------------------------- */
+/* harmonic tether F_i = -k_i (r_i - r0_i); k has numAtoms entries, r0 is xyz */
+int rbhip_set_tether(rbhip_context* ctx, const double* k, const double* r0);
+/* harmonic bonds E = k/2 (|r_b - r_a| - length)^2 (OpenMM HarmonicBondForce) */
+int rbhip_add_bonds(rbhip_context* ctx, int count, const int* atomPairs, const double* length,
+                    const double* k);
+/* evaluates forces (and, if energy != NULL, the potential energy) */
+int rbhip_compute_forces(rbhip_context* ctx, double* energy);
+
+/* ---------------------------------------------------------------------- */
+/* Context state                                                           */
+/* ---------------------------------------------------------------------- */
+/* Context::setPositions -> RigidBodyIntegrator::stateChanged(Positions)
+   (RigidBodyIntegrator.cpp:63-74): forces, then RigidBodySystem::update with
+   geometry and velocities (RigidBodySystem.cpp:120-142), then
+   uploadBodySystem (RigidBodyKernels.h:66).  The body state is built on the
+   device in FP64. */
+int rbhip_set_positions(rbhip_context* ctx, const double* xyz);
+/* Context::setVelocities -> stateChanged(Velocities): dynamics rebuild */
+int rbhip_set_velocities(rbhip_context* ctx, const double* xyz);
+int rbhip_get_positions(rbhip_context* ctx, double* xyz);
+int rbhip_get_velocities(rbhip_context* ctx, double* xyz);
+int rbhip_get_forces(rbhip_context* ctx, double* xyz);
+/* Integrator::setStepSize */
+int rbhip_set_step_size(rbhip_context* ctx, double dt);
+/* ContextImpl time and step count (ReferenceRigidBodyKernels.cpp:106-107) */
+int rbhip_get_time(rbhip_context* ctx, double* time, long long* stepCount);
+
+/* ---------------------------------------------------------------------- */
+/* The step (IntegrateRigidBodyStepKernel::execute, RigidBodyKernels.h:73) */
+/* ---------------------------------------------------------------------- */
+/* integratePart1 (RigidBodySystem.cpp:170-187) on the device */
+int rbhip_part1(rbhip_context* ctx);
+/* integratePart2 (RigidBodySystem.cpp:193-204) on the device */
+int rbhip_part2(rbhip_context* ctx);
+/* RigidBodyIntegrator::step(steps) (RigidBodyIntegrator.cpp:96-101):
+   steps x [part1, forces, part2], replayed from a HIP graph when enabled */
+int rbhip_execute(rbhip_context* ctx, int steps);
+/* 1 = capture one step in a HIP graph and replay it (default), 0 = launch */
+int rbhip_use_graph(rbhip_context* ctx, int enable);
+
+/* computeKineticEnergy + getKineticEnergies (RigidBodyKernels.h:80-86,
+   ReferenceRigidBodyKernels.cpp:110-121): out2 = {translational, rotational},
+   FP64 device reduction */
+int rbhip_kinetic_energies(rbhip_context* ctx, double* out2);
+/* potential energy of the last force evaluation */
+int rbhip_potential_energy(rbhip_context* ctx, double* energy);
+
+/* Body state of all bodies in reference body order, FP64, 36 values each:
+   N, dof, mass, I[3], invI[3], rcm[3], pcm[3], q[4], pi[4], force[3],
+   torque[4], twoKt, twoKr, loc (RigidBody.h:18-53).  pcm/pi are the
+   post-step values (the folded half-kick is taken out). */
+int rbhip_get_body_states(rbhip_context* ctx, double* out);
+/* body-fixed atom coordinates d (RigidBodySystem bodyFixedPositions),
+   numBodyAtoms x 3, reference body-atom order */
+int rbhip_get_body_fixed_positions(rbhip_context* ctx, double* xyz);
+
+/* ---------------------------------------------------------------------- */
+/* Launch configuration, autotuning, measurement                           */
+/* ---------------------------------------------------------------------- */
+/* workgroup size (64..1024, multiple of 64) and grid cap (0 = one workgroup
+   per tile) */
+int rbhip_set_launch(rbhip_context* ctx, int blockSize, int maxBlocks);
+/* lanes per body for the bodies of N atoms (power of two, 1..64) */
+int rbhip_set_lanes(rbhip_context* ctx, int atomsPerBody, int lanes);
+/* times candidate (lanes per body, workgroup size) tiles on a scratch copy of
+   the state and keeps the fastest; the context's state is not advanced */
+int rbhip_autotune(rbhip_context* ctx, int reps);
+/* describes the chosen tiling as text */
+int rbhip_describe(rbhip_context* ctx, char* buf, int len);
+/* runs `steps` plain launches with HIP events around each kernel on the
+   context's stream; avgMs[RBHIP_NUM_KERNELS] receives the mean duration of
+   each kernel and totalMs the whole span */
+int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* totalMs);
+/* streams `bytes` through a plain device copy kernel `reps` times (profiler
+   calibration of HBM traffic counters); avgMs receives its mean duration */
+int rbhip_copy_calibration(size_t bytes, int reps, double* avgMs);
+
+/* ---------------------------------------------------------------------- */
+/* Batches: many independent systems stepped by one set of launches        */
+/* (no reference counterpart: the reference steps one system per context)  */
+/* ---------------------------------------------------------------------- */
+int rbhip_batch_create(int device, rbhip_batch** out);
+int rbhip_batch_destroy(rbhip_batch* batch);
+/* replaces the member list; members must live on the batch's device, have
+   the same precision and rotation mode, and hold a built body state */
+int rbhip_batch_set_members(rbhip_batch* batch, int count, rbhip_context* const* members);
+int rbhip_batch_set_launch(rbhip_batch* batch, int blockSize, int maxBlocks);
+int rbhip_batch_use_graph(rbhip_batch* batch, int enable);
+/* steps every member `steps` times: per step one part1, one force and one
+   part2 launch over all members' tiles */
+int rbhip_batch_execute(rbhip_batch* batch, int steps);
+/* per-member {trans, rot} kinetic energies, 2*count doubles */
+int rbhip_batch_kinetic_energies(rbhip_batch* batch, double* out);
+int rbhip_batch_time_kernels(rbhip_batch* batch, int steps, double* avgMs, double* totalMs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1312 @@
+// Host side of the rbhip C ABI (include/rbhip.h): contexts, batches, tiling,
+// graph capture, timing and the tile autotuner.
+//
+// Integer bookkeeping follows the reference bit for bit:
+//   cleanBodyIndices            openmmapi/src/RigidBodySystem.cpp:28-49
+//   RigidBodySystem::initialize openmmapi/src/RigidBodySystem.cpp:55-114
+// and is what rbhip_get_body_index / rbhip_get_atom_index return.  The
+// device works in its own internal order: free atoms first, then the bodies
+// grouped by atom count (ascending), each body's atoms contiguous, bodies of
+// one size in reference order.  Tiles of one size class never mix sizes, so
+// every lane of a wave runs the same atom loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rbhip.h"
+#include "rbhip_internal.h"
+#include "rbhip_kernels.h"
+
+using namespace rbhip;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct RbError : std::runtime_error {
+    int code;
+    RbError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) throw RbError(RBHIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void require(bool ok, const std::string& msg, int code = RBHIP_E_ARG) {
+    if (!ok) throw RbError(code, msg);
+}
+
+template <class F> int guard(F&& f) {
+    try {
+        f();
+        return RBHIP_OK;
+    } catch (const RbError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return RBHIP_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return RBHIP_E_ARG;
+    }
+}
+
+// cleanBodyIndices (RigidBodySystem.cpp:28-49): distinct positive labels are
+// renumbered 1..B in ascending label order, labels <= 0 become 0
+std::vector<int> clean_body_indices(const int* in, int n) {
+    std::vector<int> labels;
+    labels.reserve(n);
+    for (int i = 0; i < n; i++)
+        if (in[i] > 0) labels.push_back(in[i]);
+    std::sort(labels.begin(), labels.end());
+    labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
+    std::vector<int> out(n, 0);
+    for (int i = 0; i < n; i++)
+        if (in[i] > 0) out[i] = (int)(std::lower_bound(labels.begin(), labels.end(), in[i]) - labels.begin()) + 1;
+    return out;
+}
+
+struct DeviceArena {
+    std::vector<void*> blocks;
+    size_t bytes = 0;
+    void* alloc(size_t n) {
+        if (n == 0) n = 256;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) throw RbError(RBHIP_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        blocks.push_back(p);
+        bytes += n;
+        return p;
+    }
+    template <class T> T* allocT(size_t count) { return reinterpret_cast<T*>(alloc(count * sizeof(T))); }
+    void release() {
+        for (void* p : blocks) (void)hipFree(p);
+        blocks.clear();
+        bytes = 0;
+    }
+};
+
+int pow2_at_least(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+struct GraphCache {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+}  // namespace
+
+namespace {
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    explicit EventSet(size_t n) : ev(n, nullptr) {
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    }
+    ~EventSet() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+template <class P1, class F, class P2>
+void time_steps(hipStream_t stream, int steps, P1 p1, F f, P2 p2, double* avgMs, double* totalMs) {
+    EventSet es(4 * (size_t)steps);
+    for (int s = 0; s < steps; s++) {
+        HIPCHK(hipEventRecord(es.ev[4 * s], stream));
+        p1();
+        HIPCHK(hipEventRecord(es.ev[4 * s + 1], stream));
+        f();
+        HIPCHK(hipEventRecord(es.ev[4 * s + 2], stream));
+        p2();
+        HIPCHK(hipEventRecord(es.ev[4 * s + 3], stream));
+    }
+    HIPCHK(hipEventSynchronize(es.ev[4 * steps - 1]));
+    double acc[3] = {0, 0, 0};
+    for (int s = 0; s < steps; s++)
+        for (int k = 0; k < 3; k++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, es.ev[4 * s + k], es.ev[4 * s + k + 1]));
+            acc[k] += ms;
+        }
+    for (int k = 0; k < 3; k++) avgMs[k] = acc[k] / steps;
+    float span = 0.f;
+    HIPCHK(hipEventElapsedTime(&span, es.ev[0], es.ev[4 * steps - 1]));
+    if (totalMs) *totalMs = span;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct rbhip_context {
+    int device = 0;
+    int precision = RBHIP_PRECISION_MIXED;
+    int rotationMode = 10;
+    double dt = 0.0;
+    double kickDt = 0.0;      // step size of the half-kick folded into the body state
+    hipStream_t ownStream = nullptr;
+    hipStream_t stream = nullptr;
+
+    int numAtoms = 0, numFree = 0, numBodies = 0, numBodyAtoms = 0, numDOF = 0;
+    std::vector<int> bodyIndex, atomIndex;
+    std::vector<int> refBodyN, refBodyLoc;       // per reference body
+    std::vector<int> internalBody;               // internal body -> reference body
+    std::vector<int> bodyN, bodyAtom0;           // per internal body
+    std::vector<int> atomOrig;                   // internal atom -> original atom
+    struct Class { int N, firstBody, count, firstAtom; };
+    std::vector<Class> classes;
+    std::map<int, int> lanes;                    // atoms per body -> lanes per body
+    int block = 256, maxBlocks = 0;
+
+    DeviceArena arena;
+    DeviceArena tileArena;
+    DevSys h{};
+    DevSys* dSys = nullptr;
+    Tile* dPartTiles = nullptr;
+    Tile* dForceTiles = nullptr;
+    Tile* dSetupTiles = nullptr;
+    int nPartTiles = 0, nForceTiles = 0, nSetupTiles = 0;
+    int maxKeSlots = 0, maxPeSlots = 0;
+    int keSlots = 0;                             // slots written by the last KE producer
+    int* dSlotCounts = nullptr;                  // [0] ke, [1] pe
+    int* dBodyN = nullptr;
+    int* dBodyAtom0 = nullptr;
+    double* dTmp = nullptr;
+    size_t tmpCount = 0;
+    double* dRed = nullptr;
+    double* hRed = nullptr;                      // pinned
+
+    bool built = false;
+    bool stageVFresh = false;
+    bool useGraph = true;
+    GraphCache graph;
+    double time = 0.0;
+    long long stepCount = 0;
+
+    // bonds in original numbering
+    std::vector<int> bondA, bondB;
+    std::vector<double> bondLen, bondK;
+
+    size_t realSize() const { return precision == RBHIP_PRECISION_MIXED ? 4 : 8; }
+
+    void setDevice() const { HIPCHK(hipSetDevice(device)); }
+
+    void uploadSys() {
+        HIPCHK(hipMemcpyAsync(dSys, &h, sizeof(DevSys), hipMemcpyHostToDevice, stream));
+    }
+
+    std::vector<Tile> partTiles(int blk, const std::map<int, int>& lanesByN) const {
+        std::vector<Tile> t;
+        for (int f = 0; f < numFree; f += blk)
+            t.push_back(Tile{0, TILE_FREE, f, std::min(blk, numFree - f), 0, 0, 1, 0});
+        for (const Class& c : classes) {
+            auto it = lanesByN.find(c.N);
+            int L = it == lanesByN.end() ? 1 : it->second;
+            int per = blk / L;
+            for (int b = 0; b < c.count; b += per)
+                t.push_back(Tile{0, TILE_BODIES, c.firstBody + b, std::min(per, c.count - b), c.N,
+                                 c.firstAtom + b * c.N, L, 0});
+        }
+        for (size_t i = 0; i < t.size(); i++) t[i].slot = (int)i;
+        return t;
+    }
+
+    std::vector<Tile> forceTiles(int blk) const {
+        std::vector<Tile> t;
+        for (int a = 0; a < numAtoms; a += blk)
+            t.push_back(Tile{0, TILE_ATOMS, a, std::min(blk, numAtoms - a), 0, 0, 1, 0});
+        for (size_t i = 0; i < t.size(); i++) t[i].slot = (int)i;
+        return t;
+    }
+
+    // every tile must address only this system's arrays and slots
+    void validateTiles(const std::vector<Tile>& tiles, int blk, int maxSlots) const {
+        for (const Tile& t : tiles) {
+            require(t.slot >= 0 && t.slot < maxSlots, "tile slot out of range", RBHIP_E_STATE);
+            require(t.count > 0, "empty tile", RBHIP_E_STATE);
+            if (t.kind == TILE_FREE) {
+                require(t.count <= blk && t.first >= 0 && t.first + t.count <= numFree, "free tile out of range",
+                        RBHIP_E_STATE);
+            } else if (t.kind == TILE_ATOMS) {
+                require(t.count <= blk && t.first >= 0 && t.first + t.count <= numAtoms, "atom tile out of range",
+                        RBHIP_E_STATE);
+            } else {
+                require(t.lanes >= 1 && t.lanes <= 64 && (t.lanes & (t.lanes - 1)) == 0 && blk % t.lanes == 0,
+                        "bad lanes per body", RBHIP_E_STATE);
+                require(t.count * t.lanes <= blk, "body tile larger than the workgroup", RBHIP_E_STATE);
+                require(t.first >= 0 && t.first + t.count <= numBodies, "body tile out of range", RBHIP_E_STATE);
+                require(t.atomFirst >= numFree && t.atomFirst + t.count * t.N <= numAtoms,
+                        "body tile atoms out of range", RBHIP_E_STATE);
+                for (int b = 0; b < t.count; b += std::max(1, t.count - 1))
+                    require(bodyN[t.first + b] == t.N && bodyAtom0[t.first + b] == t.atomFirst + b * t.N,
+                            "body tile does not match the body layout", RBHIP_E_STATE);
+            }
+        }
+    }
+
+    // largest workgroup the kernels are compiled for: 512, or 256 for the FP64
+    // exact rotation; the FP64 body-state build always runs at <= 256
+    int maxBlock() const { return rotationMode == 0 ? 256 : 512; }
+    int setupBlock() const { return std::min(block, 256); }
+
+    void rebuildTiles() {
+        require(block >= 64 && block <= maxBlock() && block % 64 == 0,
+                "block size must be a multiple of 64 in [64, " + std::to_string(maxBlock()) + "]");
+        std::vector<Tile> pt = partTiles(block, lanes);
+        std::vector<Tile> ft = forceTiles(block);
+        std::map<int, int> ones;
+        std::vector<Tile> st = partTiles(setupBlock(), ones);
+        validateTiles(pt, block, maxKeSlots);
+        validateTiles(ft, block, maxPeSlots);
+        validateTiles(st, setupBlock(), maxKeSlots);
+        tileArena.release();
+        dPartTiles = tileArena.allocT<Tile>(pt.size() + 1);
+        dForceTiles = tileArena.allocT<Tile>(ft.size() + 1);
+        dSetupTiles = tileArena.allocT<Tile>(st.size() + 1);
+        if (!pt.empty()) HIPCHK(hipMemcpy(dPartTiles, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        if (!ft.empty()) HIPCHK(hipMemcpy(dForceTiles, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        if (!st.empty()) HIPCHK(hipMemcpy(dSetupTiles, st.data(), st.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        nPartTiles = (int)pt.size();
+        nForceTiles = (int)ft.size();
+        nSetupTiles = (int)st.size();
+        graph.reset();
+    }
+
+    void setSlotCounts(int ke, int pe) {
+        int v[2] = {ke, pe};
+        HIPCHK(hipMemcpyAsync(dSlotCounts, v, sizeof(v), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void launchForces() {
+        HIPCHK(launch_forces(precision, dSys, dForceTiles, nForceTiles, block, maxBlocks, stream));
+    }
+    void launchPart1() {
+        HIPCHK(launch_part1(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+    }
+    void launchPart2() {
+        HIPCHK(launch_part2(precision, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+    }
+
+    // 4 doubles reduced from the KE (which = 0) or PE (which = 1) partials
+    void reduce(int which, double* out4) {
+        int counts[2] = {keSlots, nForceTiles};
+        HIPCHK(hipMemcpyAsync(dSlotCounts, counts, sizeof(counts), hipMemcpyHostToDevice, stream));
+        HIPCHK(launch_reduce(dSys, dSlotCounts + which, 1, which, dRed, stream));
+        HIPCHK(hipMemcpyAsync(hRed, dRed, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int c = 0; c < 4; c++) out4[c] = hRed[c];
+    }
+
+    void runSetup(int geometry) {
+        h.useStageV = stageVFresh ? 1 : 0;
+        uploadSys();
+        HIPCHK(launch_setup(precision, dSys, dSetupTiles, nSetupTiles, geometry, setupBlock(), stream));
+        keSlots = nSetupTiles;
+        kickDt = dt;
+        built = true;
+        double r[4];
+        reduce(0, r);
+        if (geometry) numDOF = numFree + (int)std::lround(r[2]);
+    }
+
+    // original-order FP64 xyz -> internal order
+    std::vector<double> toInternal(const double* xyz) const {
+        std::vector<double> out(3 * (size_t)numAtoms);
+        for (int i = 0; i < numAtoms; i++) {
+            int o = atomOrig[i];
+            out[3 * (size_t)i] = xyz[3 * (size_t)o];
+            out[3 * (size_t)i + 1] = xyz[3 * (size_t)o + 1];
+            out[3 * (size_t)i + 2] = xyz[3 * (size_t)o + 2];
+        }
+        return out;
+    }
+
+    void download(int what, double* xyz) {
+        HIPCHK(launch_stage_out(precision, dSys, dTmp, numAtoms, what, stream));
+        std::vector<double> tmp(3 * (size_t)numAtoms);
+        HIPCHK(hipMemcpyAsync(tmp.data(), dTmp, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int i = 0; i < numAtoms; i++) {
+            int o = atomOrig[i];
+            xyz[3 * (size_t)o] = tmp[3 * (size_t)i];
+            xyz[3 * (size_t)o + 1] = tmp[3 * (size_t)i + 1];
+            xyz[3 * (size_t)o + 2] = tmp[3 * (size_t)i + 2];
+        }
+    }
+
+    void ensureGraph() {
+        if (graph.exec) return;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            launchPart1();
+            launchForces();
+            launchPart2();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(stream, &graph.graph));
+        HIPCHK(hipGraphInstantiate(&graph.exec, graph.graph, nullptr, nullptr, 0));
+    }
+
+    void fixKick() {
+        if (built && kickDt != dt) {
+            HIPCHK(launch_rekick(precision, dSys, dBodyN, dBodyAtom0, numBodies, dt - kickDt, stream));
+            kickDt = dt;
+        }
+    }
+
+    void executeSteps(int steps) {
+        require(built, "positions must be set before stepping", RBHIP_E_STATE);
+        require(steps >= 0, "negative step count");
+        fixKick();
+        if (steps == 0) return;
+        if (useGraph) {
+            ensureGraph();
+            for (int s = 0; s < steps; s++) HIPCHK(hipGraphLaunch(graph.exec, stream));
+        } else {
+            for (int s = 0; s < steps; s++) {
+                launchPart1();
+                launchForces();
+                launchPart2();
+            }
+        }
+        keSlots = nPartTiles;
+        stageVFresh = false;
+        time += dt * steps;
+        stepCount += steps;
+    }
+
+    ~rbhip_context() {
+        graph.reset();
+        if (hRed) (void)hipHostFree(hRed);
+        tileArena.release();
+        arena.release();
+        if (ownStream) (void)hipStreamDestroy(ownStream);
+    }
+};
+
+// ---------------------------------------------------------------------------
+struct rbhip_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int block = 256, maxBlocks = 0;
+    bool useGraph = true;
+    std::vector<rbhip_context*> members;
+    DeviceArena arena;
+    DevSys* dSystems = nullptr;
+    Tile* dPartTiles = nullptr;
+    Tile* dForceTiles = nullptr;
+    int nPart = 0, nForce = 0;
+    std::vector<int> memberPartTiles, memberForceTiles;
+    int* dKeSlots = nullptr;
+    double* dRed = nullptr;
+    int precision = 0, rotationMode = 0;
+    GraphCache graph;
+
+    void rebuild() {
+        graph.reset();
+        arena.release();
+        nPart = nForce = 0;
+        memberPartTiles.assign(members.size(), 0);
+        memberForceTiles.assign(members.size(), 0);
+        if (members.empty()) return;
+        std::vector<DevSys> sys;
+        std::vector<Tile> pt, ft;
+        for (size_t m = 0; m < members.size(); m++) {
+            rbhip_context* c = members[m];
+            sys.push_back(c->h);
+            std::vector<Tile> p = c->partTiles(block, c->lanes);
+            std::vector<Tile> f = c->forceTiles(block);
+            c->validateTiles(p, block, c->maxKeSlots);
+            c->validateTiles(f, block, c->maxPeSlots);
+            for (Tile& t : p) { t.sys = (int)m; pt.push_back(t); }
+            for (Tile& t : f) { t.sys = (int)m; ft.push_back(t); }
+            memberPartTiles[m] = (int)p.size();
+            memberForceTiles[m] = (int)f.size();
+        }
+        dSystems = arena.allocT<DevSys>(sys.size());
+        dPartTiles = arena.allocT<Tile>(pt.size() + 1);
+        dForceTiles = arena.allocT<Tile>(ft.size() + 1);
+        dKeSlots = arena.allocT<int>(members.size());
+        dRed = arena.allocT<double>(4 * members.size());
+        HIPCHK(hipMemcpy(dSystems, sys.data(), sys.size() * sizeof(DevSys), hipMemcpyHostToDevice));
+        if (!pt.empty()) HIPCHK(hipMemcpy(dPartTiles, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        if (!ft.empty()) HIPCHK(hipMemcpy(dForceTiles, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        nPart = (int)pt.size();
+        nForce = (int)ft.size();
+    }
+
+    void launchAll() {
+        HIPCHK(launch_part1(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
+        HIPCHK(launch_forces(precision, dSystems, dForceTiles, nForce, block, maxBlocks, stream));
+        HIPCHK(launch_part2(precision, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
+    }
+
+    void ensureGraph() {
+        if (graph.exec) return;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            launchAll();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(stream, &graph.graph));
+        HIPCHK(hipGraphInstantiate(&graph.exec, graph.graph, nullptr, nullptr, 0));
+    }
+
+    void finishSteps(int steps) {
+        for (size_t m = 0; m < members.size(); m++) {
+            rbhip_context* c = members[m];
+            c->keSlots = memberPartTiles[m];
+            c->stageVFresh = false;
+            c->time += c->dt * steps;
+            c->stepCount += steps;
+        }
+    }
+
+    // the batch stream waits for work queued on member streams, and members
+    // wait for the batch, so member calls and batch steps stay ordered
+    void joinMembers() {
+        for (rbhip_context* c : members) {
+            HIPCHK(hipEventRecord(ordering, c->stream));
+            HIPCHK(hipStreamWaitEvent(stream, ordering, 0));
+        }
+    }
+    void releaseMembers() {
+        HIPCHK(hipEventRecord(ordering, stream));
+        for (rbhip_context* c : members) HIPCHK(hipStreamWaitEvent(c->stream, ordering, 0));
+    }
+
+    hipEvent_t ordering = nullptr;
+
+    ~rbhip_batch() {
+        graph.reset();
+        arena.release();
+        if (ordering) (void)hipEventDestroy(ordering);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* rbhip_last_error(void) { return g_err.c_str(); }
+
+int rbhip_version(void) { return RBHIP_VERSION; }
+
+int rbhip_device_count(int* count) {
+    return guard([&] {
+        require(count != nullptr, "null output");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+int rbhip_device_synchronize(void) {
+    return guard([&] { HIPCHK(hipDeviceSynchronize()); });
+}
+
+int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, double stepSize, int rotationMode,
+                 int precision, int device, rbhip_context** out) {
+    return guard([&] {
+        require(out != nullptr, "null output handle");
+        *out = nullptr;
+        require(numAtoms > 0 && masses != nullptr && bodyIndices != nullptr, "empty system");
+        require(rotationMode >= 0, "Rotation mode cannot be negative");
+        require(precision == RBHIP_PRECISION_MIXED || precision == RBHIP_PRECISION_DOUBLE, "unknown precision");
+        require(stepSize > 0.0, "step size must be positive");
+        for (int i = 0; i < numAtoms; i++)
+            require(masses[i] > 0.0, "massless or negative-mass particles are not supported (particle " +
+                                         std::to_string(i) + ")", RBHIP_E_UNSUPPORTED);
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        require(device >= 0 && device < ndev, "no such device");
+
+        std::unique_ptr<rbhip_context> c(new rbhip_context());
+        rbhip_context& C = *c;
+        C.device = device;
+        C.precision = precision;
+        C.rotationMode = rotationMode;
+        C.dt = C.kickDt = stepSize;
+        C.numAtoms = numAtoms;
+        C.setDevice();
+
+        // ---- reference bookkeeping (RigidBodySystem.cpp:55-105) ----
+        C.bodyIndex = clean_body_indices(bodyIndices, numAtoms);
+        for (int v : C.bodyIndex) C.numBodies = std::max(C.numBodies, v);
+        C.refBodyN.assign(C.numBodies, 0);
+        C.atomIndex.assign(numAtoms, 0);
+        for (int i = 0; i < numAtoms; i++) {
+            if (C.bodyIndex[i] == 0) C.atomIndex[C.numFree++] = i;
+            else C.refBodyN[C.bodyIndex[i] - 1]++;
+        }
+        C.numBodyAtoms = numAtoms - C.numFree;
+        C.refBodyLoc.assign(C.numBodies, 0);
+        int loc = 0;
+        for (int b = 0; b < C.numBodies; b++) {
+            require(C.refBodyN[b] >= 2, "rigid body " + std::to_string(b + 1) +
+                                            " has a single atom; single-atom bodies are not supported",
+                    RBHIP_E_UNSUPPORTED);
+            C.refBodyLoc[b] = loc;
+            loc += C.refBodyN[b];
+        }
+        std::vector<int> fill(C.numBodies, 0);
+        for (int i = 0; i < numAtoms; i++) {
+            int b = C.bodyIndex[i];
+            if (b > 0) C.atomIndex[C.numFree + C.refBodyLoc[b - 1] + fill[b - 1]++] = i;
+        }
+
+        // ---- internal order: bodies grouped by size class ----
+        std::vector<int> order(C.numBodies);
+        for (int b = 0; b < C.numBodies; b++) order[b] = b;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return C.refBodyN[x] < C.refBodyN[y]; });
+        C.internalBody = order;
+        C.bodyN.resize(C.numBodies);
+        C.bodyAtom0.resize(C.numBodies);
+        C.atomOrig.resize(numAtoms);
+        for (int k = 0; k < C.numFree; k++) C.atomOrig[k] = C.atomIndex[k];
+        int a = C.numFree;
+        for (int ib = 0; ib < C.numBodies; ib++) {
+            int r = order[ib];
+            int N = C.refBodyN[r];
+            C.bodyN[ib] = N;
+            C.bodyAtom0[ib] = a;
+            for (int j = 0; j < N; j++) C.atomOrig[a + j] = C.atomIndex[C.numFree + C.refBodyLoc[r] + j];
+            if (C.classes.empty() || C.classes.back().N != N)
+                C.classes.push_back(rbhip_context::Class{N, ib, 0, a});
+            C.classes.back().count++;
+            a += N;
+        }
+
+        // ---- device allocation ----
+        if (rotationMode == 0) C.block = 64;   // the FP64 exact rotation is register heavy
+        HIPCHK(hipStreamCreateWithFlags(&C.ownStream, hipStreamNonBlocking));
+        C.stream = C.ownStream;
+        const size_t R = C.realSize();
+        const size_t nb = std::max(C.numBodies, 1), na = (size_t)numAtoms;
+        DevSys& h = C.h;
+        h.numFree = C.numFree;
+        h.numBodies = C.numBodies;
+        h.numBodyAtoms = C.numBodyAtoms;
+        h.numAtoms = numAtoms;
+        h.rotationMode = rotationMode;
+        h.bodyStride = (int)((nb + 63) / 64 * 64);
+        h.dt = stepSize;
+        h.tetherScale = 0.0;
+        h.q = C.arena.alloc(4 * R * nb);
+        h.pi = C.arena.alloc(4 * R * nb);
+        h.pm = C.arena.alloc(4 * R * nb);
+        h.iI = C.arena.alloc(4 * R * nb);
+        h.rcm = C.arena.allocT<double>(3 * (size_t)h.bodyStride);
+        h.d = C.arena.alloc(3 * R * std::max<size_t>(C.numBodyAtoms, 1));
+        h.pos = C.arena.alloc(3 * R * na);
+        h.posLo = precision == RBHIP_PRECISION_MIXED ? C.arena.allocT<float>(3 * na) : nullptr;
+        h.vel = C.arena.alloc(3 * R * na);
+        h.force = C.arena.alloc(3 * R * na);
+        h.freeInvM = C.arena.alloc(R * std::max(C.numFree, 1));
+        h.mass = C.arena.allocT<double>(na);
+        h.stageR = C.arena.allocT<double>(3 * na);
+        h.stageV = C.arena.allocT<double>(3 * na);
+        h.tetherK = C.arena.alloc(R * na);
+        h.r0 = C.arena.alloc(3 * R * na);
+        h.r0Lo = precision == RBHIP_PRECISION_MIXED ? C.arena.allocT<float>(3 * na) : nullptr;
+        h.bondStart = nullptr;
+        h.bondOther = nullptr;
+        h.bondLen = nullptr;
+        h.bondK = nullptr;
+        size_t nFreeTilesMax = (C.numFree + 63) / 64 + 1;
+        C.maxKeSlots = (int)(nFreeTilesMax + C.numBodies + C.classes.size() + 1);
+        C.maxPeSlots = (int)((na + 63) / 64 + 1);
+        h.kePart = C.arena.allocT<double>(4 * (size_t)C.maxKeSlots);
+        h.pePart = C.arena.allocT<double>(4 * (size_t)C.maxPeSlots);
+        C.dSys = C.arena.allocT<DevSys>(1);
+        C.dSlotCounts = C.arena.allocT<int>(2);
+        C.dBodyN = C.arena.allocT<int>(nb);
+        C.dBodyAtom0 = C.arena.allocT<int>(nb);
+        C.tmpCount = std::max(3 * na, 36 * nb);
+        C.dTmp = C.arena.allocT<double>(C.tmpCount);
+        C.dRed = C.arena.allocT<double>(4);
+        HIPCHK(hipHostMalloc((void**)&C.hRed, 4 * sizeof(double), hipHostMallocDefault));
+
+        // zero state, constants
+        HIPCHK(hipMemset(h.pos, 0, 3 * R * na));
+        if (h.posLo) HIPCHK(hipMemset(h.posLo, 0, 3 * sizeof(float) * na));
+        HIPCHK(hipMemset(h.vel, 0, 3 * R * na));
+        HIPCHK(hipMemset(h.force, 0, 3 * R * na));
+        HIPCHK(hipMemset(h.tetherK, 0, R * na));
+        HIPCHK(hipMemset(h.r0, 0, 3 * R * na));
+        if (h.r0Lo) HIPCHK(hipMemset(h.r0Lo, 0, 3 * sizeof(float) * na));
+        HIPCHK(hipMemset(h.kePart, 0, 4 * sizeof(double) * C.maxKeSlots));
+        HIPCHK(hipMemset(h.pePart, 0, 4 * sizeof(double) * C.maxPeSlots));
+        std::vector<double> m(na);
+        for (size_t i = 0; i < na; i++) m[i] = masses[C.atomOrig[i]];
+        HIPCHK(hipMemcpy(h.mass, m.data(), na * sizeof(double), hipMemcpyHostToDevice));
+        if (C.numFree > 0) {
+            if (R == 4) {
+                std::vector<float> inv(C.numFree);
+                for (int k = 0; k < C.numFree; k++) inv[k] = (float)(1.0 / m[k]);
+                HIPCHK(hipMemcpy(h.freeInvM, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
+            } else {
+                std::vector<double> inv(C.numFree);
+                for (int k = 0; k < C.numFree; k++) inv[k] = 1.0 / m[k];
+                HIPCHK(hipMemcpy(h.freeInvM, inv.data(), inv.size() * 8, hipMemcpyHostToDevice));
+            }
+        }
+        if (C.numBodies > 0) {
+            HIPCHK(hipMemcpy(C.dBodyN, C.bodyN.data(), C.numBodies * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(C.dBodyAtom0, C.bodyAtom0.data(), C.numBodies * sizeof(int), hipMemcpyHostToDevice));
+        }
+        C.uploadSys();
+        HIPCHK(hipStreamSynchronize(C.stream));
+        C.rebuildTiles();
+        *out = c.release();
+    });
+}
+
+int rbhip_destroy(rbhip_context* ctx) {
+    return guard([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        delete ctx;
+    });
+}
+
+int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream) {
+    return guard([&] {
+        require(ctx, "null context");
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->stream = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->ownStream;
+        ctx->graph.reset();
+    });
+}
+
+int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes) {
+    return guard([&] {
+        require(ctx && bytes, "null argument");
+        *bytes = ctx->arena.bytes + ctx->tileArena.bytes;
+    });
+}
+
+int rbhip_get_counts(rbhip_context* ctx, int* out5) {
+    return guard([&] {
+        require(ctx && out5, "null argument");
+        out5[0] = ctx->numFree;
+        out5[1] = ctx->numBodies;
+        out5[2] = ctx->numAtoms;
+        out5[3] = ctx->numBodyAtoms;
+        out5[4] = ctx->numDOF;
+    });
+}
+
+int rbhip_get_atom_index(rbhip_context* ctx, int* out) {
+    return guard([&] {
+        require(ctx && out, "null argument");
+        std::copy(ctx->atomIndex.begin(), ctx->atomIndex.end(), out);
+    });
+}
+
+int rbhip_get_body_index(rbhip_context* ctx, int* out) {
+    return guard([&] {
+        require(ctx && out, "null argument");
+        std::copy(ctx->bodyIndex.begin(), ctx->bodyIndex.end(), out);
+    });
+}
+
+int rbhip_set_tether(rbhip_context* ctx, const double* k, const double* r0) {
+    return guard([&] {
+        require(ctx && k && r0, "null argument");
+        rbhip_context& C = *ctx;
+        C.setDevice();
+        size_t na = C.numAtoms;
+        std::vector<double> r = C.toInternal(r0);
+        if (C.precision == RBHIP_PRECISION_MIXED) {
+            std::vector<float> kk(na), hi(3 * na), lo(3 * na);
+            for (size_t i = 0; i < na; i++) kk[i] = (float)k[C.atomOrig[i]];
+            for (size_t i = 0; i < 3 * na; i++) {
+                hi[i] = (float)r[i];
+                lo[i] = (float)(r[i] - (double)hi[i]);
+            }
+            HIPCHK(hipMemcpy(C.h.tetherK, kk.data(), na * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(C.h.r0, hi.data(), 3 * na * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(C.h.r0Lo, lo.data(), 3 * na * 4, hipMemcpyHostToDevice));
+        } else {
+            std::vector<double> kk(na);
+            for (size_t i = 0; i < na; i++) kk[i] = k[C.atomOrig[i]];
+            HIPCHK(hipMemcpy(C.h.tetherK, kk.data(), na * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(C.h.r0, r.data(), 3 * na * 8, hipMemcpyHostToDevice));
+        }
+        C.h.tetherScale = 1.0;
+        C.uploadSys();
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int rbhip_add_bonds(rbhip_context* ctx, int count, const int* atomPairs, const double* length, const double* k) {
+    return guard([&] {
+        require(ctx && count >= 0 && (count == 0 || (atomPairs && length && k)), "bad bond arguments");
+        rbhip_context& C = *ctx;
+        C.setDevice();
+        for (int b = 0; b < count; b++) {
+            int x = atomPairs[2 * b], y = atomPairs[2 * b + 1];
+            require(x >= 0 && y >= 0 && x < C.numAtoms && y < C.numAtoms && x != y, "bond atom out of range");
+            C.bondA.push_back(x);
+            C.bondB.push_back(y);
+            C.bondLen.push_back(length[b]);
+            C.bondK.push_back(k[b]);
+        }
+        // CSR per internal atom: each bond listed for both of its atoms
+        std::vector<int> origToInt(C.numAtoms);
+        for (int i = 0; i < C.numAtoms; i++) origToInt[C.atomOrig[i]] = i;
+        std::vector<int> start(C.numAtoms + 1, 0);
+        for (size_t b = 0; b < C.bondA.size(); b++) {
+            start[origToInt[C.bondA[b]] + 1]++;
+            start[origToInt[C.bondB[b]] + 1]++;
+        }
+        for (int i = 0; i < C.numAtoms; i++) start[i + 1] += start[i];
+        size_t ne = start[C.numAtoms];
+        std::vector<int> other(ne), pos(start.begin(), start.end() - 1);
+        std::vector<double> len(ne), kk(ne);
+        for (size_t b = 0; b < C.bondA.size(); b++) {
+            int x = origToInt[C.bondA[b]], y = origToInt[C.bondB[b]];
+            other[pos[x]] = y; len[pos[x]] = C.bondLen[b]; kk[pos[x]++] = C.bondK[b];
+            other[pos[y]] = x; len[pos[y]] = C.bondLen[b]; kk[pos[y]++] = C.bondK[b];
+        }
+        C.h.bondStart = C.arena.allocT<int>(start.size());
+        C.h.bondOther = C.arena.allocT<int>(std::max<size_t>(ne, 1));
+        C.h.bondLen = C.arena.allocT<double>(std::max<size_t>(ne, 1));
+        C.h.bondK = C.arena.allocT<double>(std::max<size_t>(ne, 1));
+        HIPCHK(hipMemcpy(C.h.bondStart, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (ne) {
+            HIPCHK(hipMemcpy(C.h.bondOther, other.data(), ne * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(C.h.bondLen, len.data(), ne * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(C.h.bondK, kk.data(), ne * sizeof(double), hipMemcpyHostToDevice));
+        }
+        C.uploadSys();
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int rbhip_compute_forces(rbhip_context* ctx, double* energy) {
+    return guard([&] {
+        require(ctx, "null context");
+        ctx->setDevice();
+        ctx->launchForces();
+        if (energy) {
+            double r[4];
+            ctx->reduce(1, r);
+            *energy = r[0];
+        }
+    });
+}
+
+int rbhip_set_positions(rbhip_context* ctx, const double* xyz) {
+    return guard([&] {
+        require(ctx && xyz, "null argument");
+        rbhip_context& C = *ctx;
+        C.setDevice();
+        std::vector<double> r = C.toInternal(xyz);
+        HIPCHK(hipMemcpyAsync(C.h.stageR, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice, C.stream));
+        HIPCHK(launch_stage_in(C.precision, C.dSys, C.h.stageR, C.numAtoms, 0, C.stream));
+        // stateChanged(Positions): forces, then geometry + dynamics (RigidBodyIntegrator.cpp:65-68)
+        C.h.useStageR = 1;
+        C.uploadSys();
+        C.launchForces();
+        C.runSetup(1);
+        C.h.useStageR = 0;
+        C.uploadSys();
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int rbhip_set_velocities(rbhip_context* ctx, const double* xyz) {
+    return guard([&] {
+        require(ctx && xyz, "null argument");
+        rbhip_context& C = *ctx;
+        C.setDevice();
+        std::vector<double> v = C.toInternal(xyz);
+        HIPCHK(hipMemcpyAsync(C.h.stageV, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, C.stream));
+        HIPCHK(launch_stage_in(C.precision, C.dSys, C.h.stageV, C.numAtoms, 1, C.stream));
+        C.stageVFresh = true;
+        // stateChanged(Velocities): dynamics rebuild on the live body geometry
+        if (C.built) C.runSetup(0);
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int rbhip_get_positions(rbhip_context* ctx, double* xyz) {
+    return guard([&] { require(ctx && xyz, "null argument"); ctx->setDevice(); ctx->download(0, xyz); });
+}
+
+int rbhip_get_velocities(rbhip_context* ctx, double* xyz) {
+    return guard([&] { require(ctx && xyz, "null argument"); ctx->setDevice(); ctx->download(1, xyz); });
+}
+
+int rbhip_get_forces(rbhip_context* ctx, double* xyz) {
+    return guard([&] { require(ctx && xyz, "null argument"); ctx->setDevice(); ctx->download(2, xyz); });
+}
+
+int rbhip_set_step_size(rbhip_context* ctx, double dt) {
+    return guard([&] {
+        require(ctx && dt > 0.0, "step size must be positive");
+        rbhip_context& C = *ctx;
+        C.setDevice();
+        C.dt = dt;
+        C.h.dt = dt;
+        C.uploadSys();
+        C.fixKick();
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int rbhip_get_time(rbhip_context* ctx, double* time, long long* stepCount) {
+    return guard([&] {
+        require(ctx, "null context");
+        if (time) *time = ctx->time;
+        if (stepCount) *stepCount = ctx->stepCount;
+    });
+}
+
+int rbhip_part1(rbhip_context* ctx) {
+    return guard([&] {
+        require(ctx, "null context");
+        require(ctx->built, "positions must be set before stepping", RBHIP_E_STATE);
+        ctx->setDevice();
+        ctx->fixKick();
+        ctx->launchPart1();
+    });
+}
+
+int rbhip_part2(rbhip_context* ctx) {
+    return guard([&] {
+        require(ctx, "null context");
+        require(ctx->built, "positions must be set before stepping", RBHIP_E_STATE);
+        ctx->setDevice();
+        ctx->launchPart2();
+        ctx->keSlots = ctx->nPartTiles;
+        ctx->stageVFresh = false;
+    });
+}
+
+int rbhip_execute(rbhip_context* ctx, int steps) {
+    return guard([&] {
+        require(ctx, "null context");
+        ctx->setDevice();
+        ctx->executeSteps(steps);
+    });
+}
+
+int rbhip_use_graph(rbhip_context* ctx, int enable) {
+    return guard([&] {
+        require(ctx, "null context");
+        ctx->useGraph = enable != 0;
+        if (!ctx->useGraph) ctx->graph.reset();
+    });
+}
+
+int rbhip_kinetic_energies(rbhip_context* ctx, double* out2) {
+    return guard([&] {
+        require(ctx && out2, "null argument");
+        ctx->setDevice();
+        double r[4] = {0, 0, 0, 0};
+        if (ctx->keSlots > 0) ctx->reduce(0, r);
+        out2[0] = 0.5 * r[0];
+        out2[1] = 0.5 * r[1];
+    });
+}
+
+int rbhip_potential_energy(rbhip_context* ctx, double* energy) {
+    return guard([&] {
+        require(ctx && energy, "null argument");
+        ctx->setDevice();
+        double r[4];
+        ctx->reduce(1, r);
+        *energy = r[0];
+    });
+}
+
+int rbhip_get_body_states(rbhip_context* ctx, double* out) {
+    return guard([&] {
+        require(ctx && out, "null argument");
+        rbhip_context& C = *ctx;
+        require(C.built, "body state not built", RBHIP_E_STATE);
+        if (C.numBodies == 0) return;
+        C.setDevice();
+        C.fixKick();
+        HIPCHK(launch_body_states(C.precision, C.dSys, C.dBodyN, C.dBodyAtom0, C.numBodies, C.dTmp, C.stream));
+        std::vector<double> tmp(36 * (size_t)C.numBodies);
+        HIPCHK(hipMemcpyAsync(tmp.data(), C.dTmp, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        for (int ib = 0; ib < C.numBodies; ib++) {
+            int r = C.internalBody[ib];
+            std::memcpy(out + 36 * (size_t)r, tmp.data() + 36 * (size_t)ib, 36 * sizeof(double));
+            out[36 * (size_t)r + 32] = C.refBodyLoc[r];
+        }
+    });
+}
+
+int rbhip_get_body_fixed_positions(rbhip_context* ctx, double* xyz) {
+    return guard([&] {
+        require(ctx && xyz, "null argument");
+        rbhip_context& C = *ctx;
+        if (C.numBodyAtoms == 0) return;
+        C.setDevice();
+        size_t n = 3 * (size_t)C.numBodyAtoms;
+        std::vector<double> d(n);
+        if (C.precision == RBHIP_PRECISION_MIXED) {
+            std::vector<float> f(n);
+            HIPCHK(hipMemcpy(f.data(), C.h.d, n * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; i++) d[i] = f[i];
+        } else {
+            HIPCHK(hipMemcpy(d.data(), C.h.d, n * 8, hipMemcpyDeviceToHost));
+        }
+        for (int ib = 0; ib < C.numBodies; ib++) {
+            int r = C.internalBody[ib];
+            size_t src = 3 * (size_t)(C.bodyAtom0[ib] - C.numFree), dst = 3 * (size_t)C.refBodyLoc[r];
+            std::memcpy(xyz + dst, d.data() + src, 3 * C.bodyN[ib] * sizeof(double));
+        }
+    });
+}
+
+int rbhip_set_launch(rbhip_context* ctx, int blockSize, int maxBlocks) {
+    return guard([&] {
+        require(ctx, "null context");
+        require(blockSize >= 64 && blockSize <= ctx->maxBlock() && blockSize % 64 == 0,
+                "block size must be a multiple of 64 in [64, " + std::to_string(ctx->maxBlock()) + "]");
+        require(maxBlocks >= 0, "negative grid cap");
+        for (auto& kv : ctx->lanes) require(blockSize % kv.second == 0, "block size must be a multiple of the lanes per body");
+        ctx->setDevice();
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        int oldBlock = ctx->block, oldMax = ctx->maxBlocks;
+        ctx->block = blockSize;
+        ctx->maxBlocks = maxBlocks;
+        try {
+            ctx->rebuildTiles();
+        } catch (...) {
+            ctx->block = oldBlock;
+            ctx->maxBlocks = oldMax;
+            ctx->rebuildTiles();
+            throw;
+        }
+    });
+}
+
+int rbhip_set_lanes(rbhip_context* ctx, int atomsPerBody, int lanes) {
+    return guard([&] {
+        require(ctx, "null context");
+        require(lanes >= 1 && lanes <= 64 && (lanes & (lanes - 1)) == 0, "lanes per body must be a power of two in [1, 64]");
+        require(ctx->block % lanes == 0, "lanes per body must divide the block size");
+        ctx->setDevice();
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->lanes[atomsPerBody] = lanes;
+        ctx->rebuildTiles();
+    });
+}
+
+int rbhip_describe(rbhip_context* ctx, char* buf, int len) {
+    return guard([&] {
+        require(ctx && buf && len > 0, "null argument");
+        std::string s = "block=" + std::to_string(ctx->block) + " maxBlocks=" + std::to_string(ctx->maxBlocks) +
+                        " partTiles=" + std::to_string(ctx->nPartTiles) + " classes=[";
+        for (auto& c : ctx->classes) {
+            auto it = ctx->lanes.find(c.N);
+            s += "N" + std::to_string(c.N) + "x" + std::to_string(c.count) + ":L" +
+                 std::to_string(it == ctx->lanes.end() ? 1 : it->second) + " ";
+        }
+        s += "] free=" + std::to_string(ctx->numFree);
+        std::snprintf(buf, len, "%s", s.c_str());
+    });
+}
+
+
+int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* totalMs) {
+    return guard([&] {
+        require(ctx && avgMs && steps > 0, "bad arguments");
+        rbhip_context& C = *ctx;
+        require(C.built, "positions must be set before stepping", RBHIP_E_STATE);
+        C.setDevice();
+        C.fixKick();
+        time_steps(C.stream, steps, [&] { C.launchPart1(); }, [&] { C.launchForces(); }, [&] { C.launchPart2(); },
+                   avgMs, totalMs);
+        C.keSlots = C.nPartTiles;
+        C.stageVFresh = false;
+        C.time += C.dt * steps;
+        C.stepCount += steps;
+    });
+}
+
+int rbhip_autotune(rbhip_context* ctx, int reps) {
+    return guard([&] {
+        require(ctx && reps > 0, "bad arguments");
+        rbhip_context& C = *ctx;
+        require(C.built, "positions must be set before tuning", RBHIP_E_STATE);
+        C.setDevice();
+        C.fixKick();
+        HIPCHK(hipStreamSynchronize(C.stream));
+        // snapshot of everything the step writes
+        const size_t R = C.realSize();
+        const size_t nb = std::max(C.numBodies, 1), na = C.numAtoms;
+        struct Span { void* p; size_t n; };
+        std::vector<Span> spans = {{C.h.q, 4 * R * nb}, {C.h.pi, 4 * R * nb}, {C.h.pm, 4 * R * nb},
+                                   {C.h.rcm, 3 * sizeof(double) * C.h.bodyStride}, {C.h.pos, 3 * R * na},
+                                   {C.h.vel, 3 * R * na}, {C.h.force, 3 * R * na}};
+        if (C.h.posLo) spans.push_back({C.h.posLo, 3 * sizeof(float) * na});
+        spans.push_back({C.h.kePart, 4 * sizeof(double) * (size_t)C.maxKeSlots});
+        spans.push_back({C.h.pePart, 4 * sizeof(double) * (size_t)C.maxPeSlots});
+        const int savedKeSlots = C.keSlots;
+        DeviceArena snap;
+        std::vector<void*> copies;
+        for (auto& s : spans) {
+            void* p = snap.alloc(s.n);
+            HIPCHK(hipMemcpyAsync(p, s.p, s.n, hipMemcpyDeviceToDevice, C.stream));
+            copies.push_back(p);
+        }
+        auto restore = [&] {
+            for (size_t i = 0; i < spans.size(); i++)
+                HIPCHK(hipMemcpyAsync(spans[i].p, copies[i], spans[i].n, hipMemcpyDeviceToDevice, C.stream));
+            HIPCHK(hipStreamSynchronize(C.stream));
+        };
+        auto measure = [&]() -> double {
+            double avg[3], total;
+            // one untimed step, then `reps` timed steps
+            C.launchPart1(); C.launchForces(); C.launchPart2();
+            time_steps(C.stream, reps, [&] { C.launchPart1(); }, [&] { C.launchForces(); },
+                       [&] { C.launchPart2(); }, avg, &total);
+            restore();
+            return avg[0] + avg[2];
+        };
+        try {
+            const int blocks[] = {64, 128, 256, 512};
+            int bestBlock = C.block;
+            double best = 1e300;
+            std::map<int, int> savedLanes = C.lanes;
+            for (int b : blocks) {
+                bool ok = b <= C.maxBlock();
+                for (auto& kv : C.lanes) ok = ok && b % kv.second == 0;
+                if (!ok) continue;
+                C.block = b;
+                C.rebuildTiles();
+                double t = measure();
+                if (t < best) { best = t; bestBlock = b; }
+            }
+            C.block = bestBlock;
+            C.rebuildTiles();
+            for (auto& cl : C.classes) {
+                int bestL = savedLanes.count(cl.N) ? savedLanes[cl.N] : 1;
+                double bestT = 1e300;
+                int maxL = std::min(16, pow2_at_least(cl.N));
+                for (int L = 1; L <= maxL; L <<= 1) {
+                    if (C.block % L) continue;
+                    C.lanes[cl.N] = L;
+                    C.rebuildTiles();
+                    double t = measure();
+                    if (t < bestT) { bestT = t; bestL = L; }
+                }
+                C.lanes[cl.N] = bestL;
+            }
+            C.rebuildTiles();
+        } catch (...) {
+            restore();
+            snap.release();
+            throw;
+        }
+        snap.release();
+        C.keSlots = savedKeSlots;
+    });
+}
+
+int rbhip_copy_calibration(size_t bytes, int reps, double* avgMs) {
+    return guard([&] {
+        require(bytes >= 16 && reps > 0 && avgMs, "bad arguments");
+        DeviceArena a;
+        void* src = a.alloc(bytes);
+        void* dst = a.alloc(bytes);
+        HIPCHK(hipMemset(src, 1, bytes));
+        hipStream_t s;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        EventSet es(2 * (size_t)reps);
+        double acc = 0.0;
+        try {
+            for (int r = 0; r < reps; r++) {
+                HIPCHK(hipEventRecord(es.ev[2 * r], s));
+                HIPCHK(launch_copy(src, dst, bytes, s));
+                HIPCHK(hipEventRecord(es.ev[2 * r + 1], s));
+            }
+            HIPCHK(hipStreamSynchronize(s));
+            for (int r = 0; r < reps; r++) {
+                float ms;
+                HIPCHK(hipEventElapsedTime(&ms, es.ev[2 * r], es.ev[2 * r + 1]));
+                acc += ms;
+            }
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            a.release();
+            throw;
+        }
+        (void)hipStreamDestroy(s);
+        a.release();
+        *avgMs = acc / reps;
+    });
+}
+
+// ---------------------------------------------------------------------------
+int rbhip_batch_create(int device, rbhip_batch** out) {
+    return guard([&] {
+        require(out, "null output handle");
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        require(device >= 0 && device < ndev, "no such device");
+        HIPCHK(hipSetDevice(device));
+        std::unique_ptr<rbhip_batch> b(new rbhip_batch());
+        b->device = device;
+        HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&b->ordering, hipEventDisableTiming));
+        *out = b.release();
+    });
+}
+
+int rbhip_batch_destroy(rbhip_batch* batch) {
+    return guard([&] {
+        if (!batch) return;
+        (void)hipSetDevice(batch->device);
+        (void)hipStreamSynchronize(batch->stream);
+        delete batch;
+    });
+}
+
+int rbhip_batch_set_members(rbhip_batch* batch, int count, rbhip_context* const* members) {
+    return guard([&] {
+        require(batch && count >= 0 && (count == 0 || members), "bad arguments");
+        HIPCHK(hipSetDevice(batch->device));
+        HIPCHK(hipStreamSynchronize(batch->stream));
+        std::vector<rbhip_context*> m(members, members + count);
+        for (size_t i = 0; i < m.size(); i++) {
+            require(m[i] != nullptr, "null member");
+            require(m[i]->device == batch->device, "member on another device");
+            require(m[i]->built, "member body state not built", RBHIP_E_STATE);
+            require(m[i]->precision == m[0]->precision, "members must share the precision");
+            require((m[i]->rotationMode == 0) == (m[0]->rotationMode == 0), "members must share the rotation kind");
+            for (auto& kv : m[i]->lanes)
+                require(batch->block % kv.second == 0, "member lanes per body must divide the batch block size");
+            HIPCHK(hipStreamSynchronize(m[i]->stream));
+            m[i]->fixKick();
+            HIPCHK(hipStreamSynchronize(m[i]->stream));
+        }
+        if (!m.empty()) {
+            batch->precision = m[0]->precision;
+            batch->rotationMode = m[0]->rotationMode;
+            if (batch->rotationMode == 0 && batch->block > 256) batch->block = 256;
+        }
+        batch->members = m;
+        batch->rebuild();
+    });
+}
+
+int rbhip_batch_set_launch(rbhip_batch* batch, int blockSize, int maxBlocks) {
+    return guard([&] {
+        require(batch, "null batch");
+        int cap = batch->rotationMode == 0 && !batch->members.empty() ? 256 : 512;
+        require(blockSize >= 64 && blockSize <= cap && blockSize % 64 == 0,
+                "block size must be a multiple of 64 in [64, " + std::to_string(cap) + "]");
+        require(maxBlocks >= 0, "negative grid cap");
+        HIPCHK(hipSetDevice(batch->device));
+        HIPCHK(hipStreamSynchronize(batch->stream));
+        batch->block = blockSize;
+        batch->maxBlocks = maxBlocks;
+        batch->rebuild();
+    });
+}
+
+int rbhip_batch_use_graph(rbhip_batch* batch, int enable) {
+    return guard([&] {
+        require(batch, "null batch");
+        batch->useGraph = enable != 0;
+        if (!batch->useGraph) batch->graph.reset();
+    });
+}
+
+int rbhip_batch_execute(rbhip_batch* batch, int steps) {
+    return guard([&] {
+        require(batch && steps >= 0, "bad arguments");
+        if (batch->members.empty() || steps == 0) return;
+        HIPCHK(hipSetDevice(batch->device));
+        for (rbhip_context* c : batch->members)
+            require(c->kickDt == c->dt, "member step size changed since it joined the batch", RBHIP_E_STATE);
+        batch->joinMembers();
+        if (batch->useGraph) {
+            batch->ensureGraph();
+            for (int s = 0; s < steps; s++) HIPCHK(hipGraphLaunch(batch->graph.exec, batch->stream));
+        } else {
+            for (int s = 0; s < steps; s++) batch->launchAll();
+        }
+        batch->releaseMembers();
+        batch->finishSteps(steps);
+    });
+}
+
+int rbhip_batch_kinetic_energies(rbhip_batch* batch, double* out) {
+    return guard([&] {
+        require(batch && out, "null argument");
+        size_t n = batch->members.size();
+        if (n == 0) return;
+        HIPCHK(hipSetDevice(batch->device));
+        batch->joinMembers();
+        std::vector<int> slots(n);
+        for (size_t m = 0; m < n; m++) slots[m] = batch->members[m]->keSlots;
+        HIPCHK(hipMemcpyAsync(batch->dKeSlots, slots.data(), n * sizeof(int), hipMemcpyHostToDevice, batch->stream));
+        HIPCHK(launch_reduce(batch->dSystems, batch->dKeSlots, (int)n, 0, batch->dRed, batch->stream));
+        std::vector<double> r(4 * n);
+        HIPCHK(hipMemcpyAsync(r.data(), batch->dRed, r.size() * sizeof(double), hipMemcpyDeviceToHost, batch->stream));
+        HIPCHK(hipStreamSynchronize(batch->stream));
+        for (size_t m = 0; m < n; m++) {
+            out[2 * m] = 0.5 * r[4 * m];
+            out[2 * m + 1] = 0.5 * r[4 * m + 1];
+        }
+    });
+}
+
+int rbhip_batch_time_kernels(rbhip_batch* batch, int steps, double* avgMs, double* totalMs) {
+    return guard([&] {
+        require(batch && avgMs && steps > 0, "bad arguments");
+        require(!batch->members.empty(), "empty batch", RBHIP_E_STATE);
+        HIPCHK(hipSetDevice(batch->device));
+        rbhip_batch& B = *batch;
+        B.joinMembers();
+        time_steps(B.stream, steps,
+                   [&] { HIPCHK(launch_part1(B.precision, B.rotationMode, B.dSystems, B.dPartTiles, B.nPart, B.block, B.maxBlocks, B.stream)); },
+                   [&] { HIPCHK(launch_forces(B.precision, B.dSystems, B.dForceTiles, B.nForce, B.block, B.maxBlocks, B.stream)); },
+                   [&] { HIPCHK(launch_part2(B.precision, B.dSystems, B.dPartTiles, B.nPart, B.block, B.maxBlocks, B.stream)); },
+                   avgMs, totalMs);
+        B.releaseMembers();
+        B.finishSteps(steps);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,559 @@
+// Device math for the rigid-body step: quaternion algebra, free-rotor
+// rotations (NO-SQUISH and exact), FP64 elliptic integrals, 3x3 symmetric
+// eigen-decomposition.  gfx950, wave64.
+//
+// Quaternion convention (reference openmmapi/src/MatVec.cpp:495-549):
+// q = (q0, q1, q2, q3); B(q) has columns q (x) e_k, C(q) columns e_k (x) q;
+// A(q) = B(q)^T C(q) maps space -> body, A(q)^T = C(q)^T B(q) body -> space;
+// the body angular momentum is L = B(q)^T pi / 2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace rbhip {
+
+template <class T> struct V3 { T x, y, z; };
+template <class T> struct Q4 { T a, b, c, d; };  // (q0, q1, q2, q3)
+
+template <class T> __device__ __forceinline__ V3<T> v3(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T> __device__ __forceinline__ Q4<T> q4(T a, T b, T c, T d) { return Q4<T>{a, b, c, d}; }
+
+template <class T> __device__ __forceinline__ V3<T> operator+(V3<T> u, V3<T> v) { return {u.x + v.x, u.y + v.y, u.z + v.z}; }
+template <class T> __device__ __forceinline__ V3<T> operator-(V3<T> u, V3<T> v) { return {u.x - v.x, u.y - v.y, u.z - v.z}; }
+template <class T> __device__ __forceinline__ V3<T> operator*(V3<T> u, T s) { return {u.x * s, u.y * s, u.z * s}; }
+template <class T> __device__ __forceinline__ T dot(V3<T> u, V3<T> v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
+template <class T> __device__ __forceinline__ V3<T> cross(V3<T> u, V3<T> v) {
+    return {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
+}
+template <class T> __device__ __forceinline__ V3<T> hadamard(V3<T> u, V3<T> v) { return {u.x * v.x, u.y * v.y, u.z * v.z}; }
+template <class T> __device__ __forceinline__ Q4<T> operator+(Q4<T> u, Q4<T> v) { return {u.a + v.a, u.b + v.b, u.c + v.c, u.d + v.d}; }
+template <class T> __device__ __forceinline__ Q4<T> operator*(Q4<T> u, T s) { return {u.a * s, u.b * s, u.c * s, u.d * s}; }
+template <class T> __device__ __forceinline__ T dot(Q4<T> u, Q4<T> v) { return u.a * v.a + u.b * v.b + u.c * v.c + u.d * v.d; }
+
+template <class To, class From> __device__ __forceinline__ V3<To> cvt(V3<From> v) { return {(To)v.x, (To)v.y, (To)v.z}; }
+template <class To, class From> __device__ __forceinline__ Q4<To> cvt(Q4<From> v) { return {(To)v.a, (To)v.b, (To)v.c, (To)v.d}; }
+
+// B(q) x
+template <class T> __device__ __forceinline__ Q4<T> Bmul(Q4<T> q, V3<T> x) {
+    return {-q.b * x.x - q.c * x.y - q.d * x.z,
+             q.a * x.x - q.d * x.y + q.c * x.z,
+             q.d * x.x + q.a * x.y - q.b * x.z,
+            -q.c * x.x + q.b * x.y + q.a * x.z};
+}
+// C(q) x
+template <class T> __device__ __forceinline__ Q4<T> Cmul(Q4<T> q, V3<T> x) {
+    return {-q.b * x.x - q.c * x.y - q.d * x.z,
+             q.a * x.x + q.d * x.y - q.c * x.z,
+            -q.d * x.x + q.a * x.y + q.b * x.z,
+             q.c * x.x - q.b * x.y + q.a * x.z};
+}
+// B(q)^T y
+template <class T> __device__ __forceinline__ V3<T> Btmul(Q4<T> q, Q4<T> y) {
+    return {-q.b * y.a + q.a * y.b + q.d * y.c - q.c * y.d,
+            -q.c * y.a - q.d * y.b + q.a * y.c + q.b * y.d,
+            -q.d * y.a + q.c * y.b - q.b * y.c + q.a * y.d};
+}
+// C(q)^T y
+template <class T> __device__ __forceinline__ V3<T> Ctmul(Q4<T> q, Q4<T> y) {
+    return {-q.b * y.a + q.a * y.b - q.d * y.c + q.c * y.d,
+            -q.c * y.a + q.d * y.b + q.a * y.c - q.b * y.d,
+            -q.d * y.a - q.c * y.b + q.b * y.c + q.a * y.d};
+}
+// P_k q = q (x) e_k
+template <int K, class T> __device__ __forceinline__ Q4<T> perm(Q4<T> q) {
+    if (K == 1) return {-q.b, q.a, q.d, -q.c};
+    if (K == 2) return {-q.c, -q.d, q.a, q.b};
+    return {-q.d, q.c, -q.b, q.a};
+}
+template <class T> __device__ __forceinline__ V3<T> Amul(Q4<T> q, V3<T> x) { return Btmul(q, Cmul(q, x)); }
+template <class T> __device__ __forceinline__ V3<T> Atmul(Q4<T> q, V3<T> x) { return Ctmul(q, Bmul(q, x)); }
+
+// The matrix of A(q)^T (body -> space), built once per body: rows r0..r2.
+template <class T> struct Rot { V3<T> r0, r1, r2; };
+template <class T> __device__ __forceinline__ Rot<T> rotT(Q4<T> q) {
+    T aa = q.a * q.a, bb = q.b * q.b, cc = q.c * q.c, dd = q.d * q.d;
+    T ab = q.a * q.b, ac = q.a * q.c, ad = q.a * q.d;
+    T bc = q.b * q.c, bd = q.b * q.d, cd = q.c * q.d;
+    Rot<T> R;
+    R.r0 = {aa + bb - cc - dd, (T)2 * (bc - ad), (T)2 * (ac + bd)};
+    R.r1 = {(T)2 * (ad + bc), aa - bb + cc - dd, (T)2 * (cd - ab)};
+    R.r2 = {(T)2 * (bd - ac), (T)2 * (ab + cd), aa - bb - cc + dd};
+    return R;
+}
+template <class T> __device__ __forceinline__ V3<T> apply(const Rot<T>& R, V3<T> x) {
+    return {dot(R.r0, x), dot(R.r1, x), dot(R.r2, x)};
+}
+
+// ---------------------------------------------------------------------------
+// sin(phi) and cos(phi) - 1.  The rotation below updates q <- q + q (cos-1)
+// + P q sin, so the scale factor (1 + (cos-1))^2 + sin^2 stays 1 to the
+// accuracy of (cos-1) itself rather than to the rounding of cos near 1.
+// fp32: Taylor polynomials for |phi| <= 1/4 (truncation < 5e-11 relative);
+// fp64: Taylor polynomials for |phi| <= 1/16; otherwise half-angle
+// identities on the library sincos.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void sin_cosm1(float phi, float& s, float& cm1) {
+    if (fabsf(phi) <= 0.25f) {
+        float x2 = phi * phi;
+        s = phi * fmaf(x2, fmaf(x2, fmaf(x2, -1.0f / 5040.0f, 1.0f / 120.0f), -1.0f / 6.0f), 1.0f);
+        cm1 = x2 * fmaf(x2, fmaf(x2, fmaf(x2, 1.0f / 40320.0f, -1.0f / 720.0f), 1.0f / 24.0f), -0.5f);
+    } else {
+        float sh, ch;
+        sincosf(0.5f * phi, &sh, &ch);
+        s = 2.0f * sh * ch;
+        cm1 = -2.0f * sh * sh;
+    }
+}
+__device__ __forceinline__ void sin_cosm1(double phi, double& s, double& cm1) {
+    if (fabs(phi) <= 0.0625) {
+        // Taylor polynomials through phi^9 / phi^10: truncation < 2e-20 relative
+        double x2 = phi * phi;
+        s = phi * fma(x2, fma(x2, fma(x2, fma(x2, 1.0 / 362880.0, -1.0 / 5040.0), 1.0 / 120.0), -1.0 / 6.0), 1.0);
+        cm1 = x2 * fma(x2, fma(x2, fma(x2, fma(x2, -1.0 / 3628800.0, 1.0 / 40320.0), -1.0 / 720.0), 1.0 / 24.0), -0.5);
+    } else {
+        double sh, ch;
+        sincos(0.5 * phi, &sh, &ch);
+        s = 2.0 * sh * ch;
+        cm1 = -2.0 * sh * sh;
+    }
+}
+
+// uniaxial rotation about principal axis K (reference RigidBody.cpp:189-214):
+// phi = (pi . P_K q) h / (4 I_K); coef = h / (4 I_K)
+template <int K, class T> __device__ __forceinline__ void uniaxial(Q4<T>& q, Q4<T>& pi, T coef) {
+    Q4<T> Pq = perm<K>(q);
+    Q4<T> Ppi = perm<K>(pi);
+    T phi = dot(pi, Pq) * coef;
+    T s, cm1;
+    sin_cosm1(phi, s, cm1);
+    q = {fma(q.a, cm1, fma(Pq.a, s, q.a)), fma(q.b, cm1, fma(Pq.b, s, q.b)),
+         fma(q.c, cm1, fma(Pq.c, s, q.c)), fma(q.d, cm1, fma(Pq.d, s, q.d))};
+    pi = {fma(pi.a, cm1, fma(Ppi.a, s, pi.a)), fma(pi.b, cm1, fma(Ppi.b, s, pi.b)),
+          fma(pi.c, cm1, fma(Ppi.c, s, pi.c)), fma(pi.d, cm1, fma(Ppi.d, s, pi.d))};
+}
+
+// One uniaxial rotation of NO-SQUISH in body-frame variables.  For
+// pi = 2 B(q) L and |q| = 1 the reference's angle (pi . P_K q) tau / (4 I_K)
+// (RigidBody.cpp:189-214) equals L_K tau / (2 I_K), so the angles depend on L
+// only: L turns about body axis K by -2 phi, and the orientation update
+// q <- q (x) (cos phi, sin phi e_K) is accumulated into a composite rotation
+// r, applied to q once per step.
+//
+// Within a step r stays near the identity and L near its start value L0, so
+// both are carried as offsets, r = 1 + rho and L = L0 + lam: their rounding
+// errors scale with |rho| ~ w dt / 2 and |lam| ~ w dt |L| instead of with 1
+// and |L|.  In fp32 this keeps the 50 sequential updates of a 10-substep step
+// from random-walking the orientation (~1e-6 rad per 10 steps when q itself is
+// updated in fp32) at the cost of fp32 arithmetic.
+template <int K, class T>
+__device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho, T coef) {
+    T LK = K == 1 ? L0.x + lam.x : (K == 2 ? L0.y + lam.y : L0.z + lam.z);
+    T s, cm1;
+    sin_cosm1(LK * coef, s, cm1);
+    // rho <- rho + rho cm1 + P_K(rho) s + (cm1, s e_K)   [r (x) (1 + cm1, s e_K)]
+    Q4<T> Pr = perm<K>(rho);
+    Q4<T> nr = {fma(rho.a, cm1, fma(Pr.a, s, rho.a)) + cm1, fma(rho.b, cm1, fma(Pr.b, s, rho.b)),
+                fma(rho.c, cm1, fma(Pr.c, s, rho.c)), fma(rho.d, cm1, fma(Pr.d, s, rho.d))};
+    if (K == 1) nr.b += s;
+    if (K == 2) nr.c += s;
+    if (K == 3) nr.d += s;
+    rho = nr;
+    // L turns by -2 phi about axis K: sin(2 phi), cos(2 phi) - 1 from the half angle
+    T s2 = (T)2 * s * ((T)1 + cm1);
+    T c2m1 = (T)2 * cm1 * ((T)2 + cm1);
+    T& la = K == 1 ? lam.y : (K == 2 ? lam.z : lam.x);
+    T& lb = K == 1 ? lam.z : (K == 2 ? lam.x : lam.y);
+    T La = (K == 1 ? L0.y : (K == 2 ? L0.z : L0.x)) + la;
+    T Lb = (K == 1 ? L0.z : (K == 2 ? L0.x : L0.y)) + lb;
+    la = fma(La, c2m1, fma(Lb, s2, la));
+    lb = fma(Lb, c2m1, fma(-La, s2, lb));
+}
+
+// NO-SQUISH (reference RigidBody.cpp:220-231): n substeps of
+// [R3(h/2)] R2(h/2) R1(h) R2(h/2) [R3(h/2)], R3 only for dof-6 bodies
+// (invI.z == 0 marks the dof-5 linear bodies).  On return q = q (x) r
+// renormalised in FP64 and pi = 2 B(q) L, both formed in FP64.
+template <class T>
+__device__ __forceinline__ void no_squish(Q4<T>& q, Q4<T>& pi, V3<T> invI, double dt, int n) {
+    Q4<double> qd = cvt<double>(q);
+    V3<double> Ld = Btmul(qd, cvt<double>(pi)) * 0.5;
+    V3<T> L0 = cvt<T>(Ld);
+    V3<T> lam = {(T)0, (T)0, (T)0};
+    Q4<T> rho = {(T)0, (T)0, (T)0, (T)0};
+    T h = (T)(dt / (double)n);
+    T c1 = (T)0.5 * h * invI.x;
+    T c2 = (T)0.25 * h * invI.y;
+    T c3 = (T)0.25 * h * invI.z;
+    if (invI.z != (T)0) {
+        for (int i = 0; i < n; i++) {
+            body_axis_turn<3>(L0, lam, rho, c3);
+            body_axis_turn<2>(L0, lam, rho, c2);
+            body_axis_turn<1>(L0, lam, rho, c1);
+            body_axis_turn<2>(L0, lam, rho, c2);
+            body_axis_turn<3>(L0, lam, rho, c3);
+        }
+    } else {
+        for (int i = 0; i < n; i++) {
+            body_axis_turn<2>(L0, lam, rho, c2);
+            body_axis_turn<1>(L0, lam, rho, c1);
+            body_axis_turn<2>(L0, lam, rho, c2);
+        }
+    }
+    // q (x) r, renormalised in FP64; L = L0 + lam in FP64
+    Q4<double> r = {1.0 + (double)rho.a, (double)rho.b, (double)rho.c, (double)rho.d};
+    Q4<double> qn = qd * r.a + perm<1>(qd) * r.b + perm<2>(qd) * r.c + perm<3>(qd) * r.d;
+    double e = dot(qn, qn) - 1.0;
+    double sc = fabs(e) < 1e-4 ? 1.0 + e * (-0.5 + 0.375 * e) : 1.0 / sqrt(1.0 + e);
+    qn = qn * sc;
+    V3<double> L = cvt<double>(L0) + cvt<double>(lam);
+    q = cvt<T>(qn);
+    pi = cvt<T>(Bmul(qn, L) * 2.0);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 elliptic integrals, Carlson, Numer. Algorithms 10 (1995) 13-26,
+// Algorithms 1-3 (tolerance r = 1e-16), and Jacobi functions by the AGM /
+// descending Landen transformation (DLMF 22.20(ii)).
+// ---------------------------------------------------------------------------
+__device__ inline double carlson_rc(double x, double y) {
+    double A0 = (x + 2.0 * y) * (1.0 / 3.0);
+    double Q = 87.16855428717356 * fabs(A0 - x);   // (3e-16)^(-1/8)
+    double xm = x, ym = y, Am = A0, scale = 1.0;
+    for (int m = 0; m < 60 && scale * Q >= fabs(Am); m++) {
+        double lam = 2.0 * sqrt(xm) * sqrt(ym) + ym;
+        xm = 0.25 * (xm + lam);
+        ym = 0.25 * (ym + lam);
+        Am = 0.25 * (Am + lam);
+        scale *= 0.25;
+    }
+    double s = (y - A0) * scale / Am;
+    double poly = 1.0 + s * s * (3.0 / 10.0 + s * (1.0 / 7.0 + s * (3.0 / 8.0 + s * (9.0 / 22.0 +
+                  s * (159.0 / 208.0 + s * (9.0 / 8.0))))));
+    return poly / sqrt(Am);
+}
+
+__device__ inline double carlson_rf(double x, double y, double z) {
+    double A0 = (x + y + z) * (1.0 / 3.0);
+    double Q = 386.49729393434575 * fmax(fabs(A0 - x), fmax(fabs(A0 - y), fabs(A0 - z)));  // (3e-16)^(-1/6)
+    double xm = x, ym = y, zm = z, Am = A0, scale = 1.0;
+    for (int m = 0; m < 60 && scale * Q >= fabs(Am); m++) {
+        double sx = sqrt(xm), sy = sqrt(ym), sz = sqrt(zm);
+        double lam = sx * sy + sx * sz + sy * sz;
+        xm = 0.25 * (xm + lam);
+        ym = 0.25 * (ym + lam);
+        zm = 0.25 * (zm + lam);
+        Am = 0.25 * (Am + lam);
+        scale *= 0.25;
+    }
+    double X = (A0 - x) * scale / Am;
+    double Y = (A0 - y) * scale / Am;
+    double Z = -(X + Y);
+    double E2 = X * Y - Z * Z;
+    double E3 = X * Y * Z;
+    return (1.0 - E2 / 10.0 + E3 / 14.0 + E2 * E2 / 24.0 - 3.0 * E2 * E3 / 44.0) / sqrt(Am);
+}
+
+__device__ inline double carlson_rj(double x, double y, double z, double p) {
+    double A0 = (x + y + z + 2.0 * p) * 0.2;
+    double delta = (p - x) * (p - y) * (p - z);
+    double Q = 584.803547642573 *   // (2.5e-17)^(-1/6)
+               fmax(fmax(fabs(A0 - x), fabs(A0 - y)), fmax(fabs(A0 - z), fabs(A0 - p)));
+    double xm = x, ym = y, zm = z, pm = p, Am = A0, scale = 1.0, scale3 = 1.0, sum = 0.0;
+    for (int m = 0; m < 60 && scale * Q >= fabs(Am); m++) {
+        double sx = sqrt(xm), sy = sqrt(ym), sz = sqrt(zm), sp = sqrt(pm);
+        double lam = sx * sy + sx * sz + sy * sz;
+        double dm = (sp + sx) * (sp + sy) * (sp + sz);
+        double e = scale3 * delta / (dm * dm);
+        sum += scale * carlson_rc(1.0, 1.0 + e) / dm;
+        xm = 0.25 * (xm + lam);
+        ym = 0.25 * (ym + lam);
+        zm = 0.25 * (zm + lam);
+        pm = 0.25 * (pm + lam);
+        Am = 0.25 * (Am + lam);
+        scale *= 0.25;
+        scale3 *= 1.0 / 64.0;
+    }
+    double X = (A0 - x) * scale / Am, Y = (A0 - y) * scale / Am, Z = (A0 - z) * scale / Am;
+    double P = -0.5 * (X + Y + Z);
+    double E2 = X * Y + X * Z + Y * Z - 3.0 * P * P;
+    double E3 = X * Y * Z + 2.0 * E2 * P + 4.0 * P * P * P;
+    double E4 = (2.0 * X * Y * Z + E2 * P + 3.0 * P * P * P) * P;
+    double E5 = X * Y * Z * P * P;
+    double poly = 1.0 - 3.0 * E2 / 14.0 + E3 / 6.0 + 9.0 * E2 * E2 / 88.0 - 3.0 * E4 / 22.0 -
+                  9.0 * E2 * E3 / 52.0 + 3.0 * E5 / 26.0;
+    return scale * poly / (Am * sqrt(Am)) + 6.0 * sum;
+}
+
+// Omega(x, n, m) = -(n x^3 / 3) R_J(1 - x^2, 1 - m x^2, 1, 1 + n x^2)
+// (the function used by the exact rotation, reference ellipticFunctions.h:251-254)
+__device__ inline double omega_fn(double x, double n, double m) {
+    double x2 = x * x;
+    return (-1.0 / 3.0) * n * x * x2 * carlson_rj(1.0 - x2, 1.0 - m * x2, 1.0, 1.0 + n * x2);
+}
+
+__device__ inline void jacobi_fn(double u, double m, double& sn, double& cn, double& dn) {
+    const double eps = 2.220446049250313e-16;
+    if (m < 2.0 * eps) { sincos(u, &sn, &cn); dn = 1.0; return; }
+    if (1.0 - m < 2.0 * eps) { sn = tanh(u); cn = 1.0 / cosh(u); dn = cn; return; }
+    double K = carlson_rf(0.0, 1.0 - m, 1.0);
+    if (fabs(u) > 2.0 * K) u -= 4.0 * K * rint(u / (4.0 * K));
+    double a[25], c[25];
+    a[0] = 1.0;
+    c[0] = sqrt(m);
+    double b = sqrt(1.0 - m);
+    int n = 0;
+    while (fabs(c[n]) > eps * a[n] && n < 24) {
+        a[n + 1] = 0.5 * (a[n] + b);
+        c[n + 1] = 0.5 * (a[n] - b);
+        b = sqrt(a[n] * b);
+        n++;
+    }
+    double phi = ldexp(a[n] * u, n);
+    for (int i = n; i > 0; i--) phi = 0.5 * (phi + asin(c[i] / a[i] * sin(phi)));
+    sincos(phi, &sn, &cn);
+    dn = sqrt((1.0 - m) + m * cn * cn);
+}
+
+__device__ __forceinline__ int sign_of(double x) { return x >= 0.0 ? 1 : -1; }
+__device__ __forceinline__ int stair(double x) { return x > 0.0 ? (int)ceil(x - 0.5) : (int)floor(x + 0.5); }
+
+// free linear rotor (dof 5): body-frame L constant, rotation about L at
+// |w| = |L| / I0 (closed form; replaces the exact solution, which divides by
+// I0 - I1 = 0 for these bodies)
+__device__ inline void linear_rotor_exact(Q4<double>& q, Q4<double>& pi, V3<double> invI, double dt) {
+    V3<double> L = Btmul(q, pi) * 0.5;
+    V3<double> w = hadamard(invI, L);
+    double wn = sqrt(dot(w, w));
+    if (wn == 0.0) return;
+    double sh, ch;
+    sincos(0.5 * wn * dt, &sh, &ch);
+    V3<double> axis = w * (1.0 / wn);
+    q = q * ch + Bmul(q, axis) * sh;
+    double r = 1.0 / sqrt(dot(q, q));
+    q = q * r;
+    pi = Bmul(q, L * 2.0);
+}
+
+// exact free-rotor rotation over dt (reference RigidBody.cpp:241-308), FP64
+__device__ inline void exact_rotation(Q4<double>& q, Q4<double>& pi, V3<double> I, V3<double> invI, double dt) {
+    const double eps = 2.220446049250313e-16;
+    const double PI = 3.14159265358979323846;
+    V3<double> Iw = Btmul(q, pi) * 0.5;
+    V3<double> w0 = hadamard(invI, Iw);
+    double Lsq = Iw.y * Iw.y + Iw.z * Iw.z;
+    if (Lsq < eps) {
+        uniaxial<1>(q, pi, 0.25 * dt * invI.x);
+        return;
+    }
+    Lsq += Iw.x * Iw.x;
+    double L = sqrt(Lsq);
+    double twoKr = dot(Iw, w0);
+    Q4<double> z0 = {Iw.z, Iw.y, L - Iw.x, 0.0};
+    double r1 = Lsq - twoKr * I.z;
+    double r3 = twoKr * I.x - Lsq;
+    double l1 = r1 * invI.y / (I.y - I.z);
+    double l3 = r3 * invI.y / (I.x - I.y);
+    double lmin = fmin(l1, l3);
+    double c13 = 1.0 / (I.x - I.z);
+    V3<double> a = {sign_of(w0.x) * sqrt(r1 * invI.x * c13), sqrt(lmin), sign_of(w0.z) * sqrt(r3 * invI.z * c13)};
+    double m = lmin / fmax(l1, l3);
+    double K = carlson_rf(0.0, 1.0 - m, 1.0);
+    double inv2K = 0.5 / K;
+    double s0 = w0.y / a.y;
+    double c0, u0;
+    int i0;
+    if (fabs(s0) < 1.0) {
+        c0 = l1 < l3 ? w0.x / a.x : w0.z / a.z;
+        u0 = s0 * carlson_rf(1.0 - s0 * s0, 1.0 - m * s0 * s0, 1.0);
+        i0 = stair(u0 * inv2K);
+    } else {
+        a.y = fabs(w0.y);
+        s0 = sign_of(s0);
+        c0 = 0.0;
+        u0 = s0 * K;
+        i0 = 0;
+    }
+    double wp = -invI.y * a.x * a.z / (a.y * c13);
+    double u = wp * dt + u0;
+    int jump = stair(u * inv2K) - i0;
+    double sn, cn, dn, deltaF;
+    jacobi_fn(u, m, sn, cn, dn);
+    double alpha = I.x * a.x / L;
+    double eta = alpha * alpha;
+    eta /= 1.0 - eta;
+    V3<double> Ia = hadamard(I, a);
+    if (l1 < l3) {
+        double C = sqrt(m + eta);
+        deltaF = u - u0 + sign_of(cn) * omega_fn(sn, eta, m) - sign_of(c0) * omega_fn(s0, eta, m) +
+                 (alpha / C) * (atan(C * sn / dn) - atan(C * s0 * a.z / w0.z));
+        if (jump != 0) deltaF += jump * 2.0 * omega_fn(1.0, eta, m);
+        Iw = hadamard(Ia, V3<double>{cn, sn, dn});
+    } else {
+        double k2eta = m * eta;
+        double C = sqrt(1.0 + k2eta);
+        deltaF = u - u0 + sign_of(cn) * omega_fn(sn, k2eta, m) - sign_of(c0) * omega_fn(s0, k2eta, m) +
+                 (alpha / C) * (atan(C * sn / cn) - atan(C * s0 / c0));
+        if (jump != 0) deltaF += jump * (2.0 * omega_fn(1.0, k2eta, m) + (alpha / C) * PI);
+        Iw = hadamard(Ia, V3<double>{dn, sn, cn});
+    }
+    deltaF *= 1.0 + eta;
+    double theta = (Lsq * (u - u0) + r3 * deltaF) / (2.0 * L * I.x * wp);
+    double st, ct;
+    sincos(theta, &st, &ct);
+    Q4<double> z = Q4<double>{Iw.z, Iw.y, L - Iw.x, 0.0} * ct + Q4<double>{-Iw.y, Iw.z, 0.0, L - Iw.x} * st;
+    q = z * dot(z0, q) + Cmul(z, Ctmul(z0, q));
+    double r = 1.0 / sqrt(dot(q, q));
+    q = q * r;
+    pi = Bmul(q, Iw * 2.0);
+}
+
+// ---------------------------------------------------------------------------
+// Setup math in FP64 (body frame from atom positions), written to follow the
+// reference's operation order; callers disable FMA contraction.
+// Smith (1961) eigenvalues, descending (eigenDecomposition.cpp:72-100);
+// Kopp (2008) cross-product eigenvectors (eigenDecomposition.cpp:104-176);
+// Shepperd (1978) quaternion (MatVec.cpp:344-373).
+// ---------------------------------------------------------------------------
+struct M3 { V3<double> r[3]; };
+
+__device__ inline double m3_det(const M3& A) {
+    const V3<double>& a0 = A.r[0];
+    const V3<double>& a1 = A.r[1];
+    const V3<double>& a2 = A.r[2];
+    return a0.x * (a1.y * a2.z - a2.y * a1.z) - a0.y * (a1.x * a2.z - a2.x * a1.z) +
+           a0.z * (a1.x * a2.y - a2.x * a1.y);
+}
+
+__device__ inline double& m3_at(M3& A, int i, int j) {
+    V3<double>& r = A.r[i];
+    return j == 0 ? r.x : (j == 1 ? r.y : r.z);
+}
+
+__device__ inline V3<double> eigenvalues3(const M3& A) {
+    const double eps = 2.220446049250313e-16;
+    const double PI = 3.14159265358979323846;
+    double p1 = A.r[0].y * A.r[0].y + A.r[0].z * A.r[0].z + A.r[1].z * A.r[1].z;
+    if (p1 < eps) {
+        double w0 = A.r[0].x, w1 = A.r[1].y, w2 = A.r[2].z, t;
+        if (w0 < w1) { t = w0; w0 = w1; w1 = t; }
+        if (w0 < w2) { t = w0; w0 = w2; w2 = t; }
+        if (w1 < w2) { t = w1; w1 = w2; w2 = t; }
+        return {w0, w1, w2};
+    }
+    double tr = A.r[0].x + A.r[1].y + A.r[2].z;
+    double q = tr / 3.0;
+    V3<double> d = {A.r[0].x - q, A.r[1].y - q, A.r[2].z - q};
+    double p2 = dot(d, d) + 2.0 * p1;
+    double p = sqrt(p2 / 6.0);
+    M3 B = A;
+    B.r[0].x -= q; B.r[1].y -= q; B.r[2].z -= q;
+    double r = m3_det(B) * (3.0 / (p * p2));
+    double phi = r <= -1.0 ? PI / 3.0 : (r >= 1.0 ? 0.0 : acos(r) / 3.0);
+    double w0 = q + 2.0 * p * cos(phi);
+    double w2 = q + 2.0 * p * cos(phi + 2.0 * PI / 3.0);
+    return {w0, tr - (w0 + w2), w2};
+}
+
+__device__ inline void finish_eigenvector(V3<double>& v, const M3& a, double n1b, double n2b, double thresh) {
+    const double eps = 2.220446049250313e-16;
+    double norm = dot(v, v);
+    double n1 = n1b + a.r[0].x * a.r[0].x;
+    double n2 = n2b + a.r[1].y * a.r[1].y;
+    double err = n1 * n2;
+    if (n1 <= thresh) {
+        v = {1.0, 0.0, 0.0};
+    } else if (n2 <= thresh) {
+        v = {0.0, 1.0, 0.0};
+    } else if (norm < 4096.0 * eps * eps * err) {
+        double t = fabs(a.r[0].y);
+        double f = -a.r[0].x / a.r[0].y;
+        if (fabs(a.r[1].y) > t) { t = fabs(a.r[1].y); f = -a.r[0].y / a.r[1].y; }
+        if (fabs(a.r[1].z) > t) f = -a.r[0].z / a.r[1].z;
+        double s = 1.0 / sqrt(1.0 + f * f);
+        v = {s, f * s, 0.0};
+    } else {
+        v = v * sqrt(1.0 / norm);
+    }
+}
+
+// rows = eigenvectors of w.x, w.y, w.z
+__device__ inline M3 eigenvectors3(const M3& A0, V3<double> w) {
+    const double eps = 2.220446049250313e-16;
+    double wmax8eps = 8.0 * eps * fabs(w.x);
+    double thresh = wmax8eps * wmax8eps;
+    M3 a = A0;
+    a.r[1].x = a.r[0].y; a.r[2].x = a.r[0].z; a.r[2].y = a.r[1].z;
+    double n1 = a.r[0].y * a.r[0].y + a.r[0].z * a.r[0].z;
+    double n2 = a.r[0].y * a.r[0].y + a.r[1].z * a.r[1].z;
+    double base0 = a.r[0].y * a.r[1].z - a.r[0].z * a.r[1].y;
+    double base1 = a.r[0].z * a.r[0].y - a.r[1].z * a.r[0].x;
+    double a01sq = a.r[0].y * a.r[0].y;
+    a.r[0].x -= w.x;
+    a.r[1].y -= w.x;
+    V3<double> v0 = {base0 + a.r[0].z * w.x, base1 + a.r[1].z * w.x, a.r[0].x * a.r[1].y - a01sq};
+    finish_eigenvector(v0, a, n1, n2, thresh);
+    V3<double> v1;
+    double t = w.x - w.y;
+    if (fabs(t) > wmax8eps) {
+        a.r[0].x += t;
+        a.r[1].y += t;
+        v1 = {base0 + a.r[0].z * w.y, base1 + a.r[1].z * w.y, a.r[0].x * a.r[1].y - a01sq};
+        finish_eigenvector(v1, a, n1, n2, thresh);
+    } else {
+        a.r[0].x += w.x;
+        a.r[1].y += w.x;
+        bool ok = false;
+        for (int i = 0; i < 3 && !ok; i++) {
+            m3_at(a, i, i) -= w.y;
+            V3<double> ai = {m3_at(a, 0, i), m3_at(a, 1, i), m3_at(a, 2, i)};
+            double ni = dot(ai, ai);
+            ok = ni > thresh;
+            if (ok) {
+                v1 = cross(v0, ai);
+                double nn = dot(v1, v1);
+                ok = nn > 65536.0 * eps * eps * ni;
+                if (ok) v1 = v1 * sqrt(1.0 / nn);
+            }
+        }
+        if (!ok) {
+            // fully degenerate: any unit vector orthogonal to v0
+            double c[3] = {v0.x, v0.y, v0.z};
+            int i = 0;
+            while (i < 2 && c[i] == 0.0) i++;
+            int j = (i + 1) % 3;
+            double s = 1.0 / sqrt(c[i] * c[i] + c[j] * c[j]);
+            double o[3] = {0.0, 0.0, 0.0};
+            o[i] = c[j] * s;
+            o[j] = -c[i] * s;
+            v1 = {o[0], o[1], o[2]};
+        }
+    }
+    M3 R;
+    R.r[0] = v0;
+    R.r[1] = v1;
+    R.r[2] = cross(v0, v1);
+    return R;
+}
+
+__device__ inline Q4<double> quat_from_matrix(const M3& A) {
+    double a11 = A.r[0].x, a22 = A.r[1].y, a33 = A.r[2].z;
+    double t[4] = {1.0 + a11 + a22 + a33, 1.0 + a11 - a22 - a33, 1.0 - a11 + a22 - a33, 1.0 - a11 - a22 + a33};
+    int imax = 0;
+    for (int i = 1; i < 4; i++)
+        if (t[i] > t[imax]) imax = i;
+    double f = 0.5 / sqrt(t[imax]);
+    double o[4] = {0.0, 0.0, 0.0, 0.0};
+    if (imax == 0) {
+        o[1] = (A.r[1].z - A.r[2].y) * f; o[2] = (A.r[2].x - A.r[0].z) * f; o[3] = (A.r[0].y - A.r[1].x) * f;
+    } else if (imax == 1) {
+        o[0] = (A.r[1].z - A.r[2].y) * f; o[2] = (A.r[0].y + A.r[1].x) * f; o[3] = (A.r[0].z + A.r[2].x) * f;
+    } else if (imax == 2) {
+        o[0] = (A.r[2].x - A.r[0].z) * f; o[1] = (A.r[0].y + A.r[1].x) * f; o[3] = (A.r[1].z + A.r[2].y) * f;
+    } else {
+        o[0] = (A.r[0].y - A.r[1].x) * f; o[1] = (A.r[0].z + A.r[2].x) * f; o[2] = (A.r[1].z + A.r[2].y) * f;
+    }
+    o[imax] = t[imax] * f;
+    return {o[0], o[1], o[2], o[3]};
+}
+
+}  // namespace rbhip

@@ -1,0 +1,75 @@
+// Shared host/device layout of the rigid-body step library (gfx950).
+//
+// Device data layout (structure of arrays, one context = one system):
+//   bodies, internal body order (grouped by atoms-per-body class):
+//     q, pi          Real4[numBodies]   orientation quaternion, conjugate momentum
+//     pm             Real4[numBodies]   (pcm.xyz, 1/M); pcm and pi carry the next
+//                                       step's first half-kick (folded into part2)
+//     iI             Real4[numBodies]   (1/I0, 1/I1, 1/I2, dof)
+//     rcm            double[3][bodyStride]  centre of mass, FP64
+//     d              Real[3*numBodyAtoms]   body-fixed atom coordinates, packed xyz
+//   atoms, internal order (free atoms, then each body's atoms contiguously):
+//     pos            Real[3*numAtoms]   positions (hi part in mixed precision)
+//     posLo          float[3*numAtoms]  low part, mixed precision only
+//     vel, force     Real[3*numAtoms]
+//     freeInvM       Real[numFree]
+// Work is described by tiles: each workgroup iteration handles one tile of
+// free atoms, of bodies of one size class, or of atoms for the force kernel.
+#pragma once
+
+#include <cstdint>
+
+namespace rbhip {
+
+enum TileKind : int { TILE_FREE = 0, TILE_BODIES = 1, TILE_ATOMS = 2 };
+
+struct Tile {
+    int sys;        // index into the system descriptor table
+    int kind;       // TileKind
+    int first;      // first body (TILE_BODIES) or first atom (TILE_FREE / TILE_ATOMS)
+    int count;      // bodies or atoms in the tile
+    int N;          // atoms per body (TILE_BODIES)
+    int atomFirst;  // internal index of the tile's first body atom (TILE_BODIES)
+    int lanes;      // lanes per body, a power of two (TILE_BODIES)
+    int slot;       // reduction slot of this tile in the system's partials
+};
+
+// One system as the kernels see it.  Pointers to Real data are void* and
+// cast by the kernels (Real = float for mixed, double for double precision).
+struct DevSys {
+    int numFree, numBodies, numBodyAtoms, numAtoms;
+    int rotationMode;   // 0 exact, n > 0 NO-SQUISH(n)
+    int bodyStride;     // row length of rcm
+    int useStageR, useStageV;
+    double dt;
+    double tetherScale; // 1 if the tether is on, 0 otherwise
+    // bodies
+    void* q;
+    void* pi;
+    void* pm;
+    void* iI;
+    double* rcm;
+    void* d;
+    // atoms
+    void* pos;
+    float* posLo;
+    void* vel;
+    void* force;
+    void* freeInvM;
+    double* mass;
+    double* stageR;
+    double* stageV;
+    // force field
+    void* tetherK;      // Real[numAtoms]
+    void* r0;           // Real[3*numAtoms]
+    float* r0Lo;        // mixed precision low part of r0
+    int* bondStart;     // [numAtoms + 1] CSR of bonds per atom (nullptr: none)
+    int* bondOther;     // partner atom of each bond entry
+    double* bondLen;
+    double* bondK;
+    // reductions: 4 doubles per slot
+    double* kePart;     // twoKt, twoKr, dof, (unused)
+    double* pePart;     // energy, (unused) x3
+};
+
+}  // namespace rbhip

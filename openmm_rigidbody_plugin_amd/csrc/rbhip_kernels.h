@@ -1,0 +1,40 @@
+// Kernel launchers shared by the host side of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rbhip_internal.h"
+
+namespace rbhip {
+
+// hot path (rbhip_kernels.hip)
+hipError_t launch_part1(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
+                        int block, int maxBlocks, hipStream_t stream);
+hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
+                        int maxBlocks, hipStream_t stream);
+hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
+                         int maxBlocks, hipStream_t stream);
+hipError_t launch_reduce(const DevSys* systems, const int* nslots, int numSystems, int which, double* out,
+                         hipStream_t stream);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
+
+// setup and state conversion (rbhip_setup.hip, FP64 without contraction)
+// builds the body state (geometry if `geometry`, always dynamics) from the
+// staged or current atom state, records KE and dof partials, and applies the
+// first half-kick of the next step
+hipError_t launch_setup(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int geometry,
+                        int block, hipStream_t stream);
+// FP64 array (internal order) -> device state: what = 0 positions, 1 velocities
+hipError_t launch_stage_in(int precision, const DevSys* sys, const double* in, int numAtoms, int what,
+                           hipStream_t stream);
+// device state -> FP64 array (internal order): what = 0 positions, 1 velocities, 2 forces
+hipError_t launch_stage_out(int precision, const DevSys* sys, double* out, int numAtoms, int what,
+                            hipStream_t stream);
+// body states (internal body order) as 36 FP64 values each, half-kick removed
+hipError_t launch_body_states(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
+                              int numBodies, double* out, hipStream_t stream);
+// re-applies the folded half-kick for a new step size: ddt = dtNew - dtOld
+hipError_t launch_rekick(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0, int numBodies,
+                         double ddt, hipStream_t stream);
+
+}  // namespace rbhip

@@ -1,0 +1,359 @@
+// Body-state build and state conversion kernels.  This translation unit is
+// compiled with -ffp-contract=off: the body-state build runs in FP64 with the
+// operation order of the reference's host code, so the device starts from the
+// same body frames, quaternions and momenta the FP64 CPU path computes.
+//
+//   k_setup        RigidBody::buildGeometry (openmmapi/src/RigidBody.cpp:65-116)
+//                  and RigidBody::buildDynamics (RigidBody.cpp:123-142) for
+//                  each body; KE and dof partials; first half-kick of the
+//                  next step (RigidBodySystem.cpp:177-179)
+//   k_body_states  RigidBody fields for download (half-kick removed)
+//   k_rekick       re-applies the folded half-kick after a step-size change
+//   k_stage_in/out FP64 host-order staging <-> device precision state
+#include <hip/hip_runtime.h>
+
+#include "rbhip_access.h"
+#include "rbhip_kernels.h"
+
+namespace rbhip {
+
+// ---------------------------------------------------------------------------
+// geometry helpers (reference RigidBody.cpp:26-58)
+// ---------------------------------------------------------------------------
+// unit vector orthogonal to u: Projection(u) e_m / |.|, m = index of the
+// smallest (signed) component
+__device__ inline V3<double> orthonormal(V3<double> u) {
+    int m = u.x < u.y ? 0 : 1;
+    double um = m == 0 ? u.x : u.y;
+    if (u.z < um) { m = 2; um = u.z; }
+    double xtx = dot(u, u);
+    V3<double> v = {(-u.x) * um, (-u.y) * um, (-u.z) * um};
+    if (m == 0) v.x += xtx;
+    else if (m == 1) v.y += xtx;
+    else v.z += xtx;
+    return v * (1.0 / sqrt(dot(v, v)));
+}
+
+// inertia += Projection(delta) * m   (MatVec.cpp:555-564)
+__device__ inline void add_projection(M3& I, V3<double> x, double m) {
+    double xtx = dot(x, x);
+    V3<double> nx = {-x.x, -x.y, -x.z};
+    V3<double> r0 = nx * x.x, r1 = nx * x.y, r2 = nx * x.z;
+    r0.x += xtx; r1.y += xtx; r2.z += xtx;
+    I.r[0] = I.r[0] + r0 * m;
+    I.r[1] = I.r[1] + r1 * m;
+    I.r[2] = I.r[2] + r2 * m;
+}
+
+__device__ inline V3<double> m3_apply(const M3& A, V3<double> x) {
+    return {dot(A.r[0], x), dot(A.r[1], x), dot(A.r[2], x)};
+}
+
+template <class Real> __device__ __forceinline__ V3<double> setup_pos(const DevSys& S, int i) {
+    if (S.useStageR) return V3<double>{S.stageR[3 * (size_t)i], S.stageR[3 * (size_t)i + 1], S.stageR[3 * (size_t)i + 2]};
+    return load_pos<Real>(S, i);
+}
+template <class Real> __device__ __forceinline__ V3<double> setup_vel(const DevSys& S, int i) {
+    if (S.useStageV) return V3<double>{S.stageV[3 * (size_t)i], S.stageV[3 * (size_t)i + 1], S.stageV[3 * (size_t)i + 2]};
+    return cvt<double>(ld3<Real>(S.vel, i));
+}
+
+template <class Real>
+__device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometry, double (&red)[3]) {
+    const int dbase = atom0 - S.numFree;
+    const double TOL = 1.0e-5;
+    double mass = 0.0;
+    for (int j = 0; j < N; j++) mass += S.mass[atom0 + j];
+    const double invMass = 1.0 / mass;
+    Q4<double> q;
+    V3<double> invI, rcm = {0.0, 0.0, 0.0}, F = {0.0, 0.0, 0.0}, tau = {0.0, 0.0, 0.0};
+    double dof;
+    M3 A;
+    if (geometry) {
+        for (int j = 0; j < N; j++) rcm = rcm + setup_pos<Real>(S, atom0 + j) * S.mass[atom0 + j];
+        rcm = rcm * (1.0 / mass);
+        // collinearity (RigidBody.cpp:26-45): the reference keeps the last atom
+        // farther than atom 0 as the direction candidate
+        V3<double> dl0 = setup_pos<Real>(S, atom0) - rcm;
+        double d0 = dot(dl0, dl0), d2max = d0;
+        int jmax = 0;
+        for (int j = 1; j < N; j++) {
+            V3<double> dl = setup_pos<Real>(S, atom0 + j) - rcm;
+            double d2 = dot(dl, dl);
+            if (d2 > d0) { jmax = j; d2max = d2; }
+        }
+        V3<double> u = (setup_pos<Real>(S, atom0 + jmax) - rcm) * (1.0 / sqrt(d2max));
+        bool lin = true;
+        for (int j = 0; lin && j < N; j++) {
+            V3<double> dl = setup_pos<Real>(S, atom0 + j) - rcm;
+            double d2 = dot(dl, dl);
+            double ud = dot(u, dl);
+            lin = d2 < TOL * d2max || fabs(ud * ud / d2 - 1.0) < TOL;
+        }
+        if (lin) {
+            double moi = 0.0;
+            for (int j = 0; j < N; j++) {
+                V3<double> dl = setup_pos<Real>(S, atom0 + j) - rcm;
+                moi += S.mass[atom0 + j] * dot(dl, dl);
+            }
+            invI = {1.0 / moi, 1.0 / moi, 0.0};
+            V3<double> v = orthonormal(u);
+            // frame rows (v, u x v, u): the molecular axis is body axis 3, the
+            // zero principal moment (the reference stores the transpose,
+            // RigidBody.cpp:96, which breaks I = (MoI, MoI, 0))
+            A.r[0] = v;
+            A.r[1] = cross(u, v);
+            A.r[2] = u;
+            dof = 5.0;
+        } else {
+            M3 inertia;
+            inertia.r[0] = inertia.r[1] = inertia.r[2] = V3<double>{0.0, 0.0, 0.0};
+            for (int j = 0; j < N; j++)
+                add_projection(inertia, setup_pos<Real>(S, atom0 + j) - rcm, S.mass[atom0 + j]);
+            V3<double> I = eigenvalues3(inertia);
+            invI = {1.0 / I.x, 1.0 / I.y, 1.0 / I.z};
+            A = eigenvectors3(inertia, I);
+            dof = 6.0;
+        }
+        q = quat_from_matrix(A);
+        for (int j = 0; j < N; j++) {
+            V3<double> dl = setup_pos<Real>(S, atom0 + j) - rcm;
+            st3<Real>(S.d, dbase + j, cvt<Real>(m3_apply(A, dl)));
+            V3<double> f = cvt<double>(ld3<Real>(S.force, atom0 + j));
+            F = F + f;
+            tau = tau + cross(dl, f);
+        }
+        const int st = S.bodyStride;
+        S.rcm[k] = rcm.x;
+        S.rcm[k + st] = rcm.y;
+        S.rcm[k + 2 * st] = rcm.z;
+        st4<Real>(S.iI, k, Q4<Real>{(Real)invI.x, (Real)invI.y, (Real)invI.z, (Real)dof});
+        st4<Real>(S.q, k, cvt<Real>(q));
+    } else {
+        q = cvt<double>(ld4<Real>(S.q, k));
+        Q4<Real> iI = ld4<Real>(S.iI, k);
+        invI = {(double)iI.a, (double)iI.b, (double)iI.c};
+        dof = (double)iI.d;
+        for (int j = 0; j < N; j++) {
+            V3<double> dl = Atmul(q, cvt<double>(ld3<Real>(S.d, dbase + j)));
+            V3<double> f = cvt<double>(ld3<Real>(S.force, atom0 + j));
+            F = F + f;
+            tau = tau + cross(dl, f);
+        }
+    }
+    Q4<double> torque = Cmul(q, tau);
+    // dynamics (RigidBody.cpp:123-142), centre-of-mass momentum rebuilt from zero
+    V3<double> pcm = {0.0, 0.0, 0.0};
+    for (int j = 0; j < N; j++) pcm = pcm + setup_vel<Real>(S, atom0 + j) * S.mass[atom0 + j];
+    V3<double> vcm = pcm * (1.0 / mass);
+    double twoKt = dot(pcm, vcm);
+    V3<double> L = {0.0, 0.0, 0.0};
+    for (int j = 0; j < N; j++) {
+        V3<double> dj;
+        if (geometry) dj = m3_apply(A, setup_pos<Real>(S, atom0 + j) - rcm);
+        else dj = cvt<double>(ld3<Real>(S.d, dbase + j));
+        L = L + cross(dj, Amul(q, setup_vel<Real>(S, atom0 + j) - vcm) * S.mass[atom0 + j]);
+    }
+    Q4<double> pi = Bmul(q, L) * 2.0;
+    double twoKr = dot(L, hadamard(invI, L));
+    // first half-kick of the next step (RigidBodySystem.cpp:177-178)
+    const double halfDt = 0.5 * S.dt;
+    pcm = pcm + F * halfDt;
+    pi = pi + torque * S.dt;
+    st4<Real>(S.pm, k, Q4<Real>{(Real)pcm.x, (Real)pcm.y, (Real)pcm.z, (Real)invMass});
+    st4<Real>(S.pi, k, cvt<Real>(pi));
+    red[0] = twoKt;
+    red[1] = twoKr;
+    red[2] = dof;
+}
+
+__device__ inline double wave_sum_s(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <class Real>
+__global__ __launch_bounds__(256) void k_setup(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
+                                                 int numTiles, int geometry) {
+    __shared__ double red[4][4];
+    for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const DevSys& S = systems[tile.sys];
+        double v[3] = {0.0, 0.0, 0.0};
+        int i = threadIdx.x;
+        if (tile.kind == TILE_FREE) {
+            if (i < tile.count) {
+                int a = tile.first + i;
+                V3<double> vel = setup_vel<Real>(S, a);
+                v[0] = dot(vel, vel) / (double)reinterpret_cast<const Real*>(S.freeInvM)[a];
+            }
+        } else if (i < tile.count) {
+            setup_body<Real>(S, tile.first + i, tile.N, tile.atomFirst + i * tile.N, geometry, v);
+        }
+        int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int c = 0; c < 3; c++) {
+            double s = wave_sum_s(v[c]);
+            if (lane == 0) red[wave][c] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int nw = (blockDim.x + 63) >> 6;
+            for (int c = 0; c < 3; c++) {
+                double s = 0.0;
+                for (int w = 0; w < nw; w++) s += red[w][c];
+                S.kePart[4 * (size_t)tile.slot + c] = s;
+            }
+            S.kePart[4 * (size_t)tile.slot + 3] = 0.0;
+        }
+        __syncthreads();
+    }
+}
+
+// FP64 staging -> device state
+template <class Real>
+__global__ void k_stage_in(const DevSys* __restrict__ sys, const double* __restrict__ in, int n, int what) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3<double> x = {in[3 * (size_t)i], in[3 * (size_t)i + 1], in[3 * (size_t)i + 2]};
+    if (what == 0) store_pos<Real>(*sys, i, x);
+    else st3<Real>(sys->vel, i, cvt<Real>(x));
+}
+
+// device state -> FP64 staging
+template <class Real>
+__global__ void k_stage_out(const DevSys* __restrict__ sys, double* __restrict__ out, int n, int what) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3<double> x;
+    if (what == 0) x = load_pos<Real>(*sys, i);
+    else if (what == 1) x = cvt<double>(ld3<Real>(sys->vel, i));
+    else x = cvt<double>(ld3<Real>(sys->force, i));
+    out[3 * (size_t)i] = x.x;
+    out[3 * (size_t)i + 1] = x.y;
+    out[3 * (size_t)i + 2] = x.z;
+}
+
+// body force and C(q) torque from the current atom forces
+template <class Real>
+__device__ inline void body_force_torque(const DevSys& S, Q4<double> q, int N, int atom0, V3<double>& F,
+                                         Q4<double>& T) {
+    const int dbase = atom0 - S.numFree;
+    V3<double> tau = {0.0, 0.0, 0.0};
+    F = {0.0, 0.0, 0.0};
+    for (int j = 0; j < N; j++) {
+        V3<double> dl = Atmul(q, cvt<double>(ld3<Real>(S.d, dbase + j)));
+        V3<double> f = cvt<double>(ld3<Real>(S.force, atom0 + j));
+        F = F + f;
+        tau = tau + cross(dl, f);
+    }
+    T = Cmul(q, tau);
+}
+
+template <class Real>
+__global__ void k_body_states(const DevSys* __restrict__ sys, const int* __restrict__ bodyN,
+                              const int* __restrict__ bodyAtom0, int numBodies, double* __restrict__ out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= numBodies) return;
+    const DevSys& S = *sys;
+    int N = bodyN[k], atom0 = bodyAtom0[k];
+    Q4<double> q = cvt<double>(ld4<Real>(S.q, k));
+    Q4<double> pi = cvt<double>(ld4<Real>(S.pi, k));
+    Q4<Real> pm = ld4<Real>(S.pm, k);
+    Q4<Real> iI = ld4<Real>(S.iI, k);
+    V3<double> invI = {(double)iI.a, (double)iI.b, (double)iI.c};
+    double mass = 0.0;
+    for (int j = 0; j < N; j++) mass += S.mass[atom0 + j];
+    V3<double> F;
+    Q4<double> T;
+    body_force_torque<Real>(S, q, N, atom0, F, T);
+    double halfDt = 0.5 * S.dt;
+    V3<double> pcm = V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} - F * halfDt;
+    pi = pi + T * (-S.dt);
+    V3<double> L = Btmul(q, pi) * 0.5;
+    V3<double> w = hadamard(invI, L);
+    V3<double> vcm = pcm * (double)pm.d;
+    double* o = out + 36 * (size_t)k;
+    int p = 0;
+    o[p++] = N;
+    o[p++] = iI.d;
+    o[p++] = mass;
+    o[p++] = invI.x != 0.0 ? 1.0 / invI.x : 0.0;
+    o[p++] = invI.y != 0.0 ? 1.0 / invI.y : 0.0;
+    o[p++] = invI.z != 0.0 ? 1.0 / invI.z : 0.0;
+    o[p++] = invI.x; o[p++] = invI.y; o[p++] = invI.z;
+    const int st = S.bodyStride;
+    o[p++] = S.rcm[k]; o[p++] = S.rcm[k + st]; o[p++] = S.rcm[k + 2 * st];
+    o[p++] = pcm.x; o[p++] = pcm.y; o[p++] = pcm.z;
+    o[p++] = q.a; o[p++] = q.b; o[p++] = q.c; o[p++] = q.d;
+    o[p++] = pi.a; o[p++] = pi.b; o[p++] = pi.c; o[p++] = pi.d;
+    o[p++] = F.x; o[p++] = F.y; o[p++] = F.z;
+    o[p++] = T.a; o[p++] = T.b; o[p++] = T.c; o[p++] = T.d;
+    o[p++] = dot(pcm, vcm);
+    o[p++] = dot(L, w);
+    o[p++] = 0.0;
+    for (; p < 36; p++) o[p] = 0.0;
+}
+
+template <class Real>
+__global__ void k_rekick(const DevSys* __restrict__ sys, const int* __restrict__ bodyN,
+                         const int* __restrict__ bodyAtom0, int numBodies, double ddt) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= numBodies) return;
+    const DevSys& S = *sys;
+    Q4<double> q = cvt<double>(ld4<Real>(S.q, k));
+    Q4<Real> pm = ld4<Real>(S.pm, k);
+    V3<double> F;
+    Q4<double> T;
+    body_force_torque<Real>(S, q, bodyN[k], bodyAtom0[k], F, T);
+    V3<double> pcm = V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} + F * (0.5 * ddt);
+    Q4<double> pi = cvt<double>(ld4<Real>(S.pi, k)) + T * ddt;
+    st4<Real>(S.pm, k, Q4<Real>{(Real)pcm.x, (Real)pcm.y, (Real)pcm.z, pm.d});
+    st4<Real>(S.pi, k, cvt<Real>(pi));
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_setup(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int geometry,
+                        int block, hipStream_t stream) {
+    if (numTiles <= 0) return hipSuccess;
+    if (precision == 0) k_setup<float><<<numTiles, block, 0, stream>>>(systems, tiles, numTiles, geometry);
+    else k_setup<double><<<numTiles, block, 0, stream>>>(systems, tiles, numTiles, geometry);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_in(int precision, const DevSys* sys, const double* in, int numAtoms, int what,
+                           hipStream_t stream) {
+    if (numAtoms <= 0) return hipSuccess;
+    int grid = (numAtoms + 255) / 256;
+    if (precision == 0) k_stage_in<float><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
+    else k_stage_in<double><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_out(int precision, const DevSys* sys, double* out, int numAtoms, int what,
+                            hipStream_t stream) {
+    if (numAtoms <= 0) return hipSuccess;
+    int grid = (numAtoms + 255) / 256;
+    if (precision == 0) k_stage_out<float><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
+    else k_stage_out<double><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
+    return hipGetLastError();
+}
+
+hipError_t launch_body_states(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
+                              int numBodies, double* out, hipStream_t stream) {
+    if (numBodies <= 0) return hipSuccess;
+    int grid = (numBodies + 255) / 256;
+    if (precision == 0) k_body_states<float><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
+    else k_body_states<double><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_rekick(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0, int numBodies,
+                         double ddt, hipStream_t stream) {
+    if (numBodies <= 0) return hipSuccess;
+    int grid = (numBodies + 255) / 256;
+    if (precision == 0) k_rekick<float><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, ddt);
+    else k_rekick<double><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, ddt);
+    return hipGetLastError();
+}
+
+}  // namespace rbhip

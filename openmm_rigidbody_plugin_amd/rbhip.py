@@ -1,0 +1,353 @@
+"""ctypes binding of the gfx950 rigid-body step library (include/rbhip.h).
+
+Loads the in-tree ``librbhip.so``; there is no fallback: if the library is
+missing or no GPU is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "librbhip.so")
+
+MIXED = 0
+DOUBLE = 1
+PRECISIONS = {"mixed": MIXED, "double": DOUBLE}
+KERNELS = ("part1", "forces", "part2")
+BODY_FIELDS = dict(N=0, dof=1, mass=2, I=slice(3, 6), invI=slice(6, 9), rcm=slice(9, 12), pcm=slice(12, 15),
+                   q=slice(15, 19), pi=slice(19, 23), force=slice(23, 26), torque=slice(26, 30), twoKt=30,
+                   twoKr=31, loc=32)
+
+_D, _I, _P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+_DP, _IP = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+_SIZEP = ctypes.POINTER(ctypes.c_size_t)
+_LLP = ctypes.POINTER(ctypes.c_longlong)
+
+_lib = None
+
+
+class RbhipError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"rbhip error {code}: {message}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -m openmm_rigidbody_plugin_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "rbhip_last_error": (ctypes.c_char_p, []),
+            "rbhip_version": (_I, []),
+            "rbhip_device_count": (_I, [_IP]),
+            "rbhip_device_synchronize": (_I, []),
+            "rbhip_create": (_I, [_I, _DP, _IP, _D, _I, _I, _I, ctypes.POINTER(_P)]),
+            "rbhip_destroy": (_I, [_P]),
+            "rbhip_set_stream": (_I, [_P, _P]),
+            "rbhip_device_bytes": (_I, [_P, _SIZEP]),
+            "rbhip_get_counts": (_I, [_P, _IP]),
+            "rbhip_get_atom_index": (_I, [_P, _IP]),
+            "rbhip_get_body_index": (_I, [_P, _IP]),
+            "rbhip_set_tether": (_I, [_P, _DP, _DP]),
+            "rbhip_add_bonds": (_I, [_P, _I, _IP, _DP, _DP]),
+            "rbhip_compute_forces": (_I, [_P, _DP]),
+            "rbhip_set_positions": (_I, [_P, _DP]),
+            "rbhip_set_velocities": (_I, [_P, _DP]),
+            "rbhip_get_positions": (_I, [_P, _DP]),
+            "rbhip_get_velocities": (_I, [_P, _DP]),
+            "rbhip_get_forces": (_I, [_P, _DP]),
+            "rbhip_set_step_size": (_I, [_P, _D]),
+            "rbhip_get_time": (_I, [_P, _DP, _LLP]),
+            "rbhip_part1": (_I, [_P]),
+            "rbhip_part2": (_I, [_P]),
+            "rbhip_execute": (_I, [_P, _I]),
+            "rbhip_use_graph": (_I, [_P, _I]),
+            "rbhip_kinetic_energies": (_I, [_P, _DP]),
+            "rbhip_potential_energy": (_I, [_P, _DP]),
+            "rbhip_get_body_states": (_I, [_P, _DP]),
+            "rbhip_get_body_fixed_positions": (_I, [_P, _DP]),
+            "rbhip_set_launch": (_I, [_P, _I, _I]),
+            "rbhip_set_lanes": (_I, [_P, _I, _I]),
+            "rbhip_autotune": (_I, [_P, _I]),
+            "rbhip_describe": (_I, [_P, ctypes.c_char_p, _I]),
+            "rbhip_time_kernels": (_I, [_P, _I, _DP, _DP]),
+            "rbhip_copy_calibration": (_I, [ctypes.c_size_t, _I, _DP]),
+            "rbhip_batch_create": (_I, [_I, ctypes.POINTER(_P)]),
+            "rbhip_batch_destroy": (_I, [_P]),
+            "rbhip_batch_set_members": (_I, [_P, _I, ctypes.POINTER(_P)]),
+            "rbhip_batch_set_launch": (_I, [_P, _I, _I]),
+            "rbhip_batch_use_graph": (_I, [_P, _I]),
+            "rbhip_batch_execute": (_I, [_P, _I]),
+            "rbhip_batch_kinetic_energies": (_I, [_P, _DP]),
+            "rbhip_batch_time_kernels": (_I, [_P, _I, _DP, _DP]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names of the C ABI entry points this binding expects."""
+    lib()
+    return sorted(n for n in dir(_lib) if n.startswith("rbhip_"))
+
+
+def _check(rc):
+    if rc != 0:
+        raise RbhipError(rc, lib().rbhip_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def _vec_in(x, n):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    if a.shape != (n, 3):
+        raise ValueError(f"expected shape ({n}, 3), got {a.shape}")
+    return a
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(lib().rbhip_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def synchronize():
+    _check(lib().rbhip_device_synchronize())
+
+
+def copy_calibration(nbytes: int, reps: int = 20) -> float:
+    ms = ctypes.c_double()
+    _check(lib().rbhip_copy_calibration(nbytes, reps, ctypes.byref(ms)))
+    return ms.value
+
+
+class Context:
+    """One rigid-body system on one GPU: RigidBodyIntegrator + its HIP step kernel."""
+
+    def __init__(self, masses, body_indices, dt, rotation_mode=10, precision="mixed", device=0):
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        b = np.ascontiguousarray(body_indices, dtype=np.int32)
+        if m.shape != b.shape or m.ndim != 1:
+            raise ValueError("masses and body indices must be 1-D arrays of equal length")
+        self.n = m.size
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self._h = _P()
+        _check(lib().rbhip_create(self.n, _dp(m), _ip(b), float(dt), int(rotation_mode), self.precision,
+                                  int(device), ctypes.byref(self._h)))
+
+    @classmethod
+    def from_spec(cls, spec, precision="mixed", rotation_mode=None, dt=None, device=0, set_state=True):
+        c = cls(spec.masses, spec.body_indices, spec.dt if dt is None else dt,
+                spec.rotation_mode if rotation_mode is None else rotation_mode, precision, device)
+        c.set_tether(spec.tether_k, spec.tether_r0)
+        if set_state:
+            c.set_positions(spec.positions)
+            c.set_velocities(spec.velocities)
+        return c
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().rbhip_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---- bookkeeping ----
+    def counts(self):
+        out = np.empty(5, dtype=np.int32)
+        _check(lib().rbhip_get_counts(self._h, _ip(out)))
+        return dict(zip(("numFree", "numBodies", "numActualAtoms", "numBodyAtoms", "numDOF"),
+                        (int(x) for x in out)))
+
+    def atom_index(self):
+        out = np.empty(self.counts()["numActualAtoms"], dtype=np.int32)
+        _check(lib().rbhip_get_atom_index(self._h, _ip(out)))
+        return out
+
+    def body_index(self):
+        out = np.empty(self.n, dtype=np.int32)
+        _check(lib().rbhip_get_body_index(self._h, _ip(out)))
+        return out
+
+    def device_bytes(self):
+        n = ctypes.c_size_t()
+        _check(lib().rbhip_device_bytes(self._h, ctypes.byref(n)))
+        return n.value
+
+    # ---- force field ----
+    def set_tether(self, k, r0):
+        kk = np.ascontiguousarray(np.broadcast_to(np.asarray(k, dtype=np.float64), (self.n,)))
+        _check(lib().rbhip_set_tether(self._h, _dp(kk), _dp(_vec_in(r0, self.n))))
+
+    def add_bonds(self, pairs, length, k):
+        p = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        ln = np.ascontiguousarray(np.broadcast_to(np.asarray(length, dtype=np.float64), (p.shape[0],)))
+        kk = np.ascontiguousarray(np.broadcast_to(np.asarray(k, dtype=np.float64), (p.shape[0],)))
+        _check(lib().rbhip_add_bonds(self._h, p.shape[0], _ip(p), _dp(ln), _dp(kk)))
+
+    def compute_forces(self, energy=False):
+        e = ctypes.c_double()
+        _check(lib().rbhip_compute_forces(self._h, ctypes.byref(e) if energy else None))
+        return e.value if energy else None
+
+    # ---- state ----
+    def set_positions(self, x):
+        _check(lib().rbhip_set_positions(self._h, _dp(_vec_in(x, self.n))))
+
+    def set_velocities(self, v):
+        _check(lib().rbhip_set_velocities(self._h, _dp(_vec_in(v, self.n))))
+
+    def _get(self, fn):
+        out = np.empty((self.n, 3))
+        _check(fn(self._h, _dp(out)))
+        return out
+
+    def positions(self):
+        return self._get(lib().rbhip_get_positions)
+
+    def velocities(self):
+        return self._get(lib().rbhip_get_velocities)
+
+    def forces(self):
+        return self._get(lib().rbhip_get_forces)
+
+    def set_step_size(self, dt):
+        _check(lib().rbhip_set_step_size(self._h, float(dt)))
+
+    def time(self):
+        t, s = ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().rbhip_get_time(self._h, ctypes.byref(t), ctypes.byref(s)))
+        return t.value, s.value
+
+    # ---- stepping ----
+    def part1(self):
+        _check(lib().rbhip_part1(self._h))
+
+    def part2(self):
+        _check(lib().rbhip_part2(self._h))
+
+    def step(self, steps=1):
+        _check(lib().rbhip_execute(self._h, int(steps)))
+
+    def use_graph(self, enable=True):
+        _check(lib().rbhip_use_graph(self._h, int(bool(enable))))
+
+    def kinetic_energies(self):
+        out = np.empty(2)
+        _check(lib().rbhip_kinetic_energies(self._h, _dp(out)))
+        return out
+
+    def potential_energy(self):
+        e = ctypes.c_double()
+        _check(lib().rbhip_potential_energy(self._h, ctypes.byref(e)))
+        return e.value
+
+    def body_states_array(self):
+        nb = self.counts()["numBodies"]
+        out = np.empty((nb, 36))
+        if nb:
+            _check(lib().rbhip_get_body_states(self._h, _dp(out)))
+        return out
+
+    def body_states(self):
+        arr = self.body_states_array()
+        return [{k: row[sl] for k, sl in BODY_FIELDS.items()} for row in arr]
+
+    def body_fixed_positions(self):
+        nba = self.counts()["numBodyAtoms"]
+        out = np.empty((nba, 3))
+        if nba:
+            _check(lib().rbhip_get_body_fixed_positions(self._h, _dp(out)))
+        return out
+
+    # ---- launch configuration and measurement ----
+    def set_launch(self, block=256, max_blocks=0):
+        _check(lib().rbhip_set_launch(self._h, int(block), int(max_blocks)))
+
+    def set_lanes(self, atoms_per_body, lanes):
+        _check(lib().rbhip_set_lanes(self._h, int(atoms_per_body), int(lanes)))
+
+    def autotune(self, reps=5):
+        _check(lib().rbhip_autotune(self._h, int(reps)))
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().rbhip_describe(self._h, buf, len(buf)))
+        return buf.value.decode()
+
+    def time_kernels(self, steps):
+        avg = np.empty(3)
+        tot = ctypes.c_double()
+        _check(lib().rbhip_time_kernels(self._h, int(steps), _dp(avg), ctypes.byref(tot)))
+        return dict(zip(KERNELS, avg)), tot.value
+
+
+class Batch:
+    """Several contexts stepped by one set of launches."""
+
+    def __init__(self, device=0):
+        self._h = _P()
+        self.members = []
+        _check(lib().rbhip_batch_create(int(device), ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().rbhip_batch_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_members(self, contexts):
+        arr = (_P * max(len(contexts), 1))(*[c.handle for c in contexts])
+        _check(lib().rbhip_batch_set_members(self._h, len(contexts), arr))
+        self.members = list(contexts)
+
+    def set_launch(self, block=256, max_blocks=0):
+        _check(lib().rbhip_batch_set_launch(self._h, int(block), int(max_blocks)))
+
+    def use_graph(self, enable=True):
+        _check(lib().rbhip_batch_use_graph(self._h, int(bool(enable))))
+
+    def step(self, steps=1):
+        _check(lib().rbhip_batch_execute(self._h, int(steps)))
+
+    def kinetic_energies(self):
+        out = np.empty((max(len(self.members), 1), 2))
+        if self.members:
+            _check(lib().rbhip_batch_kinetic_energies(self._h, _dp(out)))
+        return out[:len(self.members)]
+
+    def time_kernels(self, steps):
+        avg = np.empty(3)
+        tot = ctypes.c_double()
+        _check(lib().rbhip_batch_time_kernels(self._h, int(steps), _dp(avg), ctypes.byref(tot)))
+        return dict(zip(KERNELS, avg)), tot.value
