@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only stream MIB MiB through the plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -128,6 +130,12 @@ def main():
     from openmm_rigidbody_plugin_amd import rbhip
     if rbhip.device_count() < 1:
         raise SystemExit("no GPU visible")
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        ms = rbhip.copy_calibration(nbytes, reps=20)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "avg_ms": ms,
+                          "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
     t0 = time.perf_counter()
     spec, cfg = make_workload(args.config)
     log(f"workload {args.config}: {spec.num_atoms} atoms built in {time.perf_counter() - t0:.1f}s")
