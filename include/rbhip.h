@@ -44,7 +44,13 @@ enum {
 enum { RBHIP_PRECISION_MIXED = 0, RBHIP_PRECISION_DOUBLE = 1 };
 
 /* indices of the per-kernel timing array filled by rbhip_time_kernels */
-enum { RBHIP_KERNEL_PART1 = 0, RBHIP_KERNEL_FORCES = 1, RBHIP_KERNEL_PART2 = 2, RBHIP_NUM_KERNELS = 3 };
+enum {
+    RBHIP_KERNEL_PART1 = 0,
+    RBHIP_KERNEL_FORCES = 1,
+    RBHIP_KERNEL_PART2 = 2,
+    RBHIP_KERNEL_FUSED = 3, /* part2 of step n + part1 of step n+1 in one pass */
+    RBHIP_NUM_KERNELS = 4
+};
 
 typedef struct rbhip_context rbhip_context;
 typedef struct rbhip_batch rbhip_batch;
@@ -128,11 +134,18 @@ int rbhip_get_time(rbhip_context* ctx, double* time, long long* stepCount);
 int rbhip_part1(rbhip_context* ctx);
 /* integratePart2 (RigidBodySystem.cpp:193-204) on the device */
 int rbhip_part2(rbhip_context* ctx);
-/* RigidBodyIntegrator::step(steps) (RigidBodyIntegrator.cpp:96-101):
-   steps x [part1, forces, part2], replayed from a HIP graph when enabled */
+/* RigidBodyIntegrator::step(steps) (RigidBodyIntegrator.cpp:96-101).
+   With fusion (default) the interior step boundaries run as one pass each:
+   part1, forces, (steps-1) x [fused part2+part1, forces], part2 -- the same
+   trajectory as steps x [part1, forces, part2] (bit for bit in mixed
+   precision, to a few ulp in FP64), without the interior steps' atom
+   velocities and kinetic energies, which nothing reads.
+   Repeated sequences are replayed from HIP graphs when enabled. */
 int rbhip_execute(rbhip_context* ctx, int steps);
-/* 1 = capture one step in a HIP graph and replay it (default), 0 = launch */
+/* 1 = capture the repeated launches in HIP graphs and replay them (default) */
 int rbhip_use_graph(rbhip_context* ctx, int enable);
+/* 1 = fuse interior step boundaries in rbhip_execute (default), 0 = two-pass */
+int rbhip_use_fusion(rbhip_context* ctx, int enable);
 
 /* computeKineticEnergy + getKineticEnergies (RigidBodyKernels.h:80-86,
    ReferenceRigidBodyKernels.cpp:110-121): out2 = {translational, rotational},
@@ -163,9 +176,10 @@ int rbhip_set_lanes(rbhip_context* ctx, int atomsPerBody, int lanes);
 int rbhip_autotune(rbhip_context* ctx, int reps);
 /* describes the chosen tiling as text */
 int rbhip_describe(rbhip_context* ctx, char* buf, int len);
-/* runs `steps` plain launches with HIP events around each kernel on the
-   context's stream; avgMs[RBHIP_NUM_KERNELS] receives the mean duration of
-   each kernel and totalMs the whole span */
+/* advances the state 2*steps+1 steps with HIP events around each kernel on
+   the context's stream: `steps` two-pass steps (part1, forces, part2), then
+   the fused path of step(steps+1); avgMs[RBHIP_NUM_KERNELS] receives the mean
+   duration of each kernel and totalMs the span of the two-pass steps */
 int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* totalMs);
 /* streams `bytes` through a plain device copy kernel `reps` times (profiler
    calibration of HBM traffic counters); avgMs receives its mean duration */
@@ -182,6 +196,7 @@ int rbhip_batch_destroy(rbhip_batch* batch);
 int rbhip_batch_set_members(rbhip_batch* batch, int count, rbhip_context* const* members);
 int rbhip_batch_set_launch(rbhip_batch* batch, int blockSize, int maxBlocks);
 int rbhip_batch_use_graph(rbhip_batch* batch, int enable);
+int rbhip_batch_use_fusion(rbhip_batch* batch, int enable);
 /* steps every member `steps` times: per step one part1, one force and one
    part2 launch over all members' tiles */
 int rbhip_batch_execute(rbhip_batch* batch, int steps);

@@ -21,34 +21,36 @@ template <class Real> __device__ __forceinline__ void st4(void* base, int k, Q4<
     v.x = q.a; v.y = q.b; v.z = q.c; v.w = q.d;
     reinterpret_cast<typename Vec4Of<Real>::type*>(base)[k] = v;
 }
-template <class Real> __device__ __forceinline__ V3<Real> ld3(const void* base, int i) {
-    const Real* p = reinterpret_cast<const Real*>(base) + 3 * (size_t)i;
-    return {p[0], p[1], p[2]};
+// element i of a component-major array Real[3][stride]
+template <class Real> __device__ __forceinline__ V3<Real> ld3(const void* base, int stride, int i) {
+    const Real* p = reinterpret_cast<const Real*>(base) + i;
+    return {p[0], p[stride], p[2 * (size_t)stride]};
 }
-template <class Real> __device__ __forceinline__ void st3(void* base, int i, V3<Real> v) {
-    Real* p = reinterpret_cast<Real*>(base) + 3 * (size_t)i;
-    p[0] = v.x; p[1] = v.y; p[2] = v.z;
+template <class Real> __device__ __forceinline__ void st3(void* base, int stride, int i, V3<Real> v) {
+    Real* p = reinterpret_cast<Real*>(base) + i;
+    p[0] = v.x;
+    p[stride] = v.y;
+    p[2 * (size_t)stride] = v.z;
 }
 
 // position of internal atom i in FP64 (hi + lo in mixed precision)
 template <class Real> __device__ __forceinline__ V3<double> load_pos(const DevSys& S, int i) {
-    V3<double> p = cvt<double>(ld3<Real>(S.pos, i));
+    V3<double> p = cvt<double>(ld3<Real>(S.pos, S.atomStride, i));
     if (sizeof(Real) == 4) {
-        const float* lo = S.posLo + 3 * (size_t)i;
-        p.x += (double)lo[0]; p.y += (double)lo[1]; p.z += (double)lo[2];
+        V3<float> lo = ld3<float>(S.posLo, S.atomStride, i);
+        p.x += (double)lo.x; p.y += (double)lo.y; p.z += (double)lo.z;
     }
     return p;
 }
 // stores an FP64 position: hi = fp32(p), lo = fp32(p - hi) in mixed precision
 template <class Real> __device__ __forceinline__ void store_pos(const DevSys& S, int i, V3<double> p) {
     if (sizeof(Real) == 4) {
-        float hx = (float)p.x, hy = (float)p.y, hz = (float)p.z;
-        float* hi = reinterpret_cast<float*>(S.pos) + 3 * (size_t)i;
-        float* lo = S.posLo + 3 * (size_t)i;
-        hi[0] = hx; hi[1] = hy; hi[2] = hz;
-        lo[0] = (float)(p.x - (double)hx); lo[1] = (float)(p.y - (double)hy); lo[2] = (float)(p.z - (double)hz);
+        V3<float> hi = {(float)p.x, (float)p.y, (float)p.z};
+        V3<float> lo = {(float)(p.x - (double)hi.x), (float)(p.y - (double)hi.y), (float)(p.z - (double)hi.z)};
+        st3<float>(S.pos, S.atomStride, i, hi);
+        st3<float>(S.posLo, S.atomStride, i, lo);
     } else {
-        st3<Real>(S.pos, i, cvt<Real>(p));
+        st3<Real>(S.pos, S.atomStride, i, cvt<Real>(p));
     }
 }
 

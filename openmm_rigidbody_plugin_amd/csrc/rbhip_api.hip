@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -128,30 +129,36 @@ struct EventSet {
     }
 };
 
-template <class P1, class F, class P2>
-void time_steps(hipStream_t stream, int steps, P1 p1, F f, P2 p2, double* avgMs, double* totalMs) {
-    EventSet es(4 * (size_t)steps);
-    for (int s = 0; s < steps; s++) {
-        HIPCHK(hipEventRecord(es.ev[4 * s], stream));
-        p1();
-        HIPCHK(hipEventRecord(es.ev[4 * s + 1], stream));
-        f();
-        HIPCHK(hipEventRecord(es.ev[4 * s + 2], stream));
-        p2();
-        HIPCHK(hipEventRecord(es.ev[4 * s + 3], stream));
-    }
-    HIPCHK(hipEventSynchronize(es.ev[4 * steps - 1]));
-    double acc[3] = {0, 0, 0};
+// times `steps` repetitions of the phases (each a launch sequence) with HIP
+// events around each phase on `stream`; avgMs[i] = mean duration of phase i
+void time_phases(hipStream_t stream, int steps, const std::vector<std::function<void()>>& phases, double* avgMs,
+                 double* totalMs) {
+    const size_t np = phases.size();
+    EventSet es((np + 1) * (size_t)steps);
     for (int s = 0; s < steps; s++)
-        for (int k = 0; k < 3; k++) {
+        for (size_t k = 0; k <= np; k++) {
+            HIPCHK(hipEventRecord(es.ev[(np + 1) * s + k], stream));
+            if (k < np) phases[k]();
+        }
+    HIPCHK(hipEventSynchronize(es.ev[(np + 1) * steps - 1]));
+    std::vector<double> acc(np, 0.0);
+    for (int s = 0; s < steps; s++)
+        for (size_t k = 0; k < np; k++) {
             float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, es.ev[4 * s + k], es.ev[4 * s + k + 1]));
+            HIPCHK(hipEventElapsedTime(&ms, es.ev[(np + 1) * s + k], es.ev[(np + 1) * s + k + 1]));
             acc[k] += ms;
         }
-    for (int k = 0; k < 3; k++) avgMs[k] = acc[k] / steps;
-    float span = 0.f;
-    HIPCHK(hipEventElapsedTime(&span, es.ev[0], es.ev[4 * steps - 1]));
-    if (totalMs) *totalMs = span;
+    for (size_t k = 0; k < np; k++) avgMs[k] = acc[k] / steps;
+    if (totalMs) {
+        float span = 0.f;
+        HIPCHK(hipEventElapsedTime(&span, es.ev[0], es.ev[(np + 1) * steps - 1]));
+        *totalMs = span;
+    }
+}
+
+template <class P1, class F, class P2>
+void time_steps(hipStream_t stream, int steps, P1 p1, F f, P2 p2, double* avgMs, double* totalMs) {
+    time_phases(stream, steps, {p1, f, p2}, avgMs, totalMs);
 }
 }  // namespace
 
@@ -197,7 +204,9 @@ struct rbhip_context {
     bool built = false;
     bool stageVFresh = false;
     bool useGraph = true;
-    GraphCache graph;
+    bool useFusion = true;
+    GraphCache graph;       // part1, forces, part2
+    GraphCache fusedGraph;  // fused part2 + part1, forces
     double time = 0.0;
     long long stepCount = 0;
 
@@ -288,6 +297,7 @@ struct rbhip_context {
         nForceTiles = (int)ft.size();
         nSetupTiles = (int)st.size();
         graph.reset();
+        fusedGraph.reset();
     }
 
     void setSlotCounts(int ke, int pe) {
@@ -304,6 +314,9 @@ struct rbhip_context {
     }
     void launchPart2() {
         HIPCHK(launch_part2(precision, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+    }
+    void launchFused() {
+        HIPCHK(launch_fused(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
     }
 
     // 4 doubles reduced from the KE (which = 0) or PE (which = 1) partials
@@ -353,21 +366,19 @@ struct rbhip_context {
         }
     }
 
-    void ensureGraph() {
-        if (graph.exec) return;
+    template <class F> void ensureGraph(GraphCache& g, F&& launches) {
+        if (g.exec) return;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
-            launchPart1();
-            launchForces();
-            launchPart2();
+            launches();
         } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(stream, &g);
-            if (g) (void)hipGraphDestroy(g);
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(stream, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
             throw;
         }
-        HIPCHK(hipStreamEndCapture(stream, &graph.graph));
-        HIPCHK(hipGraphInstantiate(&graph.exec, graph.graph, nullptr, nullptr, 0));
+        HIPCHK(hipStreamEndCapture(stream, &g.graph));
+        HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
     }
 
     void fixKick() {
@@ -377,13 +388,29 @@ struct rbhip_context {
         }
     }
 
+    // RigidBodyIntegrator::step(steps).  With fusion, the steps' interior
+    // boundaries run as one fused pass each: part1, forces,
+    // (steps - 1) x [fused, forces], part2.
     void executeSteps(int steps) {
         require(built, "positions must be set before stepping", RBHIP_E_STATE);
         require(steps >= 0, "negative step count");
         fixKick();
         if (steps == 0) return;
-        if (useGraph) {
-            ensureGraph();
+        if (useFusion && steps >= 2) {
+            launchPart1();
+            launchForces();
+            if (useGraph) {
+                ensureGraph(fusedGraph, [&] { launchFused(); launchForces(); });
+                for (int s = 1; s < steps; s++) HIPCHK(hipGraphLaunch(fusedGraph.exec, stream));
+            } else {
+                for (int s = 1; s < steps; s++) {
+                    launchFused();
+                    launchForces();
+                }
+            }
+            launchPart2();
+        } else if (useGraph) {
+            ensureGraph(graph, [&] { launchPart1(); launchForces(); launchPart2(); });
             for (int s = 0; s < steps; s++) HIPCHK(hipGraphLaunch(graph.exec, stream));
         } else {
             for (int s = 0; s < steps; s++) {
@@ -400,6 +427,7 @@ struct rbhip_context {
 
     ~rbhip_context() {
         graph.reset();
+        fusedGraph.reset();
         if (hRed) (void)hipHostFree(hRed);
         tileArena.release();
         arena.release();
@@ -423,10 +451,13 @@ struct rbhip_batch {
     int* dKeSlots = nullptr;
     double* dRed = nullptr;
     int precision = 0, rotationMode = 0;
+    bool useFusion = true;
     GraphCache graph;
+    GraphCache fusedGraph;
 
     void rebuild() {
         graph.reset();
+        fusedGraph.reset();
         arena.release();
         nPart = nForce = 0;
         memberPartTiles.assign(members.size(), 0);
@@ -458,25 +489,58 @@ struct rbhip_batch {
         nForce = (int)ft.size();
     }
 
-    void launchAll() {
+    void launchPart1() {
         HIPCHK(launch_part1(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
+    }
+    void launchForces() {
         HIPCHK(launch_forces(precision, dSystems, dForceTiles, nForce, block, maxBlocks, stream));
+    }
+    void launchPart2() {
         HIPCHK(launch_part2(precision, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
     }
+    void launchFused() {
+        HIPCHK(launch_fused(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
+    }
 
-    void ensureGraph() {
-        if (graph.exec) return;
+    template <class F> void ensureGraph(GraphCache& g, F&& launches) {
+        if (g.exec) return;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
-            launchAll();
+            launches();
         } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(stream, &g);
-            if (g) (void)hipGraphDestroy(g);
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(stream, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
             throw;
         }
-        HIPCHK(hipStreamEndCapture(stream, &graph.graph));
-        HIPCHK(hipGraphInstantiate(&graph.exec, graph.graph, nullptr, nullptr, 0));
+        HIPCHK(hipStreamEndCapture(stream, &g.graph));
+        HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    }
+
+    void executeSteps(int steps) {
+        if (useFusion && steps >= 2) {
+            launchPart1();
+            launchForces();
+            if (useGraph) {
+                ensureGraph(fusedGraph, [&] { launchFused(); launchForces(); });
+                for (int s = 1; s < steps; s++) HIPCHK(hipGraphLaunch(fusedGraph.exec, stream));
+            } else {
+                for (int s = 1; s < steps; s++) {
+                    launchFused();
+                    launchForces();
+                }
+            }
+            launchPart2();
+        } else if (useGraph) {
+            ensureGraph(graph, [&] { launchPart1(); launchForces(); launchPart2(); });
+            for (int s = 0; s < steps; s++) HIPCHK(hipGraphLaunch(graph.exec, stream));
+        } else {
+            for (int s = 0; s < steps; s++) {
+                launchPart1();
+                launchForces();
+                launchPart2();
+            }
+        }
     }
 
     void finishSteps(int steps) {
@@ -506,6 +570,7 @@ struct rbhip_batch {
 
     ~rbhip_batch() {
         graph.reset();
+        fusedGraph.reset();
         arena.release();
         if (ordering) (void)hipEventDestroy(ordering);
         if (stream) (void)hipStreamDestroy(stream);
@@ -618,6 +683,9 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         h.numAtoms = numAtoms;
         h.rotationMode = rotationMode;
         h.bodyStride = (int)((nb + 63) / 64 * 64);
+        h.atomStride = (int)((na + 63) / 64 * 64);
+        h.dStride = (int)((std::max<size_t>(C.numBodyAtoms, 1) + 63) / 64 * 64);
+        const size_t as = h.atomStride;
         h.dt = stepSize;
         h.tetherScale = 0.0;
         h.q = C.arena.alloc(4 * R * nb);
@@ -625,18 +693,18 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         h.pm = C.arena.alloc(4 * R * nb);
         h.iI = C.arena.alloc(4 * R * nb);
         h.rcm = C.arena.allocT<double>(3 * (size_t)h.bodyStride);
-        h.d = C.arena.alloc(3 * R * std::max<size_t>(C.numBodyAtoms, 1));
-        h.pos = C.arena.alloc(3 * R * na);
-        h.posLo = precision == RBHIP_PRECISION_MIXED ? C.arena.allocT<float>(3 * na) : nullptr;
-        h.vel = C.arena.alloc(3 * R * na);
-        h.force = C.arena.alloc(3 * R * na);
+        h.d = C.arena.alloc(3 * R * (size_t)h.dStride);
+        h.pos = C.arena.alloc(3 * R * as);
+        h.posLo = precision == RBHIP_PRECISION_MIXED ? C.arena.allocT<float>(3 * as) : nullptr;
+        h.vel = C.arena.alloc(3 * R * as);
+        h.force = C.arena.alloc(3 * R * as);
         h.freeInvM = C.arena.alloc(R * std::max(C.numFree, 1));
         h.mass = C.arena.allocT<double>(na);
         h.stageR = C.arena.allocT<double>(3 * na);
         h.stageV = C.arena.allocT<double>(3 * na);
         h.tetherK = C.arena.alloc(R * na);
-        h.r0 = C.arena.alloc(3 * R * na);
-        h.r0Lo = precision == RBHIP_PRECISION_MIXED ? C.arena.allocT<float>(3 * na) : nullptr;
+        h.r0 = C.arena.alloc(3 * R * as);
+        h.r0Lo = precision == RBHIP_PRECISION_MIXED ? C.arena.allocT<float>(3 * as) : nullptr;
         h.bondStart = nullptr;
         h.bondOther = nullptr;
         h.bondLen = nullptr;
@@ -656,13 +724,14 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         HIPCHK(hipHostMalloc((void**)&C.hRed, 4 * sizeof(double), hipHostMallocDefault));
 
         // zero state, constants
-        HIPCHK(hipMemset(h.pos, 0, 3 * R * na));
-        if (h.posLo) HIPCHK(hipMemset(h.posLo, 0, 3 * sizeof(float) * na));
-        HIPCHK(hipMemset(h.vel, 0, 3 * R * na));
-        HIPCHK(hipMemset(h.force, 0, 3 * R * na));
+        HIPCHK(hipMemset(h.pos, 0, 3 * R * as));
+        if (h.posLo) HIPCHK(hipMemset(h.posLo, 0, 3 * sizeof(float) * as));
+        HIPCHK(hipMemset(h.vel, 0, 3 * R * as));
+        HIPCHK(hipMemset(h.force, 0, 3 * R * as));
+        HIPCHK(hipMemset(h.d, 0, 3 * R * (size_t)h.dStride));
         HIPCHK(hipMemset(h.tetherK, 0, R * na));
-        HIPCHK(hipMemset(h.r0, 0, 3 * R * na));
-        if (h.r0Lo) HIPCHK(hipMemset(h.r0Lo, 0, 3 * sizeof(float) * na));
+        HIPCHK(hipMemset(h.r0, 0, 3 * R * as));
+        if (h.r0Lo) HIPCHK(hipMemset(h.r0Lo, 0, 3 * sizeof(float) * as));
         HIPCHK(hipMemset(h.kePart, 0, 4 * sizeof(double) * C.maxKeSlots));
         HIPCHK(hipMemset(h.pePart, 0, 4 * sizeof(double) * C.maxPeSlots));
         std::vector<double> m(na);
@@ -705,6 +774,7 @@ int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
         ctx->stream = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->ownStream;
         ctx->graph.reset();
+        ctx->fusedGraph.reset();
     });
 }
 
@@ -745,23 +815,27 @@ int rbhip_set_tether(rbhip_context* ctx, const double* k, const double* r0) {
         require(ctx && k && r0, "null argument");
         rbhip_context& C = *ctx;
         C.setDevice();
-        size_t na = C.numAtoms;
+        size_t na = C.numAtoms, as = C.h.atomStride;
         std::vector<double> r = C.toInternal(r0);
         if (C.precision == RBHIP_PRECISION_MIXED) {
-            std::vector<float> kk(na), hi(3 * na), lo(3 * na);
+            std::vector<float> kk(na), hi(3 * as, 0.f), lo(3 * as, 0.f);
             for (size_t i = 0; i < na; i++) kk[i] = (float)k[C.atomOrig[i]];
-            for (size_t i = 0; i < 3 * na; i++) {
-                hi[i] = (float)r[i];
-                lo[i] = (float)(r[i] - (double)hi[i]);
-            }
+            for (size_t i = 0; i < na; i++)
+                for (int c = 0; c < 3; c++) {
+                    double v = r[3 * i + c];
+                    hi[c * as + i] = (float)v;
+                    lo[c * as + i] = (float)(v - (double)hi[c * as + i]);
+                }
             HIPCHK(hipMemcpy(C.h.tetherK, kk.data(), na * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(C.h.r0, hi.data(), 3 * na * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(C.h.r0Lo, lo.data(), 3 * na * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(C.h.r0, hi.data(), 3 * as * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(C.h.r0Lo, lo.data(), 3 * as * 4, hipMemcpyHostToDevice));
         } else {
-            std::vector<double> kk(na);
+            std::vector<double> kk(na), rr(3 * as, 0.0);
             for (size_t i = 0; i < na; i++) kk[i] = k[C.atomOrig[i]];
+            for (size_t i = 0; i < na; i++)
+                for (int c = 0; c < 3; c++) rr[c * as + i] = r[3 * i + c];
             HIPCHK(hipMemcpy(C.h.tetherK, kk.data(), na * 8, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(C.h.r0, r.data(), 3 * na * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(C.h.r0, rr.data(), 3 * as * 8, hipMemcpyHostToDevice));
         }
         C.h.tetherScale = 1.0;
         C.uploadSys();
@@ -927,7 +1001,17 @@ int rbhip_use_graph(rbhip_context* ctx, int enable) {
     return guard([&] {
         require(ctx, "null context");
         ctx->useGraph = enable != 0;
-        if (!ctx->useGraph) ctx->graph.reset();
+        if (!ctx->useGraph) {
+            ctx->graph.reset();
+            ctx->fusedGraph.reset();
+        }
+    });
+}
+
+int rbhip_use_fusion(rbhip_context* ctx, int enable) {
+    return guard([&] {
+        require(ctx, "null context");
+        ctx->useFusion = enable != 0;
     });
 }
 
@@ -978,15 +1062,18 @@ int rbhip_get_body_fixed_positions(rbhip_context* ctx, double* xyz) {
         rbhip_context& C = *ctx;
         if (C.numBodyAtoms == 0) return;
         C.setDevice();
-        size_t n = 3 * (size_t)C.numBodyAtoms;
-        std::vector<double> d(n);
+        const size_t ds = C.h.dStride, nba = C.numBodyAtoms;
+        std::vector<double> soa(3 * ds), d(3 * nba);
+        HIPCHK(hipStreamSynchronize(C.stream));
         if (C.precision == RBHIP_PRECISION_MIXED) {
-            std::vector<float> f(n);
-            HIPCHK(hipMemcpy(f.data(), C.h.d, n * 4, hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < n; i++) d[i] = f[i];
+            std::vector<float> f(3 * ds);
+            HIPCHK(hipMemcpy(f.data(), C.h.d, 3 * ds * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < 3 * ds; i++) soa[i] = f[i];
         } else {
-            HIPCHK(hipMemcpy(d.data(), C.h.d, n * 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(soa.data(), C.h.d, 3 * ds * 8, hipMemcpyDeviceToHost));
         }
+        for (size_t i = 0; i < nba; i++)
+            for (int c = 0; c < 3; c++) d[3 * i + c] = soa[c * ds + i];
         for (int ib = 0; ib < C.numBodies; ib++) {
             int r = C.internalBody[ib];
             size_t src = 3 * (size_t)(C.bodyAtom0[ib] - C.numFree), dst = 3 * (size_t)C.refBodyLoc[r];
@@ -1055,10 +1142,17 @@ int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* tot
         C.fixKick();
         time_steps(C.stream, steps, [&] { C.launchPart1(); }, [&] { C.launchForces(); }, [&] { C.launchPart2(); },
                    avgMs, totalMs);
+        // the fused path of step(steps + 1): part1, forces, steps x [fused, forces], part2
+        C.launchPart1();
+        C.launchForces();
+        double fusedMs[2];
+        time_phases(C.stream, steps, {[&] { C.launchFused(); }, [&] { C.launchForces(); }}, fusedMs, nullptr);
+        C.launchPart2();
+        avgMs[RBHIP_KERNEL_FUSED] = fusedMs[0];
         C.keSlots = C.nPartTiles;
         C.stageVFresh = false;
-        C.time += C.dt * steps;
-        C.stepCount += steps;
+        C.time += C.dt * (2 * steps + 1);
+        C.stepCount += 2 * steps + 1;
     });
 }
 
@@ -1075,9 +1169,10 @@ int rbhip_autotune(rbhip_context* ctx, int reps) {
         const size_t nb = std::max(C.numBodies, 1), na = C.numAtoms;
         struct Span { void* p; size_t n; };
         std::vector<Span> spans = {{C.h.q, 4 * R * nb}, {C.h.pi, 4 * R * nb}, {C.h.pm, 4 * R * nb},
-                                   {C.h.rcm, 3 * sizeof(double) * C.h.bodyStride}, {C.h.pos, 3 * R * na},
-                                   {C.h.vel, 3 * R * na}, {C.h.force, 3 * R * na}};
-        if (C.h.posLo) spans.push_back({C.h.posLo, 3 * sizeof(float) * na});
+                                   {C.h.rcm, 3 * sizeof(double) * C.h.bodyStride},
+                                   {C.h.pos, 3 * R * (size_t)C.h.atomStride}, {C.h.vel, 3 * R * (size_t)C.h.atomStride},
+                                   {C.h.force, 3 * R * (size_t)C.h.atomStride}};
+        if (C.h.posLo) spans.push_back({C.h.posLo, 3 * sizeof(float) * (size_t)C.h.atomStride});
         spans.push_back({C.h.kePart, 4 * sizeof(double) * (size_t)C.maxKeSlots});
         spans.push_back({C.h.pePart, 4 * sizeof(double) * (size_t)C.maxPeSlots});
         const int savedKeSlots = C.keSlots;
@@ -1248,7 +1343,17 @@ int rbhip_batch_use_graph(rbhip_batch* batch, int enable) {
     return guard([&] {
         require(batch, "null batch");
         batch->useGraph = enable != 0;
-        if (!batch->useGraph) batch->graph.reset();
+        if (!batch->useGraph) {
+            batch->graph.reset();
+            batch->fusedGraph.reset();
+        }
+    });
+}
+
+int rbhip_batch_use_fusion(rbhip_batch* batch, int enable) {
+    return guard([&] {
+        require(batch, "null batch");
+        batch->useFusion = enable != 0;
     });
 }
 
@@ -1260,12 +1365,7 @@ int rbhip_batch_execute(rbhip_batch* batch, int steps) {
         for (rbhip_context* c : batch->members)
             require(c->kickDt == c->dt, "member step size changed since it joined the batch", RBHIP_E_STATE);
         batch->joinMembers();
-        if (batch->useGraph) {
-            batch->ensureGraph();
-            for (int s = 0; s < steps; s++) HIPCHK(hipGraphLaunch(batch->graph.exec, batch->stream));
-        } else {
-            for (int s = 0; s < steps; s++) batch->launchAll();
-        }
+        batch->executeSteps(steps);
         batch->releaseMembers();
         batch->finishSteps(steps);
     });
@@ -1299,13 +1399,16 @@ int rbhip_batch_time_kernels(rbhip_batch* batch, int steps, double* avgMs, doubl
         HIPCHK(hipSetDevice(batch->device));
         rbhip_batch& B = *batch;
         B.joinMembers();
-        time_steps(B.stream, steps,
-                   [&] { HIPCHK(launch_part1(B.precision, B.rotationMode, B.dSystems, B.dPartTiles, B.nPart, B.block, B.maxBlocks, B.stream)); },
-                   [&] { HIPCHK(launch_forces(B.precision, B.dSystems, B.dForceTiles, B.nForce, B.block, B.maxBlocks, B.stream)); },
-                   [&] { HIPCHK(launch_part2(B.precision, B.dSystems, B.dPartTiles, B.nPart, B.block, B.maxBlocks, B.stream)); },
+        time_steps(B.stream, steps, [&] { B.launchPart1(); }, [&] { B.launchForces(); }, [&] { B.launchPart2(); },
                    avgMs, totalMs);
+        B.launchPart1();
+        B.launchForces();
+        double fusedMs[2];
+        time_phases(B.stream, steps, {[&] { B.launchFused(); }, [&] { B.launchForces(); }}, fusedMs, nullptr);
+        B.launchPart2();
+        avgMs[RBHIP_KERNEL_FUSED] = fusedMs[0];
         B.releaseMembers();
-        B.finishSteps(steps);
+        B.finishSteps(2 * steps + 1);
     });
 }
 

@@ -132,6 +132,23 @@ template <int K, class T> __device__ __forceinline__ void uniaxial(Q4<T>& q, Q4<
           fma(pi.c, cm1, fma(Ppi.c, s, pi.c)), fma(pi.d, cm1, fma(Ppi.d, s, pi.d))};
 }
 
+// sin(phi) and cos(phi) - 1 on |phi| <= PHI_FAST without a branch:
+// fp32 Taylor through phi^5 / phi^6 (truncation < 3e-11 relative at 0.1),
+// fp64 through phi^9 / phi^10 (< 2e-20 relative at 1/16).
+__device__ __forceinline__ void sin_cosm1_small(float phi, float& s, float& cm1) {
+    float x2 = phi * phi;
+    s = phi * fmaf(x2, fmaf(x2, 1.0f / 120.0f, -1.0f / 6.0f), 1.0f);
+    cm1 = x2 * fmaf(x2, fmaf(x2, -1.0f / 720.0f, 1.0f / 24.0f), -0.5f);
+}
+__device__ __forceinline__ void sin_cosm1_small(double phi, double& s, double& cm1) {
+    double x2 = phi * phi;
+    s = phi * fma(x2, fma(x2, fma(x2, fma(x2, 1.0 / 362880.0, -1.0 / 5040.0), 1.0 / 120.0), -1.0 / 6.0), 1.0);
+    cm1 = x2 * fma(x2, fma(x2, fma(x2, fma(x2, -1.0 / 3628800.0, 1.0 / 40320.0), -1.0 / 720.0), 1.0 / 24.0), -0.5);
+}
+template <class T> struct SmallAngle;
+template <> struct SmallAngle<float> { static constexpr float value = 0.1f; };
+template <> struct SmallAngle<double> { static constexpr double value = 0.0625; };
+
 // One uniaxial rotation of NO-SQUISH in body-frame variables.  For
 // pi = 2 B(q) L and |q| = 1 the reference's angle (pi . P_K q) tau / (4 I_K)
 // (RigidBody.cpp:189-214) equals L_K tau / (2 I_K), so the angles depend on L
@@ -142,14 +159,15 @@ template <int K, class T> __device__ __forceinline__ void uniaxial(Q4<T>& q, Q4<
 // Within a step r stays near the identity and L near its start value L0, so
 // both are carried as offsets, r = 1 + rho and L = L0 + lam: their rounding
 // errors scale with |rho| ~ w dt / 2 and |lam| ~ w dt |L| instead of with 1
-// and |L|.  In fp32 this keeps the 50 sequential updates of a 10-substep step
+// and |L|.  In fp32 this keeps the sequential updates of a 10-substep step
 // from random-walking the orientation (~1e-6 rad per 10 steps when q itself is
 // updated in fp32) at the cost of fp32 arithmetic.
-template <int K, class T>
+template <int K, bool SMALL, class T>
 __device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho, T coef) {
     T LK = K == 1 ? L0.x + lam.x : (K == 2 ? L0.y + lam.y : L0.z + lam.z);
     T s, cm1;
-    sin_cosm1(LK * coef, s, cm1);
+    if (SMALL) sin_cosm1_small(LK * coef, s, cm1);
+    else sin_cosm1(LK * coef, s, cm1);
     // rho <- rho + rho cm1 + P_K(rho) s + (cm1, s e_K)   [r (x) (1 + cm1, s e_K)]
     Q4<T> Pr = perm<K>(rho);
     Q4<T> nr = {fma(rho.a, cm1, fma(Pr.a, s, rho.a)) + cm1, fma(rho.b, cm1, fma(Pr.b, s, rho.b)),
@@ -158,9 +176,10 @@ __device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho,
     if (K == 2) nr.c += s;
     if (K == 3) nr.d += s;
     rho = nr;
-    // L turns by -2 phi about axis K: sin(2 phi), cos(2 phi) - 1 from the half angle
-    T s2 = (T)2 * s * ((T)1 + cm1);
-    T c2m1 = (T)2 * cm1 * ((T)2 + cm1);
+    // L turns by -2 phi about axis K: sin 2phi = 2s + 2s cm1, cos 2phi - 1 = 4 cm1 + 2 cm1^2
+    T s1 = s + s, c1 = cm1 + cm1;
+    T s2 = fma(s1, cm1, s1);
+    T c2m1 = fma(c1, cm1, c1 + c1);
     T& la = K == 1 ? lam.y : (K == 2 ? lam.z : lam.x);
     T& lb = K == 1 ? lam.z : (K == 2 ? lam.x : lam.y);
     T La = (K == 1 ? L0.y : (K == 2 ? L0.z : L0.x)) + la;
@@ -169,10 +188,38 @@ __device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho,
     lb = fma(Lb, c2m1, fma(-La, s2, lb));
 }
 
-// NO-SQUISH (reference RigidBody.cpp:220-231): n substeps of
-// [R3(h/2)] R2(h/2) R1(h) R2(h/2) [R3(h/2)], R3 only for dof-6 bodies
-// (invI.z == 0 marks the dof-5 linear bodies).  On return q = q (x) r
-// renormalised in FP64 and pi = 2 B(q) L, both formed in FP64.
+// The rotation sequence of NO-SQUISH(n) (reference RigidBody.cpp:220-231):
+// n substeps of [R3(h/2)] R2(h/2) R1(h) R2(h/2) [R3(h/2)].  Rotations about one
+// axis leave that component of L unchanged, so the back-to-back R3(h/2) R3(h/2)
+// of neighbouring substeps are one R3(h) (R2 likewise for dof-5 bodies): 4n+1
+// rotations instead of 5n (2n+1 instead of 3n).
+template <bool SMALL, class T>
+__device__ __forceinline__ void no_squish_turns(V3<T> L0, V3<T>& lam, Q4<T>& rho, V3<T> invI, T h, int n) {
+    T c1 = (T)0.5 * h * invI.x;
+    T c2h = (T)0.25 * h * invI.y, c2 = (T)0.5 * h * invI.y;
+    T c3h = (T)0.25 * h * invI.z, c3 = (T)0.5 * h * invI.z;
+    if (invI.z != (T)0) {
+        body_axis_turn<3, SMALL>(L0, lam, rho, c3h);
+        for (int i = 0; i < n; i++) {
+            body_axis_turn<2, SMALL>(L0, lam, rho, c2h);
+            body_axis_turn<1, SMALL>(L0, lam, rho, c1);
+            body_axis_turn<2, SMALL>(L0, lam, rho, c2h);
+            body_axis_turn<3, SMALL>(L0, lam, rho, i + 1 < n ? c3 : c3h);
+        }
+    } else {
+        body_axis_turn<2, SMALL>(L0, lam, rho, c2h);
+        for (int i = 0; i < n; i++) {
+            body_axis_turn<1, SMALL>(L0, lam, rho, c1);
+            body_axis_turn<2, SMALL>(L0, lam, rho, i + 1 < n ? c2 : c2h);
+        }
+    }
+}
+
+// NO-SQUISH rotation of (q, pi) over dt with n substeps; dof-5 linear bodies
+// are marked by invI.z == 0.  On return q = q (x) r renormalised in FP64 and
+// pi = 2 B(q) L, both formed in FP64.  |L| is conserved by every turn, so
+// |phi| <= |L| max(coef) bounds all angles of the step and selects the
+// branch-free small-angle path for the whole body.
 template <class T>
 __device__ __forceinline__ void no_squish(Q4<T>& q, Q4<T>& pi, V3<T> invI, double dt, int n) {
     Q4<double> qd = cvt<double>(q);
@@ -181,24 +228,12 @@ __device__ __forceinline__ void no_squish(Q4<T>& q, Q4<T>& pi, V3<T> invI, doubl
     V3<T> lam = {(T)0, (T)0, (T)0};
     Q4<T> rho = {(T)0, (T)0, (T)0, (T)0};
     T h = (T)(dt / (double)n);
-    T c1 = (T)0.5 * h * invI.x;
-    T c2 = (T)0.25 * h * invI.y;
-    T c3 = (T)0.25 * h * invI.z;
-    if (invI.z != (T)0) {
-        for (int i = 0; i < n; i++) {
-            body_axis_turn<3>(L0, lam, rho, c3);
-            body_axis_turn<2>(L0, lam, rho, c2);
-            body_axis_turn<1>(L0, lam, rho, c1);
-            body_axis_turn<2>(L0, lam, rho, c2);
-            body_axis_turn<3>(L0, lam, rho, c3);
-        }
-    } else {
-        for (int i = 0; i < n; i++) {
-            body_axis_turn<2>(L0, lam, rho, c2);
-            body_axis_turn<1>(L0, lam, rho, c1);
-            body_axis_turn<2>(L0, lam, rho, c2);
-        }
-    }
+    double cmax = 0.5 * (double)h * fmax((double)invI.x, fmax((double)invI.y, (double)invI.z));
+    double phimax = 1.001 * sqrt(dot(Ld, Ld)) * cmax;
+    if (phimax <= (double)SmallAngle<T>::value)
+        no_squish_turns<true>(L0, lam, rho, invI, h, n);
+    else
+        no_squish_turns<false>(L0, lam, rho, invI, h, n);
     // q (x) r, renormalised in FP64; L = L0 + lam in FP64
     Q4<double> r = {1.0 + (double)rho.a, (double)rho.b, (double)rho.c, (double)rho.d};
     Q4<double> qn = qd * r.a + perm<1>(qd) * r.b + perm<2>(qd) * r.c + perm<3>(qd) * r.d;

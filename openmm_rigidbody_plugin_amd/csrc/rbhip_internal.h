@@ -7,12 +7,15 @@
 //                                       step's first half-kick (folded into part2)
 //     iI             Real4[numBodies]   (1/I0, 1/I1, 1/I2, dof)
 //     rcm            double[3][bodyStride]  centre of mass, FP64
-//     d              Real[3*numBodyAtoms]   body-fixed atom coordinates, packed xyz
-//   atoms, internal order (free atoms, then each body's atoms contiguously):
-//     pos            Real[3*numAtoms]   positions (hi part in mixed precision)
-//     posLo          float[3*numAtoms]  low part, mixed precision only
-//     vel, force     Real[3*numAtoms]
+//     d              Real[3][dStride]   body-fixed atom coordinates
+//   atoms, internal order (free atoms, then each body's atoms contiguously),
+//   one array per component (x[], y[], z[], row length atomStride):
+//     pos            Real[3][atomStride]   positions (hi part in mixed precision)
+//     posLo          float[3][atomStride]  low part, mixed precision only
+//     vel, force     Real[3][atomStride]
 //     freeInvM       Real[numFree]
+// With a lane per body, the N atoms of a body are N consecutive elements of
+// each component array, so a wave's loads and stores are contiguous runs.
 // Work is described by tiles: each workgroup iteration handles one tile of
 // free atoms, of bodies of one size class, or of atoms for the force kernel.
 #pragma once
@@ -41,6 +44,8 @@ struct DevSys {
     int rotationMode;   // 0 exact, n > 0 NO-SQUISH(n)
     int bodyStride;     // row length of rcm
     int useStageR, useStageV;
+    int atomStride;     // row length of the per-atom component arrays
+    int dStride;        // row length of d
     double dt;
     double tetherScale; // 1 if the tether is on, 0 otherwise
     // bodies
@@ -57,12 +62,12 @@ struct DevSys {
     void* force;
     void* freeInvM;
     double* mass;
-    double* stageR;
+    double* stageR;     // FP64 xyz-interleaved staging, internal order
     double* stageV;
     // force field
     void* tetherK;      // Real[numAtoms]
-    void* r0;           // Real[3*numAtoms]
-    float* r0Lo;        // mixed precision low part of r0
+    void* r0;           // Real[3][atomStride]
+    float* r0Lo;        // mixed precision low part of r0, float[3][atomStride]
     int* bondStart;     // [numAtoms + 1] CSR of bonds per atom (nullptr: none)
     int* bondOther;     // partner atom of each bond entry
     double* bondLen;

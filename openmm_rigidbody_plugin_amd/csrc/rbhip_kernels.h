@@ -12,6 +12,9 @@ hipError_t launch_part1(int precision, int rotationMode, const DevSys* systems, 
                         int block, int maxBlocks, hipStream_t stream);
 hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
                         int maxBlocks, hipStream_t stream);
+// part2 of step n and part1 of step n+1 in one pass (no atom velocities, no KE)
+hipError_t launch_fused(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
+                        int block, int maxBlocks, hipStream_t stream);
 hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
                          int maxBlocks, hipStream_t stream);
 hipError_t launch_reduce(const DevSys* systems, const int* nslots, int numSystems, int which, double* out,

@@ -11,6 +11,8 @@
 //            FP64 kinetic-energy partials per tile, then the next step's first
 //            half-kick (the reference applies it at the start of part1 with the
 //            same F and C(q) tau, so folding it here is the same arithmetic).
+//   k_fused  part2 of step n + part1 of step n+1 in one pass (interior steps
+//            of RigidBodyIntegrator::step(k)).
 //   k_forces the force-field slot: harmonic tethers and harmonic bonds.
 //
 // Every workgroup walks the tile list with a grid stride, so one launch
@@ -62,13 +64,13 @@ __device__ __forceinline__ void part1_free(const DevSys& S, const Tile& tile) {
     if (i >= tile.count) return;
     int a = tile.first + i;
     Real halfDt = (Real)(0.5 * S.dt);
-    V3<Real> v = ld3<Real>(S.vel, a);
-    V3<Real> f = ld3<Real>(S.force, a);
+    V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
+    V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
     Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
     v = v + f * invm * halfDt;
     V3<double> r = load_pos<Real>(S, a);
     r = r + cvt<double>(v) * S.dt;
-    st3<Real>(S.vel, a, v);
+    st3<Real>(S.vel, S.atomStride, a, v);
     store_pos<Real>(S, a, r);
 }
 
@@ -79,12 +81,12 @@ __device__ __forceinline__ void part1_atoms(const DevSys& S, const Rot<Real>& R,
     if constexpr (NC > 0) {
         V3<Real> dj[NC > 0 ? NC : 1];
 #pragma unroll
-        for (int j = 0; j < NC; j++) dj[j] = ld3<Real>(S.d, dbase + j);
+        for (int j = 0; j < NC; j++) dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
 #pragma unroll
         for (int j = 0; j < NC; j++) store_pos<Real>(S, atom0 + j, rc + cvt<double>(apply(R, dj[j])));
     } else {
         for (int j = sub; j < N; j += lanes) {
-            V3<Real> dj = ld3<Real>(S.d, dbase + j);
+            V3<Real> dj = ld3<Real>(S.d, S.dStride, dbase + j);
             store_pos<Real>(S, atom0 + j, rc + cvt<double>(apply(R, dj)));
         }
     }
@@ -166,11 +168,11 @@ __device__ __forceinline__ void part2_free(const DevSys& S, const Tile& tile, do
     if (i >= tile.count) return;
     int a = tile.first + i;
     Real halfDt = (Real)(0.5 * S.dt);
-    V3<Real> v = ld3<Real>(S.vel, a);
-    V3<Real> f = ld3<Real>(S.force, a);
+    V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
+    V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
     Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
     v = v + f * invm * halfDt;
-    st3<Real>(S.vel, a, v);
+    st3<Real>(S.vel, S.atomStride, a, v);
     V3<double> vd = cvt<double>(v);
     ke[0] = dot(vd, vd) / (double)invm;
 }
@@ -183,8 +185,8 @@ __device__ __forceinline__ void part2_body_sums(const DevSys& S, const Rot<Real>
         V3<Real> dj[NC > 0 ? NC : 1], fj[NC > 0 ? NC : 1];
 #pragma unroll
         for (int j = 0; j < NC; j++) {
-            dj[j] = ld3<Real>(S.d, dbase + j);
-            fj[j] = ld3<Real>(S.force, atom0 + j);
+            dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
+            fj[j] = ld3<Real>(S.force, S.atomStride, atom0 + j);
         }
 #pragma unroll
         for (int j = 0; j < NC; j++) {
@@ -195,8 +197,8 @@ __device__ __forceinline__ void part2_body_sums(const DevSys& S, const Rot<Real>
         }
     } else {
         for (int j = sub; j < N; j += lanes) {
-            V3<double> dl = cvt<double>(apply(R, ld3<Real>(S.d, dbase + j)));
-            V3<double> f = cvt<double>(ld3<Real>(S.force, atom0 + j));
+            V3<double> dl = cvt<double>(apply(R, ld3<Real>(S.d, S.dStride, dbase + j)));
+            V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, atom0 + j));
             F = F + f;
             tau = tau + cross(dl, f);
         }
@@ -210,13 +212,13 @@ __device__ __forceinline__ void part2_velocities(const DevSys& S, const Rot<Real
     if constexpr (NC > 0) {
 #pragma unroll
         for (int j = 0; j < NC; j++) {
-            V3<Real> dl = apply(R, ld3<Real>(S.d, dbase + j));
-            st3<Real>(S.vel, atom0 + j, vcm + cross(ws, dl));
+            V3<Real> dl = apply(R, ld3<Real>(S.d, S.dStride, dbase + j));
+            st3<Real>(S.vel, S.atomStride, atom0 + j, vcm + cross(ws, dl));
         }
     } else {
         for (int j = sub; j < N; j += lanes) {
-            V3<Real> dl = apply(R, ld3<Real>(S.d, dbase + j));
-            st3<Real>(S.vel, atom0 + j, vcm + cross(ws, dl));
+            V3<Real> dl = apply(R, ld3<Real>(S.d, S.dStride, dbase + j));
+            st3<Real>(S.vel, S.atomStride, atom0 + j, vcm + cross(ws, dl));
         }
     }
 }
@@ -305,6 +307,127 @@ __global__ __launch_bounds__(512) void k_part2(const DevSys* __restrict__ system
 }
 
 // ---------------------------------------------------------------------------
+// fused step boundary: part2 of step n followed by part1 of step n+1 for the
+// same body, one pass over its data.  Inside RigidBodyIntegrator::step(k) the
+// atom velocities and kinetic energies of the interior steps are read by
+// nothing (forces depend on positions only), so this pass writes neither;
+// the last step of the call ends with a plain part2.  The body state is
+// rounded to storage precision at the same points as part2 + part1, so the
+// trajectory is bit-identical to the two-pass path.
+// ---------------------------------------------------------------------------
+template <class Real>
+__device__ __forceinline__ void fused_free(const DevSys& S, const Tile& tile) {
+    int i = threadIdx.x;
+    if (i >= tile.count) return;
+    int a = tile.first + i;
+    Real halfDt = (Real)(0.5 * S.dt);
+    V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
+    V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
+    Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
+    v = v + f * invm * halfDt;     // end of step n
+    v = v + f * invm * halfDt;     // start of step n+1
+    V3<double> r = load_pos<Real>(S, a);
+    r = r + cvt<double>(v) * S.dt;
+    st3<Real>(S.vel, S.atomStride, a, v);
+    store_pos<Real>(S, a, r);
+}
+
+template <class Real, bool EXACT>
+__device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) {
+    const int shift = lane_shift(tile.lanes);
+    const int b = threadIdx.x >> shift;
+    const int sub = threadIdx.x & (tile.lanes - 1);
+    const bool active = b < tile.count;
+    const int k = tile.first + (active ? b : 0);
+    const int N = tile.N;
+    const int atom0 = tile.atomFirst + (active ? b : 0) * N;
+    Q4<Real> q = ld4<Real>(S.q, k);
+    Q4<Real> pi = ld4<Real>(S.pi, k);
+    Q4<Real> pm = ld4<Real>(S.pm, k);
+    Q4<Real> iI = ld4<Real>(S.iI, k);
+    const int st = S.bodyStride;
+    V3<double> rc = {S.rcm[k], S.rcm[k + st], S.rcm[k + 2 * st]};
+    // part2 of step n: force and torque, half-kick, then the first half-kick of step n+1
+    Rot<Real> R = rotT(q);
+    V3<double> F = {0.0, 0.0, 0.0}, tau = {0.0, 0.0, 0.0};
+    if (tile.lanes == 1) {
+        if (active) {
+            switch (N) {
+            case 2: part2_body_sums<Real, 2>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 3: part2_body_sums<Real, 3>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 4: part2_body_sums<Real, 4>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 6: part2_body_sums<Real, 6>(S, R, atom0, N, 0, 1, F, tau); break;
+            default: part2_body_sums<Real, 0>(S, R, atom0, N, 0, 1, F, tau); break;
+            }
+        }
+    } else {
+        if (active) part2_body_sums<Real, 0>(S, R, atom0, N, sub, tile.lanes, F, tau);
+        F = {xor_sum(F.x, tile.lanes), xor_sum(F.y, tile.lanes), xor_sum(F.z, tile.lanes)};
+        tau = {xor_sum(tau.x, tile.lanes), xor_sum(tau.y, tile.lanes), xor_sum(tau.z, tile.lanes)};
+    }
+    if (!active) return;
+    const double dt = S.dt, halfDt = 0.5 * dt;
+    Q4<double> torque = Cmul(cvt<double>(q), tau);
+    V3<double> pcm = V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} + F * halfDt;
+    Q4<double> pid = cvt<double>(pi) + torque * dt;
+    pcm = pcm + F * halfDt;
+    pid = pid + torque * dt;
+    pm = Q4<Real>{(Real)pcm.x, (Real)pcm.y, (Real)pcm.z, pm.d};
+    pi = cvt<Real>(pid);
+    // part1 of step n+1: drift, rotation, atom positions
+    double idt = (double)pm.d * dt;
+    rc = rc + V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} * idt;
+    V3<Real> invI = {iI.a, iI.b, iI.c};
+    if (EXACT) {
+        Q4<double> qd = cvt<double>(q), pd = cvt<double>(pi);
+        V3<double> invId = cvt<double>(invI);
+        if (invI.z == (Real)0) {
+            linear_rotor_exact(qd, pd, invId, dt);
+        } else {
+            V3<double> I = {1.0 / invId.x, 1.0 / invId.y, 1.0 / invId.z};
+            exact_rotation(qd, pd, I, invId, dt);
+        }
+        q = cvt<Real>(qd);
+        pi = cvt<Real>(pd);
+    } else {
+        no_squish<Real>(q, pi, invI, dt, S.rotationMode);
+    }
+    if (sub == 0) {
+        st4<Real>(S.q, k, q);
+        st4<Real>(S.pi, k, pi);
+        st4<Real>(S.pm, k, pm);
+        S.rcm[k] = rc.x;
+        S.rcm[k + st] = rc.y;
+        S.rcm[k + 2 * st] = rc.z;
+    }
+    Rot<Real> R1 = rotT(q);
+    if (tile.lanes == 1) {
+        switch (N) {
+        case 2: part1_atoms<Real, 2>(S, R1, rc, atom0, N, 0, 1); break;
+        case 3: part1_atoms<Real, 3>(S, R1, rc, atom0, N, 0, 1); break;
+        case 4: part1_atoms<Real, 4>(S, R1, rc, atom0, N, 0, 1); break;
+        case 6: part1_atoms<Real, 6>(S, R1, rc, atom0, N, 0, 1); break;
+        default: part1_atoms<Real, 0>(S, R1, rc, atom0, N, 0, 1); break;
+        }
+    } else {
+        part1_atoms<Real, 0>(S, R1, rc, atom0, N, sub, tile.lanes);
+    }
+}
+
+template <class Real, bool EXACT>
+__global__ __launch_bounds__(EXACT ? 256 : 512) void k_fused(const DevSys* __restrict__ systems,
+                                                              const Tile* __restrict__ tiles, int numTiles) {
+    for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const DevSys& S = systems[tile.sys];
+        if (tile.kind == TILE_FREE)
+            fused_free<Real>(S, tile);
+        else
+            fused_bodies<Real, EXACT>(S, tile);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // force field: harmonic tether + harmonic bonds, one lane per atom
 // ---------------------------------------------------------------------------
 template <class Real>
@@ -322,13 +445,12 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
                 Real k = reinterpret_cast<const Real*>(S.tetherK)[a];
                 V3<Real> dx;
                 if (sizeof(Real) == 4) {
-                    V3<Real> ph = ld3<Real>(S.pos, a), rh = ld3<Real>(S.r0, a);
-                    const float* pl = S.posLo + 3 * (size_t)a;
-                    const float* rl = S.r0Lo + 3 * (size_t)a;
-                    dx = {(ph.x - rh.x) + (Real)(pl[0] - rl[0]), (ph.y - rh.y) + (Real)(pl[1] - rl[1]),
-                          (ph.z - rh.z) + (Real)(pl[2] - rl[2])};
+                    V3<Real> ph = ld3<Real>(S.pos, S.atomStride, a), rh = ld3<Real>(S.r0, S.atomStride, a);
+                    V3<float> pl = ld3<float>(S.posLo, S.atomStride, a), rl = ld3<float>(S.r0Lo, S.atomStride, a);
+                    dx = {(ph.x - rh.x) + (Real)(pl.x - rl.x), (ph.y - rh.y) + (Real)(pl.y - rl.y),
+                          (ph.z - rh.z) + (Real)(pl.z - rl.z)};
                 } else {
-                    dx = ld3<Real>(S.pos, a) - ld3<Real>(S.r0, a);
+                    dx = ld3<Real>(S.pos, S.atomStride, a) - ld3<Real>(S.r0, S.atomStride, a);
                 }
                 f = dx * (-k);
                 V3<double> dd = cvt<double>(dx);
@@ -349,7 +471,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
                     f = cvt<Real>(fd);
                 }
             }
-            st3<Real>(S.force, a, f);
+            st3<Real>(S.force, S.atomStride, a, f);
         }
         block_reduce_store<1>(e, S.pePart, tile.slot);
     }
@@ -399,6 +521,20 @@ hipError_t launch_part1(int precision, int rotationMode, const DevSys* systems, 
     } else {
         if (rotationMode == 0) k_part1<double, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
         else k_part1<double, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fused(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
+                        int block, int maxBlocks, hipStream_t stream) {
+    int grid = grid_for(numTiles, maxBlocks);
+    if (grid == 0) return hipSuccess;
+    if (precision == 0) {
+        if (rotationMode == 0) k_fused<float, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_fused<float, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    } else {
+        if (rotationMode == 0) k_fused<double, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_fused<double, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
     }
     return hipGetLastError();
 }

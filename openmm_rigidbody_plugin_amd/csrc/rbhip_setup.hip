@@ -55,7 +55,7 @@ template <class Real> __device__ __forceinline__ V3<double> setup_pos(const DevS
 }
 template <class Real> __device__ __forceinline__ V3<double> setup_vel(const DevSys& S, int i) {
     if (S.useStageV) return V3<double>{S.stageV[3 * (size_t)i], S.stageV[3 * (size_t)i + 1], S.stageV[3 * (size_t)i + 2]};
-    return cvt<double>(ld3<Real>(S.vel, i));
+    return cvt<double>(ld3<Real>(S.vel, S.atomStride, i));
 }
 
 template <class Real>
@@ -118,8 +118,8 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
         q = quat_from_matrix(A);
         for (int j = 0; j < N; j++) {
             V3<double> dl = setup_pos<Real>(S, atom0 + j) - rcm;
-            st3<Real>(S.d, dbase + j, cvt<Real>(m3_apply(A, dl)));
-            V3<double> f = cvt<double>(ld3<Real>(S.force, atom0 + j));
+            st3<Real>(S.d, S.dStride, dbase + j, cvt<Real>(m3_apply(A, dl)));
+            V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, atom0 + j));
             F = F + f;
             tau = tau + cross(dl, f);
         }
@@ -135,8 +135,8 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
         invI = {(double)iI.a, (double)iI.b, (double)iI.c};
         dof = (double)iI.d;
         for (int j = 0; j < N; j++) {
-            V3<double> dl = Atmul(q, cvt<double>(ld3<Real>(S.d, dbase + j)));
-            V3<double> f = cvt<double>(ld3<Real>(S.force, atom0 + j));
+            V3<double> dl = Atmul(q, cvt<double>(ld3<Real>(S.d, S.dStride, dbase + j)));
+            V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, atom0 + j));
             F = F + f;
             tau = tau + cross(dl, f);
         }
@@ -151,7 +151,7 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
     for (int j = 0; j < N; j++) {
         V3<double> dj;
         if (geometry) dj = m3_apply(A, setup_pos<Real>(S, atom0 + j) - rcm);
-        else dj = cvt<double>(ld3<Real>(S.d, dbase + j));
+        else dj = cvt<double>(ld3<Real>(S.d, S.dStride, dbase + j));
         L = L + cross(dj, Amul(q, setup_vel<Real>(S, atom0 + j) - vcm) * S.mass[atom0 + j]);
     }
     Q4<double> pi = Bmul(q, L) * 2.0;
@@ -216,7 +216,7 @@ __global__ void k_stage_in(const DevSys* __restrict__ sys, const double* __restr
     if (i >= n) return;
     V3<double> x = {in[3 * (size_t)i], in[3 * (size_t)i + 1], in[3 * (size_t)i + 2]};
     if (what == 0) store_pos<Real>(*sys, i, x);
-    else st3<Real>(sys->vel, i, cvt<Real>(x));
+    else st3<Real>(sys->vel, sys->atomStride, i, cvt<Real>(x));
 }
 
 // device state -> FP64 staging
@@ -226,8 +226,8 @@ __global__ void k_stage_out(const DevSys* __restrict__ sys, double* __restrict__
     if (i >= n) return;
     V3<double> x;
     if (what == 0) x = load_pos<Real>(*sys, i);
-    else if (what == 1) x = cvt<double>(ld3<Real>(sys->vel, i));
-    else x = cvt<double>(ld3<Real>(sys->force, i));
+    else if (what == 1) x = cvt<double>(ld3<Real>(sys->vel, sys->atomStride, i));
+    else x = cvt<double>(ld3<Real>(sys->force, sys->atomStride, i));
     out[3 * (size_t)i] = x.x;
     out[3 * (size_t)i + 1] = x.y;
     out[3 * (size_t)i + 2] = x.z;
@@ -241,8 +241,8 @@ __device__ inline void body_force_torque(const DevSys& S, Q4<double> q, int N, i
     V3<double> tau = {0.0, 0.0, 0.0};
     F = {0.0, 0.0, 0.0};
     for (int j = 0; j < N; j++) {
-        V3<double> dl = Atmul(q, cvt<double>(ld3<Real>(S.d, dbase + j)));
-        V3<double> f = cvt<double>(ld3<Real>(S.force, atom0 + j));
+        V3<double> dl = Atmul(q, cvt<double>(ld3<Real>(S.d, S.dStride, dbase + j)));
+        V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, atom0 + j));
         F = F + f;
         tau = tau + cross(dl, f);
     }

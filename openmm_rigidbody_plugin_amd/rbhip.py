@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_PKG, "librbhip.so")
 MIXED = 0
 DOUBLE = 1
 PRECISIONS = {"mixed": MIXED, "double": DOUBLE}
-KERNELS = ("part1", "forces", "part2")
+KERNELS = ("part1", "forces", "part2", "fused")
 BODY_FIELDS = dict(N=0, dof=1, mass=2, I=slice(3, 6), invI=slice(6, 9), rcm=slice(9, 12), pcm=slice(12, 15),
                    q=slice(15, 19), pi=slice(19, 23), force=slice(23, 26), torque=slice(26, 30), twoKt=30,
                    twoKr=31, loc=32)
@@ -69,6 +69,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_part2": (_I, [_P]),
             "rbhip_execute": (_I, [_P, _I]),
             "rbhip_use_graph": (_I, [_P, _I]),
+            "rbhip_use_fusion": (_I, [_P, _I]),
             "rbhip_kinetic_energies": (_I, [_P, _DP]),
             "rbhip_potential_energy": (_I, [_P, _DP]),
             "rbhip_get_body_states": (_I, [_P, _DP]),
@@ -84,6 +85,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_batch_set_members": (_I, [_P, _I, ctypes.POINTER(_P)]),
             "rbhip_batch_set_launch": (_I, [_P, _I, _I]),
             "rbhip_batch_use_graph": (_I, [_P, _I]),
+            "rbhip_batch_use_fusion": (_I, [_P, _I]),
             "rbhip_batch_execute": (_I, [_P, _I]),
             "rbhip_batch_kinetic_energies": (_I, [_P, _DP]),
             "rbhip_batch_time_kernels": (_I, [_P, _I, _DP, _DP]),
@@ -257,6 +259,9 @@ class Context:
     def use_graph(self, enable=True):
         _check(lib().rbhip_use_graph(self._h, int(bool(enable))))
 
+    def use_fusion(self, enable=True):
+        _check(lib().rbhip_use_fusion(self._h, int(bool(enable))))
+
     def kinetic_energies(self):
         out = np.empty(2)
         _check(lib().rbhip_kinetic_energies(self._h, _dp(out)))
@@ -301,7 +306,9 @@ class Context:
         return buf.value.decode()
 
     def time_kernels(self, steps):
-        avg = np.empty(3)
+        """Mean kernel durations (ms) over `steps` two-pass steps and the fused
+        path of step(steps + 1); advances the state by 2 * steps + 1 steps."""
+        avg = np.empty(len(KERNELS))
         tot = ctypes.c_double()
         _check(lib().rbhip_time_kernels(self._h, int(steps), _dp(avg), ctypes.byref(tot)))
         return dict(zip(KERNELS, avg)), tot.value
@@ -337,6 +344,9 @@ class Batch:
     def use_graph(self, enable=True):
         _check(lib().rbhip_batch_use_graph(self._h, int(bool(enable))))
 
+    def use_fusion(self, enable=True):
+        _check(lib().rbhip_batch_use_fusion(self._h, int(bool(enable))))
+
     def step(self, steps=1):
         _check(lib().rbhip_batch_execute(self._h, int(steps)))
 
@@ -347,7 +357,7 @@ class Batch:
         return out[:len(self.members)]
 
     def time_kernels(self, steps):
-        avg = np.empty(3)
+        avg = np.empty(len(KERNELS))
         tot = ctypes.c_double()
         _check(lib().rbhip_batch_time_kernels(self._h, int(steps), _dp(avg), ctypes.byref(tot)))
         return dict(zip(KERNELS, avg)), tot.value

@@ -220,3 +220,32 @@ def test_errors_are_reported(hip):
         g.step(1)
     with pytest.raises(hip.RbhipError):
         g.set_launch(block=96)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+@pytest.mark.parametrize("mode", [10, 0])
+def test_fused_steps_equal_two_pass(hip, precision, mode):
+    """step(k) with fused interior boundaries = k x [part1, forces, part2]:
+    bit for bit in mixed precision (the state is rounded to fp32 at the same
+    points); in FP64 the compiler fuses multiply-adds differently across the
+    single kernel, so the two paths agree to a few ulp."""
+    spec = systems.heterogeneous(700, 900, seed=17, rotation_mode=mode)
+    a = hip.Context.from_spec(spec, precision=precision)
+    b = hip.Context.from_spec(spec, precision=precision)
+    b.use_fusion(False)
+    if mode == 10:
+        for c in (a, b):
+            c.set_lanes(20, 4)
+    for c in (a, b):
+        c.step(9)
+        c.step(1)
+        c.step(5)
+    if precision == "mixed":
+        assert np.array_equal(a.positions(), b.positions())
+        assert np.array_equal(a.velocities(), b.velocities())
+        assert np.array_equal(a.kinetic_energies(), b.kinetic_energies())
+        assert np.array_equal(a.body_states_array(), b.body_states_array())
+    else:
+        assert np.abs(a.positions() - b.positions()).max() < 1e-14
+        assert np.abs(a.velocities() - b.velocities()).max() < 1e-13
+        assert np.allclose(a.kinetic_energies(), b.kinetic_energies(), rtol=1e-14, atol=0)
