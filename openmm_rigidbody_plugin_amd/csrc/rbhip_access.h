@@ -33,6 +33,18 @@ template <class Real> __device__ __forceinline__ void st3(void* base, int stride
     p[2 * (size_t)stride] = v.z;
 }
 
+// element k of a packed array Real[3*n] (xyz per element)
+template <class Real> __device__ __forceinline__ V3<Real> ldp3(const void* base, int k) {
+    const Real* p = reinterpret_cast<const Real*>(base) + 3 * (size_t)k;
+    return {p[0], p[1], p[2]};
+}
+template <class Real> __device__ __forceinline__ void stp3(void* base, int k, V3<Real> v) {
+    Real* p = reinterpret_cast<Real*>(base) + 3 * (size_t)k;
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+
 // position of internal atom i in FP64 (hi + lo in mixed precision)
 template <class Real> __device__ __forceinline__ V3<double> load_pos(const DevSys& S, int i) {
     V3<double> p = cvt<double>(ld3<Real>(S.pos, S.atomStride, i));

@@ -203,6 +203,7 @@ struct rbhip_context {
 
     bool built = false;
     bool stageVFresh = false;
+    bool positionsDerived = false;               // body-atom positions hold fp32 r_cm + A(q)^T d only
     bool useGraph = true;
     bool useFusion = true;
     GraphCache graph;       // part1, forces, part2
@@ -355,6 +356,8 @@ struct rbhip_context {
 
     void download(int what, double* xyz) {
         HIPCHK(launch_stage_out(precision, dSys, dTmp, numAtoms, what, stream));
+        if (what == 0 && positionsDerived && precision == RBHIP_PRECISION_MIXED)
+            HIPCHK(launch_body_atoms_out(precision, dSys, dBodyN, dBodyAtom0, numBodies, dTmp, stream));
         std::vector<double> tmp(3 * (size_t)numAtoms);
         HIPCHK(hipMemcpyAsync(tmp.data(), dTmp, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -421,6 +424,7 @@ struct rbhip_context {
         }
         keSlots = nPartTiles;
         stageVFresh = false;
+        positionsDerived = true;
         time += dt * steps;
         stepCount += steps;
     }
@@ -548,6 +552,7 @@ struct rbhip_batch {
             rbhip_context* c = members[m];
             c->keSlots = memberPartTiles[m];
             c->stageVFresh = false;
+            if (steps > 0) c->positionsDerived = true;
             c->time += c->dt * steps;
             c->stepCount += steps;
         }
@@ -689,8 +694,8 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         h.dt = stepSize;
         h.tetherScale = 0.0;
         h.q = C.arena.alloc(4 * R * nb);
-        h.pi = C.arena.alloc(4 * R * nb);
-        h.pm = C.arena.alloc(4 * R * nb);
+        h.L = C.arena.alloc(3 * R * nb);
+        h.pcm = C.arena.alloc(3 * R * nb);
         h.iI = C.arena.alloc(4 * R * nb);
         h.rcm = C.arena.allocT<double>(3 * (size_t)h.bodyStride);
         h.d = C.arena.alloc(3 * R * (size_t)h.dStride);
@@ -838,6 +843,10 @@ int rbhip_set_tether(rbhip_context* ctx, const double* k, const double* r0) {
             HIPCHK(hipMemcpy(C.h.r0, rr.data(), 3 * as * 8, hipMemcpyHostToDevice));
         }
         C.h.tetherScale = 1.0;
+        C.h.tetherUniform = 1;
+        C.h.tetherK0 = k[0];
+        for (size_t i = 1; i < na; i++)
+            if (k[i] != k[0]) C.h.tetherUniform = 0;
         C.uploadSys();
         HIPCHK(hipStreamSynchronize(C.stream));
     });
@@ -914,6 +923,7 @@ int rbhip_set_positions(rbhip_context* ctx, const double* xyz) {
         C.uploadSys();
         C.launchForces();
         C.runSetup(1);
+        C.positionsDerived = false;
         C.h.useStageR = 0;
         C.uploadSys();
         HIPCHK(hipStreamSynchronize(C.stream));
@@ -975,6 +985,7 @@ int rbhip_part1(rbhip_context* ctx) {
         ctx->setDevice();
         ctx->fixKick();
         ctx->launchPart1();
+        ctx->positionsDerived = true;
     });
 }
 
@@ -1168,7 +1179,7 @@ int rbhip_autotune(rbhip_context* ctx, int reps) {
         const size_t R = C.realSize();
         const size_t nb = std::max(C.numBodies, 1), na = C.numAtoms;
         struct Span { void* p; size_t n; };
-        std::vector<Span> spans = {{C.h.q, 4 * R * nb}, {C.h.pi, 4 * R * nb}, {C.h.pm, 4 * R * nb},
+        std::vector<Span> spans = {{C.h.q, 4 * R * nb}, {C.h.L, 3 * R * nb}, {C.h.pcm, 3 * R * nb},
                                    {C.h.rcm, 3 * sizeof(double) * C.h.bodyStride},
                                    {C.h.pos, 3 * R * (size_t)C.h.atomStride}, {C.h.vel, 3 * R * (size_t)C.h.atomStride},
                                    {C.h.force, 3 * R * (size_t)C.h.atomStride}};
@@ -1188,14 +1199,25 @@ int rbhip_autotune(rbhip_context* ctx, int reps) {
                 HIPCHK(hipMemcpyAsync(spans[i].p, copies[i], spans[i].n, hipMemcpyDeviceToDevice, C.stream));
             HIPCHK(hipStreamSynchronize(C.stream));
         };
+        // the time of one step as rbhip_execute runs it: fused pass + forces
+        // with fusion, part1 + forces + part2 without
         auto measure = [&]() -> double {
             double avg[3], total;
-            // one untimed step, then `reps` timed steps
-            C.launchPart1(); C.launchForces(); C.launchPart2();
-            time_steps(C.stream, reps, [&] { C.launchPart1(); }, [&] { C.launchForces(); },
-                       [&] { C.launchPart2(); }, avg, &total);
+            double t;
+            if (C.useFusion) {
+                C.launchPart1(); C.launchForces();
+                C.launchFused(); C.launchForces();
+                time_phases(C.stream, reps, {[&] { C.launchFused(); }, [&] { C.launchForces(); }}, avg, &total);
+                C.launchPart2();
+                t = avg[0] + avg[1];
+            } else {
+                C.launchPart1(); C.launchForces(); C.launchPart2();
+                time_steps(C.stream, reps, [&] { C.launchPart1(); }, [&] { C.launchForces(); },
+                           [&] { C.launchPart2(); }, avg, &total);
+                t = avg[0] + avg[1] + avg[2];
+            }
             restore();
-            return avg[0] + avg[2];
+            return t;
         };
         try {
             const int blocks[] = {64, 128, 256, 512};

@@ -4,8 +4,10 @@
 //
 // Quaternion convention (reference openmmapi/src/MatVec.cpp:495-549):
 // q = (q0, q1, q2, q3); B(q) has columns q (x) e_k, C(q) columns e_k (x) q;
-// A(q) = B(q)^T C(q) maps space -> body, A(q)^T = C(q)^T B(q) body -> space;
-// the body angular momentum is L = B(q)^T pi / 2.
+// A(q) = B(q)^T C(q) maps space -> body, A(q)^T = C(q)^T B(q) body -> space.
+// The reference's quaternion momentum is pi = 2 B(q) L with L the body-frame
+// angular momentum; the device state carries L, and a torque kick
+// pi += C(q) tau dt is L += A(q) tau dt / 2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -118,20 +120,6 @@ __device__ __forceinline__ void sin_cosm1(double phi, double& s, double& cm1) {
     }
 }
 
-// uniaxial rotation about principal axis K (reference RigidBody.cpp:189-214):
-// phi = (pi . P_K q) h / (4 I_K); coef = h / (4 I_K)
-template <int K, class T> __device__ __forceinline__ void uniaxial(Q4<T>& q, Q4<T>& pi, T coef) {
-    Q4<T> Pq = perm<K>(q);
-    Q4<T> Ppi = perm<K>(pi);
-    T phi = dot(pi, Pq) * coef;
-    T s, cm1;
-    sin_cosm1(phi, s, cm1);
-    q = {fma(q.a, cm1, fma(Pq.a, s, q.a)), fma(q.b, cm1, fma(Pq.b, s, q.b)),
-         fma(q.c, cm1, fma(Pq.c, s, q.c)), fma(q.d, cm1, fma(Pq.d, s, q.d))};
-    pi = {fma(pi.a, cm1, fma(Ppi.a, s, pi.a)), fma(pi.b, cm1, fma(Ppi.b, s, pi.b)),
-          fma(pi.c, cm1, fma(Ppi.c, s, pi.c)), fma(pi.d, cm1, fma(Ppi.d, s, pi.d))};
-}
-
 // sin(phi) and cos(phi) - 1 on |phi| <= PHI_FAST without a branch:
 // fp32 Taylor through phi^5 / phi^6 (truncation < 3e-11 relative at 0.1),
 // fp64 through phi^9 / phi^10 (< 2e-20 relative at 1/16).
@@ -215,34 +203,32 @@ __device__ __forceinline__ void no_squish_turns(V3<T> L0, V3<T>& lam, Q4<T>& rho
     }
 }
 
-// NO-SQUISH rotation of (q, pi) over dt with n substeps; dof-5 linear bodies
+// NO-SQUISH rotation of (q, L) over dt with n substeps; dof-5 linear bodies
 // are marked by invI.z == 0.  On return q = q (x) r renormalised in FP64 and
-// pi = 2 B(q) L, both formed in FP64.  |L| is conserved by every turn, so
+// L = L0 + lam formed in FP64.  |L| is conserved by every turn, so
 // |phi| <= |L| max(coef) bounds all angles of the step and selects the
 // branch-free small-angle path for the whole body.
 template <class T>
-__device__ __forceinline__ void no_squish(Q4<T>& q, Q4<T>& pi, V3<T> invI, double dt, int n) {
-    Q4<double> qd = cvt<double>(q);
-    V3<double> Ld = Btmul(qd, cvt<double>(pi)) * 0.5;
-    V3<T> L0 = cvt<T>(Ld);
+__device__ __forceinline__ void no_squish(Q4<T>& q, V3<T>& L, V3<T> invI, double dt, int n) {
+    V3<T> L0 = L;
     V3<T> lam = {(T)0, (T)0, (T)0};
     Q4<T> rho = {(T)0, (T)0, (T)0, (T)0};
     T h = (T)(dt / (double)n);
     double cmax = 0.5 * (double)h * fmax((double)invI.x, fmax((double)invI.y, (double)invI.z));
+    V3<double> Ld = cvt<double>(L0);
     double phimax = 1.001 * sqrt(dot(Ld, Ld)) * cmax;
     if (phimax <= (double)SmallAngle<T>::value)
         no_squish_turns<true>(L0, lam, rho, invI, h, n);
     else
         no_squish_turns<false>(L0, lam, rho, invI, h, n);
-    // q (x) r, renormalised in FP64; L = L0 + lam in FP64
+    // q (x) r, renormalised in FP64
+    Q4<double> qd = cvt<double>(q);
     Q4<double> r = {1.0 + (double)rho.a, (double)rho.b, (double)rho.c, (double)rho.d};
     Q4<double> qn = qd * r.a + perm<1>(qd) * r.b + perm<2>(qd) * r.c + perm<3>(qd) * r.d;
     double e = dot(qn, qn) - 1.0;
     double sc = fabs(e) < 1e-4 ? 1.0 + e * (-0.5 + 0.375 * e) : 1.0 / sqrt(1.0 + e);
-    qn = qn * sc;
-    V3<double> L = cvt<double>(L0) + cvt<double>(lam);
-    q = cvt<T>(qn);
-    pi = cvt<T>(Bmul(qn, L) * 2.0);
+    q = cvt<T>(qn * sc);
+    L = cvt<T>(Ld + cvt<double>(lam));
 }
 
 // ---------------------------------------------------------------------------
@@ -355,8 +341,7 @@ __device__ __forceinline__ int stair(double x) { return x > 0.0 ? (int)ceil(x - 
 // free linear rotor (dof 5): body-frame L constant, rotation about L at
 // |w| = |L| / I0 (closed form; replaces the exact solution, which divides by
 // I0 - I1 = 0 for these bodies)
-__device__ inline void linear_rotor_exact(Q4<double>& q, Q4<double>& pi, V3<double> invI, double dt) {
-    V3<double> L = Btmul(q, pi) * 0.5;
+__device__ inline void linear_rotor_exact(Q4<double>& q, V3<double> L, V3<double> invI, double dt) {
     V3<double> w = hadamard(invI, L);
     double wn = sqrt(dot(w, w));
     if (wn == 0.0) return;
@@ -366,18 +351,25 @@ __device__ inline void linear_rotor_exact(Q4<double>& q, Q4<double>& pi, V3<doub
     q = q * ch + Bmul(q, axis) * sh;
     double r = 1.0 / sqrt(dot(q, q));
     q = q * r;
-    pi = Bmul(q, L * 2.0);
 }
 
-// exact free-rotor rotation over dt (reference RigidBody.cpp:241-308), FP64
-__device__ inline void exact_rotation(Q4<double>& q, Q4<double>& pi, V3<double> I, V3<double> invI, double dt) {
+// exact free-rotor rotation of (q, L) over dt (reference RigidBody.cpp:241-308),
+// FP64; L is the body-frame angular momentum (the reference's B(q)^T pi / 2)
+__device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> I, V3<double> invI, double dt) {
     const double eps = 2.220446049250313e-16;
     const double PI = 3.14159265358979323846;
-    V3<double> Iw = Btmul(q, pi) * 0.5;
+    V3<double> Iw = Lb;
     V3<double> w0 = hadamard(invI, Iw);
     double Lsq = Iw.y * Iw.y + Iw.z * Iw.z;
     if (Lsq < eps) {
-        uniaxial<1>(q, pi, 0.25 * dt * invI.x);
+        // rotation about body axis 1 only (RigidBody.cpp:246-247): q turns by
+        // theta = L1 dt / I1 about e1, L by -theta about e1
+        double theta = Iw.x * invI.x * dt;
+        double sh, ch, s2, c2;
+        sincos(0.5 * theta, &sh, &ch);
+        sincos(theta, &s2, &c2);
+        q = q * ch + perm<1>(q) * sh;
+        Lb = {Iw.x, Iw.y * c2 + Iw.z * s2, -Iw.y * s2 + Iw.z * c2};
         return;
     }
     Lsq += Iw.x * Iw.x;
@@ -439,7 +431,7 @@ __device__ inline void exact_rotation(Q4<double>& q, Q4<double>& pi, V3<double> 
     q = z * dot(z0, q) + Cmul(z, Ctmul(z0, q));
     double r = 1.0 / sqrt(dot(q, q));
     q = q * r;
-    pi = Bmul(q, Iw * 2.0);
+    Lb = Iw;
 }
 
 // ---------------------------------------------------------------------------

@@ -2,16 +2,23 @@
 //
 // Device data layout (structure of arrays, one context = one system):
 //   bodies, internal body order (grouped by atoms-per-body class):
-//     q, pi          Real4[numBodies]   orientation quaternion, conjugate momentum
-//     pm             Real4[numBodies]   (pcm.xyz, 1/M); pcm and pi carry the next
-//                                       step's first half-kick (folded into part2)
-//     iI             Real4[numBodies]   (1/I0, 1/I1, 1/I2, dof)
+//     q              Real4[numBodies]   orientation quaternion
+//     L              Real[3*numBodies]  body-frame angular momentum, xyz per body
+//                                       (the reference's pi = 2 B(q) L)
+//     pcm            Real[3*numBodies]  centre-of-mass momentum, xyz per body
+//                                       (L and pcm carry the next step's first
+//                                       half-kick, folded into part2)
+//     iI             Real4[numBodies]   (1/I0, 1/I1, 1/I2, 1/M); 1/I2 = 0 marks a
+//                                       linear (dof 5) body
 //     rcm            double[3][bodyStride]  centre of mass, FP64
 //     d              Real[3][dStride]   body-fixed atom coordinates
 //   atoms, internal order (free atoms, then each body's atoms contiguously),
 //   one array per component (x[], y[], z[], row length atomStride):
 //     pos            Real[3][atomStride]   positions (hi part in mixed precision)
-//     posLo          float[3][atomStride]  low part, mixed precision only
+//     posLo          float[3][atomStride]  low part, mixed precision, free atoms
+//                    (a body atom's exact position is r_cm + A(q)^T d; the
+//                    force field reads its fp32 value, as OpenMM's mixed
+//                    precision force kernels read fp32 posq)
 //     vel, force     Real[3][atomStride]
 //     freeInvM       Real[numFree]
 // With a lane per body, the N atoms of a body are N consecutive elements of
@@ -48,10 +55,13 @@ struct DevSys {
     int dStride;        // row length of d
     double dt;
     double tetherScale; // 1 if the tether is on, 0 otherwise
+    int tetherUniform;  // 1: every atom has spring constant tetherK0 (tetherK unused)
+    int pad0;
+    double tetherK0;
     // bodies
     void* q;
-    void* pi;
-    void* pm;
+    void* L;
+    void* pcm;
     void* iI;
     double* rcm;
     void* d;

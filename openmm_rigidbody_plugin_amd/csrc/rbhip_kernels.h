@@ -36,6 +36,9 @@ hipError_t launch_stage_out(int precision, const DevSys* sys, double* out, int n
 // body states (internal body order) as 36 FP64 values each, half-kick removed
 hipError_t launch_body_states(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
                               int numBodies, double* out, hipStream_t stream);
+// FP64 body-atom positions rebuilt from the body state into `out` (internal order)
+hipError_t launch_body_atoms_out(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
+                                 int numBodies, double* out, hipStream_t stream);
 // re-applies the folded half-kick for a new step size: ddt = dtNew - dtOld
 hipError_t launch_rekick(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0, int numBodies,
                          double ddt, hipStream_t stream);

@@ -2,19 +2,22 @@
 //
 //   k_part1  integratePart1 (reference openmmapi/src/RigidBodySystem.cpp:170-187):
 //            free atoms: v += F/m dt/2, r += v dt; bodies: r_cm += p_cm dt/M
-//            (FP64), free rotation (NO-SQUISH with an FP64 composite rotation,
-//            or the FP64 exact solution), FP64 renormalisation of q, atom
-//            positions r_j = r_cm + A(q)^T d_j stored as fp32 hi + lo.
+//            (FP64), free rotation (NO-SQUISH with an FP64-accurate composite
+//            rotation, or the FP64 exact solution), FP64 renormalisation of q,
+//            atom positions r_j = r_cm + A(q)^T d_j.
 //   k_part2  integratePart2 (RigidBodySystem.cpp:193-204): free atoms
 //            v += F/m dt/2; bodies: F = sum f_j, tau = sum delta_j x f_j (FP64),
-//            half-kick of p_cm and pi, atom velocities v_cm + w_s x delta_j,
+//            half-kick of p_cm and L, atom velocities v_cm + w_s x delta_j,
 //            FP64 kinetic-energy partials per tile, then the next step's first
 //            half-kick (the reference applies it at the start of part1 with the
-//            same F and C(q) tau, so folding it here is the same arithmetic).
+//            same F and torque, so folding it here is the same arithmetic).
 //   k_fused  part2 of step n + part1 of step n+1 in one pass (interior steps
 //            of RigidBodyIntegrator::step(k)).
 //   k_forces the force-field slot: harmonic tethers and harmonic bonds.
 //
+// Body state per body: q (Real4), body-frame L and p_cm (3 Reals each), iI =
+// (1/I, 1/M), r_cm (FP64).  The reference's quaternion momentum is
+// pi = 2 B(q) L, its torque kick pi += C(q) tau dt is L += A(q) tau dt / 2.
 // Every workgroup walks the tile list with a grid stride, so one launch
 // covers any number of systems (the batch path) and any grid cap.
 #include <hip/hip_runtime.h>
@@ -55,6 +58,167 @@ __device__ void block_reduce_store(double (&v)[NV], double* out, int slot) {
     __syncthreads();
 }
 
+__device__ __forceinline__ double xor_sum(double v, int lanes) {
+    for (int m = 1; m < lanes; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// body state and the per-atom loops
+// ---------------------------------------------------------------------------
+template <class Real> struct Body {
+    Q4<Real> q;
+    V3<Real> L, p, invI;
+    Real invM;
+};
+
+template <class Real> __device__ __forceinline__ Body<Real> load_body(const DevSys& S, int k) {
+    Q4<Real> iI = ld4<Real>(S.iI, k);
+    return Body<Real>{ld4<Real>(S.q, k), ldp3<Real>(S.L, k), ldp3<Real>(S.pcm, k), V3<Real>{iI.a, iI.b, iI.c}, iI.d};
+}
+
+// F = sum f_j and tau = sum delta_j x f_j in FP64, delta_j = A(q)^T d_j
+template <class Real, int NC>
+__device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, int atom0, int N, int sub, int lanes,
+                                          V3<double>& F, V3<double>& tau) {
+    const int dbase = atom0 - S.numFree;
+    if constexpr (NC > 0) {
+        V3<Real> dj[NC], fj[NC];
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
+            fj[j] = ld3<Real>(S.force, S.atomStride, atom0 + j);
+        }
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            V3<double> dl = cvt<double>(apply(R, dj[j]));
+            V3<double> f = cvt<double>(fj[j]);
+            F = F + f;
+            tau = tau + cross(dl, f);
+        }
+    } else {
+        for (int j = sub; j < N; j += lanes) {
+            V3<double> dl = cvt<double>(apply(R, ld3<Real>(S.d, S.dStride, dbase + j)));
+            V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, atom0 + j));
+            F = F + f;
+            tau = tau + cross(dl, f);
+        }
+    }
+}
+
+template <class Real>
+__device__ __forceinline__ void body_sums_any(const DevSys& S, const Rot<Real>& R, int atom0, int N, int sub,
+                                              int lanes, bool active, V3<double>& F, V3<double>& tau) {
+    F = {0.0, 0.0, 0.0};
+    tau = {0.0, 0.0, 0.0};
+    if (lanes == 1) {
+        if (active) {
+            switch (N) {
+            case 2: body_sums<Real, 2>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 3: body_sums<Real, 3>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 4: body_sums<Real, 4>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 6: body_sums<Real, 6>(S, R, atom0, N, 0, 1, F, tau); break;
+            default: body_sums<Real, 0>(S, R, atom0, N, 0, 1, F, tau); break;
+            }
+        }
+    } else {
+        if (active) body_sums<Real, 0>(S, R, atom0, N, sub, lanes, F, tau);
+        F = {xor_sum(F.x, lanes), xor_sum(F.y, lanes), xor_sum(F.z, lanes)};
+        tau = {xor_sum(tau.x, lanes), xor_sum(tau.y, lanes), xor_sum(tau.z, lanes)};
+    }
+}
+
+// atom positions r_cm + A(q)^T d_j: fp32 in mixed precision (the force field's
+// view; the exact value is rebuilt from the body state on download)
+template <class Real, int NC>
+__device__ __forceinline__ void body_positions(const DevSys& S, const Rot<Real>& R, V3<double> rc, int atom0, int N,
+                                               int sub, int lanes) {
+    const int dbase = atom0 - S.numFree;
+    if constexpr (NC > 0) {
+        V3<Real> dj[NC];
+#pragma unroll
+        for (int j = 0; j < NC; j++) dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
+#pragma unroll
+        for (int j = 0; j < NC; j++) st3<Real>(S.pos, S.atomStride, atom0 + j, cvt<Real>(rc + cvt<double>(apply(R, dj[j]))));
+    } else {
+        for (int j = sub; j < N; j += lanes) {
+            V3<Real> dj = ld3<Real>(S.d, S.dStride, dbase + j);
+            st3<Real>(S.pos, S.atomStride, atom0 + j, cvt<Real>(rc + cvt<double>(apply(R, dj))));
+        }
+    }
+}
+
+template <class Real>
+__device__ __forceinline__ void body_positions_any(const DevSys& S, const Rot<Real>& R, V3<double> rc, int atom0,
+                                                   int N, int sub, int lanes) {
+    if (lanes == 1) {
+        switch (N) {
+        case 2: body_positions<Real, 2>(S, R, rc, atom0, N, 0, 1); break;
+        case 3: body_positions<Real, 3>(S, R, rc, atom0, N, 0, 1); break;
+        case 4: body_positions<Real, 4>(S, R, rc, atom0, N, 0, 1); break;
+        case 6: body_positions<Real, 6>(S, R, rc, atom0, N, 0, 1); break;
+        default: body_positions<Real, 0>(S, R, rc, atom0, N, 0, 1); break;
+        }
+    } else {
+        body_positions<Real, 0>(S, R, rc, atom0, N, sub, lanes);
+    }
+}
+
+template <class Real, int NC>
+__device__ __forceinline__ void body_velocities(const DevSys& S, const Rot<Real>& R, V3<Real> vcm, V3<Real> ws,
+                                                int atom0, int N, int sub, int lanes) {
+    const int dbase = atom0 - S.numFree;
+    if constexpr (NC > 0) {
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            V3<Real> dl = apply(R, ld3<Real>(S.d, S.dStride, dbase + j));
+            st3<Real>(S.vel, S.atomStride, atom0 + j, vcm + cross(ws, dl));
+        }
+    } else {
+        for (int j = sub; j < N; j += lanes) {
+            V3<Real> dl = apply(R, ld3<Real>(S.d, S.dStride, dbase + j));
+            st3<Real>(S.vel, S.atomStride, atom0 + j, vcm + cross(ws, dl));
+        }
+    }
+}
+
+// A(q) x from the rows of A(q)^T
+template <class Real> __device__ __forceinline__ V3<double> to_body(const Rot<Real>& R, V3<double> x) {
+    return cvt<double>(R.r0) * x.x + cvt<double>(R.r1) * x.y + cvt<double>(R.r2) * x.z;
+}
+
+// r_cm += p_cm (dt / M), then the free rotation of (q, L)
+template <class Real, bool EXACT>
+__device__ __forceinline__ void drift_and_rotate(const DevSys& S, Body<Real>& B, V3<double>& rc) {
+    double idt = (double)B.invM * S.dt;
+    rc = rc + cvt<double>(B.p) * idt;
+    if (EXACT) {
+        Q4<double> qd = cvt<double>(B.q);
+        V3<double> Ld = cvt<double>(B.L), invId = cvt<double>(B.invI);
+        if (B.invI.z == (Real)0) {
+            linear_rotor_exact(qd, Ld, invId, S.dt);
+        } else {
+            V3<double> I = {1.0 / invId.x, 1.0 / invId.y, 1.0 / invId.z};
+            exact_rotation(qd, Ld, I, invId, S.dt);
+        }
+        B.q = cvt<Real>(qd);
+        B.L = cvt<Real>(Ld);
+    } else {
+        no_squish<Real>(B.q, B.L, B.invI, S.dt, S.rotationMode);
+    }
+}
+
+template <class Real> __device__ __forceinline__ void store_rcm(const DevSys& S, int k, V3<double> rc) {
+    const int st = S.bodyStride;
+    S.rcm[k] = rc.x;
+    S.rcm[k + st] = rc.y;
+    S.rcm[k + 2 * st] = rc.z;
+}
+template <class Real> __device__ __forceinline__ V3<double> load_rcm(const DevSys& S, int k) {
+    const int st = S.bodyStride;
+    return V3<double>{S.rcm[k], S.rcm[k + st], S.rcm[k + 2 * st]};
+}
+
 // ---------------------------------------------------------------------------
 // part 1
 // ---------------------------------------------------------------------------
@@ -74,24 +238,6 @@ __device__ __forceinline__ void part1_free(const DevSys& S, const Tile& tile) {
     store_pos<Real>(S, a, r);
 }
 
-template <class Real, int NC>
-__device__ __forceinline__ void part1_atoms(const DevSys& S, const Rot<Real>& R, V3<double> rc, int atom0,
-                                            int N, int sub, int lanes) {
-    const int dbase = atom0 - S.numFree;
-    if constexpr (NC > 0) {
-        V3<Real> dj[NC > 0 ? NC : 1];
-#pragma unroll
-        for (int j = 0; j < NC; j++) dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
-#pragma unroll
-        for (int j = 0; j < NC; j++) store_pos<Real>(S, atom0 + j, rc + cvt<double>(apply(R, dj[j])));
-    } else {
-        for (int j = sub; j < N; j += lanes) {
-            V3<Real> dj = ld3<Real>(S.d, S.dStride, dbase + j);
-            store_pos<Real>(S, atom0 + j, rc + cvt<double>(apply(R, dj)));
-        }
-    }
-}
-
 template <class Real, bool EXACT>
 __device__ __forceinline__ void part1_bodies(const DevSys& S, const Tile& tile) {
     const int shift = lane_shift(tile.lanes);
@@ -99,56 +245,21 @@ __device__ __forceinline__ void part1_bodies(const DevSys& S, const Tile& tile) 
     const int sub = threadIdx.x & (tile.lanes - 1);
     if (b >= tile.count) return;
     const int k = tile.first + b;
-    const int N = tile.N;
-    const int atom0 = tile.atomFirst + b * N;
-    Q4<Real> q = ld4<Real>(S.q, k);
-    Q4<Real> pi = ld4<Real>(S.pi, k);
-    Q4<Real> pm = ld4<Real>(S.pm, k);
-    Q4<Real> iI = ld4<Real>(S.iI, k);
-    const int st = S.bodyStride;
-    V3<double> rc = {S.rcm[k], S.rcm[k + st], S.rcm[k + 2 * st]};
-    // r_cm += p_cm (dt / M)
-    double idt = (double)pm.d * S.dt;
-    rc = rc + V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} * idt;
-    V3<Real> invI = {iI.a, iI.b, iI.c};
-    if (EXACT) {
-        Q4<double> qd = cvt<double>(q), pd = cvt<double>(pi);
-        V3<double> invId = cvt<double>(invI);
-        if (invI.z == (Real)0) {
-            linear_rotor_exact(qd, pd, invId, S.dt);
-        } else {
-            V3<double> I = {1.0 / invId.x, 1.0 / invId.y, 1.0 / invId.z};
-            exact_rotation(qd, pd, I, invId, S.dt);
-        }
-        q = cvt<Real>(qd);
-        pi = cvt<Real>(pd);
-    } else {
-        no_squish<Real>(q, pi, invI, S.dt, S.rotationMode);
-    }
+    const int atom0 = tile.atomFirst + b * tile.N;
+    Body<Real> B = load_body<Real>(S, k);
+    V3<double> rc = load_rcm<Real>(S, k);
+    drift_and_rotate<Real, EXACT>(S, B, rc);
     if (sub == 0) {
-        st4<Real>(S.q, k, q);
-        st4<Real>(S.pi, k, pi);
-        S.rcm[k] = rc.x;
-        S.rcm[k + st] = rc.y;
-        S.rcm[k + 2 * st] = rc.z;
+        st4<Real>(S.q, k, B.q);
+        stp3<Real>(S.L, k, B.L);
+        store_rcm<Real>(S, k, rc);
     }
-    Rot<Real> R = rotT(q);
-    if (tile.lanes == 1) {
-        switch (N) {
-        case 2: part1_atoms<Real, 2>(S, R, rc, atom0, N, 0, 1); break;
-        case 3: part1_atoms<Real, 3>(S, R, rc, atom0, N, 0, 1); break;
-        case 4: part1_atoms<Real, 4>(S, R, rc, atom0, N, 0, 1); break;
-        case 6: part1_atoms<Real, 6>(S, R, rc, atom0, N, 0, 1); break;
-        default: part1_atoms<Real, 0>(S, R, rc, atom0, N, 0, 1); break;
-        }
-    } else {
-        part1_atoms<Real, 0>(S, R, rc, atom0, N, sub, tile.lanes);
-    }
+    body_positions_any<Real>(S, rotT(B.q), rc, atom0, tile.N, sub, tile.lanes);
 }
 
 template <class Real, bool EXACT>
-__global__ __launch_bounds__(EXACT ? 256 : 512) void k_part1(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
-                                                 int numTiles) {
+__global__ __launch_bounds__(EXACT ? 256 : 512) void k_part1(const DevSys* __restrict__ systems,
+                                                              const Tile* __restrict__ tiles, int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const DevSys& S = systems[tile.sys];
@@ -177,57 +288,6 @@ __device__ __forceinline__ void part2_free(const DevSys& S, const Tile& tile, do
     ke[0] = dot(vd, vd) / (double)invm;
 }
 
-template <class Real, int NC>
-__device__ __forceinline__ void part2_body_sums(const DevSys& S, const Rot<Real>& R, int atom0, int N, int sub,
-                                                int lanes, V3<double>& F, V3<double>& tau) {
-    const int dbase = atom0 - S.numFree;
-    if constexpr (NC > 0) {
-        V3<Real> dj[NC > 0 ? NC : 1], fj[NC > 0 ? NC : 1];
-#pragma unroll
-        for (int j = 0; j < NC; j++) {
-            dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
-            fj[j] = ld3<Real>(S.force, S.atomStride, atom0 + j);
-        }
-#pragma unroll
-        for (int j = 0; j < NC; j++) {
-            V3<double> dl = cvt<double>(apply(R, dj[j]));
-            V3<double> f = cvt<double>(fj[j]);
-            F = F + f;
-            tau = tau + cross(dl, f);
-        }
-    } else {
-        for (int j = sub; j < N; j += lanes) {
-            V3<double> dl = cvt<double>(apply(R, ld3<Real>(S.d, S.dStride, dbase + j)));
-            V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, atom0 + j));
-            F = F + f;
-            tau = tau + cross(dl, f);
-        }
-    }
-}
-
-template <class Real, int NC>
-__device__ __forceinline__ void part2_velocities(const DevSys& S, const Rot<Real>& R, V3<Real> vcm, V3<Real> ws,
-                                                 int atom0, int N, int sub, int lanes) {
-    const int dbase = atom0 - S.numFree;
-    if constexpr (NC > 0) {
-#pragma unroll
-        for (int j = 0; j < NC; j++) {
-            V3<Real> dl = apply(R, ld3<Real>(S.d, S.dStride, dbase + j));
-            st3<Real>(S.vel, S.atomStride, atom0 + j, vcm + cross(ws, dl));
-        }
-    } else {
-        for (int j = sub; j < N; j += lanes) {
-            V3<Real> dl = apply(R, ld3<Real>(S.d, S.dStride, dbase + j));
-            st3<Real>(S.vel, S.atomStride, atom0 + j, vcm + cross(ws, dl));
-        }
-    }
-}
-
-__device__ __forceinline__ double xor_sum(double v, int lanes) {
-    for (int m = 1; m < lanes; m <<= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
-
 template <class Real>
 __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, double (&ke)[2]) {
     const int shift = lane_shift(tile.lanes);
@@ -237,63 +297,37 @@ __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, 
     const int k = tile.first + (active ? b : 0);
     const int N = tile.N;
     const int atom0 = tile.atomFirst + (active ? b : 0) * N;
-    Q4<Real> q = ld4<Real>(S.q, k);
-    Q4<Real> pi = ld4<Real>(S.pi, k);
-    Q4<Real> pm = ld4<Real>(S.pm, k);
-    Q4<Real> iI = ld4<Real>(S.iI, k);
-    Rot<Real> R = rotT(q);
-    V3<double> F = {0.0, 0.0, 0.0}, tau = {0.0, 0.0, 0.0};
-    if (tile.lanes == 1) {
-        if (active) {
-            switch (N) {
-            case 2: part2_body_sums<Real, 2>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 3: part2_body_sums<Real, 3>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 4: part2_body_sums<Real, 4>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 6: part2_body_sums<Real, 6>(S, R, atom0, N, 0, 1, F, tau); break;
-            default: part2_body_sums<Real, 0>(S, R, atom0, N, 0, 1, F, tau); break;
-            }
-        }
-    } else {
-        if (active) part2_body_sums<Real, 0>(S, R, atom0, N, sub, tile.lanes, F, tau);
-        F = {xor_sum(F.x, tile.lanes), xor_sum(F.y, tile.lanes), xor_sum(F.z, tile.lanes)};
-        tau = {xor_sum(tau.x, tile.lanes), xor_sum(tau.y, tile.lanes), xor_sum(tau.z, tile.lanes)};
-    }
+    Body<Real> B = load_body<Real>(S, k);
+    Rot<Real> R = rotT(B.q);
+    V3<double> F, tau;
+    body_sums_any<Real>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
     if (!active) return;
-    const double dt = S.dt, halfDt = 0.5 * dt;
-    Q4<double> qd = cvt<double>(q);
-    Q4<double> torque = Cmul(qd, tau);
-    V3<double> pcm = V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} + F * halfDt;
-    Q4<double> pid = cvt<double>(pi) + torque * dt;
-    double invM = (double)pm.d;
-    V3<double> L = Btmul(qd, pid) * 0.5;
-    V3<double> w = hadamard(cvt<double>(V3<Real>{iI.a, iI.b, iI.c}), L);
-    V3<double> vcm = pcm * invM;
+    const double halfDt = 0.5 * S.dt;
+    V3<double> tb = to_body(R, tau);
+    V3<double> p = cvt<double>(B.p) + F * halfDt;
+    V3<double> L = cvt<double>(B.L) + tb * halfDt;
+    V3<double> w = hadamard(cvt<double>(B.invI), L);
+    V3<double> vcm = p * (double)B.invM;
     V3<Real> ws = apply(R, cvt<Real>(w));
-    if (tile.lanes == 1) {
-        switch (N) {
-        case 2: part2_velocities<Real, 2>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-        case 3: part2_velocities<Real, 3>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-        case 4: part2_velocities<Real, 4>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-        case 6: part2_velocities<Real, 6>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-        default: part2_velocities<Real, 0>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-        }
-    } else {
-        part2_velocities<Real, 0>(S, R, cvt<Real>(vcm), ws, atom0, N, sub, tile.lanes);
+    switch (tile.lanes == 1 ? N : 0) {
+    case 2: body_velocities<Real, 2>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
+    case 3: body_velocities<Real, 3>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
+    case 4: body_velocities<Real, 4>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
+    case 6: body_velocities<Real, 6>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
+    default: body_velocities<Real, 0>(S, R, cvt<Real>(vcm), ws, atom0, N, sub, tile.lanes); break;
     }
     if (sub == 0) {
-        ke[0] = dot(pcm, vcm);
+        ke[0] = dot(p, vcm);
         ke[1] = dot(L, w);
         // first half-kick of the next step (same F and torque)
-        pcm = pcm + F * halfDt;
-        pid = pid + torque * dt;
-        st4<Real>(S.pm, k, Q4<Real>{(Real)pcm.x, (Real)pcm.y, (Real)pcm.z, pm.d});
-        st4<Real>(S.pi, k, cvt<Real>(pid));
+        stp3<Real>(S.pcm, k, cvt<Real>(p + F * halfDt));
+        stp3<Real>(S.L, k, cvt<Real>(L + tb * halfDt));
     }
 }
 
 template <class Real>
 __global__ __launch_bounds__(512) void k_part2(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
-                                                 int numTiles) {
+                                                int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const DevSys& S = systems[tile.sys];
@@ -312,8 +346,7 @@ __global__ __launch_bounds__(512) void k_part2(const DevSys* __restrict__ system
 // atom velocities and kinetic energies of the interior steps are read by
 // nothing (forces depend on positions only), so this pass writes neither;
 // the last step of the call ends with a plain part2.  The body state is
-// rounded to storage precision at the same points as part2 + part1, so the
-// trajectory is bit-identical to the two-pass path.
+// rounded to storage precision at the same points as part2 + part1.
 // ---------------------------------------------------------------------------
 template <class Real>
 __device__ __forceinline__ void fused_free(const DevSys& S, const Tile& tile) {
@@ -341,77 +374,28 @@ __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) 
     const int k = tile.first + (active ? b : 0);
     const int N = tile.N;
     const int atom0 = tile.atomFirst + (active ? b : 0) * N;
-    Q4<Real> q = ld4<Real>(S.q, k);
-    Q4<Real> pi = ld4<Real>(S.pi, k);
-    Q4<Real> pm = ld4<Real>(S.pm, k);
-    Q4<Real> iI = ld4<Real>(S.iI, k);
-    const int st = S.bodyStride;
-    V3<double> rc = {S.rcm[k], S.rcm[k + st], S.rcm[k + 2 * st]};
-    // part2 of step n: force and torque, half-kick, then the first half-kick of step n+1
-    Rot<Real> R = rotT(q);
-    V3<double> F = {0.0, 0.0, 0.0}, tau = {0.0, 0.0, 0.0};
-    if (tile.lanes == 1) {
-        if (active) {
-            switch (N) {
-            case 2: part2_body_sums<Real, 2>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 3: part2_body_sums<Real, 3>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 4: part2_body_sums<Real, 4>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 6: part2_body_sums<Real, 6>(S, R, atom0, N, 0, 1, F, tau); break;
-            default: part2_body_sums<Real, 0>(S, R, atom0, N, 0, 1, F, tau); break;
-            }
-        }
-    } else {
-        if (active) part2_body_sums<Real, 0>(S, R, atom0, N, sub, tile.lanes, F, tau);
-        F = {xor_sum(F.x, tile.lanes), xor_sum(F.y, tile.lanes), xor_sum(F.z, tile.lanes)};
-        tau = {xor_sum(tau.x, tile.lanes), xor_sum(tau.y, tile.lanes), xor_sum(tau.z, tile.lanes)};
-    }
+    Body<Real> B = load_body<Real>(S, k);
+    V3<double> rc = load_rcm<Real>(S, k);
+    Rot<Real> R = rotT(B.q);
+    V3<double> F, tau;
+    body_sums_any<Real>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
     if (!active) return;
-    const double dt = S.dt, halfDt = 0.5 * dt;
-    Q4<double> torque = Cmul(cvt<double>(q), tau);
-    V3<double> pcm = V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} + F * halfDt;
-    Q4<double> pid = cvt<double>(pi) + torque * dt;
-    pcm = pcm + F * halfDt;
-    pid = pid + torque * dt;
-    pm = Q4<Real>{(Real)pcm.x, (Real)pcm.y, (Real)pcm.z, pm.d};
-    pi = cvt<Real>(pid);
-    // part1 of step n+1: drift, rotation, atom positions
-    double idt = (double)pm.d * dt;
-    rc = rc + V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} * idt;
-    V3<Real> invI = {iI.a, iI.b, iI.c};
-    if (EXACT) {
-        Q4<double> qd = cvt<double>(q), pd = cvt<double>(pi);
-        V3<double> invId = cvt<double>(invI);
-        if (invI.z == (Real)0) {
-            linear_rotor_exact(qd, pd, invId, dt);
-        } else {
-            V3<double> I = {1.0 / invId.x, 1.0 / invId.y, 1.0 / invId.z};
-            exact_rotation(qd, pd, I, invId, dt);
-        }
-        q = cvt<Real>(qd);
-        pi = cvt<Real>(pd);
-    } else {
-        no_squish<Real>(q, pi, invI, dt, S.rotationMode);
-    }
+    // part2 of step n and the first half-kick of step n+1
+    const double halfDt = 0.5 * S.dt;
+    V3<double> tb = to_body(R, tau);
+    V3<double> p = cvt<double>(B.p) + F * halfDt;
+    V3<double> L = cvt<double>(B.L) + tb * halfDt;
+    B.p = cvt<Real>(p + F * halfDt);
+    B.L = cvt<Real>(L + tb * halfDt);
+    // part1 of step n+1
+    drift_and_rotate<Real, EXACT>(S, B, rc);
     if (sub == 0) {
-        st4<Real>(S.q, k, q);
-        st4<Real>(S.pi, k, pi);
-        st4<Real>(S.pm, k, pm);
-        S.rcm[k] = rc.x;
-        S.rcm[k + st] = rc.y;
-        S.rcm[k + 2 * st] = rc.z;
+        st4<Real>(S.q, k, B.q);
+        stp3<Real>(S.L, k, B.L);
+        stp3<Real>(S.pcm, k, B.p);
+        store_rcm<Real>(S, k, rc);
     }
-    Rot<Real> R1 = rotT(q);
-    if (tile.lanes == 1) {
-        switch (N) {
-        case 2: part1_atoms<Real, 2>(S, R1, rc, atom0, N, 0, 1); break;
-        case 3: part1_atoms<Real, 3>(S, R1, rc, atom0, N, 0, 1); break;
-        case 4: part1_atoms<Real, 4>(S, R1, rc, atom0, N, 0, 1); break;
-        case 6: part1_atoms<Real, 6>(S, R1, rc, atom0, N, 0, 1); break;
-        default: part1_atoms<Real, 0>(S, R1, rc, atom0, N, 0, 1); break;
-        }
-    } else {
-        part1_atoms<Real, 0>(S, R1, rc, atom0, N, sub, tile.lanes);
-    }
+    body_positions_any<Real>(S, rotT(B.q), rc, atom0, N, sub, tile.lanes);
 }
 
 template <class Real, bool EXACT>
@@ -428,11 +412,17 @@ __global__ __launch_bounds__(EXACT ? 256 : 512) void k_fused(const DevSys* __res
 }
 
 // ---------------------------------------------------------------------------
-// force field: harmonic tether + harmonic bonds, one lane per atom
+// force field: harmonic tether + harmonic bonds, one lane per atom.  Body
+// atoms are read at their fp32 positions, free atoms at hi + lo.
 // ---------------------------------------------------------------------------
+template <class Real> __device__ __forceinline__ V3<double> force_pos(const DevSys& S, int i) {
+    if (sizeof(Real) == 4 && i >= S.numFree) return cvt<double>(ld3<Real>(S.pos, S.atomStride, i));
+    return load_pos<Real>(S, i);
+}
+
 template <class Real>
 __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
-                                                  int numTiles) {
+                                                 int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const DevSys& S = systems[tile.sys];
@@ -442,11 +432,12 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             int a = tile.first + i;
             V3<Real> f = {(Real)0, (Real)0, (Real)0};
             if (S.tetherScale != 0.0) {
-                Real k = reinterpret_cast<const Real*>(S.tetherK)[a];
+                Real k = S.tetherUniform ? (Real)S.tetherK0 : reinterpret_cast<const Real*>(S.tetherK)[a];
                 V3<Real> dx;
                 if (sizeof(Real) == 4) {
                     V3<Real> ph = ld3<Real>(S.pos, S.atomStride, a), rh = ld3<Real>(S.r0, S.atomStride, a);
-                    V3<float> pl = ld3<float>(S.posLo, S.atomStride, a), rl = ld3<float>(S.r0Lo, S.atomStride, a);
+                    V3<float> rl = ld3<float>(S.r0Lo, S.atomStride, a);
+                    V3<float> pl = a < S.numFree ? ld3<float>(S.posLo, S.atomStride, a) : V3<float>{0.f, 0.f, 0.f};
                     dx = {(ph.x - rh.x) + (Real)(pl.x - rl.x), (ph.y - rh.y) + (Real)(pl.y - rl.y),
                           (ph.z - rh.z) + (Real)(pl.z - rl.z)};
                 } else {
@@ -459,10 +450,10 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             if (S.bondStart) {
                 int e0 = S.bondStart[a], e1 = S.bondStart[a + 1];
                 if (e0 < e1) {
-                    V3<double> pa = load_pos<Real>(S, a);
+                    V3<double> pa = force_pos<Real>(S, a);
                     V3<double> fd = cvt<double>(f);
                     for (int b = e0; b < e1; b++) {
-                        V3<double> dr = load_pos<Real>(S, S.bondOther[b]) - pa;
+                        V3<double> dr = force_pos<Real>(S, S.bondOther[b]) - pa;
                         double r = sqrt(dot(dr, dr));
                         double ext = r - S.bondLen[b];
                         fd = fd + dr * (S.bondK[b] * ext / r);

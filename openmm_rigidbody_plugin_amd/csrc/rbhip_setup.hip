@@ -10,6 +10,8 @@
 //   k_body_states  RigidBody fields for download (half-kick removed)
 //   k_rekick       re-applies the folded half-kick after a step-size change
 //   k_stage_in/out FP64 host-order staging <-> device precision state
+//   k_body_atoms_out  FP64 body-atom positions r_cm + A(q)^T d_j rebuilt from
+//                  the body state (the device keeps them in fp32 only)
 #include <hip/hip_runtime.h>
 
 #include "rbhip_access.h"
@@ -67,7 +69,6 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
     const double invMass = 1.0 / mass;
     Q4<double> q;
     V3<double> invI, rcm = {0.0, 0.0, 0.0}, F = {0.0, 0.0, 0.0}, tau = {0.0, 0.0, 0.0};
-    double dof;
     M3 A;
     if (geometry) {
         for (int j = 0; j < N; j++) rcm = rcm + setup_pos<Real>(S, atom0 + j) * S.mass[atom0 + j];
@@ -104,7 +105,6 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
             A.r[0] = v;
             A.r[1] = cross(u, v);
             A.r[2] = u;
-            dof = 5.0;
         } else {
             M3 inertia;
             inertia.r[0] = inertia.r[1] = inertia.r[2] = V3<double>{0.0, 0.0, 0.0};
@@ -113,7 +113,6 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
             V3<double> I = eigenvalues3(inertia);
             invI = {1.0 / I.x, 1.0 / I.y, 1.0 / I.z};
             A = eigenvectors3(inertia, I);
-            dof = 6.0;
         }
         q = quat_from_matrix(A);
         for (int j = 0; j < N; j++) {
@@ -127,13 +126,12 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
         S.rcm[k] = rcm.x;
         S.rcm[k + st] = rcm.y;
         S.rcm[k + 2 * st] = rcm.z;
-        st4<Real>(S.iI, k, Q4<Real>{(Real)invI.x, (Real)invI.y, (Real)invI.z, (Real)dof});
+        st4<Real>(S.iI, k, Q4<Real>{(Real)invI.x, (Real)invI.y, (Real)invI.z, (Real)invMass});
         st4<Real>(S.q, k, cvt<Real>(q));
     } else {
         q = cvt<double>(ld4<Real>(S.q, k));
         Q4<Real> iI = ld4<Real>(S.iI, k);
         invI = {(double)iI.a, (double)iI.b, (double)iI.c};
-        dof = (double)iI.d;
         for (int j = 0; j < N; j++) {
             V3<double> dl = Atmul(q, cvt<double>(ld3<Real>(S.d, S.dStride, dbase + j)));
             V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, atom0 + j));
@@ -141,7 +139,7 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
             tau = tau + cross(dl, f);
         }
     }
-    Q4<double> torque = Cmul(q, tau);
+    const double dof = invI.z == 0.0 ? 5.0 : 6.0;
     // dynamics (RigidBody.cpp:123-142), centre-of-mass momentum rebuilt from zero
     V3<double> pcm = {0.0, 0.0, 0.0};
     for (int j = 0; j < N; j++) pcm = pcm + setup_vel<Real>(S, atom0 + j) * S.mass[atom0 + j];
@@ -154,14 +152,14 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
         else dj = cvt<double>(ld3<Real>(S.d, S.dStride, dbase + j));
         L = L + cross(dj, Amul(q, setup_vel<Real>(S, atom0 + j) - vcm) * S.mass[atom0 + j]);
     }
-    Q4<double> pi = Bmul(q, L) * 2.0;
     double twoKr = dot(L, hadamard(invI, L));
-    // first half-kick of the next step (RigidBodySystem.cpp:177-178)
+    // first half-kick of the next step (RigidBodySystem.cpp:177-178); the
+    // reference's pi += C(q) tau dt is L += A(q) tau dt/2 for pi = 2 B(q) L
     const double halfDt = 0.5 * S.dt;
     pcm = pcm + F * halfDt;
-    pi = pi + torque * S.dt;
-    st4<Real>(S.pm, k, Q4<Real>{(Real)pcm.x, (Real)pcm.y, (Real)pcm.z, (Real)invMass});
-    st4<Real>(S.pi, k, cvt<Real>(pi));
+    L = L + Amul(q, tau) * halfDt;
+    stp3<Real>(S.pcm, k, cvt<Real>(pcm));
+    stp3<Real>(S.L, k, cvt<Real>(L));
     red[0] = twoKt;
     red[1] = twoKr;
     red[2] = dof;
@@ -233,12 +231,12 @@ __global__ void k_stage_out(const DevSys* __restrict__ sys, double* __restrict__
     out[3 * (size_t)i + 2] = x.z;
 }
 
-// body force and C(q) torque from the current atom forces
+// body force and space-frame torque from the current atom forces
 template <class Real>
 __device__ inline void body_force_torque(const DevSys& S, Q4<double> q, int N, int atom0, V3<double>& F,
-                                         Q4<double>& T) {
+                                         V3<double>& tau) {
     const int dbase = atom0 - S.numFree;
-    V3<double> tau = {0.0, 0.0, 0.0};
+    tau = {0.0, 0.0, 0.0};
     F = {0.0, 0.0, 0.0};
     for (int j = 0; j < N; j++) {
         V3<double> dl = Atmul(q, cvt<double>(ld3<Real>(S.d, S.dStride, dbase + j)));
@@ -246,7 +244,6 @@ __device__ inline void body_force_torque(const DevSys& S, Q4<double> q, int N, i
         F = F + f;
         tau = tau + cross(dl, f);
     }
-    T = Cmul(q, tau);
 }
 
 template <class Real>
@@ -257,25 +254,24 @@ __global__ void k_body_states(const DevSys* __restrict__ sys, const int* __restr
     const DevSys& S = *sys;
     int N = bodyN[k], atom0 = bodyAtom0[k];
     Q4<double> q = cvt<double>(ld4<Real>(S.q, k));
-    Q4<double> pi = cvt<double>(ld4<Real>(S.pi, k));
-    Q4<Real> pm = ld4<Real>(S.pm, k);
     Q4<Real> iI = ld4<Real>(S.iI, k);
     V3<double> invI = {(double)iI.a, (double)iI.b, (double)iI.c};
     double mass = 0.0;
     for (int j = 0; j < N; j++) mass += S.mass[atom0 + j];
-    V3<double> F;
-    Q4<double> T;
-    body_force_torque<Real>(S, q, N, atom0, F, T);
+    V3<double> F, tau;
+    body_force_torque<Real>(S, q, N, atom0, F, tau);
+    // take out the folded half-kick of the next step
     double halfDt = 0.5 * S.dt;
-    V3<double> pcm = V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} - F * halfDt;
-    pi = pi + T * (-S.dt);
-    V3<double> L = Btmul(q, pi) * 0.5;
+    V3<double> pcm = cvt<double>(ldp3<Real>(S.pcm, k)) - F * halfDt;
+    V3<double> L = cvt<double>(ldp3<Real>(S.L, k)) - Amul(q, tau) * halfDt;
+    Q4<double> pi = Bmul(q, L) * 2.0;
+    Q4<double> T = Cmul(q, tau);
     V3<double> w = hadamard(invI, L);
-    V3<double> vcm = pcm * (double)pm.d;
+    V3<double> vcm = pcm * (double)iI.d;
     double* o = out + 36 * (size_t)k;
     int p = 0;
     o[p++] = N;
-    o[p++] = iI.d;
+    o[p++] = invI.z == 0.0 ? 5.0 : 6.0;
     o[p++] = mass;
     o[p++] = invI.x != 0.0 ? 1.0 / invI.x : 0.0;
     o[p++] = invI.y != 0.0 ? 1.0 / invI.y : 0.0;
@@ -301,14 +297,33 @@ __global__ void k_rekick(const DevSys* __restrict__ sys, const int* __restrict__
     if (k >= numBodies) return;
     const DevSys& S = *sys;
     Q4<double> q = cvt<double>(ld4<Real>(S.q, k));
-    Q4<Real> pm = ld4<Real>(S.pm, k);
-    V3<double> F;
-    Q4<double> T;
-    body_force_torque<Real>(S, q, bodyN[k], bodyAtom0[k], F, T);
-    V3<double> pcm = V3<double>{(double)pm.a, (double)pm.b, (double)pm.c} + F * (0.5 * ddt);
-    Q4<double> pi = cvt<double>(ld4<Real>(S.pi, k)) + T * ddt;
-    st4<Real>(S.pm, k, Q4<Real>{(Real)pcm.x, (Real)pcm.y, (Real)pcm.z, pm.d});
-    st4<Real>(S.pi, k, cvt<Real>(pi));
+    V3<double> F, tau;
+    body_force_torque<Real>(S, q, bodyN[k], bodyAtom0[k], F, tau);
+    V3<double> pcm = cvt<double>(ldp3<Real>(S.pcm, k)) + F * (0.5 * ddt);
+    V3<double> L = cvt<double>(ldp3<Real>(S.L, k)) + Amul(q, tau) * (0.5 * ddt);
+    stp3<Real>(S.pcm, k, cvt<Real>(pcm));
+    stp3<Real>(S.L, k, cvt<Real>(L));
+}
+
+// body-atom positions r_cm + A(q)^T d_j in FP64 into the internal-order
+// staging buffer (the rotation is the one part1 applies)
+template <class Real>
+__global__ void k_body_atoms_out(const DevSys* __restrict__ sys, const int* __restrict__ bodyN,
+                                 const int* __restrict__ bodyAtom0, int numBodies, double* __restrict__ out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= numBodies) return;
+    const DevSys& S = *sys;
+    const int st = S.bodyStride;
+    V3<double> rc = {S.rcm[k], S.rcm[k + st], S.rcm[k + 2 * st]};
+    Rot<Real> R = rotT(ld4<Real>(S.q, k));
+    int N = bodyN[k], atom0 = bodyAtom0[k];
+    for (int j = 0; j < N; j++) {
+        V3<double> r = rc + cvt<double>(apply(R, ld3<Real>(S.d, S.dStride, atom0 + j - S.numFree)));
+        size_t a = atom0 + j;
+        out[3 * a] = r.x;
+        out[3 * a + 1] = r.y;
+        out[3 * a + 2] = r.z;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -344,6 +359,15 @@ hipError_t launch_body_states(int precision, const DevSys* sys, const int* bodyN
     int grid = (numBodies + 255) / 256;
     if (precision == 0) k_body_states<float><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
     else k_body_states<double><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_body_atoms_out(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
+                                int numBodies, double* out, hipStream_t stream) {
+    if (numBodies <= 0) return hipSuccess;
+    int grid = (numBodies + 255) / 256;
+    if (precision == 0) k_body_atoms_out<float><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
+    else k_body_atoms_out<double><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
     return hipGetLastError();
 }
 
