@@ -187,6 +187,13 @@ template <class Real> __device__ __forceinline__ V3<double> to_body(const Rot<Re
     return cvt<double>(R.r0) * x.x + cvt<double>(R.r1) * x.y + cvt<double>(R.r2) * x.z;
 }
 
+// body-frame torque; a linear body (1/I2 = 0) has no moment about its axis
+// and keeps L2 = 0 exactly (the axial torque of collinear atoms is rounding)
+template <class Real> __device__ __forceinline__ V3<double> axial_torque(V3<double> tb, V3<Real> invI) {
+    if (invI.z == (Real)0) tb.z = 0.0;
+    return tb;
+}
+
 // r_cm += p_cm (dt / M), then the free rotation of (q, L)
 template <class Real, bool EXACT>
 __device__ __forceinline__ void drift_and_rotate(const DevSys& S, Body<Real>& B, V3<double>& rc) {
@@ -303,7 +310,7 @@ __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, 
     body_sums_any<Real>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
     if (!active) return;
     const double halfDt = 0.5 * S.dt;
-    V3<double> tb = to_body(R, tau);
+    V3<double> tb = axial_torque(to_body(R, tau), B.invI);
     V3<double> p = cvt<double>(B.p) + F * halfDt;
     V3<double> L = cvt<double>(B.L) + tb * halfDt;
     V3<double> w = hadamard(cvt<double>(B.invI), L);
@@ -365,14 +372,75 @@ __device__ __forceinline__ void fused_free(const DevSys& S, const Tile& tile) {
     store_pos<Real>(S, a, r);
 }
 
+// part2 of step n and the first half-kick of step n+1 on the body momenta
+template <class Real>
+__device__ __forceinline__ void fused_kicks(const DevSys& S, Body<Real>& B, const Rot<Real>& R, V3<double> F,
+                                            V3<double> tau) {
+    const double halfDt = 0.5 * S.dt;
+    V3<double> tb = axial_torque(to_body(R, tau), B.invI);
+    V3<double> p = cvt<double>(B.p) + F * halfDt;
+    V3<double> L = cvt<double>(B.L) + tb * halfDt;
+    B.p = cvt<Real>(p + F * halfDt);
+    B.L = cvt<Real>(L + tb * halfDt);
+}
+
+template <class Real> __device__ __forceinline__ void store_body(const DevSys& S, int k, const Body<Real>& B,
+                                                                 V3<double> rc) {
+    st4<Real>(S.q, k, B.q);
+    stp3<Real>(S.L, k, B.L);
+    stp3<Real>(S.pcm, k, B.p);
+    store_rcm<Real>(S, k, rc);
+}
+
+// one lane per body of NC atoms: the body-fixed coordinates stay in registers
+// from the torque sum to the position rebuild, so d is read once
+template <class Real, bool EXACT, int NC>
+__device__ __forceinline__ void fused_body_regs(const DevSys& S, int k, int atom0) {
+    const int dbase = atom0 - S.numFree;
+    Body<Real> B = load_body<Real>(S, k);
+    V3<double> rc = load_rcm<Real>(S, k);
+    V3<Real> dj[NC], fj[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
+        fj[j] = ld3<Real>(S.force, S.atomStride, atom0 + j);
+    }
+    Rot<Real> R = rotT(B.q);
+    V3<double> F = {0.0, 0.0, 0.0}, tau = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        V3<double> dl = cvt<double>(apply(R, dj[j]));
+        V3<double> f = cvt<double>(fj[j]);
+        F = F + f;
+        tau = tau + cross(dl, f);
+    }
+    fused_kicks<Real>(S, B, R, F, tau);
+    drift_and_rotate<Real, EXACT>(S, B, rc);
+    store_body<Real>(S, k, B, rc);
+    Rot<Real> R1 = rotT(B.q);
+#pragma unroll
+    for (int j = 0; j < NC; j++)
+        st3<Real>(S.pos, S.atomStride, atom0 + j, cvt<Real>(rc + cvt<double>(apply(R1, dj[j]))));
+}
+
 template <class Real, bool EXACT>
 __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) {
+    const int N = tile.N;
+    // (the FP64 exact rotation leaves no registers to spare: it re-reads d)
+    if (!EXACT && tile.lanes == 1 && (N == 2 || N == 3 || N == 4)) {
+        const int b = threadIdx.x;
+        if (b >= tile.count) return;
+        const int k = tile.first + b, atom0 = tile.atomFirst + b * N;
+        if (N == 2) fused_body_regs<Real, EXACT, 2>(S, k, atom0);
+        else if (N == 3) fused_body_regs<Real, EXACT, 3>(S, k, atom0);
+        else fused_body_regs<Real, EXACT, 4>(S, k, atom0);
+        return;
+    }
     const int shift = lane_shift(tile.lanes);
     const int b = threadIdx.x >> shift;
     const int sub = threadIdx.x & (tile.lanes - 1);
     const bool active = b < tile.count;
     const int k = tile.first + (active ? b : 0);
-    const int N = tile.N;
     const int atom0 = tile.atomFirst + (active ? b : 0) * N;
     Body<Real> B = load_body<Real>(S, k);
     V3<double> rc = load_rcm<Real>(S, k);
@@ -380,21 +448,9 @@ __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) 
     V3<double> F, tau;
     body_sums_any<Real>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
     if (!active) return;
-    // part2 of step n and the first half-kick of step n+1
-    const double halfDt = 0.5 * S.dt;
-    V3<double> tb = to_body(R, tau);
-    V3<double> p = cvt<double>(B.p) + F * halfDt;
-    V3<double> L = cvt<double>(B.L) + tb * halfDt;
-    B.p = cvt<Real>(p + F * halfDt);
-    B.L = cvt<Real>(L + tb * halfDt);
-    // part1 of step n+1
+    fused_kicks<Real>(S, B, R, F, tau);
     drift_and_rotate<Real, EXACT>(S, B, rc);
-    if (sub == 0) {
-        st4<Real>(S.q, k, B.q);
-        stp3<Real>(S.L, k, B.L);
-        stp3<Real>(S.pcm, k, B.p);
-        store_rcm<Real>(S, k, rc);
-    }
+    if (sub == 0) store_body<Real>(S, k, B, rc);
     body_positions_any<Real>(S, rotT(B.q), rc, atom0, N, sub, tile.lanes);
 }
 
@@ -413,7 +469,8 @@ __global__ __launch_bounds__(EXACT ? 256 : 512) void k_fused(const DevSys* __res
 
 // ---------------------------------------------------------------------------
 // force field: harmonic tether + harmonic bonds, one lane per atom.  Body
-// atoms are read at their fp32 positions, free atoms at hi + lo.
+// atoms are read at their fp32 positions (as a mixed-precision force field
+// reads fp32 coordinates), free atoms at hi + lo.
 // ---------------------------------------------------------------------------
 template <class Real> __device__ __forceinline__ V3<double> force_pos(const DevSys& S, int i) {
     if (sizeof(Real) == 4 && i >= S.numFree) return cvt<double>(ld3<Real>(S.pos, S.atomStride, i));

@@ -51,6 +51,13 @@ __device__ inline V3<double> m3_apply(const M3& A, V3<double> x) {
     return {dot(A.r[0], x), dot(A.r[1], x), dot(A.r[2], x)};
 }
 
+// body-frame torque A(q) tau without the axial part of a linear body
+__device__ inline V3<double> body_torque(Q4<double> q, V3<double> tau, double invIz) {
+    V3<double> tb = Amul(q, tau);
+    if (invIz == 0.0) tb.z = 0.0;
+    return tb;
+}
+
 template <class Real> __device__ __forceinline__ V3<double> setup_pos(const DevSys& S, int i) {
     if (S.useStageR) return V3<double>{S.stageR[3 * (size_t)i], S.stageR[3 * (size_t)i + 1], S.stageR[3 * (size_t)i + 2]};
     return load_pos<Real>(S, i);
@@ -152,12 +159,14 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
         else dj = cvt<double>(ld3<Real>(S.d, S.dStride, dbase + j));
         L = L + cross(dj, Amul(q, setup_vel<Real>(S, atom0 + j) - vcm) * S.mass[atom0 + j]);
     }
+    // a linear body has no angular momentum about its axis (body axis 3)
+    if (invI.z == 0.0) L.z = 0.0;
     double twoKr = dot(L, hadamard(invI, L));
     // first half-kick of the next step (RigidBodySystem.cpp:177-178); the
     // reference's pi += C(q) tau dt is L += A(q) tau dt/2 for pi = 2 B(q) L
     const double halfDt = 0.5 * S.dt;
     pcm = pcm + F * halfDt;
-    L = L + Amul(q, tau) * halfDt;
+    L = L + body_torque(q, tau, invI.z) * halfDt;
     stp3<Real>(S.pcm, k, cvt<Real>(pcm));
     stp3<Real>(S.L, k, cvt<Real>(L));
     red[0] = twoKt;
@@ -263,7 +272,7 @@ __global__ void k_body_states(const DevSys* __restrict__ sys, const int* __restr
     // take out the folded half-kick of the next step
     double halfDt = 0.5 * S.dt;
     V3<double> pcm = cvt<double>(ldp3<Real>(S.pcm, k)) - F * halfDt;
-    V3<double> L = cvt<double>(ldp3<Real>(S.L, k)) - Amul(q, tau) * halfDt;
+    V3<double> L = cvt<double>(ldp3<Real>(S.L, k)) - body_torque(q, tau, invI.z) * halfDt;
     Q4<double> pi = Bmul(q, L) * 2.0;
     Q4<double> T = Cmul(q, tau);
     V3<double> w = hadamard(invI, L);
@@ -300,7 +309,7 @@ __global__ void k_rekick(const DevSys* __restrict__ sys, const int* __restrict__
     V3<double> F, tau;
     body_force_torque<Real>(S, q, bodyN[k], bodyAtom0[k], F, tau);
     V3<double> pcm = cvt<double>(ldp3<Real>(S.pcm, k)) + F * (0.5 * ddt);
-    V3<double> L = cvt<double>(ldp3<Real>(S.L, k)) + Amul(q, tau) * (0.5 * ddt);
+    V3<double> L = cvt<double>(ldp3<Real>(S.L, k)) + body_torque(q, tau, (double)ld4<Real>(S.iI, k).c) * (0.5 * ddt);
     stp3<Real>(S.pcm, k, cvt<Real>(pcm));
     stp3<Real>(S.L, k, cvt<Real>(L));
 }
