@@ -4,8 +4,9 @@ The compute path is ``librbhip.so`` (include/rbhip.h), bound by
 :mod:`openmm_rigidbody_plugin_amd.rbhip` (the library itself is loaded on
 first use); the C++ plugin layer that mirrors the reference's
 RigidBodyIntegrator / IntegrateRigidBodyStepKernel API lives in ``host/`` and
-builds into ``librigidbody_hip.so``.
+builds into ``librigidbody_hip.so``, bound by
+:mod:`openmm_rigidbody_plugin_amd.plugin`.
 """
-from . import rbhip, systems  # noqa: F401
+from . import plugin, rbhip, systems  # noqa: F401
 
-__all__ = ["rbhip", "systems"]
+__all__ = ["plugin", "rbhip", "systems"]

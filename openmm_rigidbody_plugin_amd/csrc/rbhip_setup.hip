@@ -23,11 +23,13 @@ namespace rbhip {
 // geometry helpers (reference RigidBody.cpp:26-58)
 // ---------------------------------------------------------------------------
 // unit vector orthogonal to u: Projection(u) e_m / |.|, m = index of the
-// smallest (signed) component
+// smallest |u_k| (the reference takes the smallest signed component, which
+// is parallel to u when u = -e_k and gives 0/0; with |u_m| <= 1/sqrt(3) the
+// projection keeps a length >= sqrt(2/3))
 __device__ inline V3<double> orthonormal(V3<double> u) {
-    int m = u.x < u.y ? 0 : 1;
+    int m = fabs(u.x) < fabs(u.y) ? 0 : 1;
     double um = m == 0 ? u.x : u.y;
-    if (u.z < um) { m = 2; um = u.z; }
+    if (fabs(u.z) < fabs(um)) { m = 2; um = u.z; }
     double xtx = dot(u, u);
     V3<double> v = {(-u.x) * um, (-u.y) * um, (-u.z) * um};
     if (m == 0) v.x += xtx;

@@ -1,0 +1,62 @@
+// HipIntegrateRigidBodyStepKernel (see HipRigidBodyKernels.h).
+#include "HipRigidBodyKernels.h"
+
+namespace RigidBodyPlugin {
+
+using rbhost::Exception;
+using rbhost::rbhip_check;
+
+void HipIntegrateRigidBodyStepKernel::initialize(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator) {
+    if (context.getSystem().getNumConstraints() > 0)
+        throw Exception("The HIP platform does not support constraints between free atoms");
+    stepSize = integrator.getStepSize();
+    data.createSimulation(context.getSystem(), integrator.getBodyIndices(), stepSize, integrator.getRotationMode());
+}
+
+void HipIntegrateRigidBodyStepKernel::uploadBodySystem(RigidBodySystem& bodySystem) {
+    bodySystem.attach(data.require());
+}
+
+void HipIntegrateRigidBodyStepKernel::syncStepSize(const RigidBodyIntegrator& integrator) {
+    double dt = integrator.getStepSize();
+    if (dt != stepSize) {
+        rbhip_check(rbhip_set_step_size(data.require(), dt));
+        stepSize = dt;
+    }
+}
+
+void HipIntegrateRigidBodyStepKernel::execute(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator) {
+    execute(context, integrator, 1);
+}
+
+void HipIntegrateRigidBodyStepKernel::execute(rbhost::ContextImpl&, const RigidBodyIntegrator& integrator, int steps) {
+    syncStepSize(integrator);
+    rbhip_check(rbhip_execute(data.require(), steps));
+}
+
+double HipIntegrateRigidBodyStepKernel::computeKineticEnergy(rbhost::ContextImpl&, const RigidBodyIntegrator&) {
+    double ke[2];
+    rbhip_check(rbhip_kinetic_energies(data.require(), ke));
+    return ke[0] + ke[1];
+}
+
+std::vector<double> HipIntegrateRigidBodyStepKernel::getKineticEnergies(const RigidBodyIntegrator&) {
+    double ke[2];
+    rbhip_check(rbhip_kinetic_energies(data.require(), ke));
+    return {ke[0], ke[1]};
+}
+
+std::vector<double> HipIntegrateRigidBodyStepKernel::getRefinedKineticEnergies(const RigidBodyIntegrator& integrator) {
+    return getKineticEnergies(integrator);
+}
+
+double HipIntegrateRigidBodyStepKernel::getPotentialEnergyRefinement(const RigidBodyIntegrator&) { return 0.0; }
+
+rbhost::KernelImpl* HipRigidBodyKernelFactory::createKernelImpl(std::string name, const rbhost::Platform& platform,
+                                                                rbhost::ContextImpl& context) const {
+    auto& data = rbhost::HipPlatform::data(context);
+    if (name == IntegrateRigidBodyStepKernel::Name()) return new HipIntegrateRigidBodyStepKernel(name, platform, data);
+    throw Exception("Tried to create kernel with illegal kernel name '" + name + "'");
+}
+
+}  // namespace RigidBodyPlugin

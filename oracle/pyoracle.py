@@ -18,7 +18,8 @@ FIX_PCM_RESET = 1
 FIX_LINEAR_FRAME = 2
 FIX_LINEAR_EXACT = 4
 FIX_STALE_SYSTEM = 8
-FIX_ALL = 15
+FIX_ORTHONORMAL = 16
+FIX_ALL = 31
 
 _D = ctypes.c_double
 _I = ctypes.c_int

@@ -467,10 +467,20 @@ bool collinear(const std::vector<Vec3>& delta, const std::vector<double>& d2, Ve
     return all;
 }
 
-/* unit vector orthogonal to u, RigidBody.cpp:51-58 */
-Vec3 orthonormal(const Vec3& u) {
-    int imin = u[0] < u[1] ? 0 : 1;
-    if (u[2] < u[imin]) imin = 2;
+/* unit vector orthogonal to u, RigidBody.cpp:51-58: Projection(u) e_m
+   normalised, m = index of the smallest signed component of u.  For u = -e_k
+   that e_m is parallel to u and the result is 0/0 (NaN); with
+   RBREF_FIX_ORTHONORMAL m is the index of the smallest |u_k|, which keeps
+   |Projection(u) e_m| >= sqrt(2/3). */
+Vec3 orthonormal(const Vec3& u, unsigned flags) {
+    int imin;
+    if (flags & RBREF_FIX_ORTHONORMAL) {
+        imin = std::fabs(u[0]) < std::fabs(u[1]) ? 0 : 1;
+        if (std::fabs(u[2]) < std::fabs(u[imin])) imin = 2;
+    } else {
+        imin = u[0] < u[1] ? 0 : 1;
+        if (u[2] < u[imin]) imin = 2;
+    }
     Vec3 v;
     v[imin] = 1.0;
     v = projection(u) * v;
@@ -756,7 +766,7 @@ struct BodySystem {
             for (int j = 0; j < b.N; j++) moi += M[atomOf(b, j)] * d2[j];
             b.I = Vec3(moi, moi, 0.0);
             b.invI = Vec3(1.0 / moi, 1.0 / moi, 0.0);
-            Vec3 v = orthonormal(u);
+            Vec3 v = orthonormal(u, flags);
             Mat3 rows(v, u.cross(v), u);
             A = (flags & RBREF_FIX_LINEAR_FRAME) ? rows : rows.t();
             b.dof = 5;

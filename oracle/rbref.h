@@ -44,7 +44,11 @@ extern "C" {
 /* velocity-only rebuilds use the live body geometry instead of the
    integrator's stale copy (RigidBodyIntegrator.cpp:70-72 + TODO.md:5) */
 #define RBREF_FIX_STALE_SYSTEM   8u
-#define RBREF_FIX_ALL            15u
+/* linear bodies: the axis-orthogonal frame vector starts from the axis of
+   the smallest |u_k| (reference RigidBody.cpp:51-58 takes the smallest signed
+   component and returns NaN for a body axis along -e_k) */
+#define RBREF_FIX_ORTHONORMAL    16u
+#define RBREF_FIX_ALL            31u
 
 typedef struct rbref_sim rbref_sim;
 

@@ -1,0 +1,258 @@
+// C++ tests of the host plugin layer on the HIP platform, written as the
+// reference's tests/TestRigidBodyIntegrator.h is: System + RigidBodyIntegrator
+// + Context, checked against closed forms.  Exit status 0 = all passed.
+//
+//   testSingleBond       the reference's known-answer test
+//                        (tests/TestRigidBodyIntegrator.h:49-83): two free
+//                        atoms on a harmonic bond are an analytic oscillator
+//   testRigidDimerSpin   a free rigid dimer keeps its bond length, its centre
+//                        of mass moves uniformly, its angular momentum and
+//                        kinetic energy are conserved, it rotates at |L|/I
+//   testApiErrors        the reference's exceptions (negative rotation mode,
+//                        mode change after binding, body-index count, body
+//                        constraints) and this platform's documented limits
+//   testBookkeeping      RigidBodySystem counts, atom order and DOF
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "HipRigidBodyKernels.h"
+#include "rigidbody_hip_plugin.h"
+
+using namespace RigidBodyPlugin;
+using namespace rbhost;
+
+static int failures = 0;
+
+// scaled comparisons: relative when |expected| > 1, absolute otherwise
+static bool close(double expected, double found, double tol) {
+    double scale = std::fabs(expected) > 1.0 ? std::fabs(expected) : 1.0;
+    return std::fabs(expected - found) / scale <= tol;
+}
+#define CHECK_TOL(expected, found, tol)                                                                         \
+    do {                                                                                                        \
+        double e_ = (expected), f_ = (found);                                                                   \
+        if (!close(e_, f_, (tol))) {                                                                            \
+            std::printf("  %s:%d expected %.10g found %.10g (tol %g)\n", __FILE__, __LINE__, e_, f_, (double)(tol)); \
+            throw Exception("check failed");                                                                    \
+        }                                                                                                       \
+    } while (0)
+#define CHECK_VEC(expected, found, tol)                                                                 \
+    do {                                                                                                \
+        Vec3 e_ = (expected), f_ = (found);                                                             \
+        double s_ = std::sqrt(e_.dot(e_));                                                              \
+        s_ = s_ > 1.0 ? s_ : 1.0;                                                                       \
+        Vec3 d_ = e_ - f_;                                                                              \
+        if (!(std::sqrt(d_.dot(d_)) / s_ <= (tol))) {                                                   \
+            std::printf("  %s:%d expected (%g %g %g) found (%g %g %g)\n", __FILE__, __LINE__, e_[0], e_[1], \
+                        e_[2], f_[0], f_[1], f_[2]);                                                    \
+            throw Exception("check failed");                                                            \
+        }                                                                                               \
+    } while (0)
+#define CHECK_THROWS(stmt)                                                                    \
+    do {                                                                                      \
+        bool thrown_ = false;                                                                 \
+        try {                                                                                 \
+            stmt;                                                                             \
+        } catch (const Exception&) {                                                          \
+            thrown_ = true;                                                                   \
+        }                                                                                     \
+        if (!thrown_) {                                                                       \
+            std::printf("  %s:%d expected an exception from: %s\n", __FILE__, __LINE__, #stmt); \
+            throw Exception("check failed");                                                  \
+        }                                                                                     \
+    } while (0)
+
+static std::map<std::string, std::string> props(const std::string& precision) {
+    return {{"Precision", precision}};
+}
+
+void testSingleBond(Platform& platform, const std::string& precision) {
+    System system;
+    system.addParticle(2.0);
+    system.addParticle(2.0);
+    std::vector<int> bodyIndices(2, 0);
+    RigidBodyIntegrator integrator(0.01, bodyIndices);
+    HarmonicBondForce* forceField = new HarmonicBondForce();
+    forceField->addBond(0, 1, 1.5, 1);
+    system.addForce(forceField);
+    Context context(system, integrator, platform, props(precision));
+    std::vector<Vec3> positions(2);
+    positions[0] = Vec3(-1, 0, 0);
+    positions[1] = Vec3(1, 0, 0);
+    context.setPositions(positions);
+
+    const double freq = 1.0;
+    State state = context.getState(State::Energy);
+    const double initialEnergy = state.getKineticEnergy() + state.getPotentialEnergy();
+    for (int i = 0; i < 1000; ++i) {
+        state = context.getState(State::Positions | State::Velocities | State::Energy);
+        double time = state.getTime();
+        double expectedDist = 1.5 + 0.5 * std::cos(freq * time);
+        CHECK_VEC(Vec3(-0.5 * expectedDist, 0, 0), state.getPositions()[0], 0.02);
+        CHECK_VEC(Vec3(0.5 * expectedDist, 0, 0), state.getPositions()[1], 0.02);
+        double expectedSpeed = -0.5 * freq * std::sin(freq * time);
+        CHECK_VEC(Vec3(-0.5 * expectedSpeed, 0, 0), state.getVelocities()[0], 0.02);
+        CHECK_VEC(Vec3(0.5 * expectedSpeed, 0, 0), state.getVelocities()[1], 0.02);
+        double energy = state.getKineticEnergy() + state.getPotentialEnergy();
+        CHECK_TOL(initialEnergy, energy, 0.01);
+        integrator.step(1);
+    }
+    CHECK_TOL(10.0, context.getState(0).getTime(), 1e-5);
+}
+
+// a rigid dimer (masses m1, m2 at distance d) spinning freely about an axis
+// perpendicular to the bond: |L| = I w with I = mu d^2, the bond turns by w t
+void testRigidDimerSpin(Platform& platform, const std::string& precision, int mode) {
+    System system;
+    const double m1 = 3.0, m2 = 5.0, d = 0.4, w = 2.0;
+    system.addParticle(m1);
+    system.addParticle(m2);
+    RigidBodyIntegrator integrator(0.002, std::vector<int>{7, 7});
+    integrator.setRotationMode(mode);
+    Context context(system, integrator, platform, props(precision));
+    const double M = m1 + m2, r1 = d * m2 / M, r2 = d * m1 / M;
+    const Vec3 vcm(0.3, -0.2, 0.1), c0(1.0, 2.0, 3.0);
+    context.setPositions({c0 + Vec3(-r1, 0, 0), c0 + Vec3(r2, 0, 0)});
+    // rotation about z at rate w: v_i = vcm + w e_z x (r_i - c)
+    context.setVelocities({vcm + Vec3(0, -w * r1, 0), vcm + Vec3(0, w * r2, 0)});
+    const double I = m1 * r1 * r1 + m2 * r2 * r2;
+    const double ke0 = 0.5 * M * vcm.dot(vcm) + 0.5 * I * w * w;
+    RigidBody b0 = integrator.getRigidBodySystem().getRigidBody(0);
+    if (b0.dof != 5) throw Exception("a dimer is a linear body (dof 5)");
+    CHECK_TOL(M, b0.mass, 1e-12);
+    CHECK_TOL(I, b0.I[0], 1e-6);
+    for (int s = 0; s < 500; s++) integrator.step(1);
+    State st = context.getState(State::Positions | State::Velocities | State::Energy);
+    const std::vector<Vec3>& R = st.getPositions();
+    const std::vector<Vec3>& V = st.getVelocities();
+    double t = st.getTime();
+    Vec3 c = (R[0] * m1 + R[1] * m2) * (1.0 / M);
+    CHECK_VEC(c0 + vcm * t, c, 1e-5);
+    Vec3 bond = R[1] - R[0];
+    CHECK_TOL(d, std::sqrt(bond.dot(bond)), 1e-6);
+    CHECK_VEC(Vec3(d * std::cos(w * t), d * std::sin(w * t), 0), bond, 2e-4);
+    Vec3 vc = (V[0] * m1 + V[1] * m2) * (1.0 / M);
+    CHECK_VEC(vcm, vc, 1e-5);
+    CHECK_TOL(ke0, st.getKineticEnergy(), 1e-5);
+    std::vector<double> ke = integrator.getKineticEnergies();
+    CHECK_TOL(0.5 * I * w * w, ke[1], 1e-5);
+    std::vector<double> rk = integrator.getRefinedKineticEnergies();
+    CHECK_TOL(ke[0], rk[0], 0.0);
+    CHECK_TOL(ke[1], rk[1], 0.0);
+    CHECK_TOL(0.0, integrator.getPotentialEnergyRefinement(), 0.0);
+}
+
+void testApiErrors(Platform& platform) {
+    RigidBodyIntegrator a(0.001, std::vector<int>{1, 1});
+    CHECK_THROWS(a.setRotationMode(-1));
+    {
+        System system;
+        system.addParticle(1.0);
+        system.addParticle(1.0);
+        RigidBodyIntegrator integrator(0.001, std::vector<int>{1, 1});
+        Context context(system, integrator, platform);
+        CHECK_THROWS(integrator.setRotationMode(3));
+        CHECK_THROWS(integrator.setComputeRefinedEnergies(true));
+        RigidBodyIntegrator other(0.001, std::vector<int>{1, 1});
+        CHECK_THROWS(other.step(1));
+    }
+    {
+        System system;
+        system.addParticle(1.0);
+        system.addParticle(1.0);
+        RigidBodyIntegrator integrator(0.001, std::vector<int>{1, 1, 0});
+        CHECK_THROWS(Context(system, integrator, platform));
+    }
+    {
+        System system;
+        for (int i = 0; i < 3; i++) system.addParticle(1.0);
+        system.addConstraint(0, 2, 1.0);
+        RigidBodyIntegrator integrator(0.001, std::vector<int>{1, 1, 0});
+        CHECK_THROWS(Context(system, integrator, platform));
+    }
+    {
+        // this platform's limits: single-atom bodies, massless particles
+        System system;
+        system.addParticle(1.0);
+        system.addParticle(1.0);
+        RigidBodyIntegrator integrator(0.001, std::vector<int>{1, 2});
+        CHECK_THROWS(Context(system, integrator, platform));
+        System massless;
+        massless.addParticle(1.0);
+        massless.addParticle(0.0);
+        RigidBodyIntegrator integrator2(0.001, std::vector<int>{0, 0});
+        CHECK_THROWS(Context(massless, integrator2, platform));
+    }
+    {
+        System system;
+        system.addParticle(1.0);
+        RigidBodyIntegrator integrator(0.001, std::vector<int>{0});
+        CHECK_THROWS(Context(system, integrator, platform, {{"Precision", "half"}}));
+    }
+    {
+        // the integrator is free again once its context is gone
+        System system;
+        system.addParticle(1.0);
+        system.addParticle(1.0);
+        RigidBodyIntegrator integrator(0.001, std::vector<int>{4, 4});
+        { Context context(system, integrator, platform); }
+        integrator.setRotationMode(2);
+        Context context(system, integrator, platform);
+        if (integrator.getRotationMode() != 2) throw Exception("rotation mode not kept");
+    }
+}
+
+void testBookkeeping(Platform& platform) {
+    // labels with gaps and out of order; free atoms interleaved
+    std::vector<int> labels = {9, 0, 4, 9, -1, 4, 4, 0, 9, 12, 12};
+    System system;
+    for (size_t i = 0; i < labels.size(); i++) system.addParticle(1.0 + i);
+    RigidBodyIntegrator integrator(0.001, labels);
+    integrator.setRotationMode(5);
+    Context context(system, integrator, platform);
+    std::vector<Vec3> pos;
+    for (size_t i = 0; i < labels.size(); i++) pos.push_back(Vec3(0.3 * i, 0.1 * (i % 3), 0.05 * (i * i % 5)));
+    context.setPositions(pos);
+    const RigidBodySystem& bs = integrator.getRigidBodySystem();
+    if (bs.getNumFree() != 3 || bs.getNumBodies() != 3 || bs.getNumActualAtoms() != 11 || bs.getNumBodyAtoms() != 8)
+        throw Exception("counts");
+    // free atoms ascending, then body 1 (label 4), body 2 (label 9), body 3 (label 12)
+    const int expected[] = {1, 4, 7, 2, 5, 6, 0, 3, 8, 9, 10};
+    for (int i = 0; i < 11; i++)
+        if (bs.getAtomIndex(i) != expected[i]) throw Exception("atom index");
+    int dof = 3;
+    for (int b = 0; b < 3; b++) dof += bs.getRigidBody(b).dof;
+    if (bs.getNumDOF() != dof) throw Exception("numDOF");
+    if (bs.getRigidBody(2).dof != 5 || bs.getRigidBody(0).dof != 6) throw Exception("body dof");
+}
+
+int main() {
+    registerRigidBodyHipKernelFactories();
+    Platform& platform = Platform::getPlatformByName("HIP");
+    std::vector<std::pair<std::string, std::function<void()>>> tests = {
+        {"testSingleBond mixed", [&] { testSingleBond(platform, "mixed"); }},
+        {"testSingleBond double", [&] { testSingleBond(platform, "double"); }},
+        {"testRigidDimerSpin mixed NO-SQUISH", [&] { testRigidDimerSpin(platform, "mixed", 4); }},
+        {"testRigidDimerSpin double exact", [&] { testRigidDimerSpin(platform, "double", 0); }},
+        {"testApiErrors", [&] { testApiErrors(platform); }},
+        {"testBookkeeping", [&] { testBookkeeping(platform); }},
+    };
+    for (auto& t : tests) {
+        std::printf("%s ...\n", t.first.c_str());
+        std::fflush(stdout);
+        try {
+            t.second();
+            std::printf("  ok\n");
+        } catch (const std::exception& e) {
+            std::printf("  FAILED: %s\n", e.what());
+            failures++;
+        }
+        std::fflush(stdout);
+    }
+    std::printf(failures ? "FAIL (%d)\n" : "Done\n", failures);
+    return failures ? 1 : 0;
+}

@@ -156,6 +156,42 @@ def test_linear_exact_deviation(oracle):
     assert np.all(np.isfinite(fixed.positions()))
 
 
+def test_orthonormal_deviation(oracle):
+    """Reference orthonormal (RigidBody.cpp:51-58) projects e_m for m = the
+    smallest signed component of the axis u: for a diatomic whose axis points
+    along -x that e_m is parallel to u and the frame is 0/0.  The fixed rule
+    (smallest |u_k|) gives a proper frame with the same dynamics."""
+    masses = [3.0, 5.0]
+    x = np.array([[0.75, 2.0, 3.0], [1.15, 2.0, 3.0]])   # axis from the centre of mass to atom 0 is -x
+    v = np.array([[0.3, -0.5, 0.1], [0.3, 0.3, 0.1]])
+    for mode in (0, 4):
+        faithful = oracle.OracleSim(masses, [1, 1], 0.002, mode, flags=oracle.FIX_ALL & ~oracle.FIX_ORTHONORMAL)
+        faithful.set_positions(x)
+        assert not np.all(np.isfinite(faithful.body_state(0)["q"]))
+        fixed = oracle.OracleSim(masses, [1, 1], 0.002, mode)
+        fixed.set_positions(x)
+        fixed.set_velocities(v)
+        q = fixed.body_state(0)["q"]
+        assert np.all(np.isfinite(q)) and np.linalg.norm(q) == pytest.approx(1.0, abs=1e-15)
+        d = fixed.body_fixed_positions()
+        assert np.allclose(d[:, :2], 0.0, atol=1e-15)        # atoms on body axis 3
+        fixed.step(100)
+        r = fixed.positions()
+        assert np.linalg.norm(r[1] - r[0]) == pytest.approx(0.4, abs=1e-12)
+    # an axis whose smallest signed component is not parallel to it: both
+    # rules give a valid frame (rotated about the axis) and, with the exact
+    # rotation (invariant under that choice), the same motion
+    x2 = np.array([[0.8, 1.9, 3.05], [1.1, 2.07, 2.95]])
+    runs = []
+    for flags in (oracle.FIX_ALL, oracle.FIX_ALL & ~oracle.FIX_ORTHONORMAL):
+        s = oracle.OracleSim(masses, [1, 1], 0.002, 0, flags=flags)
+        s.set_positions(x2)
+        s.set_velocities(v)
+        s.step(50)
+        runs.append(s.positions())
+    assert np.abs(runs[0] - runs[1]).max() < 1e-12
+
+
 def test_stale_system_deviation(oracle):
     """Reference: setVelocities after stepping rebuilds from the integrator's
     stale copy of the body system, so the bodies jump back to the geometry of
