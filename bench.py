@@ -122,6 +122,61 @@ def traffic_from_profiles(config, kernel):
         return json.load(f).get("bytes_per_launch")
 
 
+def batched(steps, warmup, count=256, log_every=64):
+    """256 independent replicas (units ~ LogUniform(1e3, 2e5), 80% water bodies
+    / 20% argon, k ~ U[500, 2000]) stepped `steps` times: first one system at a
+    time through its own step(K) call (each context on its own stream), then
+    all of them as one batch (one launch per phase over every replica's tiles,
+    HIP-graph replay, launch shape from the batch autotuner)."""
+    from openmm_rigidbody_plugin_amd import rbhip, systems
+    t = time.perf_counter()
+    specs = systems.replica_set(count)
+    ctxs = []
+    for i, s in enumerate(specs):
+        ctxs.append(rbhip.Context.from_spec(s, precision="mixed"))
+        if (i + 1) % log_every == 0:
+            log(f"  replicas built: {i + 1}/{count}")
+    rbhip.synchronize()
+    nb = sum(s.num_bodies for s in specs)
+    nf = sum(s.num_free for s in specs)
+    log(f"batched: {count} replicas, {nb} bodies + {nf} free atoms, built in {time.perf_counter() - t:.1f}s")
+    for c in ctxs:
+        c.step(warmup)
+    rbhip.synchronize()
+    t = time.perf_counter()
+    for c in ctxs:
+        c.step(steps)
+    rbhip.synchronize()
+    plain_s = time.perf_counter() - t
+    log(f"  plain (one system per step call): {plain_s:.3f}s")
+    batch = rbhip.Batch()
+    batch.set_members(ctxs)
+    t = time.perf_counter()
+    batch.autotune(reps=2)
+    log(f"  batch autotune {time.perf_counter() - t:.1f}s: {batch.describe()}")
+    batch.step(warmup)
+    rbhip.synchronize()
+    t = time.perf_counter()
+    batch.step(steps)
+    rbhip.synchronize()
+    batch_s = time.perf_counter() - t
+    ke = batch.kinetic_energies()
+    assert np.all(np.isfinite(ke)), "non-finite kinetic energy in the batch"
+    log(f"  batched: {batch_s:.3f}s")
+    units = nb + nf
+    return {
+        "workload": f"{count} replicas, units LogUniform(1e3, 2e5), 80% 3-site bodies / 20% free atoms, "
+                    "k ~ U[500, 2000], NO-SQUISH n=10, dt 2 fs, mixed",
+        "replicas": count, "bodies": nb, "free_atoms": nf, "steps": steps,
+        "value": nb * steps / batch_s, "unit": "body-steps/s",
+        "units_per_s": units * steps / batch_s,
+        "plain_value": nb * steps / plain_s,
+        "speedup_vs_plain": plain_s / batch_s,
+        "ms_per_step": batch_s / steps * 1e3, "plain_ms_per_step": plain_s / steps * 1e3,
+        "launch": batch.describe(),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +186,8 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica batched measurement")
+    ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only stream MIB MiB through the plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -220,6 +277,10 @@ def main():
             "kernel_achieved_GBs": achieved,
         },
     }
+    if not args.no_batch and args.config == "water1M":
+        del ctx
+        log("batched replicas ...")
+        result["batched"] = batched(args.batch_steps, args.warmup)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(spec)

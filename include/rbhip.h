@@ -202,7 +202,15 @@ int rbhip_batch_use_fusion(rbhip_batch* batch, int enable);
 int rbhip_batch_execute(rbhip_batch* batch, int steps);
 /* per-member {trans, rot} kinetic energies, 2*count doubles */
 int rbhip_batch_kinetic_energies(rbhip_batch* batch, double* out);
+/* as rbhip_time_kernels, over all members (advances them 2*steps+1 steps) */
 int rbhip_batch_time_kernels(rbhip_batch* batch, int steps, double* avgMs, double* totalMs);
+/* times candidate launch shapes on the members' real state and keeps the
+   fastest: workgroup size, grid cap (0 = one workgroup per tile; a cap makes
+   the workgroups loop over the tile list, a persistent grid) and lanes per
+   body for each body size present; the members' state is restored */
+int rbhip_batch_autotune(rbhip_batch* batch, int reps);
+/* describes the batch's launch shape as text */
+int rbhip_batch_describe(rbhip_batch* batch, char* buf, int len);
 
 #ifdef __cplusplus
 }

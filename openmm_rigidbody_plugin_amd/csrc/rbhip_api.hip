@@ -219,8 +219,47 @@ struct rbhip_context {
 
     void setDevice() const { HIPCHK(hipSetDevice(device)); }
 
+    int sysVersion = 0;                          // bumped on every change of h
     void uploadSys() {
         HIPCHK(hipMemcpyAsync(dSys, &h, sizeof(DevSys), hipMemcpyHostToDevice, stream));
+        sysVersion++;
+    }
+
+    // device copy of everything a step writes, so tuning can run real steps
+    // and put the state back
+    struct Snapshot {
+        DeviceArena arena;
+        std::vector<void*> live, copy;
+        std::vector<size_t> bytes;
+        int keSlots = 0;
+        bool positionsDerived = false;
+        ~Snapshot() { arena.release(); }
+    };
+    void takeSnapshot(Snapshot& s) {
+        const size_t R = realSize();
+        const size_t nb = std::max(numBodies, 1), as = (size_t)h.atomStride;
+        std::vector<std::pair<void*, size_t>> spans = {
+            {h.q, 4 * R * nb}, {h.L, 3 * R * nb}, {h.pcm, 3 * R * nb},
+            {h.rcm, 3 * sizeof(double) * (size_t)h.bodyStride}, {h.pos, 3 * R * as}, {h.vel, 3 * R * as},
+            {h.force, 3 * R * as}, {h.kePart, 4 * sizeof(double) * (size_t)maxKeSlots},
+            {h.pePart, 4 * sizeof(double) * (size_t)maxPeSlots}};
+        if (h.posLo) spans.push_back({h.posLo, 3 * sizeof(float) * as});
+        for (auto& sp : spans) {
+            void* c = s.arena.alloc(sp.second);
+            HIPCHK(hipMemcpyAsync(c, sp.first, sp.second, hipMemcpyDeviceToDevice, stream));
+            s.live.push_back(sp.first);
+            s.copy.push_back(c);
+            s.bytes.push_back(sp.second);
+        }
+        s.keSlots = keSlots;
+        s.positionsDerived = positionsDerived;
+    }
+    // enqueues the copies back on `on` (the caller orders and synchronises)
+    void restoreSnapshot(const Snapshot& s, hipStream_t on) {
+        for (size_t i = 0; i < s.live.size(); i++)
+            HIPCHK(hipMemcpyAsync(s.live[i], s.copy[i], s.bytes[i], hipMemcpyDeviceToDevice, on));
+        keSlots = s.keSlots;
+        positionsDerived = s.positionsDerived;
     }
 
     std::vector<Tile> partTiles(int blk, const std::map<int, int>& lanesByN) const {
@@ -452,6 +491,7 @@ struct rbhip_batch {
     Tile* dForceTiles = nullptr;
     int nPart = 0, nForce = 0;
     std::vector<int> memberPartTiles, memberForceTiles;
+    std::vector<int> memberVersions;             // sysVersion of each member's DevSys copy
     int* dKeSlots = nullptr;
     double* dRed = nullptr;
     int precision = 0, rotationMode = 0;
@@ -469,9 +509,11 @@ struct rbhip_batch {
         if (members.empty()) return;
         std::vector<DevSys> sys;
         std::vector<Tile> pt, ft;
+        memberVersions.assign(members.size(), 0);
         for (size_t m = 0; m < members.size(); m++) {
             rbhip_context* c = members[m];
             sys.push_back(c->h);
+            memberVersions[m] = c->sysVersion;
             std::vector<Tile> p = c->partTiles(block, c->lanes);
             std::vector<Tile> f = c->forceTiles(block);
             c->validateTiles(p, block, c->maxKeSlots);
@@ -491,6 +533,19 @@ struct rbhip_batch {
         if (!ft.empty()) HIPCHK(hipMemcpy(dForceTiles, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
         nPart = (int)pt.size();
         nForce = (int)ft.size();
+    }
+
+    // re-sends the descriptor of every member whose DevSys changed since it
+    // was copied (step size, force field); the table's address is unchanged,
+    // so captured graphs stay valid
+    void refreshSystems() {
+        for (size_t m = 0; m < members.size(); m++) {
+            rbhip_context* c = members[m];
+            if (c->sysVersion == memberVersions[m]) continue;
+            HIPCHK(hipMemcpyAsync(dSystems + m, &c->h, sizeof(DevSys), hipMemcpyHostToDevice, stream));
+            memberVersions[m] = c->sysVersion;
+        }
+        HIPCHK(hipStreamSynchronize(stream));
     }
 
     void launchPart1() {
@@ -1175,28 +1230,10 @@ int rbhip_autotune(rbhip_context* ctx, int reps) {
         C.setDevice();
         C.fixKick();
         HIPCHK(hipStreamSynchronize(C.stream));
-        // snapshot of everything the step writes
-        const size_t R = C.realSize();
-        const size_t nb = std::max(C.numBodies, 1), na = C.numAtoms;
-        struct Span { void* p; size_t n; };
-        std::vector<Span> spans = {{C.h.q, 4 * R * nb}, {C.h.L, 3 * R * nb}, {C.h.pcm, 3 * R * nb},
-                                   {C.h.rcm, 3 * sizeof(double) * C.h.bodyStride},
-                                   {C.h.pos, 3 * R * (size_t)C.h.atomStride}, {C.h.vel, 3 * R * (size_t)C.h.atomStride},
-                                   {C.h.force, 3 * R * (size_t)C.h.atomStride}};
-        if (C.h.posLo) spans.push_back({C.h.posLo, 3 * sizeof(float) * (size_t)C.h.atomStride});
-        spans.push_back({C.h.kePart, 4 * sizeof(double) * (size_t)C.maxKeSlots});
-        spans.push_back({C.h.pePart, 4 * sizeof(double) * (size_t)C.maxPeSlots});
-        const int savedKeSlots = C.keSlots;
-        DeviceArena snap;
-        std::vector<void*> copies;
-        for (auto& s : spans) {
-            void* p = snap.alloc(s.n);
-            HIPCHK(hipMemcpyAsync(p, s.p, s.n, hipMemcpyDeviceToDevice, C.stream));
-            copies.push_back(p);
-        }
+        rbhip_context::Snapshot snap;
+        C.takeSnapshot(snap);
         auto restore = [&] {
-            for (size_t i = 0; i < spans.size(); i++)
-                HIPCHK(hipMemcpyAsync(spans[i].p, copies[i], spans[i].n, hipMemcpyDeviceToDevice, C.stream));
+            C.restoreSnapshot(snap, C.stream);
             HIPCHK(hipStreamSynchronize(C.stream));
         };
         // the time of one step as rbhip_execute runs it: fused pass + forces
@@ -1251,11 +1288,8 @@ int rbhip_autotune(rbhip_context* ctx, int reps) {
             C.rebuildTiles();
         } catch (...) {
             restore();
-            snap.release();
             throw;
         }
-        snap.release();
-        C.keSlots = savedKeSlots;
     });
 }
 
@@ -1387,6 +1421,7 @@ int rbhip_batch_execute(rbhip_batch* batch, int steps) {
         for (rbhip_context* c : batch->members)
             require(c->kickDt == c->dt, "member step size changed since it joined the batch", RBHIP_E_STATE);
         batch->joinMembers();
+        batch->refreshSystems();
         batch->executeSteps(steps);
         batch->releaseMembers();
         batch->finishSteps(steps);
@@ -1420,7 +1455,10 @@ int rbhip_batch_time_kernels(rbhip_batch* batch, int steps, double* avgMs, doubl
         require(!batch->members.empty(), "empty batch", RBHIP_E_STATE);
         HIPCHK(hipSetDevice(batch->device));
         rbhip_batch& B = *batch;
+        for (rbhip_context* c : B.members)
+            require(c->kickDt == c->dt, "member step size changed since it joined the batch", RBHIP_E_STATE);
         B.joinMembers();
+        B.refreshSystems();
         time_steps(B.stream, steps, [&] { B.launchPart1(); }, [&] { B.launchForces(); }, [&] { B.launchPart2(); },
                    avgMs, totalMs);
         B.launchPart1();
@@ -1431,6 +1469,124 @@ int rbhip_batch_time_kernels(rbhip_batch* batch, int steps, double* avgMs, doubl
         avgMs[RBHIP_KERNEL_FUSED] = fusedMs[0];
         B.releaseMembers();
         B.finishSteps(2 * steps + 1);
+    });
+}
+
+int rbhip_batch_autotune(rbhip_batch* batch, int reps) {
+    return guard([&] {
+        require(batch && reps > 0, "bad arguments");
+        rbhip_batch& B = *batch;
+        require(!B.members.empty(), "empty batch", RBHIP_E_STATE);
+        HIPCHK(hipSetDevice(B.device));
+        for (rbhip_context* c : B.members) {
+            require(c->kickDt == c->dt, "member step size changed since it joined the batch", RBHIP_E_STATE);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        std::vector<std::unique_ptr<rbhip_context::Snapshot>> snaps;
+        for (rbhip_context* c : B.members) {
+            snaps.emplace_back(new rbhip_context::Snapshot());
+            c->takeSnapshot(*snaps.back());
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        B.joinMembers();
+        B.refreshSystems();
+        auto restore = [&] {
+            for (size_t m = 0; m < B.members.size(); m++) B.members[m]->restoreSnapshot(*snaps[m], B.stream);
+            HIPCHK(hipStreamSynchronize(B.stream));
+        };
+        // time of one step as rbhip_batch_execute runs it
+        auto measure = [&]() -> double {
+            double avg[3], total, t;
+            if (B.useFusion) {
+                B.launchPart1(); B.launchForces();
+                B.launchFused(); B.launchForces();
+                time_phases(B.stream, reps, {[&] { B.launchFused(); }, [&] { B.launchForces(); }}, avg, &total);
+                B.launchPart2();
+                t = avg[0] + avg[1];
+            } else {
+                B.launchPart1(); B.launchForces(); B.launchPart2();
+                time_steps(B.stream, reps, [&] { B.launchPart1(); }, [&] { B.launchForces(); },
+                           [&] { B.launchPart2(); }, avg, &total);
+                t = avg[0] + avg[1] + avg[2];
+            }
+            restore();
+            return t;
+        };
+        const int cap = B.rotationMode == 0 ? 256 : 512;
+        auto lanesFit = [&](int blk) {
+            for (rbhip_context* c : B.members)
+                for (auto& kv : c->lanes)
+                    if (blk % kv.second || c->block % kv.second) return false;
+            return true;
+        };
+        try {
+            // workgroup size and grid cap (0 = one workgroup per tile; a cap
+            // makes each workgroup loop over tiles, a persistent grid)
+            int bestBlock = B.block, bestGrid = B.maxBlocks;
+            double best = 1e300;
+            for (int blk : {128, 256, 512}) {
+                if (blk > cap || !lanesFit(blk)) continue;
+                for (int grid : {0, 2048, 4096, 8192, 16384, 32768}) {
+                    B.block = blk;
+                    B.maxBlocks = grid;
+                    B.rebuild();
+                    double t = measure();
+                    if (t < best) { best = t; bestBlock = blk; bestGrid = grid; }
+                }
+            }
+            B.block = bestBlock;
+            B.maxBlocks = bestGrid;
+            // lanes per body for each body size present in the batch
+            std::map<int, int> sizes;
+            for (rbhip_context* c : B.members)
+                for (auto& cl : c->classes) sizes[cl.N] = c->lanes.count(cl.N) ? c->lanes[cl.N] : 1;
+            for (auto& kv : sizes) {
+                const int N = kv.first;
+                int bestL = kv.second;
+                double bestT = 1e300;
+                for (int L = 1; L <= std::min(16, pow2_at_least(N)); L <<= 1) {
+                    bool ok = B.block % L == 0;
+                    for (rbhip_context* c : B.members) ok = ok && c->block % L == 0;
+                    if (!ok) continue;
+                    for (rbhip_context* c : B.members)
+                        if (c->lanes.count(N) || std::any_of(c->classes.begin(), c->classes.end(),
+                                                             [&](const rbhip_context::Class& cl) { return cl.N == N; }))
+                            c->lanes[N] = L;
+                    B.rebuild();
+                    double t = measure();
+                    if (t < bestT) { bestT = t; bestL = L; }
+                }
+                for (rbhip_context* c : B.members)
+                    if (c->lanes.count(N)) c->lanes[N] = bestL;
+            }
+            for (rbhip_context* c : B.members) c->rebuildTiles();
+            B.rebuild();
+        } catch (...) {
+            restore();
+            B.releaseMembers();
+            throw;
+        }
+        B.releaseMembers();
+    });
+}
+
+int rbhip_batch_describe(rbhip_batch* batch, char* buf, int len) {
+    return guard([&] {
+        require(batch && buf && len > 0, "bad arguments");
+        std::map<int, int> lanes;
+        long long bodies = 0, free = 0;
+        for (rbhip_context* c : batch->members) {
+            for (auto& kv : c->lanes) lanes[kv.first] = kv.second;
+            bodies += c->numBodies;
+            free += c->numFree;
+        }
+        std::string s = "members=" + std::to_string(batch->members.size()) + " bodies=" + std::to_string(bodies) +
+                        " free=" + std::to_string(free) + " block=" + std::to_string(batch->block) +
+                        " maxBlocks=" + std::to_string(batch->maxBlocks) + " partTiles=" + std::to_string(batch->nPart) +
+                        " forceTiles=" + std::to_string(batch->nForce) + " lanes=[";
+        for (auto& kv : lanes) s += "N" + std::to_string(kv.first) + ":L" + std::to_string(kv.second) + " ";
+        s += "]";
+        std::snprintf(buf, (size_t)len, "%s", s.c_str());
     });
 }
 

@@ -89,6 +89,8 @@ def lib() -> ctypes.CDLL:
             "rbhip_batch_execute": (_I, [_P, _I]),
             "rbhip_batch_kinetic_energies": (_I, [_P, _DP]),
             "rbhip_batch_time_kernels": (_I, [_P, _I, _DP, _DP]),
+            "rbhip_batch_autotune": (_I, [_P, _I]),
+            "rbhip_batch_describe": (_I, [_P, ctypes.c_char_p, _I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -355,6 +357,14 @@ class Batch:
         if self.members:
             _check(lib().rbhip_batch_kinetic_energies(self._h, _dp(out)))
         return out[:len(self.members)]
+
+    def autotune(self, reps=3):
+        _check(lib().rbhip_batch_autotune(self._h, int(reps)))
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(512)
+        _check(lib().rbhip_batch_describe(self._h, buf, 512))
+        return buf.value.decode()
 
     def time_kernels(self, steps):
         avg = np.empty(len(KERNELS))
