@@ -212,6 +212,50 @@ int rbhip_batch_autotune(rbhip_batch* batch, int reps);
 /* describes the batch's launch shape as text */
 int rbhip_batch_describe(rbhip_batch* batch, char* buf, int len);
 
+/* ---------------------------------------------------------------------- */
+/* Job server: admission by HBM budget and continuous batching             */
+/* (no reference counterpart: the reference runs one system per context)   */
+/* ---------------------------------------------------------------------- */
+typedef struct rbhip_server rbhip_server;
+
+/* device bytes rbhip_create would hold for this system (an upper bound of
+   what rbhip_device_bytes reports before bonds are added) */
+int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, size_t* bytes);
+
+/* A server on `device`: jobs wait in arrival order and are admitted while the
+   device bytes of the resident jobs stay within hbmBudget; resident jobs are
+   stepped together as one batch, each round advancing all of them by the
+   same count (at most stepsPerRound, never past the first job's end); a job
+   that reaches its step count leaves with its final state copied to host
+   memory and its device memory freed.  All jobs share the precision and the
+   rotation kind (exact or NO-SQUISH) and run with workgroup size blockSize,
+   so each job's result equals the job stepped alone with that block size.
+   Errors of the rbhip_server_* calls are described by
+   rbhip_server_last_error(). */
+const char* rbhip_server_last_error(void);
+int rbhip_server_create(int device, size_t hbmBudget, int stepsPerRound, int precision, int blockSize,
+                        rbhip_server** out);
+int rbhip_server_destroy(rbhip_server* server);
+/* queues a job (the inputs are copied; velocities and the tether may be NULL);
+   rejects a job larger than the whole budget */
+int rbhip_server_submit(rbhip_server* server, int numAtoms, const double* masses, const int* bodyIndices,
+                        const double* positions, const double* velocities, const double* tetherK,
+                        const double* tetherR0, double stepSize, int rotationMode, long long steps,
+                        long long* jobId);
+/* one round: admission, one batched advance, retirement; *finished = jobs completed */
+int rbhip_server_run_round(rbhip_server* server, int* finished);
+/* counts3 = {waiting, resident, completed and not yet taken}; device bytes of
+   the resident jobs; body-steps done so far.  Any output may be NULL. */
+int rbhip_server_status(rbhip_server* server, long long* counts3, size_t* bytesInUse, long long* bodySteps);
+/* ids of completed jobs not yet taken, up to max */
+int rbhip_server_completed(rbhip_server* server, long long* ids, int max, int* count);
+/* final positions and velocities (numAtoms x 3, original order), {KE_t, KE_r}
+   and times3 = {seconds waiting for admission, seconds resident, completion
+   latency from submission} of a completed job, which is then released; any
+   output may be NULL */
+int rbhip_server_take(rbhip_server* server, long long jobId, double* positions, double* velocities, double* ke2,
+                      double* times3);
+
 #ifdef __cplusplus
 }
 #endif

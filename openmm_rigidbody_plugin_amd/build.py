@@ -44,6 +44,8 @@ UNITS = [
     # FP64 body-state build: unfused operations, in the reference's order
     ("rbhip_setup.hip", ["-ffp-contract=off"]),
     ("rbhip_api.hip", []),
+    # job server: host code over the C ABI
+    ("rbhip_server.cpp", []),
 ]
 
 
@@ -62,7 +64,7 @@ def build_hip(verbose: bool = False) -> str:
 
     def compile_unit(unit):
         src, extra = unit
-        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         out = _run([cc, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         if verbose and out.strip():
             print(out)

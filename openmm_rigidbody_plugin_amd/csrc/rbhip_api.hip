@@ -162,6 +162,37 @@ void time_steps(hipStream_t stream, int steps, P1 p1, F f, P2 p2, double* avgMs,
 }
 }  // namespace
 
+// device bytes rbhip_create allocates for a system (the arena; tiles and
+// bonds come on top), kept in step with rbhip_create by a check there
+static size_t context_arena_bytes(size_t na, int numBodies, int numFree, int numBodyAtoms, int numClasses, size_t R) {
+    auto a = [](size_t n) { return n == 0 ? (size_t)256 : n; };
+    const size_t nb = std::max(numBodies, 1);
+    const size_t bodyStride = (nb + 63) / 64 * 64, atomStride = (na + 63) / 64 * 64;
+    const size_t dStride = (std::max<size_t>(numBodyAtoms, 1) + 63) / 64 * 64;
+    const bool mixed = R == 4;
+    const size_t maxKe = ((size_t)numFree + 63) / 64 + 1 + numBodies + numClasses + 1;
+    const size_t maxPe = (na + 63) / 64 + 1;
+    size_t b = 0;
+    b += a(4 * R * nb) + a(3 * R * nb) + a(3 * R * nb) + a(4 * R * nb);           // q, L, pcm, iI
+    b += a(3 * sizeof(double) * bodyStride) + a(3 * R * dStride);                  // rcm, d
+    b += a(3 * R * atomStride) + (mixed ? a(3 * sizeof(float) * atomStride) : 0);  // pos, posLo
+    b += a(3 * R * atomStride) + a(3 * R * atomStride);                            // vel, force
+    b += a(R * std::max(numFree, 1)) + a(sizeof(double) * na);                    // freeInvM, mass
+    b += a(3 * sizeof(double) * na) * 2;                                           // stageR, stageV
+    b += a(R * na) + a(3 * R * atomStride) + (mixed ? a(3 * sizeof(float) * atomStride) : 0);  // tether
+    b += a(4 * sizeof(double) * maxKe) + a(4 * sizeof(double) * maxPe);            // partials
+    b += a(sizeof(DevSys)) + a(2 * sizeof(int)) + a(sizeof(int) * nb) * 2;
+    b += a(sizeof(double) * std::max(3 * na, 36 * nb)) + a(4 * sizeof(double));  // staging, reduction
+    return b;
+}
+
+// upper bound of the tile tables (64-wide workgroups, one lane per body)
+static size_t tile_bytes_bound(size_t na, int numBodies, int numFree, int numClasses) {
+    size_t part = ((size_t)numFree + 63) / 64 + ((size_t)numBodies + 63) / 64 + numClasses + 1;
+    size_t force = (na + 63) / 64 + 1;
+    return (2 * part + force) * sizeof(Tile);
+}
+
 // ---------------------------------------------------------------------------
 struct rbhip_context {
     int device = 0;
@@ -782,6 +813,9 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         C.dTmp = C.arena.allocT<double>(C.tmpCount);
         C.dRed = C.arena.allocT<double>(4);
         HIPCHK(hipHostMalloc((void**)&C.hRed, 4 * sizeof(double), hipHostMallocDefault));
+        require(C.arena.bytes == context_arena_bytes(na, C.numBodies, C.numFree, C.numBodyAtoms,
+                                                     (int)C.classes.size(), R),
+                "internal: device size estimate out of step with the allocation", RBHIP_E_STATE);
 
         // zero state, constants
         HIPCHK(hipMemset(h.pos, 0, 3 * R * as));
@@ -842,6 +876,27 @@ int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes) {
     return guard([&] {
         require(ctx && bytes, "null argument");
         *bytes = ctx->arena.bytes + ctx->tileArena.bytes;
+    });
+}
+
+int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, size_t* bytes) {
+    return guard([&] {
+        require(numAtoms > 0 && bodyIndices && bytes, "bad arguments");
+        require(precision == RBHIP_PRECISION_MIXED || precision == RBHIP_PRECISION_DOUBLE, "unknown precision");
+        std::vector<int> bi = clean_body_indices(bodyIndices, numAtoms);
+        int numBodies = 0, numFree = 0;
+        for (int v : bi) numBodies = std::max(numBodies, v);
+        std::vector<int> n(numBodies, 0);
+        for (int v : bi) {
+            if (v == 0) numFree++;
+            else n[v - 1]++;
+        }
+        std::vector<int> sizes(n);
+        std::sort(sizes.begin(), sizes.end());
+        int classes = (int)(std::unique(sizes.begin(), sizes.end()) - sizes.begin());
+        size_t R = precision == RBHIP_PRECISION_MIXED ? 4 : 8;
+        *bytes = context_arena_bytes(numAtoms, numBodies, numFree, numAtoms - numFree, classes, R) +
+                 tile_bytes_bound(numAtoms, numBodies, numFree, classes);
     });
 }
 

@@ -1,0 +1,318 @@
+// Job server over the rbhip C ABI (include/rbhip.h, rbhip_server_*).
+//
+// Jobs are independent rigid-body systems with a step count.  They wait in
+// arrival order and are admitted while their device state fits the HBM
+// budget; admitted jobs stay resident and are stepped together as one batch
+// (one launch per phase over all their tiles).  Each round advances every
+// resident job by the same number of steps, at most stepsPerRound and never
+// past the end of the job that finishes first, so a job leaves at exactly
+// its step count; finished jobs hand their final state to host memory and
+// free their device state, and the next waiting jobs are admitted in the
+// following round (continuous batching).
+//
+// A resident job is stepped with the same kernels and tiles as when it is
+// stepped alone with the same workgroup size, so its result does not depend
+// on which other jobs share its rounds.
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rbhip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Job {
+    long long id = 0;
+    int numAtoms = 0;
+    std::vector<double> masses, positions, velocities, tetherK, tetherR0;
+    std::vector<int> bodyIndices;
+    double stepSize = 0.0;
+    int rotationMode = 0;
+    long long steps = 0, done = 0;
+    long long bodies = 0;
+    size_t bytes = 0;                 // admission estimate
+    rbhip_context* ctx = nullptr;
+    Clock::time_point submitted, admitted, finished;
+    std::vector<double> outPos, outVel;
+    double ke[2] = {0.0, 0.0};
+};
+
+struct ServerError : std::runtime_error {
+    int code;
+    ServerError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+thread_local std::string g_err;
+
+void need(bool ok, const std::string& msg, int code = RBHIP_E_ARG) {
+    if (!ok) throw ServerError(code, msg);
+}
+
+void call(int status) {
+    if (status != RBHIP_OK) throw ServerError(status, rbhip_last_error());
+}
+
+template <class F> int guard(F&& f) {
+    try {
+        f();
+        return RBHIP_OK;
+    } catch (const ServerError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return RBHIP_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return RBHIP_E_ARG;
+    }
+}
+
+double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+}  // namespace
+
+struct rbhip_server {
+    int device = 0;
+    size_t budget = 0;
+    int stepsPerRound = 0;
+    int precision = RBHIP_PRECISION_MIXED;
+    int block = 256;
+    int exactKind = -1;                 // rotation kind of the jobs: 1 exact, 0 NO-SQUISH
+    long long nextId = 1;
+    size_t inUse = 0;
+    long long bodySteps = 0;
+    std::deque<std::unique_ptr<Job>> queue;
+    std::vector<std::unique_ptr<Job>> resident;
+    std::map<long long, std::unique_ptr<Job>> completed;
+    rbhip_batch* batch = nullptr;
+    bool membersChanged = false;
+
+    ~rbhip_server() {
+        for (auto& j : resident)
+            if (j->ctx) (void)rbhip_destroy(j->ctx);
+        if (batch) (void)rbhip_batch_destroy(batch);
+    }
+
+    void admit(Job& j) {
+        rbhip_context* c = nullptr;
+        call(rbhip_create(j.numAtoms, j.masses.data(), j.bodyIndices.data(), j.stepSize, j.rotationMode, precision,
+                          device, &c));
+        try {
+            call(rbhip_set_launch(c, block, 0));
+            if (!j.tetherK.empty()) call(rbhip_set_tether(c, j.tetherK.data(), j.tetherR0.data()));
+            call(rbhip_set_positions(c, j.positions.data()));
+            if (!j.velocities.empty()) call(rbhip_set_velocities(c, j.velocities.data()));
+            size_t b = 0;
+            call(rbhip_device_bytes(c, &b));
+            j.bytes = std::max(j.bytes, b);
+        } catch (...) {
+            (void)rbhip_destroy(c);
+            throw;
+        }
+        j.ctx = c;
+        j.admitted = Clock::now();
+        // host inputs are no longer needed
+        std::vector<double>().swap(j.masses);
+        std::vector<double>().swap(j.positions);
+        std::vector<double>().swap(j.velocities);
+        std::vector<double>().swap(j.tetherK);
+        std::vector<double>().swap(j.tetherR0);
+        std::vector<int>().swap(j.bodyIndices);
+    }
+
+    void retire(Job& j) {
+        j.outPos.resize(3 * (size_t)j.numAtoms);
+        j.outVel.resize(3 * (size_t)j.numAtoms);
+        call(rbhip_get_positions(j.ctx, j.outPos.data()));
+        call(rbhip_get_velocities(j.ctx, j.outVel.data()));
+        call(rbhip_kinetic_energies(j.ctx, j.ke));
+        call(rbhip_destroy(j.ctx));
+        j.ctx = nullptr;
+        j.finished = Clock::now();
+    }
+
+    int round() {
+        // admission in arrival order while the budget holds
+        while (!queue.empty() && inUse + queue.front()->bytes <= budget) {
+            std::unique_ptr<Job> j = std::move(queue.front());
+            queue.pop_front();
+            admit(*j);
+            inUse += j->bytes;
+            resident.push_back(std::move(j));
+            membersChanged = true;
+        }
+        if (resident.empty()) return 0;
+        long long n = stepsPerRound;
+        for (auto& j : resident) n = std::min(n, j->steps - j->done);
+        if (membersChanged) {
+            std::vector<rbhip_context*> m;
+            for (auto& j : resident) m.push_back(j->ctx);
+            call(rbhip_batch_set_members(batch, (int)m.size(), m.data()));
+            membersChanged = false;
+        }
+        if (n > 0) {
+            call(rbhip_batch_execute(batch, (int)n));
+            for (auto& j : resident) {
+                j->done += n;
+                bodySteps += j->bodies * n;
+            }
+        }
+        // jobs at their step count leave
+        int finished = 0;
+        std::vector<std::unique_ptr<Job>> keep;
+        for (auto& j : resident) {
+            if (j->done >= j->steps) {
+                if (!membersChanged) {
+                    call(rbhip_batch_set_members(batch, 0, nullptr));
+                    membersChanged = true;
+                }
+                retire(*j);
+                inUse -= std::min(inUse, j->bytes);
+                completed[j->id] = std::move(j);
+                finished++;
+            } else {
+                keep.push_back(std::move(j));
+            }
+        }
+        resident.swap(keep);
+        return finished;
+    }
+};
+
+extern "C" {
+
+const char* rbhip_server_last_error(void) { return g_err.c_str(); }
+
+int rbhip_server_create(int device, size_t hbmBudget, int stepsPerRound, int precision, int blockSize,
+                        rbhip_server** out) {
+    return guard([&] {
+        need(out != nullptr, "null output handle");
+        *out = nullptr;
+        need(hbmBudget > 0, "HBM budget must be positive");
+        need(stepsPerRound > 0, "steps per round must be positive");
+        need(precision == RBHIP_PRECISION_MIXED || precision == RBHIP_PRECISION_DOUBLE, "unknown precision");
+        need(blockSize >= 64 && blockSize <= 512 && blockSize % 64 == 0, "block size must be 64..512, a multiple of 64");
+        std::unique_ptr<rbhip_server> s(new rbhip_server());
+        s->device = device;
+        s->budget = hbmBudget;
+        s->stepsPerRound = stepsPerRound;
+        s->precision = precision;
+        s->block = blockSize;
+        call(rbhip_batch_create(device, &s->batch));
+        call(rbhip_batch_set_launch(s->batch, blockSize, 0));
+        *out = s.release();
+    });
+}
+
+int rbhip_server_destroy(rbhip_server* s) {
+    return guard([&] { delete s; });
+}
+
+int rbhip_server_submit(rbhip_server* s, int numAtoms, const double* masses, const int* bodyIndices,
+                        const double* positions, const double* velocities, const double* tetherK,
+                        const double* tetherR0, double stepSize, int rotationMode, long long steps,
+                        long long* jobId) {
+    return guard([&] {
+        need(s && numAtoms > 0 && masses && bodyIndices && positions && jobId, "bad arguments");
+        need(steps > 0, "a job needs a positive step count");
+        need(rotationMode >= 0, "Rotation mode cannot be negative");
+        need(!tetherK || tetherR0, "tether anchors missing");
+        int kind = rotationMode == 0 ? 1 : 0;
+        need(s->exactKind < 0 || s->exactKind == kind,
+             "jobs of one server share the rotation kind (exact or NO-SQUISH)", RBHIP_E_UNSUPPORTED);
+        need(!(kind == 1 && s->block > 256), "exact-rotation jobs need a block size <= 256", RBHIP_E_UNSUPPORTED);
+        size_t bytes = 0;
+        call(rbhip_estimate_bytes(numAtoms, bodyIndices, s->precision, &bytes));
+        need(bytes <= s->budget, "job needs " + std::to_string(bytes) + " device bytes, more than the whole budget",
+             RBHIP_E_NOMEM);
+        std::unique_ptr<Job> j(new Job());
+        j->id = s->nextId++;
+        j->numAtoms = numAtoms;
+        j->masses.assign(masses, masses + numAtoms);
+        j->bodyIndices.assign(bodyIndices, bodyIndices + numAtoms);
+        j->positions.assign(positions, positions + 3 * (size_t)numAtoms);
+        if (velocities) j->velocities.assign(velocities, velocities + 3 * (size_t)numAtoms);
+        if (tetherK) {
+            j->tetherK.assign(tetherK, tetherK + numAtoms);
+            j->tetherR0.assign(tetherR0, tetherR0 + 3 * (size_t)numAtoms);
+        }
+        std::vector<int> labels;
+        for (int i = 0; i < numAtoms; i++)
+            if (bodyIndices[i] > 0) labels.push_back(bodyIndices[i]);
+        std::sort(labels.begin(), labels.end());
+        j->bodies = std::unique(labels.begin(), labels.end()) - labels.begin();
+        j->stepSize = stepSize;
+        j->rotationMode = rotationMode;
+        j->steps = steps;
+        j->bytes = bytes;
+        j->submitted = Clock::now();
+        s->exactKind = kind;
+        *jobId = j->id;
+        s->queue.push_back(std::move(j));
+    });
+}
+
+int rbhip_server_run_round(rbhip_server* s, int* finished) {
+    return guard([&] {
+        need(s != nullptr, "null server");
+        int f = s->round();
+        if (finished) *finished = f;
+    });
+}
+
+int rbhip_server_status(rbhip_server* s, long long* counts3, size_t* bytesInUse, long long* bodySteps) {
+    return guard([&] {
+        need(s != nullptr, "null server");
+        if (counts3) {
+            counts3[0] = (long long)s->queue.size();
+            counts3[1] = (long long)s->resident.size();
+            counts3[2] = (long long)s->completed.size();
+        }
+        if (bytesInUse) *bytesInUse = s->inUse;
+        if (bodySteps) *bodySteps = s->bodySteps;
+    });
+}
+
+int rbhip_server_completed(rbhip_server* s, long long* ids, int max, int* count) {
+    return guard([&] {
+        need(s && count && (max == 0 || ids), "bad arguments");
+        int n = 0;
+        for (auto& kv : s->completed) {
+            if (n >= max) break;
+            ids[n++] = kv.first;
+        }
+        *count = n;
+    });
+}
+
+int rbhip_server_take(rbhip_server* s, long long jobId, double* positions, double* velocities, double* ke2,
+                      double* times3) {
+    return guard([&] {
+        need(s != nullptr, "null server");
+        auto it = s->completed.find(jobId);
+        need(it != s->completed.end(), "no completed job with id " + std::to_string(jobId), RBHIP_E_STATE);
+        Job& j = *it->second;
+        if (positions) std::copy(j.outPos.begin(), j.outPos.end(), positions);
+        if (velocities) std::copy(j.outVel.begin(), j.outVel.end(), velocities);
+        if (ke2) {
+            ke2[0] = j.ke[0];
+            ke2[1] = j.ke[1];
+        }
+        if (times3) {
+            times3[0] = seconds(j.submitted, j.admitted);    // waiting for admission
+            times3[1] = seconds(j.admitted, j.finished);     // resident
+            times3[2] = seconds(j.submitted, j.finished);    // completion latency
+        }
+        s->completed.erase(it);
+    });
+}
+
+}  // extern "C"

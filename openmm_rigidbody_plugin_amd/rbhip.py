@@ -91,6 +91,15 @@ def lib() -> ctypes.CDLL:
             "rbhip_batch_time_kernels": (_I, [_P, _I, _DP, _DP]),
             "rbhip_batch_autotune": (_I, [_P, _I]),
             "rbhip_batch_describe": (_I, [_P, ctypes.c_char_p, _I]),
+            "rbhip_estimate_bytes": (_I, [_I, _IP, _I, _SIZEP]),
+            "rbhip_server_last_error": (ctypes.c_char_p, []),
+            "rbhip_server_create": (_I, [_I, ctypes.c_size_t, _I, _I, _I, ctypes.POINTER(_P)]),
+            "rbhip_server_destroy": (_I, [_P]),
+            "rbhip_server_submit": (_I, [_P, _I, _DP, _IP, _DP, _DP, _DP, _DP, _D, _I, ctypes.c_longlong, _LLP]),
+            "rbhip_server_run_round": (_I, [_P, _IP]),
+            "rbhip_server_status": (_I, [_P, _LLP, _SIZEP, _LLP]),
+            "rbhip_server_completed": (_I, [_P, _LLP, _I, _IP]),
+            "rbhip_server_take": (_I, [_P, ctypes.c_longlong, _DP, _DP, _DP, _DP]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -371,3 +380,82 @@ class Batch:
         tot = ctypes.c_double()
         _check(lib().rbhip_batch_time_kernels(self._h, int(steps), _dp(avg), ctypes.byref(tot)))
         return dict(zip(KERNELS, avg)), tot.value
+
+
+def estimate_bytes(body_indices, precision="mixed"):
+    """Device bytes a context for this system holds (upper bound before bonds)."""
+    bi = np.ascontiguousarray(body_indices, dtype=np.int32)
+    n = ctypes.c_size_t()
+    _check(lib().rbhip_estimate_bytes(bi.size, _ip(bi), PRECISIONS[precision], ctypes.byref(n)))
+    return n.value
+
+
+class ServerError(RbhipError):
+    pass
+
+
+def _scheck(status):
+    if status != 0:
+        raise ServerError(status, lib().rbhip_server_last_error().decode())
+
+
+class Server:
+    """Job server: admission by HBM budget, continuous batching (rbhip_server_*)."""
+
+    def __init__(self, hbm_budget, steps_per_round=100, precision="mixed", block=256, device=0):
+        self._h = _P()
+        _scheck(lib().rbhip_server_create(int(device), int(hbm_budget), int(steps_per_round),
+                                          PRECISIONS[precision], int(block), ctypes.byref(self._h)))
+        self.num_atoms = {}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().rbhip_server_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, spec, steps):
+        """Queues a SystemSpec for `steps` steps; returns the job id."""
+        n = spec.num_atoms
+        m = np.ascontiguousarray(spec.masses, dtype=np.float64)
+        bi = np.ascontiguousarray(spec.body_indices, dtype=np.int32)
+        x = np.ascontiguousarray(spec.positions, dtype=np.float64)
+        v = np.ascontiguousarray(spec.velocities, dtype=np.float64)
+        k = np.ascontiguousarray(spec.tether_k, dtype=np.float64)
+        r0 = np.ascontiguousarray(spec.tether_r0, dtype=np.float64)
+        jid = ctypes.c_longlong()
+        _scheck(lib().rbhip_server_submit(self._h, n, _dp(m), _ip(bi), _dp(x), _dp(v), _dp(k), _dp(r0),
+                                          float(spec.dt), int(spec.rotation_mode), int(steps), ctypes.byref(jid)))
+        self.num_atoms[jid.value] = n
+        return jid.value
+
+    def run_round(self):
+        f = ctypes.c_int()
+        _scheck(lib().rbhip_server_run_round(self._h, ctypes.byref(f)))
+        return f.value
+
+    def status(self):
+        c = (ctypes.c_longlong * 3)()
+        b, bs = ctypes.c_size_t(), ctypes.c_longlong()
+        _scheck(lib().rbhip_server_status(self._h, c, ctypes.byref(b), ctypes.byref(bs)))
+        return {"waiting": c[0], "resident": c[1], "completed": c[2], "bytes_in_use": b.value,
+                "body_steps": bs.value}
+
+    def completed(self, max_ids=4096):
+        ids = (ctypes.c_longlong * max_ids)()
+        n = ctypes.c_int()
+        _scheck(lib().rbhip_server_completed(self._h, ids, max_ids, ctypes.byref(n)))
+        return [ids[i] for i in range(n.value)]
+
+    def take(self, job_id):
+        """Final positions, velocities, {KE_t, KE_r} and timings (wait, resident,
+        latency in seconds) of a completed job."""
+        n = self.num_atoms.pop(job_id)
+        x, v, ke, t = np.empty((n, 3)), np.empty((n, 3)), np.empty(2), np.empty(3)
+        _scheck(lib().rbhip_server_take(self._h, int(job_id), _dp(x), _dp(v), _dp(ke), _dp(t)))
+        return {"positions": x, "velocities": v, "kinetic": ke, "wait": t[0], "resident": t[1], "latency": t[2]}
