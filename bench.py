@@ -177,6 +177,91 @@ def batched(steps, warmup, count=256, log_every=64):
     }
 
 
+def served(num_jobs=96, rate=25.0, budget=1 << 30, steps_per_round=100, seed=11):
+    """Serving: a Poisson stream of jobs (bodies ~ LogUniform(500, 1e5) with the
+    heterogeneous body mix and 2.5 free atoms per body, steps ~
+    LogUniform(100, 1e4), exact rotation, dt 1 fs, mixed) submitted to the job
+    server in real time; admission against a fixed HBM budget, continuous
+    batching.  Reports completion-latency percentiles and aggregate
+    body-steps/s; then the same jobs all at once (saturated server) and one
+    after another through the plain per-system path."""
+    from openmm_rigidbody_plugin_amd import rbhip, systems
+    rng = np.random.default_rng(seed)
+    nbs = np.exp(rng.uniform(np.log(500), np.log(1e5), num_jobs)).astype(int)
+    steps = np.exp(rng.uniform(np.log(100), np.log(1e4), num_jobs)).astype(int)
+    t = time.perf_counter()
+    specs = [systems.heterogeneous(int(nb), int(2.5 * nb), seed=int(seed * 1000 + i), rotation_mode=0,
+                                   name=f"job{i}") for i, nb in enumerate(nbs)]
+    log(f"served: {num_jobs} jobs generated in {time.perf_counter() - t:.1f}s "
+        f"({int(nbs.sum())} bodies, {int((nbs * steps).sum())} body-steps)")
+    work = int((nbs * steps).sum())
+    block = 64
+
+    def run_server(arrivals):
+        srv = rbhip.Server(hbm_budget=budget, steps_per_round=steps_per_round, block=block)
+        ids, lat, peak = {}, {}, 0
+        nxt = 0
+        t0 = time.perf_counter()
+        while len(lat) < num_jobs:
+            now = time.perf_counter() - t0
+            while nxt < num_jobs and arrivals[nxt] <= now:
+                ids[srv.submit(specs[nxt], int(steps[nxt]))] = nxt
+                nxt += 1
+            st = srv.status()
+            peak = max(peak, st["resident"])
+            if st["waiting"] == 0 and st["resident"] == 0:
+                if nxt < num_jobs:
+                    time.sleep(max(0.0, min(arrivals[nxt] - now, 0.001)))
+                continue
+            srv.run_round()
+            for jid in srv.completed():
+                r = srv.take(jid)
+                lat[ids[jid]] = r
+        makespan = time.perf_counter() - t0
+        log(f"    server phases: {srv.profile()}")
+        return srv, lat, makespan, peak
+
+    gaps = rng.exponential(1.0 / rate, num_jobs)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    _, lat, makespan, peak = run_server(arrivals)
+    L = np.array([lat[i]["latency"] for i in range(num_jobs)])
+    log(f"  poisson: makespan {makespan:.2f}s, p50 {np.percentile(L, 50) * 1e3:.1f} ms, "
+        f"p99 {np.percentile(L, 99) * 1e3:.1f} ms, peak resident {peak}")
+    # saturated: every job waiting at t = 0
+    _, lat_s, makespan_s, peak_s = run_server(np.zeros(num_jobs))
+    log(f"  saturated: makespan {makespan_s:.2f}s, peak resident {peak_s}")
+    # serial plain: one job at a time through its own context
+    t = time.perf_counter()
+    alone = {}
+    sample = {int(np.argmin(nbs * steps)), int(np.argsort(nbs * steps)[num_jobs // 2])}
+    for i, spec in enumerate(specs):
+        c = rbhip.Context.from_spec(spec, precision="mixed")
+        c.set_launch(block)
+        c.step(int(steps[i]))
+        x = c.positions()
+        if i in sample:
+            alone[i] = x
+        del c
+    serial = time.perf_counter() - t
+    log(f"  serial plain: {serial:.2f}s")
+    same = all(np.array_equal(alone[i], lat_s[i]["positions"]) and np.array_equal(alone[i], lat[i]["positions"])
+               for i in alone)
+    return {
+        "workload": f"{num_jobs} jobs, bodies LogUniform(500, 1e5) (30% 2-atom, 40% 3-site, 20% 6-ring, "
+                    "10% 20-atom) + 2.5 free atoms per body, steps LogUniform(100, 1e4), exact rotation, "
+                    "dt 1 fs, mixed",
+        "arrival_rate_per_s": rate, "hbm_budget_bytes": budget, "steps_per_round": steps_per_round,
+        "block": block, "jobs": num_jobs, "body_steps": work,
+        "p50_latency_ms": float(np.percentile(L, 50) * 1e3), "p99_latency_ms": float(np.percentile(L, 99) * 1e3),
+        "p50_wait_ms": float(np.percentile([lat[i]["wait"] for i in range(num_jobs)], 50) * 1e3),
+        "aggregate_body_steps_per_s": work / makespan, "makespan_s": makespan, "peak_resident": peak,
+        "saturated_body_steps_per_s": work / makespan_s,
+        "serial_plain_body_steps_per_s": work / serial,
+        "saturated_speedup_vs_serial_plain": serial / makespan_s,
+        "sampled_jobs_equal_alone": bool(same), "sampled_jobs": sorted(alone),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +273,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica batched measurement")
     ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served job-stream measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only stream MIB MiB through the plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -281,6 +367,9 @@ def main():
         del ctx
         log("batched replicas ...")
         result["batched"] = batched(args.batch_steps, args.warmup)
+    if not args.no_serve and args.config == "water1M":
+        log("served job stream ...")
+        result["served"] = served()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(spec)

@@ -79,7 +79,8 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
                  int rotationMode, int precision, int device, rbhip_context** out);
 int rbhip_destroy(rbhip_context* ctx);
 /* launch stream for all work of this context (hipStream_t; NULL restores the
-   context's own stream) */
+   stream the library assigned, one of a per-device pool of non-blocking
+   streams shared by contexts and batches) */
 int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream);
 /* device bytes held by the context (for admission against an HBM budget) */
 int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes);
@@ -194,6 +195,8 @@ int rbhip_batch_destroy(rbhip_batch* batch);
 /* replaces the member list; members must live on the batch's device, have
    the same precision and rotation mode, and hold a built body state */
 int rbhip_batch_set_members(rbhip_batch* batch, int count, rbhip_context* const* members);
+/* launch stream of the batch (NULL = a library pool stream, the default) */
+int rbhip_batch_set_stream(rbhip_batch* batch, rbhip_stream stream);
 int rbhip_batch_set_launch(rbhip_batch* batch, int blockSize, int maxBlocks);
 int rbhip_batch_use_graph(rbhip_batch* batch, int enable);
 int rbhip_batch_use_fusion(rbhip_batch* batch, int enable);
@@ -242,11 +245,17 @@ int rbhip_server_submit(rbhip_server* server, int numAtoms, const double* masses
                         const double* positions, const double* velocities, const double* tetherK,
                         const double* tetherR0, double stepSize, int rotationMode, long long steps,
                         long long* jobId);
-/* one round: admission, one batched advance, retirement; *finished = jobs completed */
+/* one round: admits waiting jobs while the steps queued by the previous
+   round run, collects the jobs that reached their step count, and queues the
+   next batched advance without waiting for it; *finished = jobs completed */
 int rbhip_server_run_round(rbhip_server* server, int* finished);
 /* counts3 = {waiting, resident, completed and not yet taken}; device bytes of
    the resident jobs; body-steps done so far.  Any output may be NULL. */
 int rbhip_server_status(rbhip_server* server, long long* counts3, size_t* bytesInUse, long long* bodySteps);
+/* rounds run so far and the host seconds spent in their phases: seconds4 =
+   {admitting, collecting finished jobs (includes waiting for the device),
+   updating the batch members, queueing the steps} */
+int rbhip_server_profile(rbhip_server* server, long long* rounds, double* seconds4);
 /* ids of completed jobs not yet taken, up to max */
 int rbhip_server_completed(rbhip_server* server, long long* ids, int max, int* count);
 /* final positions and velocities (numAtoms x 3, original order), {KE_t, KE_r}

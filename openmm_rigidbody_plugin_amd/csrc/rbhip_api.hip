@@ -18,6 +18,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -43,6 +44,10 @@ struct RbError : std::runtime_error {
         if (e_ != hipSuccess) throw RbError(RBHIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// (messages are built only on failure: the checks run in per-atom loops)
+void require(bool ok, const char* msg, int code = RBHIP_E_ARG) {
+    if (!ok) throw RbError(code, msg);
+}
 void require(bool ok, const std::string& msg, int code = RBHIP_E_ARG) {
     if (!ok) throw RbError(code, msg);
 }
@@ -66,15 +71,37 @@ template <class F> int guard(F&& f) {
 // cleanBodyIndices (RigidBodySystem.cpp:28-49): distinct positive labels are
 // renumbered 1..B in ascending label order, labels <= 0 become 0
 std::vector<int> clean_body_indices(const int* in, int n) {
-    std::vector<int> labels;
-    labels.reserve(n);
+    // (label, atom) pairs of the body atoms sorted by label with a stable
+    // 4 x 8-bit radix sort, then ranked in one pass: O(n), no per-atom search
+    std::vector<unsigned> key, atom;
+    key.reserve(n);
+    atom.reserve(n);
     for (int i = 0; i < n; i++)
-        if (in[i] > 0) labels.push_back(in[i]);
-    std::sort(labels.begin(), labels.end());
-    labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
+        if (in[i] > 0) {
+            key.push_back((unsigned)in[i]);
+            atom.push_back((unsigned)i);
+        }
+    const size_t m = key.size();
+    std::vector<unsigned> key2(m), atom2(m);
+    for (int shift = 0; shift < 32 && m > 0; shift += 8) {
+        size_t count[257] = {0};
+        for (size_t i = 0; i < m; i++) count[((key[i] >> shift) & 255u) + 1]++;
+        if (count[((key[0] >> shift) & 255u) + 1] == m) continue;  // one bucket: the pass keeps the order
+        for (int b = 0; b < 256; b++) count[b + 1] += count[b];
+        for (size_t i = 0; i < m; i++) {
+            size_t d = count[(key[i] >> shift) & 255u]++;
+            key2[d] = key[i];
+            atom2[d] = atom[i];
+        }
+        key.swap(key2);
+        atom.swap(atom2);
+    }
     std::vector<int> out(n, 0);
-    for (int i = 0; i < n; i++)
-        if (in[i] > 0) out[i] = (int)(std::lower_bound(labels.begin(), labels.end(), in[i]) - labels.begin()) + 1;
+    int rank = 0;
+    for (size_t i = 0; i < m; i++) {
+        if (i == 0 || key[i] != key[i - 1]) rank++;
+        out[atom[i]] = rank;
+    }
     return out;
 }
 
@@ -97,6 +124,27 @@ struct DeviceArena {
         bytes = 0;
     }
 };
+
+// Streams come from a per-device pool created once per process (creating a
+// HIP stream costs milliseconds, which dominated the set-up of small
+// systems); a context or batch takes the next pool stream in turn.  Pool
+// streams are non-blocking and live until the process ends.
+hipStream_t pooled_stream(int device) {
+    static std::mutex mtx;
+    static std::map<int, std::vector<hipStream_t>> pool;
+    static std::map<int, unsigned> next;
+    constexpr int kPoolSize = 4;
+    std::lock_guard<std::mutex> lock(mtx);
+    std::vector<hipStream_t>& v = pool[device];
+    if (v.empty()) {
+        for (int i = 0; i < kPoolSize; i++) {
+            hipStream_t st = nullptr;
+            HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            v.push_back(st);
+        }
+    }
+    return v[next[device]++ % v.size()];
+}
 
 int pow2_at_least(int v) {
     int p = 1;
@@ -357,13 +405,19 @@ struct rbhip_context {
         validateTiles(pt, block, maxKeSlots);
         validateTiles(ft, block, maxPeSlots);
         validateTiles(st, setupBlock(), maxKeSlots);
+        // the old tables may still be read by queued work
+        HIPCHK(hipStreamSynchronize(stream));
         tileArena.release();
         dPartTiles = tileArena.allocT<Tile>(pt.size() + 1);
         dForceTiles = tileArena.allocT<Tile>(ft.size() + 1);
         dSetupTiles = tileArena.allocT<Tile>(st.size() + 1);
-        if (!pt.empty()) HIPCHK(hipMemcpy(dPartTiles, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
-        if (!ft.empty()) HIPCHK(hipMemcpy(dForceTiles, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
-        if (!st.empty()) HIPCHK(hipMemcpy(dSetupTiles, st.data(), st.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        if (!pt.empty())
+            HIPCHK(hipMemcpyAsync(dPartTiles, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+        if (!ft.empty())
+            HIPCHK(hipMemcpyAsync(dForceTiles, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+        if (!st.empty())
+            HIPCHK(hipMemcpyAsync(dSetupTiles, st.data(), st.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         nPartTiles = (int)pt.size();
         nForceTiles = (int)ft.size();
         nSetupTiles = (int)st.size();
@@ -505,7 +559,6 @@ struct rbhip_context {
         if (hRed) (void)hipHostFree(hRed);
         tileArena.release();
         arena.release();
-        if (ownStream) (void)hipStreamDestroy(ownStream);
     }
 };
 
@@ -570,13 +623,15 @@ struct rbhip_batch {
     // was copied (step size, force field); the table's address is unchanged,
     // so captured graphs stay valid
     void refreshSystems() {
+        bool copied = false;
         for (size_t m = 0; m < members.size(); m++) {
             rbhip_context* c = members[m];
             if (c->sysVersion == memberVersions[m]) continue;
             HIPCHK(hipMemcpyAsync(dSystems + m, &c->h, sizeof(DevSys), hipMemcpyHostToDevice, stream));
             memberVersions[m] = c->sysVersion;
+            copied = true;
         }
-        HIPCHK(hipStreamSynchronize(stream));
+        if (copied) HIPCHK(hipStreamSynchronize(stream));
     }
 
     void launchPart1() {
@@ -664,7 +719,6 @@ struct rbhip_batch {
         fusedGraph.reset();
         arena.release();
         if (ordering) (void)hipEventDestroy(ordering);
-        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -698,8 +752,9 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         require(precision == RBHIP_PRECISION_MIXED || precision == RBHIP_PRECISION_DOUBLE, "unknown precision");
         require(stepSize > 0.0, "step size must be positive");
         for (int i = 0; i < numAtoms; i++)
-            require(masses[i] > 0.0, "massless or negative-mass particles are not supported (particle " +
-                                         std::to_string(i) + ")", RBHIP_E_UNSUPPORTED);
+            if (!(masses[i] > 0.0))
+                throw RbError(RBHIP_E_UNSUPPORTED, "massless or negative-mass particles are not supported (particle " +
+                                                       std::to_string(i) + ")");
         int ndev = 0;
         HIPCHK(hipGetDeviceCount(&ndev));
         require(device >= 0 && device < ndev, "no such device");
@@ -726,9 +781,9 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         C.refBodyLoc.assign(C.numBodies, 0);
         int loc = 0;
         for (int b = 0; b < C.numBodies; b++) {
-            require(C.refBodyN[b] >= 2, "rigid body " + std::to_string(b + 1) +
-                                            " has a single atom; single-atom bodies are not supported",
-                    RBHIP_E_UNSUPPORTED);
+            if (C.refBodyN[b] < 2)
+                throw RbError(RBHIP_E_UNSUPPORTED, "rigid body " + std::to_string(b + 1) +
+                                                       " has a single atom; single-atom bodies are not supported");
             C.refBodyLoc[b] = loc;
             loc += C.refBodyN[b];
         }
@@ -763,7 +818,7 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
 
         // ---- device allocation ----
         if (rotationMode == 0) C.block = 64;   // the FP64 exact rotation is register heavy
-        HIPCHK(hipStreamCreateWithFlags(&C.ownStream, hipStreamNonBlocking));
+        C.ownStream = pooled_stream(device);
         C.stream = C.ownStream;
         const size_t R = C.realSize();
         const size_t nb = std::max(C.numBodies, 1), na = (size_t)numAtoms;
@@ -817,37 +872,42 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
                                                      (int)C.classes.size(), R),
                 "internal: device size estimate out of step with the allocation", RBHIP_E_STATE);
 
-        // zero state, constants
-        HIPCHK(hipMemset(h.pos, 0, 3 * R * as));
-        if (h.posLo) HIPCHK(hipMemset(h.posLo, 0, 3 * sizeof(float) * as));
-        HIPCHK(hipMemset(h.vel, 0, 3 * R * as));
-        HIPCHK(hipMemset(h.force, 0, 3 * R * as));
-        HIPCHK(hipMemset(h.d, 0, 3 * R * (size_t)h.dStride));
-        HIPCHK(hipMemset(h.tetherK, 0, R * na));
-        HIPCHK(hipMemset(h.r0, 0, 3 * R * as));
-        if (h.r0Lo) HIPCHK(hipMemset(h.r0Lo, 0, 3 * sizeof(float) * as));
-        HIPCHK(hipMemset(h.kePart, 0, 4 * sizeof(double) * C.maxKeSlots));
-        HIPCHK(hipMemset(h.pePart, 0, 4 * sizeof(double) * C.maxPeSlots));
+        // zero state, constants (queued on the context's stream; the host
+        // buffers live until the synchronisation below)
+        hipStream_t st = C.stream;
+        HIPCHK(hipMemsetAsync(h.pos, 0, 3 * R * as, st));
+        if (h.posLo) HIPCHK(hipMemsetAsync(h.posLo, 0, 3 * sizeof(float) * as, st));
+        HIPCHK(hipMemsetAsync(h.vel, 0, 3 * R * as, st));
+        HIPCHK(hipMemsetAsync(h.force, 0, 3 * R * as, st));
+        HIPCHK(hipMemsetAsync(h.d, 0, 3 * R * (size_t)h.dStride, st));
+        HIPCHK(hipMemsetAsync(h.tetherK, 0, R * na, st));
+        HIPCHK(hipMemsetAsync(h.r0, 0, 3 * R * as, st));
+        if (h.r0Lo) HIPCHK(hipMemsetAsync(h.r0Lo, 0, 3 * sizeof(float) * as, st));
+        HIPCHK(hipMemsetAsync(h.kePart, 0, 4 * sizeof(double) * C.maxKeSlots, st));
+        HIPCHK(hipMemsetAsync(h.pePart, 0, 4 * sizeof(double) * C.maxPeSlots, st));
         std::vector<double> m(na);
         for (size_t i = 0; i < na; i++) m[i] = masses[C.atomOrig[i]];
-        HIPCHK(hipMemcpy(h.mass, m.data(), na * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(h.mass, m.data(), na * sizeof(double), hipMemcpyHostToDevice, st));
+        std::vector<float> invf;
+        std::vector<double> invd;
         if (C.numFree > 0) {
             if (R == 4) {
-                std::vector<float> inv(C.numFree);
-                for (int k = 0; k < C.numFree; k++) inv[k] = (float)(1.0 / m[k]);
-                HIPCHK(hipMemcpy(h.freeInvM, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
+                invf.resize(C.numFree);
+                for (int k = 0; k < C.numFree; k++) invf[k] = (float)(1.0 / m[k]);
+                HIPCHK(hipMemcpyAsync(h.freeInvM, invf.data(), invf.size() * 4, hipMemcpyHostToDevice, st));
             } else {
-                std::vector<double> inv(C.numFree);
-                for (int k = 0; k < C.numFree; k++) inv[k] = 1.0 / m[k];
-                HIPCHK(hipMemcpy(h.freeInvM, inv.data(), inv.size() * 8, hipMemcpyHostToDevice));
+                invd.resize(C.numFree);
+                for (int k = 0; k < C.numFree; k++) invd[k] = 1.0 / m[k];
+                HIPCHK(hipMemcpyAsync(h.freeInvM, invd.data(), invd.size() * 8, hipMemcpyHostToDevice, st));
             }
         }
         if (C.numBodies > 0) {
-            HIPCHK(hipMemcpy(C.dBodyN, C.bodyN.data(), C.numBodies * sizeof(int), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(C.dBodyAtom0, C.bodyAtom0.data(), C.numBodies * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpyAsync(C.dBodyN, C.bodyN.data(), C.numBodies * sizeof(int), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(C.dBodyAtom0, C.bodyAtom0.data(), C.numBodies * sizeof(int), hipMemcpyHostToDevice,
+                                  st));
         }
         C.uploadSys();
-        HIPCHK(hipStreamSynchronize(C.stream));
+        HIPCHK(hipStreamSynchronize(st));
         C.rebuildTiles();
         *out = c.release();
     });
@@ -1392,7 +1452,7 @@ int rbhip_batch_create(int device, rbhip_batch** out) {
         HIPCHK(hipSetDevice(device));
         std::unique_ptr<rbhip_batch> b(new rbhip_batch());
         b->device = device;
-        HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        b->stream = pooled_stream(device);
         HIPCHK(hipEventCreateWithFlags(&b->ordering, hipEventDisableTiming));
         *out = b.release();
     });
@@ -1432,6 +1492,17 @@ int rbhip_batch_set_members(rbhip_batch* batch, int count, rbhip_context* const*
         }
         batch->members = m;
         batch->rebuild();
+    });
+}
+
+int rbhip_batch_set_stream(rbhip_batch* batch, rbhip_stream stream) {
+    return guard([&] {
+        require(batch, "null batch");
+        HIPCHK(hipSetDevice(batch->device));
+        HIPCHK(hipStreamSynchronize(batch->stream));
+        batch->stream = stream ? reinterpret_cast<hipStream_t>(stream) : pooled_stream(batch->device);
+        batch->graph.reset();
+        batch->fusedGraph.reset();
     });
 }
 

@@ -312,11 +312,11 @@ __device__ inline double omega_fn(double x, double n, double m) {
     return (-1.0 / 3.0) * n * x * x2 * carlson_rj(1.0 - x2, 1.0 - m * x2, 1.0, 1.0 + n * x2);
 }
 
-__device__ inline void jacobi_fn(double u, double m, double& sn, double& cn, double& dn) {
+// K = K(m) = R_F(0, 1 - m, 1), which the caller has already computed
+__device__ inline void jacobi_fn(double u, double m, double K, double& sn, double& cn, double& dn) {
     const double eps = 2.220446049250313e-16;
     if (m < 2.0 * eps) { sincos(u, &sn, &cn); dn = 1.0; return; }
     if (1.0 - m < 2.0 * eps) { sn = tanh(u); cn = 1.0 / cosh(u); dn = cn; return; }
-    double K = carlson_rf(0.0, 1.0 - m, 1.0);
     if (fabs(u) > 2.0 * K) u -= 4.0 * K * rint(u / (4.0 * K));
     double a[25], c[25];
     a[0] = 1.0;
@@ -404,7 +404,7 @@ __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> 
     double u = wp * dt + u0;
     int jump = stair(u * inv2K) - i0;
     double sn, cn, dn, deltaF;
-    jacobi_fn(u, m, sn, cn, dn);
+    jacobi_fn(u, m, K, sn, cn, dn);
     double alpha = I.x * a.x / L;
     double eta = alpha * alpha;
     eta /= 1.0 - eta;

@@ -12,7 +12,12 @@
 //
 // A resident job is stepped with the same kernels and tiles as when it is
 // stepped alone with the same workgroup size, so its result does not depend
-// on which other jobs share its rounds.
+// on which other jobs share its rounds.  A round returns once its steps are
+// queued: the next round's admissions overlap with them on the device.
+//
+// Streams: the batch runs on a stream of its own, the resident jobs share a
+// second one (the batch orders it after each advance) and admissions use a
+// third, so setting up a new job never waits behind the running steps.
 #include <algorithm>
 #include <chrono>
 #include <deque>
@@ -21,6 +26,8 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <hip/hip_runtime.h>
 
 #include "rbhip.h"
 
@@ -89,16 +96,23 @@ struct rbhip_server {
     long long nextId = 1;
     size_t inUse = 0;
     long long bodySteps = 0;
+    long long rounds = 0;
+    double phase[4] = {0.0, 0.0, 0.0, 0.0};   // host seconds: admit, collect, members, launch
     std::deque<std::unique_ptr<Job>> queue;
-    std::vector<std::unique_ptr<Job>> resident;
+    std::vector<std::unique_ptr<Job>> resident;     // batch members
+    std::vector<std::unique_ptr<Job>> pending;      // admitted this round, members from the next launch
     std::map<long long, std::unique_ptr<Job>> completed;
     rbhip_batch* batch = nullptr;
-    bool membersChanged = false;
+    hipStream_t batchStream = nullptr, memberStream = nullptr, admitStream = nullptr;
 
     ~rbhip_server() {
+        if (batch) (void)rbhip_batch_destroy(batch);
         for (auto& j : resident)
             if (j->ctx) (void)rbhip_destroy(j->ctx);
-        if (batch) (void)rbhip_batch_destroy(batch);
+        for (auto& j : pending)
+            if (j->ctx) (void)rbhip_destroy(j->ctx);
+        for (hipStream_t st : {batchStream, memberStream, admitStream})
+            if (st) (void)hipStreamDestroy(st);
     }
 
     void admit(Job& j) {
@@ -106,6 +120,7 @@ struct rbhip_server {
         call(rbhip_create(j.numAtoms, j.masses.data(), j.bodyIndices.data(), j.stepSize, j.rotationMode, precision,
                           device, &c));
         try {
+            call(rbhip_set_stream(c, admitStream));
             call(rbhip_set_launch(c, block, 0));
             if (!j.tetherK.empty()) call(rbhip_set_tether(c, j.tetherK.data(), j.tetherR0.data()));
             call(rbhip_set_positions(c, j.positions.data()));
@@ -113,6 +128,7 @@ struct rbhip_server {
             size_t b = 0;
             call(rbhip_device_bytes(c, &b));
             j.bytes = std::max(j.bytes, b);
+            call(rbhip_set_stream(c, memberStream));
         } catch (...) {
             (void)rbhip_destroy(c);
             throw;
@@ -128,61 +144,74 @@ struct rbhip_server {
         std::vector<int>().swap(j.bodyIndices);
     }
 
-    void retire(Job& j) {
-        j.outPos.resize(3 * (size_t)j.numAtoms);
-        j.outVel.resize(3 * (size_t)j.numAtoms);
-        call(rbhip_get_positions(j.ctx, j.outPos.data()));
-        call(rbhip_get_velocities(j.ctx, j.outVel.data()));
-        call(rbhip_kinetic_energies(j.ctx, j.ke));
-        call(rbhip_destroy(j.ctx));
-        j.ctx = nullptr;
-        j.finished = Clock::now();
-    }
-
+    // One round.  The previous round's steps run on the device while this
+    // call admits waiting jobs (host set-up and their own set-up kernels);
+    // then the jobs that reached their step count are collected, the batch
+    // takes its new member list and the next steps are queued without
+    // waiting for them.
     int round() {
+        rounds++;
+        Clock::time_point t0 = Clock::now();
         // admission in arrival order while the budget holds
         while (!queue.empty() && inUse + queue.front()->bytes <= budget) {
             std::unique_ptr<Job> j = std::move(queue.front());
             queue.pop_front();
             admit(*j);
             inUse += j->bytes;
-            resident.push_back(std::move(j));
-            membersChanged = true;
+            pending.push_back(std::move(j));
         }
-        if (resident.empty()) return 0;
-        long long n = stepsPerRound;
-        for (auto& j : resident) n = std::min(n, j->steps - j->done);
-        if (membersChanged) {
+        Clock::time_point t1 = Clock::now();
+        // collect finished jobs (waits for the steps queued last round)
+        int finished = 0;
+        std::vector<std::unique_ptr<Job>> keep, done;
+        for (auto& j : resident) {
+            if (j->done >= j->steps) {
+                j->outPos.resize(3 * (size_t)j->numAtoms);
+                j->outVel.resize(3 * (size_t)j->numAtoms);
+                call(rbhip_get_positions(j->ctx, j->outPos.data()));
+                call(rbhip_get_velocities(j->ctx, j->outVel.data()));
+                call(rbhip_kinetic_energies(j->ctx, j->ke));
+                j->finished = Clock::now();
+                done.push_back(std::move(j));
+            } else {
+                keep.push_back(std::move(j));
+            }
+        }
+        for (auto& j : pending) keep.push_back(std::move(j));
+        pending.clear();
+        Clock::time_point t2 = Clock::now();
+        const bool changed = !done.empty() || keep.size() != resident.size();
+        resident.swap(keep);
+        if (changed) {
             std::vector<rbhip_context*> m;
             for (auto& j : resident) m.push_back(j->ctx);
             call(rbhip_batch_set_members(batch, (int)m.size(), m.data()));
-            membersChanged = false;
         }
-        if (n > 0) {
+        // the finished jobs are no longer members: free their device state
+        for (auto& j : done) {
+            call(rbhip_destroy(j->ctx));
+            j->ctx = nullptr;
+            inUse -= std::min(inUse, j->bytes);
+            completed[j->id] = std::move(j);
+            finished++;
+        }
+        Clock::time_point t3 = Clock::now();
+        // queue the next steps: all resident jobs advance together, never
+        // past the end of the one that finishes first
+        long long n = stepsPerRound;
+        for (auto& j : resident) n = std::min(n, j->steps - j->done);
+        if (!resident.empty() && n > 0) {
             call(rbhip_batch_execute(batch, (int)n));
             for (auto& j : resident) {
                 j->done += n;
                 bodySteps += j->bodies * n;
             }
         }
-        // jobs at their step count leave
-        int finished = 0;
-        std::vector<std::unique_ptr<Job>> keep;
-        for (auto& j : resident) {
-            if (j->done >= j->steps) {
-                if (!membersChanged) {
-                    call(rbhip_batch_set_members(batch, 0, nullptr));
-                    membersChanged = true;
-                }
-                retire(*j);
-                inUse -= std::min(inUse, j->bytes);
-                completed[j->id] = std::move(j);
-                finished++;
-            } else {
-                keep.push_back(std::move(j));
-            }
-        }
-        resident.swap(keep);
+        Clock::time_point t4 = Clock::now();
+        phase[0] += seconds(t0, t1);
+        phase[1] += seconds(t1, t2);
+        phase[2] += seconds(t2, t3);
+        phase[3] += seconds(t3, t4);
         return finished;
     }
 };
@@ -207,6 +236,10 @@ int rbhip_server_create(int device, size_t hbmBudget, int stepsPerRound, int pre
         s->precision = precision;
         s->block = blockSize;
         call(rbhip_batch_create(device, &s->batch));
+        for (hipStream_t* st : {&s->batchStream, &s->memberStream, &s->admitStream})
+            if (hipStreamCreateWithFlags(st, hipStreamNonBlocking) != hipSuccess)
+                throw ServerError(RBHIP_E_HIP, "stream creation failed");
+        call(rbhip_batch_set_stream(s->batch, s->batchStream));
         call(rbhip_batch_set_launch(s->batch, blockSize, 0));
         *out = s.release();
     });
@@ -278,6 +311,15 @@ int rbhip_server_status(rbhip_server* s, long long* counts3, size_t* bytesInUse,
         }
         if (bytesInUse) *bytesInUse = s->inUse;
         if (bodySteps) *bodySteps = s->bodySteps;
+    });
+}
+
+int rbhip_server_profile(rbhip_server* s, long long* rounds, double* seconds4) {
+    return guard([&] {
+        need(s != nullptr, "null server");
+        if (rounds) *rounds = s->rounds;
+        if (seconds4)
+            for (int i = 0; i < 4; i++) seconds4[i] = s->phase[i];
     });
 }
 

@@ -84,6 +84,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_batch_destroy": (_I, [_P]),
             "rbhip_batch_set_members": (_I, [_P, _I, ctypes.POINTER(_P)]),
             "rbhip_batch_set_launch": (_I, [_P, _I, _I]),
+            "rbhip_batch_set_stream": (_I, [_P, _P]),
             "rbhip_batch_use_graph": (_I, [_P, _I]),
             "rbhip_batch_use_fusion": (_I, [_P, _I]),
             "rbhip_batch_execute": (_I, [_P, _I]),
@@ -99,6 +100,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_server_run_round": (_I, [_P, _IP]),
             "rbhip_server_status": (_I, [_P, _LLP, _SIZEP, _LLP]),
             "rbhip_server_completed": (_I, [_P, _LLP, _I, _IP]),
+            "rbhip_server_profile": (_I, [_P, _LLP, _DP]),
             "rbhip_server_take": (_I, [_P, ctypes.c_longlong, _DP, _DP, _DP, _DP]),
         }
         for name, (res, args) in sig.items():
@@ -445,6 +447,13 @@ class Server:
         _scheck(lib().rbhip_server_status(self._h, c, ctypes.byref(b), ctypes.byref(bs)))
         return {"waiting": c[0], "resident": c[1], "completed": c[2], "bytes_in_use": b.value,
                 "body_steps": bs.value}
+
+    def profile(self):
+        """Rounds run and host seconds per phase (admit, collect, members, launch)."""
+        r = ctypes.c_longlong()
+        t = np.empty(4)
+        _scheck(lib().rbhip_server_profile(self._h, ctypes.byref(r), _dp(t)))
+        return {"rounds": r.value, "admit_s": t[0], "collect_s": t[1], "members_s": t[2], "launch_s": t[3]}
 
     def completed(self, max_ids=4096):
         ids = (ctypes.c_longlong * max_ids)()
