@@ -245,9 +245,10 @@ int rbhip_server_submit(rbhip_server* server, int numAtoms, const double* masses
                         const double* positions, const double* velocities, const double* tetherK,
                         const double* tetherR0, double stepSize, int rotationMode, long long steps,
                         long long* jobId);
-/* one round: admits waiting jobs while the steps queued by the previous
-   round run, collects the jobs that reached their step count, and queues the
-   next batched advance without waiting for it; *finished = jobs completed */
+/* one round: admits up to 4 waiting jobs (in arrival order, within the
+   budget) while the steps queued by the previous round run, collects the
+   jobs that reached their step count, and queues the next batched advance
+   without waiting for it; *finished = jobs completed */
 int rbhip_server_run_round(rbhip_server* server, int* finished);
 /* counts3 = {waiting, resident, completed and not yet taken}; device bytes of
    the resident jobs; body-steps done so far.  Any output may be NULL. */

@@ -343,8 +343,9 @@ struct rbhip_context {
 
     std::vector<Tile> partTiles(int blk, const std::map<int, int>& lanesByN) const {
         std::vector<Tile> t;
-        for (int f = 0; f < numFree; f += blk)
-            t.push_back(Tile{0, TILE_FREE, f, std::min(blk, numFree - f), 0, 0, 1, 0});
+        const int freeTile = FREE_PER_THREAD * blk;
+        for (int f = 0; f < numFree; f += freeTile)
+            t.push_back(Tile{0, TILE_FREE, f, std::min(freeTile, numFree - f), 0, 0, 1, 0});
         for (const Class& c : classes) {
             auto it = lanesByN.find(c.N);
             int L = it == lanesByN.end() ? 1 : it->second;
@@ -359,8 +360,9 @@ struct rbhip_context {
 
     std::vector<Tile> forceTiles(int blk) const {
         std::vector<Tile> t;
-        for (int a = 0; a < numAtoms; a += blk)
-            t.push_back(Tile{0, TILE_ATOMS, a, std::min(blk, numAtoms - a), 0, 0, 1, 0});
+        const int atomTile = FORCE_PER_THREAD * blk;
+        for (int a = 0; a < numAtoms; a += atomTile)
+            t.push_back(Tile{0, TILE_ATOMS, a, std::min(atomTile, numAtoms - a), 0, 0, 1, 0});
         for (size_t i = 0; i < t.size(); i++) t[i].slot = (int)i;
         return t;
     }
@@ -371,10 +373,12 @@ struct rbhip_context {
             require(t.slot >= 0 && t.slot < maxSlots, "tile slot out of range", RBHIP_E_STATE);
             require(t.count > 0, "empty tile", RBHIP_E_STATE);
             if (t.kind == TILE_FREE) {
-                require(t.count <= blk && t.first >= 0 && t.first + t.count <= numFree, "free tile out of range",
+                require(t.count <= FREE_PER_THREAD * blk && t.first >= 0 && t.first + t.count <= numFree,
+                        "free tile out of range",
                         RBHIP_E_STATE);
             } else if (t.kind == TILE_ATOMS) {
-                require(t.count <= blk && t.first >= 0 && t.first + t.count <= numAtoms, "atom tile out of range",
+                require(t.count <= FORCE_PER_THREAD * blk && t.first >= 0 && t.first + t.count <= numAtoms,
+                        "atom tile out of range",
                         RBHIP_E_STATE);
             } else {
                 require(t.lanes >= 1 && t.lanes <= 64 && (t.lanes & (t.lanes - 1)) == 0 && blk % t.lanes == 0,
@@ -388,6 +392,27 @@ struct rbhip_context {
                             "body tile does not match the body layout", RBHIP_E_STATE);
             }
         }
+    }
+
+    // this context's validated tiles for workgroup size `blk` and its lanes per
+    // body, kept for reuse (a batch concatenates them on every change of its
+    // member list)
+    struct TileCache {
+        int block = -1;
+        std::map<int, int> lanes;
+        std::vector<Tile> part, force;
+    };
+    TileCache tileCache;
+    const TileCache& tilesFor(int blk) {
+        if (tileCache.block != blk || tileCache.lanes != lanes) {
+            tileCache.part = partTiles(blk, lanes);
+            tileCache.force = forceTiles(blk);
+            validateTiles(tileCache.part, blk, maxKeSlots);
+            validateTiles(tileCache.force, blk, maxPeSlots);
+            tileCache.block = blk;
+            tileCache.lanes = lanes;
+        }
+        return tileCache;
     }
 
     // largest workgroup the kernels are compiled for: 512, or 256 for the FP64
@@ -594,18 +619,23 @@ struct rbhip_batch {
         std::vector<DevSys> sys;
         std::vector<Tile> pt, ft;
         memberVersions.assign(members.size(), 0);
+        size_t np = 0, nf = 0;
+        for (rbhip_context* c : members) {
+            const auto& tc = c->tilesFor(block);
+            np += tc.part.size();
+            nf += tc.force.size();
+        }
+        pt.reserve(np);
+        ft.reserve(nf);
         for (size_t m = 0; m < members.size(); m++) {
             rbhip_context* c = members[m];
             sys.push_back(c->h);
             memberVersions[m] = c->sysVersion;
-            std::vector<Tile> p = c->partTiles(block, c->lanes);
-            std::vector<Tile> f = c->forceTiles(block);
-            c->validateTiles(p, block, c->maxKeSlots);
-            c->validateTiles(f, block, c->maxPeSlots);
-            for (Tile& t : p) { t.sys = (int)m; pt.push_back(t); }
-            for (Tile& t : f) { t.sys = (int)m; ft.push_back(t); }
-            memberPartTiles[m] = (int)p.size();
-            memberForceTiles[m] = (int)f.size();
+            const auto& tc = c->tilesFor(block);
+            for (Tile t : tc.part) { t.sys = (int)m; pt.push_back(t); }
+            for (Tile t : tc.force) { t.sys = (int)m; ft.push_back(t); }
+            memberPartTiles[m] = (int)tc.part.size();
+            memberForceTiles[m] = (int)tc.force.size();
         }
         dSystems = arena.allocT<DevSys>(sys.size());
         dPartTiles = arena.allocT<Tile>(pt.size() + 1);
@@ -814,6 +844,14 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
                 C.classes.push_back(rbhip_context::Class{N, ib, 0, a});
             C.classes.back().count++;
             a += N;
+        }
+
+        // lanes per body: one lane per body up to 7 atoms, then a power of
+        // two that leaves each lane 4-7 atoms (shorter serial loops)
+        for (const auto& cl : C.classes) {
+            int L = 1;
+            while (L < 16 && cl.N >= 8 * L) L <<= 1;
+            if (L > 1) C.lanes[cl.N] = L;
         }
 
         // ---- device allocation ----
@@ -1481,9 +1519,11 @@ int rbhip_batch_set_members(rbhip_batch* batch, int count, rbhip_context* const*
             require((m[i]->rotationMode == 0) == (m[0]->rotationMode == 0), "members must share the rotation kind");
             for (auto& kv : m[i]->lanes)
                 require(batch->block % kv.second == 0, "member lanes per body must divide the batch block size");
-            HIPCHK(hipStreamSynchronize(m[i]->stream));
-            m[i]->fixKick();
-            HIPCHK(hipStreamSynchronize(m[i]->stream));
+            if (m[i]->kickDt != m[i]->dt) {
+                HIPCHK(hipStreamSynchronize(m[i]->stream));
+                m[i]->fixKick();
+                HIPCHK(hipStreamSynchronize(m[i]->stream));
+            }
         }
         if (!m.empty()) {
             batch->precision = m[0]->precision;

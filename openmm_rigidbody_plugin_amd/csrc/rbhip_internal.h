@@ -33,6 +33,13 @@ namespace rbhip {
 
 enum TileKind : int { TILE_FREE = 0, TILE_BODIES = 1, TILE_ATOMS = 2 };
 
+// free-atom tiles cover this many atoms per workgroup thread (the integrator
+// kernels loop over them), so the per-tile work of the kinetic-energy
+// partials is spread over more atoms
+constexpr int FREE_PER_THREAD = 4;
+// atoms per workgroup thread in the force kernel's tiles
+constexpr int FORCE_PER_THREAD = 4;
+
 struct Tile {
     int sys;        // index into the system descriptor table
     int kind;       // TileKind

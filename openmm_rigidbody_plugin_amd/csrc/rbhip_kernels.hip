@@ -229,20 +229,22 @@ template <class Real> __device__ __forceinline__ V3<double> load_rcm(const DevSy
 // ---------------------------------------------------------------------------
 // part 1
 // ---------------------------------------------------------------------------
+// free-atom tiles hold up to FREE_PER_THREAD atoms per thread (strided by
+// the workgroup size, so each pass over the tile is coalesced)
 template <class Real>
 __device__ __forceinline__ void part1_free(const DevSys& S, const Tile& tile) {
-    int i = threadIdx.x;
-    if (i >= tile.count) return;
-    int a = tile.first + i;
     Real halfDt = (Real)(0.5 * S.dt);
-    V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
-    V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
-    Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
-    v = v + f * invm * halfDt;
-    V3<double> r = load_pos<Real>(S, a);
-    r = r + cvt<double>(v) * S.dt;
-    st3<Real>(S.vel, S.atomStride, a, v);
-    store_pos<Real>(S, a, r);
+    for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
+        int a = tile.first + i;
+        V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
+        V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
+        Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
+        v = v + f * invm * halfDt;
+        V3<double> r = load_pos<Real>(S, a);
+        r = r + cvt<double>(v) * S.dt;
+        st3<Real>(S.vel, S.atomStride, a, v);
+        store_pos<Real>(S, a, r);
+    }
 }
 
 template <class Real, bool EXACT>
@@ -282,17 +284,17 @@ __global__ __launch_bounds__(EXACT ? 256 : 512) void k_part1(const DevSys* __res
 // ---------------------------------------------------------------------------
 template <class Real>
 __device__ __forceinline__ void part2_free(const DevSys& S, const Tile& tile, double (&ke)[2]) {
-    int i = threadIdx.x;
-    if (i >= tile.count) return;
-    int a = tile.first + i;
     Real halfDt = (Real)(0.5 * S.dt);
-    V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
-    V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
-    Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
-    v = v + f * invm * halfDt;
-    st3<Real>(S.vel, S.atomStride, a, v);
-    V3<double> vd = cvt<double>(v);
-    ke[0] = dot(vd, vd) / (double)invm;
+    for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
+        int a = tile.first + i;
+        V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
+        V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
+        Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
+        v = v + f * invm * halfDt;
+        st3<Real>(S.vel, S.atomStride, a, v);
+        V3<double> vd = cvt<double>(v);
+        ke[0] += dot(vd, vd) / (double)invm;
+    }
 }
 
 template <class Real>
@@ -357,19 +359,19 @@ __global__ __launch_bounds__(512) void k_part2(const DevSys* __restrict__ system
 // ---------------------------------------------------------------------------
 template <class Real>
 __device__ __forceinline__ void fused_free(const DevSys& S, const Tile& tile) {
-    int i = threadIdx.x;
-    if (i >= tile.count) return;
-    int a = tile.first + i;
     Real halfDt = (Real)(0.5 * S.dt);
-    V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
-    V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
-    Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
-    v = v + f * invm * halfDt;     // end of step n
-    v = v + f * invm * halfDt;     // start of step n+1
-    V3<double> r = load_pos<Real>(S, a);
-    r = r + cvt<double>(v) * S.dt;
-    st3<Real>(S.vel, S.atomStride, a, v);
-    store_pos<Real>(S, a, r);
+    for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
+        int a = tile.first + i;
+        V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
+        V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
+        Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
+        v = v + f * invm * halfDt;     // end of step n
+        v = v + f * invm * halfDt;     // start of step n+1
+        V3<double> r = load_pos<Real>(S, a);
+        r = r + cvt<double>(v) * S.dt;
+        st3<Real>(S.vel, S.atomStride, a, v);
+        store_pos<Real>(S, a, r);
+    }
 }
 
 // part2 of step n and the first half-kick of step n+1 on the body momenta
@@ -484,8 +486,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
         const Tile tile = tiles[t];
         const DevSys& S = systems[tile.sys];
         double e[1] = {0.0};
-        int i = threadIdx.x;
-        if (i < tile.count) {
+        for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
             int a = tile.first + i;
             V3<Real> f = {(Real)0, (Real)0, (Real)0};
             if (S.tetherScale != 0.0) {

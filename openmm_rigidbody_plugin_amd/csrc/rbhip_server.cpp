@@ -86,6 +86,8 @@ double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::d
 
 }  // namespace
 
+constexpr int kAdmitPerRound = 4;
+
 struct rbhip_server {
     int device = 0;
     size_t budget = 0;
@@ -152,8 +154,12 @@ struct rbhip_server {
     int round() {
         rounds++;
         Clock::time_point t0 = Clock::now();
-        // admission in arrival order while the budget holds
-        while (!queue.empty() && inUse + queue.front()->bytes <= budget) {
+        // admission in arrival order while the budget holds, a few jobs per
+        // round so the device starts on the first ones while later ones are
+        // set up
+        int admitted = 0;
+        while (!queue.empty() && inUse + queue.front()->bytes <= budget && admitted < kAdmitPerRound) {
+            admitted++;
             std::unique_ptr<Job> j = std::move(queue.front());
             queue.pop_front();
             admit(*j);

@@ -191,10 +191,10 @@ __global__ __launch_bounds__(256) void k_setup(const DevSys* __restrict__ system
         double v[3] = {0.0, 0.0, 0.0};
         int i = threadIdx.x;
         if (tile.kind == TILE_FREE) {
-            if (i < tile.count) {
-                int a = tile.first + i;
+            for (int f = i; f < tile.count; f += blockDim.x) {
+                int a = tile.first + f;
                 V3<double> vel = setup_vel<Real>(S, a);
-                v[0] = dot(vel, vel) / (double)reinterpret_cast<const Real*>(S.freeInvM)[a];
+                v[0] += dot(vel, vel) / (double)reinterpret_cast<const Real*>(S.freeInvM)[a];
             }
         } else if (i < tile.count) {
             setup_body<Real>(S, tile.first + i, tile.N, tile.atomFirst + i * tile.N, geometry, v);
