@@ -177,7 +177,7 @@ def batched(steps, warmup, count=256, log_every=64):
     }
 
 
-def served(num_jobs=96, rate=25.0, budget=1 << 30, steps_per_round=100, seed=11):
+def served(num_jobs=96, rate=25.0, budget=1 << 30, steps_per_round=250, seed=11):
     """Serving: a Poisson stream of jobs (bodies ~ LogUniform(500, 1e5) with the
     heterogeneous body mix and 2.5 free atoms per body, steps ~
     LogUniform(100, 1e4), exact rotation, dt 1 fs, mixed) submitted to the job

@@ -80,7 +80,10 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
 int rbhip_destroy(rbhip_context* ctx);
 /* launch stream for all work of this context (hipStream_t; NULL restores the
    stream the library assigned, one of a per-device pool of non-blocking
-   streams shared by contexts and batches) */
+   streams shared by contexts and batches).  Work queued on the previous
+   stream completes before work on the new one; the call does not wait for
+   it, unless the context holds captured graphs (then it waits for the old
+   stream and drops them). */
 int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream);
 /* device bytes held by the context (for admission against an HBM budget) */
 int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes);
@@ -222,8 +225,9 @@ int rbhip_batch_describe(rbhip_batch* batch, char* buf, int len);
 typedef struct rbhip_server rbhip_server;
 
 /* device bytes rbhip_create would hold for this system (an upper bound of
-   what rbhip_device_bytes reports before bonds are added) */
-int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, size_t* bytes);
+   what rbhip_device_bytes reports before bonds are added) and, if numBodies
+   is not NULL, the number of rigid bodies */
+int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, size_t* bytes, int* numBodies);
 
 /* A server on `device`: jobs wait in arrival order and are admitted while the
    device bytes of the resident jobs stay within hbmBudget; resident jobs are
@@ -245,7 +249,7 @@ int rbhip_server_submit(rbhip_server* server, int numAtoms, const double* masses
                         const double* positions, const double* velocities, const double* tetherK,
                         const double* tetherR0, double stepSize, int rotationMode, long long steps,
                         long long* jobId);
-/* one round: admits up to 4 waiting jobs (in arrival order, within the
+/* one round: admits up to 8 waiting jobs (in arrival order, within the
    budget) while the steps queued by the previous round run, collects the
    jobs that reached their step count, and queues the next batched advance
    without waiting for it; *finished = jobs completed */

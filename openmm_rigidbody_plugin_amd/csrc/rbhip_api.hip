@@ -105,21 +105,28 @@ std::vector<int> clean_body_indices(const int* in, int n) {
     return out;
 }
 
+// Device memory of one owner.  With a stream set, blocks are allocated and
+// freed in that stream's order (hipMallocAsync / hipFreeAsync from the
+// device's memory pool, which keeps freed memory for reuse), so creating or
+// destroying a context never waits for work queued on other streams; the
+// owner's device work runs on that stream or is ordered after it by events.
+// Without a stream the blocks use hipMalloc / hipFree.
 struct DeviceArena {
     std::vector<void*> blocks;
     size_t bytes = 0;
+    hipStream_t stream = nullptr;
     void* alloc(size_t n) {
         if (n == 0) n = 256;
         void* p = nullptr;
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) throw RbError(RBHIP_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        hipError_t e = stream ? hipMallocAsync(&p, n, stream) : hipMalloc(&p, n);
+        if (e != hipSuccess) throw RbError(RBHIP_E_NOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
         blocks.push_back(p);
         bytes += n;
         return p;
     }
     template <class T> T* allocT(size_t count) { return reinterpret_cast<T*>(alloc(count * sizeof(T))); }
     void release() {
-        for (void* p : blocks) (void)hipFree(p);
+        for (void* p : blocks) (void)(stream ? hipFreeAsync(p, stream) : hipFree(p));
         blocks.clear();
         bytes = 0;
     }
@@ -141,6 +148,13 @@ hipStream_t pooled_stream(int device) {
             hipStream_t st = nullptr;
             HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
             v.push_back(st);
+        }
+        // the stream-ordered allocations keep freed memory in the device's
+        // pool for the next context instead of returning it
+        hipMemPool_t mp = nullptr;
+        if (hipDeviceGetDefaultMemPool(&mp, device) == hipSuccess && mp) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep);
         }
     }
     return v[next[device]++ % v.size()];
@@ -278,7 +292,8 @@ struct rbhip_context {
     double* dTmp = nullptr;
     size_t tmpCount = 0;
     double* dRed = nullptr;
-    double* hRed = nullptr;                      // pinned
+    double hRed[4] = {0.0, 0.0, 0.0, 0.0};
+    hipEvent_t handoff = nullptr;                // orders a new stream after the old one
 
     bool built = false;
     bool stageVFresh = false;
@@ -315,6 +330,7 @@ struct rbhip_context {
         ~Snapshot() { arena.release(); }
     };
     void takeSnapshot(Snapshot& s) {
+        s.arena.stream = stream;
         const size_t R = realSize();
         const size_t nb = std::max(numBodies, 1), as = (size_t)h.atomStride;
         std::vector<std::pair<void*, size_t>> spans = {
@@ -581,9 +597,9 @@ struct rbhip_context {
     ~rbhip_context() {
         graph.reset();
         fusedGraph.reset();
-        if (hRed) (void)hipHostFree(hRed);
         tileArena.release();
         arena.release();
+        if (handoff) (void)hipEventDestroy(handoff);
     }
 };
 
@@ -642,9 +658,12 @@ struct rbhip_batch {
         dForceTiles = arena.allocT<Tile>(ft.size() + 1);
         dKeSlots = arena.allocT<int>(members.size());
         dRed = arena.allocT<double>(4 * members.size());
-        HIPCHK(hipMemcpy(dSystems, sys.data(), sys.size() * sizeof(DevSys), hipMemcpyHostToDevice));
-        if (!pt.empty()) HIPCHK(hipMemcpy(dPartTiles, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
-        if (!ft.empty()) HIPCHK(hipMemcpy(dForceTiles, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(dSystems, sys.data(), sys.size() * sizeof(DevSys), hipMemcpyHostToDevice, stream));
+        if (!pt.empty())
+            HIPCHK(hipMemcpyAsync(dPartTiles, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+        if (!ft.empty())
+            HIPCHK(hipMemcpyAsync(dForceTiles, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         nPart = (int)pt.size();
         nForce = (int)ft.size();
     }
@@ -858,6 +877,7 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         if (rotationMode == 0) C.block = 64;   // the FP64 exact rotation is register heavy
         C.ownStream = pooled_stream(device);
         C.stream = C.ownStream;
+        C.arena.stream = C.tileArena.stream = C.stream;
         const size_t R = C.realSize();
         const size_t nb = std::max(C.numBodies, 1), na = (size_t)numAtoms;
         DevSys& h = C.h;
@@ -905,7 +925,6 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         C.tmpCount = std::max(3 * na, 36 * nb);
         C.dTmp = C.arena.allocT<double>(C.tmpCount);
         C.dRed = C.arena.allocT<double>(4);
-        HIPCHK(hipHostMalloc((void**)&C.hRed, 4 * sizeof(double), hipHostMallocDefault));
         require(C.arena.bytes == context_arena_bytes(na, C.numBodies, C.numFree, C.numBodyAtoms,
                                                      (int)C.classes.size(), R),
                 "internal: device size estimate out of step with the allocation", RBHIP_E_STATE);
@@ -955,7 +974,9 @@ int rbhip_destroy(rbhip_context* ctx) {
     return guard([&] {
         if (!ctx) return;
         (void)hipSetDevice(ctx->device);
-        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        // device memory is freed in the context's stream order; only
+        // instantiated graphs wait for the stream before they go
+        if (ctx->graph.exec || ctx->fusedGraph.exec) (void)hipStreamSynchronize(ctx->stream);
         delete ctx;
     });
 }
@@ -963,10 +984,22 @@ int rbhip_destroy(rbhip_context* ctx) {
 int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream) {
     return guard([&] {
         require(ctx, "null context");
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        ctx->stream = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->ownStream;
-        ctx->graph.reset();
-        ctx->fusedGraph.reset();
+        hipStream_t next = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->ownStream;
+        if (next == ctx->stream) return;
+        HIPCHK(hipSetDevice(ctx->device));
+        if (ctx->graph.exec || ctx->fusedGraph.exec) {
+            // captured graphs may still be running on the old stream
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->graph.reset();
+            ctx->fusedGraph.reset();
+        } else {
+            // the new stream starts after the work queued on the old one
+            if (!ctx->handoff) HIPCHK(hipEventCreateWithFlags(&ctx->handoff, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(ctx->handoff, ctx->stream));
+            HIPCHK(hipStreamWaitEvent(next, ctx->handoff, 0));
+        }
+        ctx->stream = next;
+        ctx->arena.stream = ctx->tileArena.stream = ctx->stream;
     });
 }
 
@@ -977,7 +1010,7 @@ int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes) {
     });
 }
 
-int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, size_t* bytes) {
+int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, size_t* bytes, int* numBodiesOut) {
     return guard([&] {
         require(numAtoms > 0 && bodyIndices && bytes, "bad arguments");
         require(precision == RBHIP_PRECISION_MIXED || precision == RBHIP_PRECISION_DOUBLE, "unknown precision");
@@ -995,6 +1028,7 @@ int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, si
         size_t R = precision == RBHIP_PRECISION_MIXED ? 4 : 8;
         *bytes = context_arena_bytes(numAtoms, numBodies, numFree, numAtoms - numFree, classes, R) +
                  tile_bytes_bound(numAtoms, numBodies, numFree, classes);
+        if (numBodiesOut) *numBodiesOut = numBodies;
     });
 }
 
@@ -1039,16 +1073,18 @@ int rbhip_set_tether(rbhip_context* ctx, const double* k, const double* r0) {
                     hi[c * as + i] = (float)v;
                     lo[c * as + i] = (float)(v - (double)hi[c * as + i]);
                 }
-            HIPCHK(hipMemcpy(C.h.tetherK, kk.data(), na * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(C.h.r0, hi.data(), 3 * as * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(C.h.r0Lo, lo.data(), 3 * as * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpyAsync(C.h.tetherK, kk.data(), na * 4, hipMemcpyHostToDevice, C.stream));
+            HIPCHK(hipMemcpyAsync(C.h.r0, hi.data(), 3 * as * 4, hipMemcpyHostToDevice, C.stream));
+            HIPCHK(hipMemcpyAsync(C.h.r0Lo, lo.data(), 3 * as * 4, hipMemcpyHostToDevice, C.stream));
+            HIPCHK(hipStreamSynchronize(C.stream));
         } else {
             std::vector<double> kk(na), rr(3 * as, 0.0);
             for (size_t i = 0; i < na; i++) kk[i] = k[C.atomOrig[i]];
             for (size_t i = 0; i < na; i++)
                 for (int c = 0; c < 3; c++) rr[c * as + i] = r[3 * i + c];
-            HIPCHK(hipMemcpy(C.h.tetherK, kk.data(), na * 8, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(C.h.r0, rr.data(), 3 * as * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpyAsync(C.h.tetherK, kk.data(), na * 8, hipMemcpyHostToDevice, C.stream));
+            HIPCHK(hipMemcpyAsync(C.h.r0, rr.data(), 3 * as * 8, hipMemcpyHostToDevice, C.stream));
+            HIPCHK(hipStreamSynchronize(C.stream));
         }
         C.h.tetherScale = 1.0;
         C.h.tetherUniform = 1;
@@ -1094,11 +1130,11 @@ int rbhip_add_bonds(rbhip_context* ctx, int count, const int* atomPairs, const d
         C.h.bondOther = C.arena.allocT<int>(std::max<size_t>(ne, 1));
         C.h.bondLen = C.arena.allocT<double>(std::max<size_t>(ne, 1));
         C.h.bondK = C.arena.allocT<double>(std::max<size_t>(ne, 1));
-        HIPCHK(hipMemcpy(C.h.bondStart, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(C.h.bondStart, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, C.stream));
         if (ne) {
-            HIPCHK(hipMemcpy(C.h.bondOther, other.data(), ne * sizeof(int), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(C.h.bondLen, len.data(), ne * sizeof(double), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(C.h.bondK, kk.data(), ne * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpyAsync(C.h.bondOther, other.data(), ne * sizeof(int), hipMemcpyHostToDevice, C.stream));
+            HIPCHK(hipMemcpyAsync(C.h.bondLen, len.data(), ne * sizeof(double), hipMemcpyHostToDevice, C.stream));
+            HIPCHK(hipMemcpyAsync(C.h.bondK, kk.data(), ne * sizeof(double), hipMemcpyHostToDevice, C.stream));
         }
         C.uploadSys();
         HIPCHK(hipStreamSynchronize(C.stream));
@@ -1286,10 +1322,12 @@ int rbhip_get_body_fixed_positions(rbhip_context* ctx, double* xyz) {
         HIPCHK(hipStreamSynchronize(C.stream));
         if (C.precision == RBHIP_PRECISION_MIXED) {
             std::vector<float> f(3 * ds);
-            HIPCHK(hipMemcpy(f.data(), C.h.d, 3 * ds * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpyAsync(f.data(), C.h.d, 3 * ds * 4, hipMemcpyDeviceToHost, C.stream));
+            HIPCHK(hipStreamSynchronize(C.stream));
             for (size_t i = 0; i < 3 * ds; i++) soa[i] = f[i];
         } else {
-            HIPCHK(hipMemcpy(soa.data(), C.h.d, 3 * ds * 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpyAsync(soa.data(), C.h.d, 3 * ds * 8, hipMemcpyDeviceToHost, C.stream));
+            HIPCHK(hipStreamSynchronize(C.stream));
         }
         for (size_t i = 0; i < nba; i++)
             for (int c = 0; c < 3; c++) d[3 * i + c] = soa[c * ds + i];
@@ -1491,6 +1529,7 @@ int rbhip_batch_create(int device, rbhip_batch** out) {
         std::unique_ptr<rbhip_batch> b(new rbhip_batch());
         b->device = device;
         b->stream = pooled_stream(device);
+        b->arena.stream = b->stream;
         HIPCHK(hipEventCreateWithFlags(&b->ordering, hipEventDisableTiming));
         *out = b.release();
     });
@@ -1541,6 +1580,7 @@ int rbhip_batch_set_stream(rbhip_batch* batch, rbhip_stream stream) {
         HIPCHK(hipSetDevice(batch->device));
         HIPCHK(hipStreamSynchronize(batch->stream));
         batch->stream = stream ? reinterpret_cast<hipStream_t>(stream) : pooled_stream(batch->device);
+        batch->arena.stream = batch->stream;
         batch->graph.reset();
         batch->fusedGraph.reset();
     });

@@ -15,9 +15,10 @@
 // on which other jobs share its rounds.  A round returns once its steps are
 // queued: the next round's admissions overlap with them on the device.
 //
-// Streams: the batch runs on a stream of its own, the resident jobs share a
-// second one (the batch orders it after each advance) and admissions use a
-// third, so setting up a new job never waits behind the running steps.
+// Streams: each of the two batches runs on a stream of its own, the
+// resident jobs share one (the batch orders it after each advance),
+// admissions use another and finished jobs a last one, so neither setting up
+// a new job nor collecting a finished one waits behind the queued steps.
 #include <algorithm>
 #include <chrono>
 #include <deque>
@@ -86,7 +87,7 @@ double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::d
 
 }  // namespace
 
-constexpr int kAdmitPerRound = 4;
+constexpr int kAdmitPerRound = 8;
 
 struct rbhip_server {
     int device = 0;
@@ -104,17 +105,28 @@ struct rbhip_server {
     std::vector<std::unique_ptr<Job>> resident;     // batch members
     std::vector<std::unique_ptr<Job>> pending;      // admitted this round, members from the next launch
     std::map<long long, std::unique_ptr<Job>> completed;
-    rbhip_batch* batch = nullptr;
-    hipStream_t batchStream = nullptr, memberStream = nullptr, admitStream = nullptr;
+    // two batches: one runs the queued round while the other takes a new
+    // member list; members[b] is the list batch b was last given
+    rbhip_batch* batches[2] = {nullptr, nullptr};
+    std::vector<rbhip_context*> members[2];
+    int cur = 0;
+    hipStream_t batchStreams[2] = {nullptr, nullptr};
+    hipStream_t memberStream = nullptr, admitStream = nullptr, retireStream = nullptr;
 
     ~rbhip_server() {
-        if (batch) (void)rbhip_batch_destroy(batch);
+        for (hipStream_t st : {batchStreams[0], batchStreams[1], memberStream, admitStream, retireStream})
+            if (st) (void)hipStreamSynchronize(st);
+        for (rbhip_batch* b : batches)
+            if (b) (void)rbhip_batch_destroy(b);
         for (auto& j : resident)
             if (j->ctx) (void)rbhip_destroy(j->ctx);
         for (auto& j : pending)
             if (j->ctx) (void)rbhip_destroy(j->ctx);
-        for (hipStream_t st : {batchStream, memberStream, admitStream})
-            if (st) (void)hipStreamDestroy(st);
+        for (hipStream_t st : {batchStreams[0], batchStreams[1], memberStream, admitStream, retireStream})
+            if (st) {
+                (void)hipStreamSynchronize(st);
+                (void)hipStreamDestroy(st);
+            }
     }
 
     void admit(Job& j) {
@@ -146,17 +158,19 @@ struct rbhip_server {
         std::vector<int>().swap(j.bodyIndices);
     }
 
-    // One round.  The previous round's steps run on the device while this
-    // call admits waiting jobs (host set-up and their own set-up kernels);
-    // then the jobs that reached their step count are collected, the batch
-    // takes its new member list and the next steps are queued without
-    // waiting for them.
+    // One round.
+    //  1. Admission: waiting jobs are set up while the round queued by the
+    //     previous call runs on the device.
+    //  2. Jobs that reached their step count move to the retire stream,
+    //     which then continues after the round they finished in.
+    //  3. The next round is queued: on the same batch if the member list is
+    //     unchanged, else on the other batch (idle, so its tables can be
+    //     rebuilt while the queued round runs); it starts when that round ends.
+    //  4. The finished jobs are collected (their downloads wait only for the
+    //     round they finished in) and their device memory is released.
     int round() {
         rounds++;
         Clock::time_point t0 = Clock::now();
-        // admission in arrival order while the budget holds, a few jobs per
-        // round so the device starts on the first ones while later ones are
-        // set up
         int admitted = 0;
         while (!queue.empty() && inUse + queue.front()->bytes <= budget && admitted < kAdmitPerRound) {
             admitted++;
@@ -167,57 +181,55 @@ struct rbhip_server {
             pending.push_back(std::move(j));
         }
         Clock::time_point t1 = Clock::now();
-        // collect finished jobs (waits for the steps queued last round)
-        int finished = 0;
-        std::vector<std::unique_ptr<Job>> keep, done;
-        for (auto& j : resident) {
-            if (j->done >= j->steps) {
-                j->outPos.resize(3 * (size_t)j->numAtoms);
-                j->outVel.resize(3 * (size_t)j->numAtoms);
-                call(rbhip_get_positions(j->ctx, j->outPos.data()));
-                call(rbhip_get_velocities(j->ctx, j->outVel.data()));
-                call(rbhip_kinetic_energies(j->ctx, j->ke));
-                j->finished = Clock::now();
-                done.push_back(std::move(j));
-            } else {
-                keep.push_back(std::move(j));
+        std::vector<std::unique_ptr<Job>> next, done;
+        for (auto& j : resident) (j->done >= j->steps ? done : next).push_back(std::move(j));
+        for (auto& j : pending) next.push_back(std::move(j));
+        pending.clear();
+        for (auto& j : done) call(rbhip_set_stream(j->ctx, retireStream));
+        std::vector<rbhip_context*> m;
+        for (auto& j : next) m.push_back(j->ctx);
+        if (!m.empty() && m != members[cur]) {
+            cur = 1 - cur;
+            if (m != members[cur]) {
+                call(rbhip_batch_set_members(batches[cur], (int)m.size(), m.data()));
+                members[cur] = m;
             }
         }
-        for (auto& j : pending) keep.push_back(std::move(j));
-        pending.clear();
         Clock::time_point t2 = Clock::now();
-        const bool changed = !done.empty() || keep.size() != resident.size();
-        resident.swap(keep);
-        if (changed) {
-            std::vector<rbhip_context*> m;
-            for (auto& j : resident) m.push_back(j->ctx);
-            call(rbhip_batch_set_members(batch, (int)m.size(), m.data()));
+        long long n = stepsPerRound;
+        for (auto& j : next) n = std::min(n, j->steps - j->done);
+        if (!next.empty() && n > 0) {
+            call(rbhip_batch_execute(batches[cur], (int)n));
+            for (auto& j : next) {
+                j->done += n;
+                bodySteps += j->bodies * n;
+            }
         }
-        // the finished jobs are no longer members: free their device state
+        resident.swap(next);
+        Clock::time_point t3 = Clock::now();
+        int finished = 0;
         for (auto& j : done) {
+            j->outPos.resize(3 * (size_t)j->numAtoms);
+            j->outVel.resize(3 * (size_t)j->numAtoms);
+            call(rbhip_get_positions(j->ctx, j->outPos.data()));
+            call(rbhip_get_velocities(j->ctx, j->outVel.data()));
+            call(rbhip_kinetic_energies(j->ctx, j->ke));
+            j->finished = Clock::now();
             call(rbhip_destroy(j->ctx));
             j->ctx = nullptr;
             inUse -= std::min(inUse, j->bytes);
             completed[j->id] = std::move(j);
             finished++;
         }
-        Clock::time_point t3 = Clock::now();
-        // queue the next steps: all resident jobs advance together, never
-        // past the end of the one that finishes first
-        long long n = stepsPerRound;
-        for (auto& j : resident) n = std::min(n, j->steps - j->done);
-        if (!resident.empty() && n > 0) {
-            call(rbhip_batch_execute(batch, (int)n));
-            for (auto& j : resident) {
-                j->done += n;
-                bodySteps += j->bodies * n;
-            }
-        }
+        // a batch whose member list held finished jobs forgets it (their
+        // handles are gone)
+        for (int b = 0; b < 2; b++)
+            if (b != cur || resident.empty()) members[b].clear();
         Clock::time_point t4 = Clock::now();
         phase[0] += seconds(t0, t1);
-        phase[1] += seconds(t1, t2);
-        phase[2] += seconds(t2, t3);
-        phase[3] += seconds(t3, t4);
+        phase[1] += seconds(t3, t4);
+        phase[2] += seconds(t1, t2);
+        phase[3] += seconds(t2, t3);
         return finished;
     }
 };
@@ -241,12 +253,15 @@ int rbhip_server_create(int device, size_t hbmBudget, int stepsPerRound, int pre
         s->stepsPerRound = stepsPerRound;
         s->precision = precision;
         s->block = blockSize;
-        call(rbhip_batch_create(device, &s->batch));
-        for (hipStream_t* st : {&s->batchStream, &s->memberStream, &s->admitStream})
+        for (hipStream_t* st : {&s->batchStreams[0], &s->batchStreams[1], &s->memberStream, &s->admitStream,
+                                &s->retireStream})
             if (hipStreamCreateWithFlags(st, hipStreamNonBlocking) != hipSuccess)
                 throw ServerError(RBHIP_E_HIP, "stream creation failed");
-        call(rbhip_batch_set_stream(s->batch, s->batchStream));
-        call(rbhip_batch_set_launch(s->batch, blockSize, 0));
+        for (int b = 0; b < 2; b++) {
+            call(rbhip_batch_create(device, &s->batches[b]));
+            call(rbhip_batch_set_stream(s->batches[b], s->batchStreams[b]));
+            call(rbhip_batch_set_launch(s->batches[b], blockSize, 0));
+        }
         *out = s.release();
     });
 }
@@ -269,7 +284,8 @@ int rbhip_server_submit(rbhip_server* s, int numAtoms, const double* masses, con
              "jobs of one server share the rotation kind (exact or NO-SQUISH)", RBHIP_E_UNSUPPORTED);
         need(!(kind == 1 && s->block > 256), "exact-rotation jobs need a block size <= 256", RBHIP_E_UNSUPPORTED);
         size_t bytes = 0;
-        call(rbhip_estimate_bytes(numAtoms, bodyIndices, s->precision, &bytes));
+        int numBodies = 0;
+        call(rbhip_estimate_bytes(numAtoms, bodyIndices, s->precision, &bytes, &numBodies));
         need(bytes <= s->budget, "job needs " + std::to_string(bytes) + " device bytes, more than the whole budget",
              RBHIP_E_NOMEM);
         std::unique_ptr<Job> j(new Job());
@@ -283,11 +299,7 @@ int rbhip_server_submit(rbhip_server* s, int numAtoms, const double* masses, con
             j->tetherK.assign(tetherK, tetherK + numAtoms);
             j->tetherR0.assign(tetherR0, tetherR0 + 3 * (size_t)numAtoms);
         }
-        std::vector<int> labels;
-        for (int i = 0; i < numAtoms; i++)
-            if (bodyIndices[i] > 0) labels.push_back(bodyIndices[i]);
-        std::sort(labels.begin(), labels.end());
-        j->bodies = std::unique(labels.begin(), labels.end()) - labels.begin();
+        j->bodies = numBodies;
         j->stepSize = stepSize;
         j->rotationMode = rotationMode;
         j->steps = steps;

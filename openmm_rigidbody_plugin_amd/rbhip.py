@@ -92,7 +92,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_batch_time_kernels": (_I, [_P, _I, _DP, _DP]),
             "rbhip_batch_autotune": (_I, [_P, _I]),
             "rbhip_batch_describe": (_I, [_P, ctypes.c_char_p, _I]),
-            "rbhip_estimate_bytes": (_I, [_I, _IP, _I, _SIZEP]),
+            "rbhip_estimate_bytes": (_I, [_I, _IP, _I, _SIZEP, _IP]),
             "rbhip_server_last_error": (ctypes.c_char_p, []),
             "rbhip_server_create": (_I, [_I, ctypes.c_size_t, _I, _I, _I, ctypes.POINTER(_P)]),
             "rbhip_server_destroy": (_I, [_P]),
@@ -388,7 +388,7 @@ def estimate_bytes(body_indices, precision="mixed"):
     """Device bytes a context for this system holds (upper bound before bonds)."""
     bi = np.ascontiguousarray(body_indices, dtype=np.int32)
     n = ctypes.c_size_t()
-    _check(lib().rbhip_estimate_bytes(bi.size, _ip(bi), PRECISIONS[precision], ctypes.byref(n)))
+    _check(lib().rbhip_estimate_bytes(bi.size, _ip(bi), PRECISIONS[precision], ctypes.byref(n), None))
     return n.value
 
 
