@@ -274,6 +274,28 @@ __device__ inline double carlson_rf(double x, double y, double z) {
     return (1.0 - E2 / 10.0 + E3 / 14.0 + E2 * E2 / 24.0 - 3.0 * E2 * E3 / 44.0) / sqrt(Am);
 }
 
+// R_C(1, 1 + e) in closed form (DLMF 19.2.17-19.2.18): atan(sqrt(e))/sqrt(e)
+// for e > 0, atanh(sqrt(-e))/sqrt(-e) for -1 < e < 0 (1 - sqrt(-e) written
+// as (1 + e)/(1 + sqrt(-e)) against cancellation).  Not inlined: the caller
+// takes this branch rarely and keeps its own registers free of it.
+__device__ __noinline__ double carlson_rc_one_closed(double e) {
+    if (e > 0.0) {
+        double t = sqrt(e);
+        return atan(t) / t;
+    }
+    double t = sqrt(-e);
+    return 0.5 * log1p(2.0 * t * (1.0 + t) / (1.0 + e)) / t;
+}
+
+// R_C(1, 1 + e): Taylor series 1 - e/3 + e^2/5 - e^3/7 + e^4/9 for |e| < 1e-3
+// (truncation below 1e-16), the closed form otherwise.  In the duplication
+// loop of R_J, e shrinks by about 64 per step, so after the first steps the
+// series applies.
+__device__ __forceinline__ double carlson_rc_one(double e) {
+    if (fabs(e) < 1e-3) return 1.0 + e * (-1.0 / 3.0 + e * (1.0 / 5.0 + e * (-1.0 / 7.0 + e * (1.0 / 9.0))));
+    return carlson_rc_one_closed(e);
+}
+
 __device__ inline double carlson_rj(double x, double y, double z, double p) {
     double A0 = (x + y + z + 2.0 * p) * 0.2;
     double delta = (p - x) * (p - y) * (p - z);
@@ -285,7 +307,7 @@ __device__ inline double carlson_rj(double x, double y, double z, double p) {
         double lam = sx * sy + sx * sz + sy * sz;
         double dm = (sp + sx) * (sp + sy) * (sp + sz);
         double e = scale3 * delta / (dm * dm);
-        sum += scale * carlson_rc(1.0, 1.0 + e) / dm;
+        sum += scale * carlson_rc_one(e) / dm;
         xm = 0.25 * (xm + lam);
         ym = 0.25 * (ym + lam);
         zm = 0.25 * (zm + lam);
@@ -318,19 +340,28 @@ __device__ inline void jacobi_fn(double u, double m, double K, double& sn, doubl
     if (m < 2.0 * eps) { sincos(u, &sn, &cn); dn = 1.0; return; }
     if (1.0 - m < 2.0 * eps) { sn = tanh(u); cn = 1.0 / cosh(u); dn = cn; return; }
     if (fabs(u) > 2.0 * K) u -= 4.0 * K * rint(u / (4.0 * K));
-    double a[25], c[25];
-    a[0] = 1.0;
-    c[0] = sqrt(m);
-    double b = sqrt(1.0 - m);
+    // AGM levels with fixed trip counts so the ratios c_i / a_i stay in
+    // registers; 10 levels converge for any m < 1 - 2 eps (the AGM doubles
+    // the number of correct digits per level)
+    constexpr int kMaxLevels = 10;
+    double ratio[kMaxLevels + 1];
+    double a = 1.0, c = sqrt(m), b = sqrt(1.0 - m);
     int n = 0;
-    while (fabs(c[n]) > eps * a[n] && n < 24) {
-        a[n + 1] = 0.5 * (a[n] + b);
-        c[n + 1] = 0.5 * (a[n] - b);
-        b = sqrt(a[n] * b);
-        n++;
+#pragma unroll
+    for (int i = 0; i < kMaxLevels; i++) {
+        if (fabs(c) > eps * a) {
+            double an = 0.5 * (a + b);
+            c = 0.5 * (a - b);
+            b = sqrt(a * b);
+            a = an;
+            n = i + 1;
+        }
+        ratio[i + 1] = c / a;
     }
-    double phi = ldexp(a[n] * u, n);
-    for (int i = n; i > 0; i--) phi = 0.5 * (phi + asin(c[i] / a[i] * sin(phi)));
+    double phi = ldexp(a * u, n);
+#pragma unroll
+    for (int i = kMaxLevels; i > 0; i--)
+        if (i <= n) phi = 0.5 * (phi + asin(ratio[i] * sin(phi)));
     sincos(phi, &sn, &cn);
     dn = sqrt((1.0 - m) + m * cn * cn);
 }
