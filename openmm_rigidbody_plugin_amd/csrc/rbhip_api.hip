@@ -1593,6 +1593,9 @@ int rbhip_batch_set_launch(rbhip_batch* batch, int blockSize, int maxBlocks) {
         require(blockSize >= 64 && blockSize <= cap && blockSize % 64 == 0,
                 "block size must be a multiple of 64 in [64, " + std::to_string(cap) + "]");
         require(maxBlocks >= 0, "negative grid cap");
+        for (rbhip_context* c : batch->members)
+            for (auto& kv : c->lanes)
+                require(blockSize % kv.second == 0, "member lanes per body must divide the batch block size");
         HIPCHK(hipSetDevice(batch->device));
         HIPCHK(hipStreamSynchronize(batch->stream));
         batch->block = blockSize;
