@@ -111,15 +111,17 @@ def test_single_bond_known_answer_on_gpu(hip):
         assert g.time()[0] == pytest.approx(10.0, abs=1e-5)
 
 
-def test_energy_drift_matches_reference(hip, oracle):
-    """10^4 steps of the 512-water + 64-argon system (NO-SQUISH n = 10).
+@pytest.mark.parametrize("mode", [10, 0])
+def test_energy_drift_matches_reference(hip, oracle, mode):
+    """10^4 steps of the 512-water + 64-argon system (NO-SQUISH n = 10 and
+    the exact rotation).
 
     The tethered rigid bodies are chaotic: after ~10^3 steps any two runs
     (FP64 GPU included) decorrelate, so the energy records are compared as
     statistics, sampled every 100 steps: same mean energy, same fluctuation,
     and a fitted drift over the run that is small against the fluctuation in
     both and equal within it."""
-    spec = systems.water_c1(rotation_mode=10)
+    spec = systems.water_c1(rotation_mode=mode)
     o = oracle.OracleSim.from_spec(spec)
     g = hip.Context.from_spec(spec, precision="mixed")
     eo, eg = [], []
