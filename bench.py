@@ -114,6 +114,14 @@ def cpu_baseline(spec, budget_s=15.0, sample_bodies=100_000):
                       f"(single thread, g++ -O3) on {cpu}", "seconds": round(el, 2)}
 
 
+def valu_from_profiles(config, kernel):
+    path = os.path.join(ROOT, "profiles", f"valu_{kernel}_{config}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("valu_issue_frac")
+
+
 def traffic_from_profiles(config, kernel):
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{config}.json")
     if not os.path.exists(path):
@@ -349,6 +357,7 @@ def main():
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(args.config, dominant),
+            "valu_issue_frac": valu_from_profiles(args.config, dominant),
             "algorithmic_bytes_per_launch": algorithmic_bytes(dominant, natoms_body, nb, nf),
             "avg_launch_ms": kms[dominant],
         },
