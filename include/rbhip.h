@@ -158,6 +158,29 @@ int rbhip_kinetic_energies(rbhip_context* ctx, double* out2);
 /* potential energy of the last force evaluation */
 int rbhip_potential_energy(rbhip_context* ctx, double* energy);
 
+/* Refined energies (RigidBodyIntegrator::setComputeRefinedEnergies,
+   RigidBodyIntegrator.cpp:34-38): 1 = accumulate, on the last step of every
+   rbhip_execute (or each rbhip_part1/rbhip_part2 pair), the CUDA platform's
+   COMPMOD terms (platforms/cuda/src/kernels/rigidbodyintegrator.cu:285-292,
+   318-320, 380-383).  Enabling allocates 32 B per body and 12 B per free
+   atom in single precision (twice that in double); the accumulators start
+   empty and every state change (rbhip_set_positions, rbhip_set_velocities)
+   empties them again, as the reference's upload zeroes them
+   (CudaRigidBodyKernels.cpp:293-335). */
+int rbhip_set_refined_energies(rbhip_context* ctx, int enable);
+/* getRefinedKineticEnergies (RigidBodyKernels.h:92; CudaRigidBodyKernels.cpp:
+   469-477): {translational, rotational} from the position derivative
+   estimated over the last step, divided by 6 dt at the current step size;
+   {0, 0} before the first step; the plain kinetic energies when refinement
+   is off.  Free atoms take the sign of the body terms (the reference's free
+   term, CudaRigidBodyKernels.cpp:411-413, counts the backward displacement
+   with -1, which gives 4/3 of the kinetic energy at constant velocity). */
+int rbhip_refined_kinetic_energies(rbhip_context* ctx, double* out2);
+/* getPotentialEnergyRefinement (RigidBodyKernels.h:98; CudaRigidBodyKernels.cpp:
+   481-494): -dt^2/24 [sum_free |F|^2/m + sum_bodies |F|^2/M + tau.(I^-1 tau)]
+   of the current forces; 0 when refinement is off */
+int rbhip_potential_energy_refinement(rbhip_context* ctx, double* energy);
+
 /* Body state of all bodies in reference body order, FP64, 36 values each:
    N, dof, mass, I[3], invI[3], rcm[3], pcm[3], q[4], pi[4], force[3],
    torque[4], twoKt, twoKr, loc (RigidBody.h:18-53).  pcm/pi are the

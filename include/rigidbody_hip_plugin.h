@@ -40,6 +40,12 @@ int rbplugin_create(int numAtoms, const double* masses, const int* bodyIndices, 
                     int rotationMode, int precision, int device, int numBonds, const int* bondPairs,
                     const double* bondLength, const double* bondK, const double* tetherK,
                     const double* tetherR0, rbplugin_sim** out);
+/* as rbplugin_create, with RigidBodyIntegrator::setComputeRefinedEnergies
+   (RigidBodyIntegrator.cpp:34-38) called before the Context binds it */
+int rbplugin_create_ex(int numAtoms, const double* masses, const int* bodyIndices, double stepSize,
+                       int rotationMode, int computeRefinedEnergies, int precision, int device, int numBonds,
+                       const int* bondPairs, const double* bondLength, const double* bondK, const double* tetherK,
+                       const double* tetherR0, rbplugin_sim** out);
 int rbplugin_destroy(rbplugin_sim* sim);
 /* Context::setPositions / setVelocities (-> RigidBodyIntegrator::stateChanged) */
 int rbplugin_set_positions(rbplugin_sim* sim, const double* xyz);

@@ -243,7 +243,7 @@ static size_t context_arena_bytes(size_t na, int numBodies, int numFree, int num
     b += a(3 * sizeof(double) * na) * 2;                                           // stageR, stageV
     b += a(R * na) + a(3 * R * atomStride) + (mixed ? a(3 * sizeof(float) * atomStride) : 0);  // tether
     b += a(4 * sizeof(double) * maxKe) + a(4 * sizeof(double) * maxPe);            // partials
-    b += a(sizeof(DevSys)) + a(2 * sizeof(int)) + a(sizeof(int) * nb) * 2;
+    b += a(sizeof(DevSys)) + a(4 * sizeof(int)) + a(sizeof(int) * nb) * 2;
     b += a(sizeof(double) * std::max(3 * na, 36 * nb)) + a(4 * sizeof(double));  // staging, reduction
     return b;
 }
@@ -286,7 +286,7 @@ struct rbhip_context {
     int nPartTiles = 0, nForceTiles = 0, nSetupTiles = 0;
     int maxKeSlots = 0, maxPeSlots = 0;
     int keSlots = 0;                             // slots written by the last KE producer
-    int* dSlotCounts = nullptr;                  // [0] ke, [1] pe
+    int* dSlotCounts = nullptr;                  // [0] ke, [1] pe, [2] refined ke, [3] refined pe
     int* dBodyN = nullptr;
     int* dBodyAtom0 = nullptr;
     double* dTmp = nullptr;
@@ -294,6 +294,15 @@ struct rbhip_context {
     double* dRed = nullptr;
     double hRed[4] = {0.0, 0.0, 0.0, 0.0};
     hipEvent_t handoff = nullptr;                // orders a new stream after the old one
+
+    // refined energies (setComputeRefinedEnergies): buffers in their own
+    // arena, allocated when enabled; the accumulators are valid after a
+    // step and reset by every state change, as the reference's upload zeroes them
+    bool refined = false;
+    bool refinedValid = false;
+    int refKeSlots = 0;
+    DeviceArena refArena;
+    void* refBuf[4] = {nullptr, nullptr, nullptr, nullptr};
 
     bool built = false;
     bool stageVFresh = false;
@@ -325,8 +334,8 @@ struct rbhip_context {
         DeviceArena arena;
         std::vector<void*> live, copy;
         std::vector<size_t> bytes;
-        int keSlots = 0;
-        bool positionsDerived = false;
+        int keSlots = 0, refKeSlots = 0;
+        bool positionsDerived = false, refinedValid = false;
         ~Snapshot() { arena.release(); }
     };
     void takeSnapshot(Snapshot& s) {
@@ -339,6 +348,7 @@ struct rbhip_context {
             {h.force, 3 * R * as}, {h.kePart, 4 * sizeof(double) * (size_t)maxKeSlots},
             {h.pePart, 4 * sizeof(double) * (size_t)maxPeSlots}};
         if (h.posLo) spans.push_back({h.posLo, 3 * sizeof(float) * as});
+        if (h.refKePart) spans.push_back({h.refKePart, 4 * sizeof(double) * (size_t)maxKeSlots});
         for (auto& sp : spans) {
             void* c = s.arena.alloc(sp.second);
             HIPCHK(hipMemcpyAsync(c, sp.first, sp.second, hipMemcpyDeviceToDevice, stream));
@@ -347,14 +357,18 @@ struct rbhip_context {
             s.bytes.push_back(sp.second);
         }
         s.keSlots = keSlots;
+        s.refKeSlots = refKeSlots;
         s.positionsDerived = positionsDerived;
+        s.refinedValid = refinedValid;
     }
     // enqueues the copies back on `on` (the caller orders and synchronises)
     void restoreSnapshot(const Snapshot& s, hipStream_t on) {
         for (size_t i = 0; i < s.live.size(); i++)
             HIPCHK(hipMemcpyAsync(s.live[i], s.copy[i], s.bytes[i], hipMemcpyDeviceToDevice, on));
         keSlots = s.keSlots;
+        refKeSlots = s.refKeSlots;
         positionsDerived = s.positionsDerived;
+        refinedValid = s.refinedValid;
     }
 
     std::vector<Tile> partTiles(int blk, const std::map<int, int>& lanesByN) const {
@@ -484,10 +498,18 @@ struct rbhip_context {
     void launchFused() {
         HIPCHK(launch_fused(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
     }
+    void launchRefineBegin(bool folded) {
+        HIPCHK(launch_refine_begin(precision, rotationMode, dSys, dPartTiles, nPartTiles, folded ? 1 : 0, block,
+                                   maxBlocks, stream));
+    }
+    void launchRefineEnd() {
+        HIPCHK(launch_refine_end(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+    }
 
-    // 4 doubles reduced from the KE (which = 0) or PE (which = 1) partials
+    // 4 doubles reduced from the KE (which = 0), PE (1), refined KE (2) or
+    // refined PE (3) partials
     void reduce(int which, double* out4) {
-        int counts[2] = {keSlots, nForceTiles};
+        int counts[4] = {keSlots, nForceTiles, refKeSlots, nPartTiles};
         HIPCHK(hipMemcpyAsync(dSlotCounts, counts, sizeof(counts), hipMemcpyHostToDevice, stream));
         HIPCHK(launch_reduce(dSys, dSlotCounts + which, 1, which, dRed, stream));
         HIPCHK(hipMemcpyAsync(hRed, dRed, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -502,6 +524,7 @@ struct rbhip_context {
         keSlots = nSetupTiles;
         kickDt = dt;
         built = true;
+        refinedValid = false;
         double r[4];
         reduce(0, r);
         if (geometry) numDOF = numFree + (int)std::lround(r[2]);
@@ -559,33 +582,54 @@ struct rbhip_context {
     // RigidBodyIntegrator::step(steps).  With fusion, the steps' interior
     // boundaries run as one fused pass each: part1, forces,
     // (steps - 1) x [fused, forces], part2.
+    // With refined energies the last step is bracketed by the refined-energy
+    // kernels.
     void executeSteps(int steps) {
         require(built, "positions must be set before stepping", RBHIP_E_STATE);
         require(steps >= 0, "negative step count");
         fixKick();
         if (steps == 0) return;
+        const int tail = refined ? 1 : 0;
         if (useFusion && steps >= 2) {
             launchPart1();
             launchForces();
             if (useGraph) {
                 ensureGraph(fusedGraph, [&] { launchFused(); launchForces(); });
-                for (int s = 1; s < steps; s++) HIPCHK(hipGraphLaunch(fusedGraph.exec, stream));
+                for (int s = 1; s < steps - tail; s++) HIPCHK(hipGraphLaunch(fusedGraph.exec, stream));
             } else {
-                for (int s = 1; s < steps; s++) {
+                for (int s = 1; s < steps - tail; s++) {
                     launchFused();
                     launchForces();
                 }
             }
+            if (tail) {
+                launchRefineBegin(false);
+                launchFused();
+                launchForces();
+            }
             launchPart2();
-        } else if (useGraph) {
-            ensureGraph(graph, [&] { launchPart1(); launchForces(); launchPart2(); });
-            for (int s = 0; s < steps; s++) HIPCHK(hipGraphLaunch(graph.exec, stream));
         } else {
-            for (int s = 0; s < steps; s++) {
+            if (useGraph) {
+                ensureGraph(graph, [&] { launchPart1(); launchForces(); launchPart2(); });
+                for (int s = 0; s < steps - tail; s++) HIPCHK(hipGraphLaunch(graph.exec, stream));
+            } else {
+                for (int s = 0; s < steps - tail; s++) {
+                    launchPart1();
+                    launchForces();
+                    launchPart2();
+                }
+            }
+            if (tail) {
+                launchRefineBegin(true);
                 launchPart1();
                 launchForces();
                 launchPart2();
             }
+        }
+        if (tail) {
+            launchRefineEnd();
+            refKeSlots = nPartTiles;
+            refinedValid = true;
         }
         keSlots = nPartTiles;
         stageVFresh = false;
@@ -594,10 +638,33 @@ struct rbhip_context {
         stepCount += steps;
     }
 
+    // turns the refined-energy accumulation on (allocating its buffers once)
+    // or off; either way the accumulators start empty.  Kernels see the
+    // buffers only while it is on.
+    void setRefined(bool on) {
+        if (on && !refBuf[0]) {
+            const size_t R = realSize();
+            refArena.stream = stream;
+            refBuf[0] = refArena.alloc(8 * R * (size_t)std::max(numBodies, 1));
+            refBuf[1] = refArena.alloc(3 * R * (size_t)std::max(numFree, 1));
+            refBuf[2] = refArena.alloc(4 * sizeof(double) * (size_t)maxKeSlots);
+            refBuf[3] = refArena.alloc(4 * sizeof(double) * (size_t)maxKeSlots);
+        }
+        refined = on;
+        refinedValid = false;
+        refKeSlots = 0;
+        h.refBody = on ? refBuf[0] : nullptr;
+        h.refFree = on ? refBuf[1] : nullptr;
+        h.refKePart = on ? static_cast<double*>(refBuf[2]) : nullptr;
+        h.refPePart = on ? static_cast<double*>(refBuf[3]) : nullptr;
+        uploadSys();   // (captured graphs hold only plain launches and stay valid)
+    }
+
     ~rbhip_context() {
         graph.reset();
         fusedGraph.reset();
         tileArena.release();
+        refArena.release();
         arena.release();
         if (handoff) (void)hipEventDestroy(handoff);
     }
@@ -695,6 +762,18 @@ struct rbhip_batch {
     void launchFused() {
         HIPCHK(launch_fused(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
     }
+    void launchRefineBegin(bool folded) {
+        HIPCHK(launch_refine_begin(precision, rotationMode, dSystems, dPartTiles, nPart, folded ? 1 : 0, block,
+                                   maxBlocks, stream));
+    }
+    void launchRefineEnd() {
+        HIPCHK(launch_refine_end(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
+    }
+    bool anyRefined() const {
+        for (const rbhip_context* c : members)
+            if (c->refined) return true;
+        return false;
+    }
 
     template <class F> void ensureGraph(GraphCache& g, F&& launches) {
         if (g.exec) return;
@@ -711,35 +790,57 @@ struct rbhip_batch {
         HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
     }
 
+    // as rbhip_context::executeSteps; the refined-energy kernels around the
+    // last step skip the members that have refinement off
     void executeSteps(int steps) {
+        if (steps <= 0) return;
+        const int tail = anyRefined() ? 1 : 0;
         if (useFusion && steps >= 2) {
             launchPart1();
             launchForces();
             if (useGraph) {
                 ensureGraph(fusedGraph, [&] { launchFused(); launchForces(); });
-                for (int s = 1; s < steps; s++) HIPCHK(hipGraphLaunch(fusedGraph.exec, stream));
+                for (int s = 1; s < steps - tail; s++) HIPCHK(hipGraphLaunch(fusedGraph.exec, stream));
             } else {
-                for (int s = 1; s < steps; s++) {
+                for (int s = 1; s < steps - tail; s++) {
                     launchFused();
                     launchForces();
                 }
             }
+            if (tail) {
+                launchRefineBegin(false);
+                launchFused();
+                launchForces();
+            }
             launchPart2();
-        } else if (useGraph) {
-            ensureGraph(graph, [&] { launchPart1(); launchForces(); launchPart2(); });
-            for (int s = 0; s < steps; s++) HIPCHK(hipGraphLaunch(graph.exec, stream));
         } else {
-            for (int s = 0; s < steps; s++) {
+            if (useGraph) {
+                ensureGraph(graph, [&] { launchPart1(); launchForces(); launchPart2(); });
+                for (int s = 0; s < steps - tail; s++) HIPCHK(hipGraphLaunch(graph.exec, stream));
+            } else {
+                for (int s = 0; s < steps - tail; s++) {
+                    launchPart1();
+                    launchForces();
+                    launchPart2();
+                }
+            }
+            if (tail) {
+                launchRefineBegin(true);
                 launchPart1();
                 launchForces();
                 launchPart2();
             }
         }
+        if (tail) launchRefineEnd();
     }
 
     void finishSteps(int steps) {
         for (size_t m = 0; m < members.size(); m++) {
             rbhip_context* c = members[m];
+            if (steps > 0 && c->refined) {
+                c->refKeSlots = memberPartTiles[m];
+                c->refinedValid = true;
+            }
             c->keSlots = memberPartTiles[m];
             c->stageVFresh = false;
             if (steps > 0) c->positionsDerived = true;
@@ -913,13 +1014,15 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         h.bondOther = nullptr;
         h.bondLen = nullptr;
         h.bondK = nullptr;
+        h.refBody = h.refFree = nullptr;
+        h.refKePart = h.refPePart = nullptr;
         size_t nFreeTilesMax = (C.numFree + 63) / 64 + 1;
         C.maxKeSlots = (int)(nFreeTilesMax + C.numBodies + C.classes.size() + 1);
         C.maxPeSlots = (int)((na + 63) / 64 + 1);
         h.kePart = C.arena.allocT<double>(4 * (size_t)C.maxKeSlots);
         h.pePart = C.arena.allocT<double>(4 * (size_t)C.maxPeSlots);
         C.dSys = C.arena.allocT<DevSys>(1);
-        C.dSlotCounts = C.arena.allocT<int>(2);
+        C.dSlotCounts = C.arena.allocT<int>(4);
         C.dBodyN = C.arena.allocT<int>(nb);
         C.dBodyAtom0 = C.arena.allocT<int>(nb);
         C.tmpCount = std::max(3 * na, 36 * nb);
@@ -999,14 +1102,14 @@ int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream) {
             HIPCHK(hipStreamWaitEvent(next, ctx->handoff, 0));
         }
         ctx->stream = next;
-        ctx->arena.stream = ctx->tileArena.stream = ctx->stream;
+        ctx->arena.stream = ctx->tileArena.stream = ctx->refArena.stream = ctx->stream;
     });
 }
 
 int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes) {
     return guard([&] {
         require(ctx && bytes, "null argument");
-        *bytes = ctx->arena.bytes + ctx->tileArena.bytes;
+        *bytes = ctx->arena.bytes + ctx->tileArena.bytes + ctx->refArena.bytes;
     });
 }
 
@@ -1228,6 +1331,7 @@ int rbhip_part1(rbhip_context* ctx) {
         require(ctx->built, "positions must be set before stepping", RBHIP_E_STATE);
         ctx->setDevice();
         ctx->fixKick();
+        if (ctx->refined) ctx->launchRefineBegin(true);
         ctx->launchPart1();
         ctx->positionsDerived = true;
     });
@@ -1241,6 +1345,11 @@ int rbhip_part2(rbhip_context* ctx) {
         ctx->launchPart2();
         ctx->keSlots = ctx->nPartTiles;
         ctx->stageVFresh = false;
+        if (ctx->refined) {
+            ctx->launchRefineEnd();
+            ctx->refKeSlots = ctx->nPartTiles;
+            ctx->refinedValid = true;
+        }
     });
 }
 
@@ -1278,6 +1387,48 @@ int rbhip_kinetic_energies(rbhip_context* ctx, double* out2) {
         if (ctx->keSlots > 0) ctx->reduce(0, r);
         out2[0] = 0.5 * r[0];
         out2[1] = 0.5 * r[1];
+    });
+}
+
+int rbhip_set_refined_energies(rbhip_context* ctx, int enable) {
+    return guard([&] {
+        require(ctx, "null context");
+        ctx->setDevice();
+        ctx->setRefined(enable != 0);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int rbhip_refined_kinetic_energies(rbhip_context* ctx, double* out2) {
+    return guard([&] {
+        require(ctx && out2, "null argument");
+        rbhip_context& C = *ctx;
+        C.setDevice();
+        double r[4] = {0, 0, 0, 0};
+        if (!C.refined) {
+            if (C.keSlots > 0) C.reduce(0, r);
+            out2[0] = 0.5 * r[0];
+            out2[1] = 0.5 * r[1];
+            return;
+        }
+        if (C.refinedValid && C.refKeSlots > 0) C.reduce(2, r);
+        const double f = 1.0 / (6.0 * C.dt);
+        out2[0] = r[0] * f;
+        out2[1] = r[1] * f;
+    });
+}
+
+int rbhip_potential_energy_refinement(rbhip_context* ctx, double* energy) {
+    return guard([&] {
+        require(ctx && energy, "null argument");
+        rbhip_context& C = *ctx;
+        *energy = 0.0;
+        if (!C.refined || !C.built || C.nPartTiles == 0) return;
+        C.setDevice();
+        HIPCHK(launch_refine_pe(C.precision, C.dSys, C.dPartTiles, C.nPartTiles, C.block, C.maxBlocks, C.stream));
+        double r[4];
+        C.reduce(3, r);
+        *energy = -r[0] * C.dt * C.dt / 24.0;
     });
 }
 

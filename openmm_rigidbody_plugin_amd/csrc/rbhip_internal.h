@@ -92,6 +92,12 @@ struct DevSys {
     // reductions: 4 doubles per slot
     double* kePart;     // twoKt, twoKr, dof, (unused)
     double* pePart;     // energy, (unused) x3
+    // refined energies (all nullptr unless enabled)
+    void* refBody;      // Real4[2 * numBodies]: (a_r, 0), a_q per body, written at the
+                        // start of a call's last step, read at its end
+    void* refFree;      // Real[3 * numFree] packed: a_v per free atom, likewise
+    double* refKePart;  // per part tile: 6 dt KE_t refined, 6 dt KE_r refined
+    double* refPePart;  // per part tile: sum |F|^2/m + tau.(I^-1 tau)
 };
 
 }  // namespace rbhip

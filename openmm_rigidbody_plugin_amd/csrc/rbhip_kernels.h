@@ -15,8 +15,19 @@ hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles,
 // part2 of step n and part1 of step n+1 in one pass (no atom velocities, no KE)
 hipError_t launch_fused(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
                         int block, int maxBlocks, hipStream_t stream);
+// refined energies of the systems whose refBody is set: the start terms of
+// the next step (`folded` = the stored momenta carry its first half-kick,
+// i.e. part1 follows; 0 = a fused boundary follows), the end terms after
+// its part2, and the potential refinement of the current forces
+hipError_t launch_refine_begin(int precision, int rotationMode, const DevSys* systems, const Tile* tiles,
+                               int numTiles, int folded, int block, int maxBlocks, hipStream_t stream);
+hipError_t launch_refine_end(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
+                             int block, int maxBlocks, hipStream_t stream);
+hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
+                            int maxBlocks, hipStream_t stream);
 hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
                          int maxBlocks, hipStream_t stream);
+// which: 0 kePart, 1 pePart, 2 refKePart, 3 refPePart
 hipError_t launch_reduce(const DevSys* systems, const int* nslots, int numSystems, int which, double* out,
                          hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);

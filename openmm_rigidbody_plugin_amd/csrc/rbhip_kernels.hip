@@ -194,25 +194,31 @@ template <class Real> __device__ __forceinline__ V3<double> axial_torque(V3<doub
     return tb;
 }
 
+// free rotation of (q, L) over h: the FP64 exact solution or NO-SQUISH(mode)
+template <class Real, bool EXACT>
+__device__ __forceinline__ void free_rotation(Q4<Real>& q, V3<Real>& L, V3<Real> invI, double h, int mode) {
+    if (EXACT) {
+        Q4<double> qd = cvt<double>(q);
+        V3<double> Ld = cvt<double>(L), invId = cvt<double>(invI);
+        if (invI.z == (Real)0) {
+            linear_rotor_exact(qd, Ld, invId, h);
+        } else {
+            V3<double> I = {1.0 / invId.x, 1.0 / invId.y, 1.0 / invId.z};
+            exact_rotation(qd, Ld, I, invId, h);
+        }
+        q = cvt<Real>(qd);
+        L = cvt<Real>(Ld);
+    } else {
+        no_squish<Real>(q, L, invI, h, mode);
+    }
+}
+
 // r_cm += p_cm (dt / M), then the free rotation of (q, L)
 template <class Real, bool EXACT>
 __device__ __forceinline__ void drift_and_rotate(const DevSys& S, Body<Real>& B, V3<double>& rc) {
     double idt = (double)B.invM * S.dt;
     rc = rc + cvt<double>(B.p) * idt;
-    if (EXACT) {
-        Q4<double> qd = cvt<double>(B.q);
-        V3<double> Ld = cvt<double>(B.L), invId = cvt<double>(B.invI);
-        if (B.invI.z == (Real)0) {
-            linear_rotor_exact(qd, Ld, invId, S.dt);
-        } else {
-            V3<double> I = {1.0 / invId.x, 1.0 / invId.y, 1.0 / invId.z};
-            exact_rotation(qd, Ld, I, invId, S.dt);
-        }
-        B.q = cvt<Real>(qd);
-        B.L = cvt<Real>(Ld);
-    } else {
-        no_squish<Real>(B.q, B.L, B.invI, S.dt, S.rotationMode);
-    }
+    free_rotation<Real, EXACT>(B.q, B.L, B.invI, S.dt, S.rotationMode);
 }
 
 template <class Real> __device__ __forceinline__ void store_rcm(const DevSys& S, int k, V3<double> rc) {
@@ -470,6 +476,206 @@ __global__ __launch_bounds__(EXACT ? 256 : 512) void k_fused(const DevSys* __res
 }
 
 // ---------------------------------------------------------------------------
+// Refined energies: the CUDA platform's COMPMOD terms
+// (platforms/cuda/src/kernels/rigidbodyintegrator.cu:270-292, 318-320,
+// 380-383, 433-469).  The accumulators of a step depend on that step alone,
+// so they are evaluated for the last step of a call only, by two kernels
+// that bracket it; the step kernels themselves are the same for every
+// system.  In displacement form, equal in exact arithmetic to the
+// reference's 2.5 r(t) + ... and 0.5 q(t-dt) - 3 q(t) + ...:
+//   rdot = dt/2 [5 v(t+dt/2) - v(t-dt/2) + 2 v(t+3dt/2)]
+//   qdot = (q(t-dt) - q(t))/2 + 5 (q(t+dt) - q(t))/2 + (q(t+2dt) - q(t+dt))
+// where q(t-dt) and q(t+2dt) are virtual rotations from (q, L -/+ the
+// torque half-kick) over -/+dt.  Free atoms use posDot = 2 rdot of the same
+// form (with the sign of v(t-dt/2) the body terms have).
+//   k_refine_begin, before the step: from its starting state and the
+//     current forces, a_r = 5 p(t+dt/2) - p(t-dt/2) and the two q(t) terms
+//     a_q per body (q(t+dt) by the step's own kick and rotation), and
+//     a_v = 5 v(t+dt/2) - v(t-dt/2) per free atom.  `folded` = the stored
+//     momenta carry the step's first half-kick (after part2 or the set-up);
+//     otherwise the step is a fused boundary, which applies both half-kicks.
+//   k_refine_end, after the step's part2: per tile
+//     [0] sum m dt/2 (a + 2 v(t+3dt/2)) . v(t+dt)   (= 6 dt KE_t refined)
+//     [1] sum qdot . pi(t+dt), pi = 2 B(q) L         (= 6 dt KE_r refined)
+//     (qdot's component along q, which the reference projects out, has no
+//     effect: B(q)^T q = 0).
+// ---------------------------------------------------------------------------
+template <class Real> __device__ __forceinline__ void store_refine_body(const DevSys& S, int k, V3<double> ar,
+                                                                        Q4<double> aq) {
+    st4<Real>(S.refBody, 2 * k, Q4<Real>{(Real)ar.x, (Real)ar.y, (Real)ar.z, (Real)0});
+    st4<Real>(S.refBody, 2 * k + 1, cvt<Real>(aq));
+}
+
+template <class Real>
+__device__ __forceinline__ void refine_begin_free(const DevSys& S, const Tile& tile, bool folded) {
+    Real halfDt = (Real)(0.5 * S.dt);
+    for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
+        int a = tile.first + i;
+        V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
+        V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
+        Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
+        V3<Real> dv = f * invm * halfDt;
+        V3<double> av;
+        if (folded) {   // v = v(t): 5 (v + dv) - (v - dv)
+            av = cvt<double>(v) * 4.0 + cvt<double>(dv) * 6.0;
+        } else {        // v = v(t-dt/2), v(t+dt/2) as the fused boundary forms it
+            V3<Real> vp = v + dv;
+            vp = vp + dv;
+            av = cvt<double>(vp) * 5.0 + cvt<double>(v) * -1.0;
+        }
+        stp3<Real>(S.refFree, a, cvt<Real>(av));
+    }
+}
+
+template <class Real, bool EXACT>
+__device__ __forceinline__ void refine_begin_bodies(const DevSys& S, const Tile& tile, bool folded) {
+    const int shift = lane_shift(tile.lanes);
+    const int b = threadIdx.x >> shift;
+    const int sub = threadIdx.x & (tile.lanes - 1);
+    const bool active = b < tile.count;
+    const int k = tile.first + (active ? b : 0);
+    const int atom0 = tile.atomFirst + (active ? b : 0) * tile.N;
+    Body<Real> B = load_body<Real>(S, k);
+    Rot<Real> R = rotT(B.q);
+    V3<double> F, tau;
+    body_sums_any<Real>(S, R, atom0, tile.N, sub, tile.lanes, active, F, tau);
+    if (!active || sub != 0) return;
+    V3<double> tb = axial_torque(to_body(R, tau), B.invI);
+    V3<double> ar;
+    V3<Real> Lback;
+    Body<Real> B1 = B;
+    if (folded) {
+        // stored p = p(t+dt/2), L = L(t) + tau dt/2
+        ar = cvt<double>(B.p) * 4.0 + F * S.dt;
+        Lback = cvt<Real>(cvt<double>(B.L) + tb * -S.dt);
+    } else {
+        // stored p = p(t-dt/2), L = L(t) - tau dt/2 (after the last rotation)
+        fused_kicks<Real>(S, B1, R, F, tau);
+        ar = cvt<double>(B1.p) * 5.0 + cvt<double>(B.p) * -1.0;
+        Lback = B.L;
+    }
+    Q4<Real> qm = B.q;
+    free_rotation<Real, EXACT>(qm, Lback, B.invI, -S.dt, S.rotationMode);
+    free_rotation<Real, EXACT>(B1.q, B1.L, B.invI, S.dt, S.rotationMode);
+    Q4<double> q0 = cvt<double>(B.q) * -1.0;
+    Q4<double> aq = (cvt<double>(qm) + q0) * 0.5 + (cvt<double>(B1.q) + q0) * 2.5;
+    store_refine_body<Real>(S, k, ar, aq);
+}
+
+template <class Real, bool EXACT>
+__global__ __launch_bounds__(EXACT ? 256 : 512) void k_refine_begin(const DevSys* __restrict__ systems,
+                                                                     const Tile* __restrict__ tiles, int numTiles,
+                                                                     int folded) {
+    for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const DevSys& S = systems[tile.sys];
+        if (!S.refBody) continue;
+        if (tile.kind == TILE_FREE)
+            refine_begin_free<Real>(S, tile, folded != 0);
+        else
+            refine_begin_bodies<Real, EXACT>(S, tile, folded != 0);
+    }
+}
+
+template <class Real>
+__device__ __forceinline__ void refine_end_free(const DevSys& S, const Tile& tile, double (&rk)[2]) {
+    const double halfDt = 0.5 * S.dt;
+    for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
+        int a = tile.first + i;
+        V3<double> v = cvt<double>(ld3<Real>(S.vel, S.atomStride, a));
+        V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, a));
+        double invm = (double)reinterpret_cast<const Real*>(S.freeInvM)[a];
+        // posDot / dt = a_v + 2 v(t+3dt/2)
+        V3<double> pd = cvt<double>(ldp3<Real>(S.refFree, a)) + (v + f * (invm * halfDt)) * 2.0;
+        rk[0] += 0.5 * S.dt * dot(pd, v) / invm;
+    }
+}
+
+template <class Real, bool EXACT>
+__device__ __forceinline__ void refine_end_bodies(const DevSys& S, const Tile& tile, double (&rk)[2]) {
+    const int shift = lane_shift(tile.lanes);
+    const int b = threadIdx.x >> shift;
+    const int sub = threadIdx.x & (tile.lanes - 1);
+    const bool active = b < tile.count;
+    const int k = tile.first + (active ? b : 0);
+    const int atom0 = tile.atomFirst + (active ? b : 0) * tile.N;
+    // stored after part2: q(t+dt), p and L with the next half-kick folded in
+    Body<Real> B = load_body<Real>(S, k);
+    Rot<Real> R = rotT(B.q);
+    V3<double> F, tau;
+    body_sums_any<Real>(S, R, atom0, tile.N, sub, tile.lanes, active, F, tau);
+    if (!active || sub != 0) return;
+    const double halfDt = 0.5 * S.dt;
+    V3<double> tb = axial_torque(to_body(R, tau), B.invI);
+    V3<double> p1 = cvt<double>(B.p) + F * -halfDt;
+    V3<double> L1 = cvt<double>(B.L) + tb * -halfDt;
+    Q4<Real> r0 = ld4<Real>(S.refBody, 2 * k), r1 = ld4<Real>(S.refBody, 2 * k + 1);
+    // M rdot . v = dt/2 (a_r + 2 p(t+3dt/2)) . p(t+dt) / M
+    V3<double> ar = {(double)r0.a, (double)r0.b, (double)r0.c};
+    rk[0] = 0.5 * S.dt * (double)B.invM * dot(ar + cvt<double>(B.p) * 2.0, p1);
+    // q(t+2dt): the next step's rotation from (q, L + tau dt/2)
+    Q4<Real> q2 = B.q;
+    V3<Real> L2 = B.L;
+    free_rotation<Real, EXACT>(q2, L2, B.invI, S.dt, S.rotationMode);
+    Q4<double> q1 = cvt<double>(B.q);
+    Q4<double> qdot = cvt<double>(r1) + cvt<double>(q2) + q1 * -1.0;
+    rk[1] = 2.0 * dot(Btmul(q1, qdot), L1);
+}
+
+template <class Real, bool EXACT>
+__global__ __launch_bounds__(EXACT ? 256 : 512) void k_refine_end(const DevSys* __restrict__ systems,
+                                                                   const Tile* __restrict__ tiles, int numTiles) {
+    for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const DevSys& S = systems[tile.sys];
+        if (!S.refBody) continue;
+        double rk[2] = {0.0, 0.0};
+        if (tile.kind == TILE_FREE)
+            refine_end_free<Real>(S, tile, rk);
+        else
+            refine_end_bodies<Real, EXACT>(S, tile, rk);
+        block_reduce_store<2>(rk, S.refKePart, tile.slot);
+    }
+}
+
+// potential-energy refinement partials (rigidbodyintegrator.cu:449-469):
+// sum |F|^2/m over free atoms and |F|^2/M + tau.(I^-1 tau) over bodies, with
+// the body-frame torque tau = B(q)^T C(q) tau_space of the current forces
+template <class Real>
+__global__ __launch_bounds__(512) void k_refine_pe(const DevSys* __restrict__ systems,
+                                                    const Tile* __restrict__ tiles, int numTiles) {
+    for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const DevSys& S = systems[tile.sys];
+        if (!S.refPePart) continue;
+        double u[1] = {0.0};
+        if (tile.kind == TILE_FREE) {
+            for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
+                int a = tile.first + i;
+                V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, a));
+                u[0] += dot(f, f) * (double)reinterpret_cast<const Real*>(S.freeInvM)[a];
+            }
+        } else {
+            const int shift = lane_shift(tile.lanes);
+            const int b = threadIdx.x >> shift;
+            const int sub = threadIdx.x & (tile.lanes - 1);
+            const bool active = b < tile.count;
+            const int k = tile.first + (active ? b : 0);
+            const int atom0 = tile.atomFirst + (active ? b : 0) * tile.N;
+            Body<Real> B = load_body<Real>(S, k);
+            Rot<Real> R = rotT(B.q);
+            V3<double> F, tau;
+            body_sums_any<Real>(S, R, atom0, tile.N, sub, tile.lanes, active, F, tau);
+            if (active && sub == 0) {
+                V3<double> tb = axial_torque(to_body(R, tau), B.invI);
+                u[0] = dot(F, F) * (double)B.invM + dot(hadamard(tb, cvt<double>(B.invI)), tb);
+            }
+        }
+        block_reduce_store<1>(u, S.refPePart, tile.slot);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // force field: harmonic tether + harmonic bonds, one lane per atom.  Body
 // atoms are read at their fp32 positions (as a mixed-precision force field
 // reads fp32 coordinates), free atoms at hi + lo.
@@ -530,7 +736,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
 __global__ __launch_bounds__(256) void k_reduce(const DevSys* __restrict__ systems, const int* __restrict__ nslots,
                                                  int which, double* __restrict__ out) {
     const DevSys& S = systems[blockIdx.x];
-    const double* part = which == 0 ? S.kePart : S.pePart;
+    const double* part = which == 0 ? S.kePart : which == 1 ? S.pePart : which == 2 ? S.refKePart : S.refPePart;
     int n = nslots[blockIdx.x];
     __shared__ double red[256][4];
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -594,6 +800,43 @@ hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles,
     if (grid == 0) return hipSuccess;
     if (precision == 0) k_part2<float><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
     else k_part2<double><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_refine_begin(int precision, int rotationMode, const DevSys* systems, const Tile* tiles,
+                               int numTiles, int folded, int block, int maxBlocks, hipStream_t stream) {
+    int grid = grid_for(numTiles, maxBlocks);
+    if (grid == 0) return hipSuccess;
+    if (precision == 0) {
+        if (rotationMode == 0) k_refine_begin<float, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
+        else k_refine_begin<float, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
+    } else {
+        if (rotationMode == 0) k_refine_begin<double, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
+        else k_refine_begin<double, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_refine_end(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
+                             int block, int maxBlocks, hipStream_t stream) {
+    int grid = grid_for(numTiles, maxBlocks);
+    if (grid == 0) return hipSuccess;
+    if (precision == 0) {
+        if (rotationMode == 0) k_refine_end<float, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_refine_end<float, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    } else {
+        if (rotationMode == 0) k_refine_end<double, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_refine_end<double, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
+                            int maxBlocks, hipStream_t stream) {
+    int grid = grid_for(numTiles, maxBlocks);
+    if (grid == 0) return hipSuccess;
+    if (precision == 0) k_refine_pe<float><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    else k_refine_pe<double><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
     return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@ void HipIntegrateRigidBodyStepKernel::initialize(rbhost::ContextImpl& context, c
         throw Exception("The HIP platform does not support constraints between free atoms");
     stepSize = integrator.getStepSize();
     data.createSimulation(context.getSystem(), integrator.getBodyIndices(), stepSize, integrator.getRotationMode());
+    if (integrator.getComputeRefinedEnergies()) rbhip_check(rbhip_set_refined_energies(data.require(), 1));
 }
 
 void HipIntegrateRigidBodyStepKernel::uploadBodySystem(RigidBodySystem& bodySystem) {
@@ -46,11 +47,21 @@ std::vector<double> HipIntegrateRigidBodyStepKernel::getKineticEnergies(const Ri
     return {ke[0], ke[1]};
 }
 
+// both divide by the integrator's current step size, as the reference does
+// (CudaRigidBodyKernels.cpp:159, 492)
 std::vector<double> HipIntegrateRigidBodyStepKernel::getRefinedKineticEnergies(const RigidBodyIntegrator& integrator) {
-    return getKineticEnergies(integrator);
+    syncStepSize(integrator);
+    double ke[2];
+    rbhip_check(rbhip_refined_kinetic_energies(data.require(), ke));
+    return {ke[0], ke[1]};
 }
 
-double HipIntegrateRigidBodyStepKernel::getPotentialEnergyRefinement(const RigidBodyIntegrator&) { return 0.0; }
+double HipIntegrateRigidBodyStepKernel::getPotentialEnergyRefinement(const RigidBodyIntegrator& integrator) {
+    syncStepSize(integrator);
+    double u = 0.0;
+    rbhip_check(rbhip_potential_energy_refinement(data.require(), &u));
+    return u;
+}
 
 rbhost::KernelImpl* HipRigidBodyKernelFactory::createKernelImpl(std::string name, const rbhost::Platform& platform,
                                                                 rbhost::ContextImpl& context) const {

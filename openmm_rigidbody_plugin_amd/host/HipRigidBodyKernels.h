@@ -26,8 +26,9 @@ public:
     void execute(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator, int steps) override;
     double computeKineticEnergy(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator) override;
     std::vector<double> getKineticEnergies(const RigidBodyIntegrator& integrator) override;
-    // unrefined values, as the reference platform returns
-    // (ReferenceRigidBodyKernels.cpp:123-132)
+    // with setComputeRefinedEnergies(true), the CUDA platform's refined values
+    // (CudaRigidBodyKernels.cpp:469-494), otherwise the plain kinetic energies
+    // and 0, as the reference platform returns (ReferenceRigidBodyKernels.cpp:123-132)
     std::vector<double> getRefinedKineticEnergies(const RigidBodyIntegrator& integrator) override;
     double getPotentialEnergyRefinement(const RigidBodyIntegrator& integrator) override;
 

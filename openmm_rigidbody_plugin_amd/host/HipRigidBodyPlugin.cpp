@@ -79,6 +79,14 @@ extern "C" int rbplugin_create(int numAtoms, const double* masses, const int* bo
                                int rotationMode, int precision, int device, int numBonds, const int* bondPairs,
                                const double* bondLength, const double* bondK, const double* tetherK,
                                const double* tetherR0, rbplugin_sim** out) {
+    return rbplugin_create_ex(numAtoms, masses, bodyIndices, stepSize, rotationMode, 0, precision, device, numBonds,
+                              bondPairs, bondLength, bondK, tetherK, tetherR0, out);
+}
+
+extern "C" int rbplugin_create_ex(int numAtoms, const double* masses, const int* bodyIndices, double stepSize,
+                                  int rotationMode, int computeRefinedEnergies, int precision, int device,
+                                  int numBonds, const int* bondPairs, const double* bondLength, const double* bondK,
+                                  const double* tetherK, const double* tetherR0, rbplugin_sim** out) {
     return guard([&] {
         need(out != nullptr, "null output handle");
         *out = nullptr;
@@ -104,6 +112,7 @@ extern "C" int rbplugin_create(int numAtoms, const double* masses, const int* bo
         }
         s->integrator.reset(new RigidBodyIntegrator(stepSize, std::vector<int>(bodyIndices, bodyIndices + numAtoms)));
         s->integrator->setRotationMode(rotationMode);
+        s->integrator->setComputeRefinedEnergies(computeRefinedEnergies != 0);
         std::map<std::string, std::string> props = {{"DeviceIndex", std::to_string(device)},
                                                     {"Precision", precision == 0 ? "mixed" : "double"}};
         s->context.reset(new rbhost::Context(s->system, *s->integrator, rbhost::Platform::getPlatformByName("HIP"), props));

@@ -26,6 +26,8 @@ _SIG = {
     "registerRigidBodyHipKernelFactories": (None, []),
     "rbplugin_last_error": (ctypes.c_char_p, []),
     "rbplugin_create": (_I, [_I, _DP, _IP, _D, _I, _I, _I, _I, _IP, _DP, _DP, _DP, _DP, ctypes.POINTER(_P)]),
+    "rbplugin_create_ex": (_I, [_I, _DP, _IP, _D, _I, _I, _I, _I, _I, _IP, _DP, _DP, _DP, _DP,
+                                ctypes.POINTER(_P)]),
     "rbplugin_destroy": (_I, [_P]),
     "rbplugin_set_positions": (_I, [_P, _DP]),
     "rbplugin_set_velocities": (_I, [_P, _DP]),
@@ -87,7 +89,7 @@ class Simulation:
     """System + RigidBodyIntegrator + Context (HIP platform) of the C++ host layer."""
 
     def __init__(self, masses, body_indices, step_size, rotation_mode=0, precision="mixed", device=0,
-                 bonds=None, tether_k=None, tether_r0=None):
+                 bonds=None, tether_k=None, tether_r0=None, refined_energies=False):
         self._h = None
         m = np.ascontiguousarray(masses, dtype=np.float64)
         bi = np.ascontiguousarray(body_indices, dtype=np.int32)
@@ -110,16 +112,17 @@ class Simulation:
         else:
             tkp, trp = None, None
         h = _P()
-        _check(lib().rbplugin_create(n, _dp(m), _ip(bi), float(step_size), int(rotation_mode), prec, int(device),
-                                     nb, pp, lp, kp, tkp, trp, ctypes.byref(h)))
+        _check(lib().rbplugin_create_ex(n, _dp(m), _ip(bi), float(step_size), int(rotation_mode),
+                                        int(bool(refined_energies)), prec, int(device), nb, pp, lp, kp, tkp, trp,
+                                        ctypes.byref(h)))
         self._h = h
 
     @classmethod
-    def from_spec(cls, spec, precision="mixed", device=0):
+    def from_spec(cls, spec, precision="mixed", device=0, refined_energies=False):
         """A SystemSpec (openmm_rigidbody_plugin_amd.systems) with its tether,
         positions then velocities set, as the reference's tests do."""
         s = cls(spec.masses, spec.body_indices, spec.dt, spec.rotation_mode, precision, device,
-                tether_k=spec.tether_k, tether_r0=spec.tether_r0)
+                tether_k=spec.tether_k, tether_r0=spec.tether_r0, refined_energies=refined_energies)
         s.set_positions(spec.positions)
         if spec.velocities is not None:
             s.set_velocities(spec.velocities)

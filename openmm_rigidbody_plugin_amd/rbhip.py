@@ -72,6 +72,9 @@ def lib() -> ctypes.CDLL:
             "rbhip_use_fusion": (_I, [_P, _I]),
             "rbhip_kinetic_energies": (_I, [_P, _DP]),
             "rbhip_potential_energy": (_I, [_P, _DP]),
+            "rbhip_set_refined_energies": (_I, [_P, _I]),
+            "rbhip_refined_kinetic_energies": (_I, [_P, _DP]),
+            "rbhip_potential_energy_refinement": (_I, [_P, _DP]),
             "rbhip_get_body_states": (_I, [_P, _DP]),
             "rbhip_get_body_fixed_positions": (_I, [_P, _DP]),
             "rbhip_set_launch": (_I, [_P, _I, _I]),
@@ -283,6 +286,23 @@ class Context:
     def potential_energy(self):
         e = ctypes.c_double()
         _check(lib().rbhip_potential_energy(self._h, ctypes.byref(e)))
+        return e.value
+
+    def set_refined(self, enable=True):
+        """setComputeRefinedEnergies: accumulate the refined-energy terms on the
+        last step of every call (the accumulators start empty)."""
+        _check(lib().rbhip_set_refined_energies(self._h, int(bool(enable))))
+
+    def refined_kinetic_energies(self):
+        """getRefinedKineticEnergies: {translational, rotational}."""
+        out = np.empty(2)
+        _check(lib().rbhip_refined_kinetic_energies(self._h, _dp(out)))
+        return out
+
+    def potential_energy_refinement(self):
+        """getPotentialEnergyRefinement: -dt^2/24 sum F.M^-1 F of the current forces."""
+        e = ctypes.c_double()
+        _check(lib().rbhip_potential_energy_refinement(self._h, ctypes.byref(e)))
         return e.value
 
     def body_states_array(self):

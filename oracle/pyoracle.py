@@ -19,7 +19,8 @@ FIX_LINEAR_FRAME = 2
 FIX_LINEAR_EXACT = 4
 FIX_STALE_SYSTEM = 8
 FIX_ORTHONORMAL = 16
-FIX_ALL = 31
+FIX_REFINED_FREE = 32
+FIX_ALL = 63
 
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -51,6 +52,9 @@ def lib():
             "rbref_get_forces": (_I, [_P, _DP]),
             "rbref_kinetic_energies": (_I, [_P, _DP]),
             "rbref_potential_energy": (_D, [_P]),
+            "rbref_set_refined": (_I, [_P, _I]),
+            "rbref_refined_kinetic_energies": (_I, [_P, _DP]),
+            "rbref_potential_energy_refinement": (_D, [_P]),
             "rbref_time": (_D, [_P]),
             "rbref_counts": (_I, [_P, _IP]),
             "rbref_atom_index": (_I, [_P, _IP]),
@@ -171,6 +175,17 @@ class OracleSim:
 
     def potential_energy(self):
         return lib().rbref_potential_energy(self._h)
+
+    def set_refined(self, enable=True):
+        _check(lib().rbref_set_refined(self._h, int(bool(enable))))
+
+    def refined_kinetic_energies(self):
+        out = np.empty(2)
+        _check(lib().rbref_refined_kinetic_energies(self._h, _dp(out)))
+        return out
+
+    def potential_energy_refinement(self):
+        return lib().rbref_potential_energy_refinement(self._h)
 
     def time(self):
         return lib().rbref_time(self._h)

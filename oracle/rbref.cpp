@@ -43,6 +43,13 @@
  *                           → testSingleBond KAT (tests/TestRigidBodyIntegrator.h:49-83),
  *                             energy conservation, time reversibility
  *   computeKineticEnergies  RigidBodySystem.cpp:210-220 → KE identity
+ *   refined energies        the CUDA platform's COMPMOD terms,
+ *                           rigidbodyintegrator.cu:270-292, 318-320, 380-383,
+ *                           433-469, CudaRigidBodyKernels.cpp:405-494 (the
+ *                           reference platform returns plain values)
+ *                           → test_oracle_refined: exact at constant velocity,
+ *                             third order for free rotors, shadow-energy
+ *                             fluctuation O(dt^3) vs O(dt^2), closed-form U_ref
  *
  * Parity unpinned (no check fixes them beyond the restatement itself): the
  * bit-level FP64 output of the original exactRotation composition and of the
@@ -137,6 +144,7 @@ struct Quat {
     double operator[](int i) const { return c[i]; }
     double& operator[](int i) { return c[i]; }
     Quat operator+(const Quat& o) const { return Quat(c[0] + o[0], c[1] + o[1], c[2] + o[2], c[3] + o[3]); }
+    Quat operator-(const Quat& o) const { return Quat(c[0] - o[0], c[1] - o[1], c[2] - o[2], c[3] - o[3]); }
     Quat operator*(double s) const { return Quat(c[0] * s, c[1] * s, c[2] * s, c[3] * s); }
     Quat& operator+=(const Quat& o) { for (int i = 0; i < 4; i++) c[i] += o[i]; return *this; }
     Quat& operator/=(double s) { double r = 1.0 / s; for (int i = 0; i < 4; i++) c[i] *= r; return *this; }
@@ -890,6 +898,10 @@ struct rbref_sim {
     /* RigidBodyIntegrator::stateChanged, RigidBodyIntegrator.cpp:63-74, with
        uploadBodySystem = RigidBodySystem::copy (ReferenceRigidBodyKernels.cpp:78-80) */
     void stateChanged(bool positions) {
+        stateChangedBodies(positions);
+        refineReset();
+    }
+    void stateChangedBodies(bool positions) {
         if (positions) {
             ctx.calcForces();
             integratorSystem.update(ctx, true, true);
@@ -903,8 +915,10 @@ struct rbref_sim {
         }
     }
 
-    /* ReferenceIntegrateRigidBodyStepKernel::execute, ReferenceRigidBodyKernels.cpp:82-108 */
+    /* ReferenceIntegrateRigidBodyStepKernel::execute, ReferenceRigidBodyKernels.cpp:82-108,
+       with the refined-energy accumulation of the CUDA platform around it */
     void execute() {
+        if (refined) refineBegin();
         for (int k = 0; k < kernelSystem.numFree; k++) {
             int i = kernelSystem.atomIndex[k];
             oldPos[i] = ctx.R[i];
@@ -913,8 +927,105 @@ struct rbref_sim {
         /* constraints / virtual sites: none in this system model */
         ctx.calcForces();
         kernelSystem.integratePart2(dt, ctx.R, ctx.F, ctx.V);
+        if (refined) refineEnd();
         ctx.time += dt;
         ctx.stepCount++;
+    }
+
+    /* ---- refined energies (IntegrateRigidBodyStepKernel::getRefinedKineticEnergies
+       and getPotentialEnergyRefinement).  The reference platform returns the plain
+       values (ReferenceRigidBodyKernels.cpp:123-132); the CUDA platform's COMPMOD
+       build computes these, restated here in FP64:
+         - free atoms: posDot = s*D(-dt) + 5 D(dt) + 2 D'(dt), D(h) = (v + F/m h/2) h the
+           displacement over h from the current state (freeAtomsDelta/freeAtomsDot,
+           rigidbodyintegrator.cu:270-292, sequence CudaRigidBodyKernels.cpp:405-437),
+           D' from the state after the step.  The reference uses s = -1; with
+           RBREF_FIX_REFINED_FREE s = +1, the sign the body terms have;
+         - bodies: rdot = 2.5 r(t) + (v - F/M dt/2) dt/2 before part 1 and
+           rdot = 2.5 r(t+dt) + (v + F/M dt/2) dt - rdot after part 2; qdot =
+           q_virt(-dt)/2 - 3 q(t), then qdot += 1.5 q(t+dt) + q_virt(+dt) and
+           qdot -= q (qdot . q) (rigidbodyintegrator.cu:318-320, 380-383), q_virt(h) =
+           q after the kick pi + T h and a free rotation over h (:238-243);
+         - KE_t = [sum_free posDot.v m/2 + sum_bodies M rdot.v] / (6 dt),
+           KE_r = sum_bodies qdot.pi / (6 dt) (rigidbodyintegrator.cu:433-446,
+           CudaRigidBodyKernels.cpp:159-163);
+         - U_ref = -dt^2/24 [sum_free |F|^2/m + sum_bodies |F|^2/M + tau.(I^-1 tau)],
+           tau = B(q)^T T (rigidbodyintegrator.cu:449-469, CudaRigidBodyKernels.cpp:481-491).
+       The accumulators are zeroed whenever the body system is rebuilt
+       (uploadBodySystem, CudaRigidBodyKernels.cpp:293-335). */
+    bool refined = false;
+    std::vector<Vec3> posDot, posDelta;   /* per free atom */
+    std::vector<Vec3> rdot;               /* per body */
+    std::vector<Quat> qdot;
+
+    void refineReset() {
+        posDot.assign(kernelSystem.numFree, Vec3());
+        posDelta.assign(kernelSystem.numFree, Vec3());
+        rdot.assign(kernelSystem.numBodies, Vec3());
+        qdot.assign(kernelSystem.numBodies, Quat());
+    }
+    Vec3 freeDelta(const FreeAtom& a, double h) const {
+        return (ctx.V[a.index] + ctx.F[a.index] * ((0.5 * h) * a.invMass)) * h;
+    }
+    Quat virtualRotation(const Body& b, double h) const {
+        Quat q = b.q, pi = b.pi + b.torque * h;
+        rotateBody(q, pi, b.I, b.invI, b.dof, h, rotationMode, flags);
+        return q;
+    }
+    void refineBegin() {
+        const double halfDt = 0.5 * dt;
+        const double s = (flags & RBREF_FIX_REFINED_FREE) ? 1.0 : -1.0;
+        for (int k = 0; k < kernelSystem.numFree; k++) {
+            const FreeAtom& a = kernelSystem.freeAtom[k];
+            posDot[k] = freeDelta(a, -dt) * s;
+            posDelta[k] = freeDelta(a, dt);
+        }
+        for (int k = 0; k < kernelSystem.numBodies; k++) {
+            const Body& b = kernelSystem.body[k];
+            Vec3 v = b.pcm * b.invMass;
+            Vec3 dv = b.force * (b.invMass * halfDt);
+            rdot[k] = b.rcm * 2.5 + (v - dv) * halfDt;
+            qdot[k] = virtualRotation(b, -dt) * 0.5 - b.q * 3.0;
+        }
+    }
+    void refineEnd() {
+        const double halfDt = 0.5 * dt;
+        for (int k = 0; k < kernelSystem.numFree; k++) {
+            posDot[k] += posDelta[k] * 5.0;
+            posDot[k] += freeDelta(kernelSystem.freeAtom[k], dt) * 2.0;
+        }
+        for (int k = 0; k < kernelSystem.numBodies; k++) {
+            const Body& b = kernelSystem.body[k];
+            Vec3 v = b.pcm * b.invMass;
+            Vec3 dv = b.force * (b.invMass * halfDt);
+            rdot[k] = b.rcm * 2.5 + (v + dv) * dt - rdot[k];
+            qdot[k] += b.q * 1.5 + virtualRotation(b, dt);
+            qdot[k] += b.q * (-qdot[k].dot(b.q));
+        }
+    }
+    void refinedKineticEnergies(double* out2) const {
+        double KEt = 0.0, KEr = 0.0;
+        for (int k = 0; k < kernelSystem.numFree; k++) {
+            const FreeAtom& a = kernelSystem.freeAtom[k];
+            KEt += posDot[k].dot(ctx.V[a.index]) * (0.5 / a.invMass);
+        }
+        for (int k = 0; k < kernelSystem.numBodies; k++) {
+            const Body& b = kernelSystem.body[k];
+            KEt += rdot[k].dot(b.pcm * b.invMass) / b.invMass;
+        }
+        for (int k = 0; k < kernelSystem.numBodies; k++) KEr += qdot[k].dot(kernelSystem.body[k].pi);
+        double factor = 1.0 / (6.0 * dt);
+        out2[0] = KEt * factor;
+        out2[1] = KEr * factor;
+    }
+    double potentialEnergyRefinement() const {
+        double U = 0.0;
+        for (const FreeAtom& a : kernelSystem.freeAtom) U += ctx.F[a.index].dot(ctx.F[a.index]) * a.invMass;
+        for (const Body& b : kernelSystem.body) {
+            Vec3 tau = b.q.Bt(b.torque);
+            U += b.force.dot(b.force) * b.invMass + diagMul(tau, b.invI).dot(tau);
+        }
+        return -U * dt * dt / 24.0;
     }
 };
 
@@ -998,6 +1109,22 @@ int rbref_kinetic_energies(rbref_sim* s, double* out2) {
     out2[0] = s->kernelSystem.transKE;
     out2[1] = s->kernelSystem.rotKE;
     return 0;
+}
+
+int rbref_set_refined(rbref_sim* s, int enable) {
+    s->refined = enable != 0;
+    s->refineReset();
+    return 0;
+}
+
+int rbref_refined_kinetic_energies(rbref_sim* s, double* out2) {
+    if (s->refined) s->refinedKineticEnergies(out2);
+    else rbref_kinetic_energies(s, out2);
+    return 0;
+}
+
+double rbref_potential_energy_refinement(rbref_sim* s) {
+    return s->refined ? s->potentialEnergyRefinement() : 0.0;
 }
 
 double rbref_potential_energy(rbref_sim* s) { return s->ctx.pe; }

@@ -48,7 +48,12 @@ extern "C" {
    the smallest |u_k| (reference RigidBody.cpp:51-58 takes the smallest signed
    component and returns NaN for a body axis along -e_k) */
 #define RBREF_FIX_ORTHONORMAL    16u
-#define RBREF_FIX_ALL            31u
+/* refined kinetic energy of free atoms: the backward displacement enters the
+   position derivative with +1, as in the body terms (the CUDA platform's
+   CudaRigidBodyKernels.cpp:411-413 uses -1, which gives 4/3 of the kinetic
+   energy at constant velocity) */
+#define RBREF_FIX_REFINED_FREE   32u
+#define RBREF_FIX_ALL            63u
 
 typedef struct rbref_sim rbref_sim;
 
@@ -76,6 +81,11 @@ int rbref_get_forces(rbref_sim* s, double* xyz);
 /* computeKineticEnergy() then getKineticEnergies(): out = {trans, rot} */
 int rbref_kinetic_energies(rbref_sim* s, double* out2);
 double rbref_potential_energy(rbref_sim* s);
+/* refined energies (setComputeRefinedEnergies): the CUDA platform's COMPMOD
+   accumulators, zeroed on enable and on every state change */
+int rbref_set_refined(rbref_sim* s, int enable);
+int rbref_refined_kinetic_energies(rbref_sim* s, double* out2);   /* plain KE when off */
+double rbref_potential_energy_refinement(rbref_sim* s);           /* 0 when off */
 double rbref_time(rbref_sim* s);
 /* counts: numFree, numBodies, numActualAtoms, numBodyAtoms, numDOF */
 int rbref_counts(rbref_sim* s, int* out5);

@@ -12,6 +12,11 @@
 //                        mode change after binding, body-index count, body
 //                        constraints) and this platform's documented limits
 //   testBookkeeping      RigidBodySystem counts, atom order and DOF
+//   testSingleBondRefined  setComputeRefinedEnergies on testSingleBond's
+//                        oscillator: refined kinetic energy + potential +
+//                        potential refinement is the velocity-Verlet shadow
+//                        energy, conserved far better than the plain energy
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -146,6 +151,37 @@ void testRigidDimerSpin(Platform& platform, const std::string& precision, int mo
     CHECK_TOL(0.0, integrator.getPotentialEnergyRefinement(), 0.0);
 }
 
+void testSingleBondRefined(Platform& platform) {
+    System system;
+    system.addParticle(2.0);
+    system.addParticle(2.0);
+    RigidBodyIntegrator integrator(0.01, std::vector<int>(2, 0));
+    integrator.setComputeRefinedEnergies(true);
+    HarmonicBondForce* forceField = new HarmonicBondForce();
+    forceField->addBond(0, 1, 1.5, 1);
+    system.addForce(forceField);
+    Context context(system, integrator, platform, props("double"));
+    context.setPositions({Vec3(-1, 0, 0), Vec3(1, 0, 0)});
+    std::vector<double> rk = integrator.getRefinedKineticEnergies();
+    CHECK_TOL(0.0, rk[0], 0.0);   // empty until the first step
+    double plainMin = 1e30, plainMax = -1e30, shadowMin = 1e30, shadowMax = -1e30;
+    for (int i = 0; i < 1000; ++i) {
+        integrator.step(1);
+        State state = context.getState(State::Energy);
+        rk = integrator.getRefinedKineticEnergies();
+        double plain = state.getKineticEnergy() + state.getPotentialEnergy();
+        double shadow = rk[0] + rk[1] + state.getPotentialEnergy() + integrator.getPotentialEnergyRefinement();
+        plainMin = std::min(plainMin, plain);
+        plainMax = std::max(plainMax, plain);
+        shadowMin = std::min(shadowMin, shadow);
+        shadowMax = std::max(shadowMax, shadow);
+    }
+    if (!(plainMax - plainMin > 0.0 && shadowMax - shadowMin < (plainMax - plainMin) / 20)) {
+        std::printf("  plain energy range %g, shadow energy range %g\n", plainMax - plainMin, shadowMax - shadowMin);
+        throw Exception("the shadow energy is not conserved better than the plain energy");
+    }
+}
+
 void testApiErrors(Platform& platform) {
     RigidBodyIntegrator a(0.001, std::vector<int>{1, 1});
     CHECK_THROWS(a.setRotationMode(-1));
@@ -240,6 +276,7 @@ int main() {
         {"testRigidDimerSpin double exact", [&] { testRigidDimerSpin(platform, "double", 0); }},
         {"testApiErrors", [&] { testApiErrors(platform); }},
         {"testBookkeeping", [&] { testBookkeeping(platform); }},
+        {"testSingleBondRefined", [&] { testSingleBondRefined(platform); }},
     };
     for (auto& t : tests) {
         std::printf("%s ...\n", t.first.c_str());
