@@ -260,6 +260,8 @@ int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, si
    memory and its device memory freed.  All jobs share the precision and the
    rotation kind (exact or NO-SQUISH) and run with workgroup size blockSize,
    so each job's result equals the job stepped alone with that block size.
+   New jobs are set up by four threads of the server and finished jobs
+   collected by a fifth, overlapping the queued rounds.
    Errors of the rbhip_server_* calls are described by
    rbhip_server_last_error(). */
 const char* rbhip_server_last_error(void);
@@ -272,17 +274,25 @@ int rbhip_server_submit(rbhip_server* server, int numAtoms, const double* masses
                         const double* positions, const double* velocities, const double* tetherK,
                         const double* tetherR0, double stepSize, int rotationMode, long long steps,
                         long long* jobId);
-/* one round: admits up to 8 waiting jobs (in arrival order, within the
-   budget) while the steps queued by the previous round run, collects the
-   jobs that reached their step count, and queues the next batched advance
-   without waiting for it; *finished = jobs completed */
+/* one round: hands up to 8 waiting jobs (in arrival order, within the
+   budget) to the admission thread, hands the jobs that reached their step
+   count to the collection thread, and queues the next batched advance of
+   the resident and newly set-up jobs without waiting for it (it waits only
+   to keep at most two rounds queued, or, with nothing resident, for the
+   first admission); *finished = jobs completed since the previous call.
+   Jobs complete on the collection thread, so rbhip_server_completed can
+   list more of them between calls. */
 int rbhip_server_run_round(rbhip_server* server, int* finished);
-/* counts3 = {waiting, resident, completed and not yet taken}; device bytes of
-   the resident jobs; body-steps done so far.  Any output may be NULL. */
+/* counts3 = {waiting, resident (including finished jobs not yet collected,
+   which still hold device memory), completed and not yet taken}; device
+   bytes of the resident jobs; body-steps done so far.  Any output may be
+   NULL. */
 int rbhip_server_status(rbhip_server* server, long long* counts3, size_t* bytesInUse, long long* bodySteps);
-/* rounds run so far and the host seconds spent in their phases: seconds4 =
-   {admitting, collecting finished jobs (includes waiting for the device),
-   updating the batch members, queueing the steps} */
+/* rounds queued so far and the host seconds spent in their phases:
+   seconds4 = {admitting (admission thread), collecting finished jobs
+   (collection thread, includes waiting for the device), updating the batch
+   members (includes waiting for the oldest queued round), queueing the
+   steps} */
 int rbhip_server_profile(rbhip_server* server, long long* rounds, double* seconds4);
 /* ids of completed jobs not yet taken, up to max */
 int rbhip_server_completed(rbhip_server* server, long long* ids, int max, int* count);

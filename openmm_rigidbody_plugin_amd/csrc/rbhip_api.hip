@@ -150,11 +150,16 @@ hipStream_t pooled_stream(int device) {
             v.push_back(st);
         }
         // the stream-ordered allocations keep freed memory in the device's
-        // pool for the next context instead of returning it
+        // pool for the next context instead of returning it.  Memory freed on
+        // one stream is reused on another only once the free has completed:
+        // the pool inserts no waits between streams, so setting up a context
+        // never waits behind the steps queued before another context's free.
         hipMemPool_t mp = nullptr;
         if (hipDeviceGetDefaultMemPool(&mp, device) == hipSuccess && mp) {
             uint64_t keep = UINT64_MAX;
             (void)hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep);
+            int no = 0;
+            (void)hipMemPoolSetAttribute(mp, hipMemPoolReuseAllowInternalDependencies, &no);
         }
     }
     return v[next[device]++ % v.size()];
@@ -176,6 +181,105 @@ struct GraphCache {
         graph = nullptr;
     }
 };
+
+// captures the launches `launches` makes on `stream` into g, once
+template <class F> void ensure_graph(GraphCache& g, hipStream_t stream, F&& launches) {
+    if (g.exec) return;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+        launches();
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(stream, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(stream, &g.graph));
+    HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+}
+
+// The repeated launch sequences of a call, as HIP graphs: the two-pass step
+// (part1, forces, part2), one fused step boundary (fused pass, forces), and
+// kFusedChunk of those in one graph.  A graph launch costs the host tens of
+// microseconds, as much as a small system's whole step, so long calls replay
+// the chunk.
+struct StepGraphs {
+    static constexpr int kFusedChunk = 16;
+    GraphCache twoPass, fused, chunk;
+    void reset() {
+        twoPass.reset();
+        fused.reset();
+        chunk.reset();
+    }
+    bool any() const { return twoPass.exec || fused.exec || chunk.exec; }
+};
+
+// RigidBodyIntegrator::step(steps) on a context or a batch (`o` provides the
+// launches, its stream, graphs and switches).  With fusion the steps'
+// interior boundaries run as one fused pass each: part1, forces,
+// (steps - 1) x [fused, forces], part2.  With `refine` the last step is
+// bracketed by the refined-energy kernels.
+template <class O> void run_steps(O& o, int steps, bool refine) {
+    if (steps <= 0) return;
+    const int tail = refine ? 1 : 0;
+    if (o.useFusion && steps >= 2) {
+        o.launchPart1();
+        o.launchForces();
+        int interior = steps - 1 - tail;
+        if (o.useGraph) {
+            if (interior >= StepGraphs::kFusedChunk) {
+                ensure_graph(o.graphs.chunk, o.stream, [&] {
+                    for (int i = 0; i < StepGraphs::kFusedChunk; i++) {
+                        o.launchFused();
+                        o.launchForces();
+                    }
+                });
+                for (; interior >= StepGraphs::kFusedChunk; interior -= StepGraphs::kFusedChunk)
+                    HIPCHK(hipGraphLaunch(o.graphs.chunk.exec, o.stream));
+            }
+            if (interior > 0) {
+                ensure_graph(o.graphs.fused, o.stream, [&] {
+                    o.launchFused();
+                    o.launchForces();
+                });
+                for (; interior > 0; interior--) HIPCHK(hipGraphLaunch(o.graphs.fused.exec, o.stream));
+            }
+        } else {
+            for (; interior > 0; interior--) {
+                o.launchFused();
+                o.launchForces();
+            }
+        }
+        if (tail) {
+            o.launchRefineBegin(false);
+            o.launchFused();
+            o.launchForces();
+        }
+        o.launchPart2();
+    } else {
+        if (o.useGraph) {
+            ensure_graph(o.graphs.twoPass, o.stream, [&] {
+                o.launchPart1();
+                o.launchForces();
+                o.launchPart2();
+            });
+            for (int s = 0; s < steps - tail; s++) HIPCHK(hipGraphLaunch(o.graphs.twoPass.exec, o.stream));
+        } else {
+            for (int s = 0; s < steps - tail; s++) {
+                o.launchPart1();
+                o.launchForces();
+                o.launchPart2();
+            }
+        }
+        if (tail) {
+            o.launchRefineBegin(true);
+            o.launchPart1();
+            o.launchForces();
+            o.launchPart2();
+        }
+    }
+    if (tail) o.launchRefineEnd();
+}
 
 }  // namespace
 
@@ -309,8 +413,7 @@ struct rbhip_context {
     bool positionsDerived = false;               // body-atom positions hold fp32 r_cm + A(q)^T d only
     bool useGraph = true;
     bool useFusion = true;
-    GraphCache graph;       // part1, forces, part2
-    GraphCache fusedGraph;  // fused part2 + part1, forces
+    StepGraphs graphs;
     double time = 0.0;
     long long stepCount = 0;
 
@@ -476,8 +579,7 @@ struct rbhip_context {
         nPartTiles = (int)pt.size();
         nForceTiles = (int)ft.size();
         nSetupTiles = (int)st.size();
-        graph.reset();
-        fusedGraph.reset();
+        graphs.reset();
     }
 
     void setSlotCounts(int ke, int pe) {
@@ -557,21 +659,6 @@ struct rbhip_context {
         }
     }
 
-    template <class F> void ensureGraph(GraphCache& g, F&& launches) {
-        if (g.exec) return;
-        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        try {
-            launches();
-        } catch (...) {
-            hipGraph_t tmp = nullptr;
-            (void)hipStreamEndCapture(stream, &tmp);
-            if (tmp) (void)hipGraphDestroy(tmp);
-            throw;
-        }
-        HIPCHK(hipStreamEndCapture(stream, &g.graph));
-        HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
-    }
-
     void fixKick() {
         if (built && kickDt != dt) {
             HIPCHK(launch_rekick(precision, dSys, dBodyN, dBodyAtom0, numBodies, dt - kickDt, stream));
@@ -579,55 +666,14 @@ struct rbhip_context {
         }
     }
 
-    // RigidBodyIntegrator::step(steps).  With fusion, the steps' interior
-    // boundaries run as one fused pass each: part1, forces,
-    // (steps - 1) x [fused, forces], part2.
-    // With refined energies the last step is bracketed by the refined-energy
-    // kernels.
+    // RigidBodyIntegrator::step(steps) (run_steps)
     void executeSteps(int steps) {
         require(built, "positions must be set before stepping", RBHIP_E_STATE);
         require(steps >= 0, "negative step count");
         fixKick();
         if (steps == 0) return;
-        const int tail = refined ? 1 : 0;
-        if (useFusion && steps >= 2) {
-            launchPart1();
-            launchForces();
-            if (useGraph) {
-                ensureGraph(fusedGraph, [&] { launchFused(); launchForces(); });
-                for (int s = 1; s < steps - tail; s++) HIPCHK(hipGraphLaunch(fusedGraph.exec, stream));
-            } else {
-                for (int s = 1; s < steps - tail; s++) {
-                    launchFused();
-                    launchForces();
-                }
-            }
-            if (tail) {
-                launchRefineBegin(false);
-                launchFused();
-                launchForces();
-            }
-            launchPart2();
-        } else {
-            if (useGraph) {
-                ensureGraph(graph, [&] { launchPart1(); launchForces(); launchPart2(); });
-                for (int s = 0; s < steps - tail; s++) HIPCHK(hipGraphLaunch(graph.exec, stream));
-            } else {
-                for (int s = 0; s < steps - tail; s++) {
-                    launchPart1();
-                    launchForces();
-                    launchPart2();
-                }
-            }
-            if (tail) {
-                launchRefineBegin(true);
-                launchPart1();
-                launchForces();
-                launchPart2();
-            }
-        }
-        if (tail) {
-            launchRefineEnd();
+        run_steps(*this, steps, refined);
+        if (refined) {
             refKeSlots = nPartTiles;
             refinedValid = true;
         }
@@ -661,8 +707,7 @@ struct rbhip_context {
     }
 
     ~rbhip_context() {
-        graph.reset();
-        fusedGraph.reset();
+        graphs.reset();
         tileArena.release();
         refArena.release();
         arena.release();
@@ -688,12 +733,10 @@ struct rbhip_batch {
     double* dRed = nullptr;
     int precision = 0, rotationMode = 0;
     bool useFusion = true;
-    GraphCache graph;
-    GraphCache fusedGraph;
+    StepGraphs graphs;
 
     void rebuild() {
-        graph.reset();
-        fusedGraph.reset();
+        graphs.reset();
         arena.release();
         nPart = nForce = 0;
         memberPartTiles.assign(members.size(), 0);
@@ -775,64 +818,7 @@ struct rbhip_batch {
         return false;
     }
 
-    template <class F> void ensureGraph(GraphCache& g, F&& launches) {
-        if (g.exec) return;
-        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        try {
-            launches();
-        } catch (...) {
-            hipGraph_t tmp = nullptr;
-            (void)hipStreamEndCapture(stream, &tmp);
-            if (tmp) (void)hipGraphDestroy(tmp);
-            throw;
-        }
-        HIPCHK(hipStreamEndCapture(stream, &g.graph));
-        HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
-    }
-
-    // as rbhip_context::executeSteps; the refined-energy kernels around the
-    // last step skip the members that have refinement off
-    void executeSteps(int steps) {
-        if (steps <= 0) return;
-        const int tail = anyRefined() ? 1 : 0;
-        if (useFusion && steps >= 2) {
-            launchPart1();
-            launchForces();
-            if (useGraph) {
-                ensureGraph(fusedGraph, [&] { launchFused(); launchForces(); });
-                for (int s = 1; s < steps - tail; s++) HIPCHK(hipGraphLaunch(fusedGraph.exec, stream));
-            } else {
-                for (int s = 1; s < steps - tail; s++) {
-                    launchFused();
-                    launchForces();
-                }
-            }
-            if (tail) {
-                launchRefineBegin(false);
-                launchFused();
-                launchForces();
-            }
-            launchPart2();
-        } else {
-            if (useGraph) {
-                ensureGraph(graph, [&] { launchPart1(); launchForces(); launchPart2(); });
-                for (int s = 0; s < steps - tail; s++) HIPCHK(hipGraphLaunch(graph.exec, stream));
-            } else {
-                for (int s = 0; s < steps - tail; s++) {
-                    launchPart1();
-                    launchForces();
-                    launchPart2();
-                }
-            }
-            if (tail) {
-                launchRefineBegin(true);
-                launchPart1();
-                launchForces();
-                launchPart2();
-            }
-        }
-        if (tail) launchRefineEnd();
-    }
+    void executeSteps(int steps) { run_steps(*this, steps, anyRefined()); }
 
     void finishSteps(int steps) {
         for (size_t m = 0; m < members.size(); m++) {
@@ -865,8 +851,7 @@ struct rbhip_batch {
     hipEvent_t ordering = nullptr;
 
     ~rbhip_batch() {
-        graph.reset();
-        fusedGraph.reset();
+        graphs.reset();
         arena.release();
         if (ordering) (void)hipEventDestroy(ordering);
     }
@@ -1079,7 +1064,7 @@ int rbhip_destroy(rbhip_context* ctx) {
         (void)hipSetDevice(ctx->device);
         // device memory is freed in the context's stream order; only
         // instantiated graphs wait for the stream before they go
-        if (ctx->graph.exec || ctx->fusedGraph.exec) (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->graphs.any()) (void)hipStreamSynchronize(ctx->stream);
         delete ctx;
     });
 }
@@ -1090,11 +1075,10 @@ int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream) {
         hipStream_t next = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->ownStream;
         if (next == ctx->stream) return;
         HIPCHK(hipSetDevice(ctx->device));
-        if (ctx->graph.exec || ctx->fusedGraph.exec) {
+        if (ctx->graphs.any()) {
             // captured graphs may still be running on the old stream
             HIPCHK(hipStreamSynchronize(ctx->stream));
-            ctx->graph.reset();
-            ctx->fusedGraph.reset();
+            ctx->graphs.reset();
         } else {
             // the new stream starts after the work queued on the old one
             if (!ctx->handoff) HIPCHK(hipEventCreateWithFlags(&ctx->handoff, hipEventDisableTiming));
@@ -1365,10 +1349,7 @@ int rbhip_use_graph(rbhip_context* ctx, int enable) {
     return guard([&] {
         require(ctx, "null context");
         ctx->useGraph = enable != 0;
-        if (!ctx->useGraph) {
-            ctx->graph.reset();
-            ctx->fusedGraph.reset();
-        }
+        if (!ctx->useGraph) ctx->graphs.reset();
     });
 }
 
@@ -1732,8 +1713,7 @@ int rbhip_batch_set_stream(rbhip_batch* batch, rbhip_stream stream) {
         HIPCHK(hipStreamSynchronize(batch->stream));
         batch->stream = stream ? reinterpret_cast<hipStream_t>(stream) : pooled_stream(batch->device);
         batch->arena.stream = batch->stream;
-        batch->graph.reset();
-        batch->fusedGraph.reset();
+        batch->graphs.reset();
     });
 }
 
@@ -1759,10 +1739,7 @@ int rbhip_batch_use_graph(rbhip_batch* batch, int enable) {
     return guard([&] {
         require(batch, "null batch");
         batch->useGraph = enable != 0;
-        if (!batch->useGraph) {
-            batch->graph.reset();
-            batch->fusedGraph.reset();
-        }
+        if (!batch->useGraph) batch->graphs.reset();
     });
 }
 

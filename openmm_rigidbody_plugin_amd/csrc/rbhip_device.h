@@ -232,48 +232,13 @@ __device__ __forceinline__ void no_squish(Q4<T>& q, V3<T>& L, V3<T> invI, double
 }
 
 // ---------------------------------------------------------------------------
-// FP64 elliptic integrals, Carlson, Numer. Algorithms 10 (1995) 13-26,
-// Algorithms 1-3 (tolerance r = 1e-16), and Jacobi functions by the AGM /
-// descending Landen transformation (DLMF 22.20(ii)).
+// FP64 elliptic integrals by Carlson's duplication algorithms (Numer.
+// Algorithms 10 (1995) 13-26, Algorithms 1 and 4, tolerance r = 1e-16), and
+// Jacobi functions by Bulirsch's algorithm (Numer. Math. 7 (1965) 78-90):
+// the arithmetic-geometric mean of (1, sqrt(1 - m)), which also gives
+// K(m) = pi / (2 AGM) (DLMF 19.8.5), then one sin/cos and a backward
+// recurrence with one division per level.
 // ---------------------------------------------------------------------------
-__device__ inline double carlson_rc(double x, double y) {
-    double A0 = (x + 2.0 * y) * (1.0 / 3.0);
-    double Q = 87.16855428717356 * fabs(A0 - x);   // (3e-16)^(-1/8)
-    double xm = x, ym = y, Am = A0, scale = 1.0;
-    for (int m = 0; m < 60 && scale * Q >= fabs(Am); m++) {
-        double lam = 2.0 * sqrt(xm) * sqrt(ym) + ym;
-        xm = 0.25 * (xm + lam);
-        ym = 0.25 * (ym + lam);
-        Am = 0.25 * (Am + lam);
-        scale *= 0.25;
-    }
-    double s = (y - A0) * scale / Am;
-    double poly = 1.0 + s * s * (3.0 / 10.0 + s * (1.0 / 7.0 + s * (3.0 / 8.0 + s * (9.0 / 22.0 +
-                  s * (159.0 / 208.0 + s * (9.0 / 8.0))))));
-    return poly / sqrt(Am);
-}
-
-__device__ inline double carlson_rf(double x, double y, double z) {
-    double A0 = (x + y + z) * (1.0 / 3.0);
-    double Q = 386.49729393434575 * fmax(fabs(A0 - x), fmax(fabs(A0 - y), fabs(A0 - z)));  // (3e-16)^(-1/6)
-    double xm = x, ym = y, zm = z, Am = A0, scale = 1.0;
-    for (int m = 0; m < 60 && scale * Q >= fabs(Am); m++) {
-        double sx = sqrt(xm), sy = sqrt(ym), sz = sqrt(zm);
-        double lam = sx * sy + sx * sz + sy * sz;
-        xm = 0.25 * (xm + lam);
-        ym = 0.25 * (ym + lam);
-        zm = 0.25 * (zm + lam);
-        Am = 0.25 * (Am + lam);
-        scale *= 0.25;
-    }
-    double X = (A0 - x) * scale / Am;
-    double Y = (A0 - y) * scale / Am;
-    double Z = -(X + Y);
-    double E2 = X * Y - Z * Z;
-    double E3 = X * Y * Z;
-    return (1.0 - E2 / 10.0 + E3 / 14.0 + E2 * E2 / 24.0 - 3.0 * E2 * E3 / 44.0) / sqrt(Am);
-}
-
 // R_C(1, 1 + e) in closed form (DLMF 19.2.17-19.2.18): atan(sqrt(e))/sqrt(e)
 // for e > 0, atanh(sqrt(-e))/sqrt(-e) for -1 < e < 0 (1 - sqrt(-e) written
 // as (1 + e)/(1 + sqrt(-e)) against cancellation).  Not inlined: the caller
@@ -327,6 +292,48 @@ __device__ inline double carlson_rj(double x, double y, double z, double p) {
     return scale * poly / (Am * sqrt(Am)) + 6.0 * sum;
 }
 
+// R_F(x, y, z) and R_J(x, y, z, p) from one duplication sequence: both
+// algorithms duplicate (x, y, z) with the same lambda (R_J adds p), so one
+// loop serves both; it runs until both meet their tolerances
+__device__ inline void carlson_rf_rj(double x, double y, double z, double p, double& rf, double& rj) {
+    double AF0 = (x + y + z) * (1.0 / 3.0);
+    double AJ0 = (x + y + z + 2.0 * p) * 0.2;
+    double delta = (p - x) * (p - y) * (p - z);
+    double QF = 386.49729393434575 * fmax(fabs(AF0 - x), fmax(fabs(AF0 - y), fabs(AF0 - z)));  // (3e-16)^(-1/6)
+    double QJ = 584.803547642573 *   // (2.5e-17)^(-1/6)
+                fmax(fmax(fabs(AJ0 - x), fabs(AJ0 - y)), fmax(fabs(AJ0 - z), fabs(AJ0 - p)));
+    double xm = x, ym = y, zm = z, pm = p, AF = AF0, AJ = AJ0, scale = 1.0, scale3 = 1.0, sum = 0.0;
+    for (int m = 0; m < 60 && (scale * QJ >= fabs(AJ) || scale * QF >= fabs(AF)); m++) {
+        double sx = sqrt(xm), sy = sqrt(ym), sz = sqrt(zm), sp = sqrt(pm);
+        double lam = sx * sy + sx * sz + sy * sz;
+        double dm = (sp + sx) * (sp + sy) * (sp + sz);
+        double e = scale3 * delta / (dm * dm);
+        sum += scale * carlson_rc_one(e) / dm;
+        xm = 0.25 * (xm + lam);
+        ym = 0.25 * (ym + lam);
+        zm = 0.25 * (zm + lam);
+        pm = 0.25 * (pm + lam);
+        AF = 0.25 * (AF + lam);
+        AJ = 0.25 * (AJ + lam);
+        scale *= 0.25;
+        scale3 *= 1.0 / 64.0;
+    }
+    {
+        double X = (AF0 - x) * scale / AF, Y = (AF0 - y) * scale / AF, Z = -(X + Y);
+        double E2 = X * Y - Z * Z, E3 = X * Y * Z;
+        rf = (1.0 - E2 / 10.0 + E3 / 14.0 + E2 * E2 / 24.0 - 3.0 * E2 * E3 / 44.0) / sqrt(AF);
+    }
+    double X = (AJ0 - x) * scale / AJ, Y = (AJ0 - y) * scale / AJ, Z = (AJ0 - z) * scale / AJ;
+    double P = -0.5 * (X + Y + Z);
+    double E2 = X * Y + X * Z + Y * Z - 3.0 * P * P;
+    double E3 = X * Y * Z + 2.0 * E2 * P + 4.0 * P * P * P;
+    double E4 = (2.0 * X * Y * Z + E2 * P + 3.0 * P * P * P) * P;
+    double E5 = X * Y * Z * P * P;
+    double poly = 1.0 - 3.0 * E2 / 14.0 + E3 / 6.0 + 9.0 * E2 * E2 / 88.0 - 3.0 * E4 / 22.0 -
+                  9.0 * E2 * E3 / 52.0 + 3.0 * E5 / 26.0;
+    rj = scale * poly / (AJ * sqrt(AJ)) + 6.0 * sum;
+}
+
 // Omega(x, n, m) = -(n x^3 / 3) R_J(1 - x^2, 1 - m x^2, 1, 1 + n x^2)
 // (the function used by the exact rotation, reference ellipticFunctions.h:251-254)
 __device__ inline double omega_fn(double x, double n, double m) {
@@ -334,36 +341,65 @@ __device__ inline double omega_fn(double x, double n, double m) {
     return (-1.0 / 3.0) * n * x * x2 * carlson_rj(1.0 - x2, 1.0 - m * x2, 1.0, 1.0 + n * x2);
 }
 
-// K = K(m) = R_F(0, 1 - m, 1), which the caller has already computed
-__device__ inline void jacobi_fn(double u, double m, double K, double& sn, double& cn, double& dn) {
-    const double eps = 2.220446049250313e-16;
-    if (m < 2.0 * eps) { sincos(u, &sn, &cn); dn = 1.0; return; }
-    if (1.0 - m < 2.0 * eps) { sn = tanh(u); cn = 1.0 / cosh(u); dn = cn; return; }
-    if (fabs(u) > 2.0 * K) u -= 4.0 * K * rint(u / (4.0 * K));
-    // AGM levels with fixed trip counts so the ratios c_i / a_i stay in
-    // registers; 10 levels converge for any m < 1 - 2 eps (the AGM doubles
-    // the number of correct digits per level)
-    constexpr int kMaxLevels = 10;
-    double ratio[kMaxLevels + 1];
-    double a = 1.0, c = sqrt(m), b = sqrt(1.0 - m);
-    int n = 0;
+// The AGM levels of (1, sqrt(1 - m)), kept for the Jacobi functions; the
+// fixed trip count keeps the arrays in registers.  The mean c is exact to
+// about CA^2 once |a - b| <= CA a; 8 levels reach that for any 1 - m >= 2 eps
+// (smaller 1 - m takes the m = 1 limit in jacobi_agm).
+constexpr int kAgmLevels = 9;
+struct AgmLevels {
+    double a[kAgmLevels], b[kAgmLevels];
+    double c;    // the arithmetic-geometric mean
+    int last;    // index of the last level used
+};
+__device__ inline void agm_levels(double m, AgmLevels& g) {
+    const double CA = 1.0e-8;
+    double a = 1.0, bb = 1.0 - m, c = 1.0;
+    int last = 0;
+    bool alive = true;
 #pragma unroll
-    for (int i = 0; i < kMaxLevels; i++) {
-        if (fabs(c) > eps * a) {
-            double an = 0.5 * (a + b);
-            c = 0.5 * (a - b);
-            b = sqrt(a * b);
-            a = an;
-            n = i + 1;
+    for (int i = 0; i < kAgmLevels; i++) {
+        if (alive) {
+            g.a[i] = a;
+            bb = sqrt(bb);
+            g.b[i] = bb;
+            c = 0.5 * (a + bb);
+            last = i;
+            if (fabs(a - bb) <= CA * a) {
+                alive = false;
+            } else {
+                bb *= a;
+                a = c;
+            }
         }
-        ratio[i + 1] = c / a;
     }
-    double phi = ldexp(a * u, n);
+    g.c = c;
+    g.last = last;
+}
+
+// sn, cn, dn of (u | m) from the AGM levels of m (Bulirsch's recurrence)
+__device__ inline void jacobi_agm(double u, double m, const AgmLevels& g, double& sn, double& cn, double& dn) {
+    const double eps = 2.220446049250313e-16;
+    if (1.0 - m < 2.0 * eps) { sn = tanh(u); cn = 1.0 / cosh(u); dn = cn; return; }
+    double su, cu;
+    sincos(u * g.c, &su, &cu);
+    sn = su;
+    cn = cu;
+    dn = 1.0;
+    if (su == 0.0) return;
+    double a = cu / su, c = g.c * a;
 #pragma unroll
-    for (int i = kMaxLevels; i > 0; i--)
-        if (i <= n) phi = 0.5 * (phi + asin(ratio[i] * sin(phi)));
-    sincos(phi, &sn, &cn);
-    dn = sqrt((1.0 - m) + m * cn * cn);
+    for (int i = kAgmLevels - 1; i >= 0; i--) {
+        if (i <= g.last) {
+            double b = g.a[i];
+            a *= c;
+            c *= dn;
+            dn = (g.b[i] + a) / (b + a);
+            a = c / b;
+        }
+    }
+    a = 1.0 / sqrt(c * c + 1.0);
+    sn = su >= 0.0 ? a : -a;
+    cn = c * sn;
 }
 
 __device__ __forceinline__ int sign_of(double x) { return x >= 0.0 ? 1 : -1; }
@@ -385,7 +421,12 @@ __device__ inline void linear_rotor_exact(Q4<double>& q, V3<double> L, V3<double
 }
 
 // exact free-rotor rotation of (q, L) over dt (reference RigidBody.cpp:241-308),
-// FP64; L is the body-frame angular momentum (the reference's B(q)^T pi / 2)
+// FP64; L is the body-frame angular momentum (the reference's B(q)^T pi / 2).
+// The reference's two branches on l1 < l3 (:286-300) differ only in their
+// parameters, so they are one computation with selected parameters here (the
+// lanes of a wave then do not run both); K(m) comes from the AGM that the
+// Jacobi functions need anyway, and F(s0 | m) (:270) and Omega(s0) (:289,
+// :297) share one duplication sequence.
 __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> I, V3<double> invI, double dt) {
     const double eps = 2.220446049250313e-16;
     const double PI = 3.14159265358979323846;
@@ -411,49 +452,42 @@ __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> 
     double r3 = twoKr * I.x - Lsq;
     double l1 = r1 * invI.y / (I.y - I.z);
     double l3 = r3 * invI.y / (I.x - I.y);
+    const bool low = l1 < l3;
     double lmin = fmin(l1, l3);
     double c13 = 1.0 / (I.x - I.z);
     V3<double> a = {sign_of(w0.x) * sqrt(r1 * invI.x * c13), sqrt(lmin), sign_of(w0.z) * sqrt(r3 * invI.z * c13)};
     double m = lmin / fmax(l1, l3);
-    double K = carlson_rf(0.0, 1.0 - m, 1.0);
+    AgmLevels g;
+    agm_levels(m, g);
+    double K = (0.5 * PI) / g.c;
     double inv2K = 0.5 / K;
-    double s0 = w0.y / a.y;
-    double c0, u0;
-    int i0;
-    if (fabs(s0) < 1.0) {
-        c0 = l1 < l3 ? w0.x / a.x : w0.z / a.z;
-        u0 = s0 * carlson_rf(1.0 - s0 * s0, 1.0 - m * s0 * s0, 1.0);
-        i0 = stair(u0 * inv2K);
-    } else {
-        a.y = fabs(w0.y);
-        s0 = sign_of(s0);
-        c0 = 0.0;
-        u0 = s0 * K;
-        i0 = 0;
-    }
-    double wp = -invI.y * a.x * a.z / (a.y * c13);
-    double u = wp * dt + u0;
-    int jump = stair(u * inv2K) - i0;
-    double sn, cn, dn, deltaF;
-    jacobi_fn(u, m, K, sn, cn, dn);
     double alpha = I.x * a.x / L;
     double eta = alpha * alpha;
     eta /= 1.0 - eta;
-    V3<double> Ia = hadamard(I, a);
-    if (l1 < l3) {
-        double C = sqrt(m + eta);
-        deltaF = u - u0 + sign_of(cn) * omega_fn(sn, eta, m) - sign_of(c0) * omega_fn(s0, eta, m) +
-                 (alpha / C) * (atan(C * sn / dn) - atan(C * s0 * a.z / w0.z));
-        if (jump != 0) deltaF += jump * 2.0 * omega_fn(1.0, eta, m);
-        Iw = hadamard(Ia, V3<double>{cn, sn, dn});
-    } else {
-        double k2eta = m * eta;
-        double C = sqrt(1.0 + k2eta);
-        deltaF = u - u0 + sign_of(cn) * omega_fn(sn, k2eta, m) - sign_of(c0) * omega_fn(s0, k2eta, m) +
-                 (alpha / C) * (atan(C * sn / cn) - atan(C * s0 / c0));
-        if (jump != 0) deltaF += jump * (2.0 * omega_fn(1.0, k2eta, m) + (alpha / C) * PI);
-        Iw = hadamard(Ia, V3<double>{dn, sn, cn});
+    double npar = low ? eta : m * eta;   // the characteristic of Omega in each branch
+    double s0 = w0.y / a.y;
+    const bool inside = fabs(s0) < 1.0;
+    if (!inside) {
+        a.y = fabs(w0.y);
+        s0 = sign_of(s0);
     }
+    double S0 = s0 * s0, rf0, rj0;
+    carlson_rf_rj(1.0 - S0, 1.0 - m * S0, 1.0, 1.0 + npar * S0, rf0, rj0);
+    double c0 = inside ? (low ? w0.x / a.x : w0.z / a.z) : 0.0;
+    double u0 = inside ? s0 * rf0 : s0 * K;
+    int i0 = inside ? stair(u0 * inv2K) : 0;
+    double wp = -invI.y * a.x * a.z / (a.y * c13);
+    double u = wp * dt + u0;
+    int jump = stair(u * inv2K) - i0;
+    double sn, cn, dn;
+    jacobi_agm(u, m, g, sn, cn, dn);
+    V3<double> Ia = hadamard(I, a);
+    double C = sqrt(low ? m + eta : 1.0 + m * eta);
+    double om0 = (-1.0 / 3.0) * npar * s0 * S0 * rj0;
+    double deltaF = u - u0 + sign_of(cn) * omega_fn(sn, npar, m) - sign_of(c0) * om0 +
+                    (alpha / C) * (atan(C * sn / (low ? dn : cn)) - atan(C * s0 * a.z / w0.z));
+    if (jump != 0) deltaF += jump * (2.0 * omega_fn(1.0, npar, m) + (low ? 0.0 : (alpha / C) * PI));
+    Iw = low ? hadamard(Ia, V3<double>{cn, sn, dn}) : hadamard(Ia, V3<double>{dn, sn, cn});
     deltaF *= 1.0 + eta;
     double theta = (Lsq * (u - u0) + r3 * deltaF) / (2.0 * L * I.x * wp);
     double st, ct;

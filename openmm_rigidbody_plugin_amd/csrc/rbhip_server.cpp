@@ -12,20 +12,37 @@
 //
 // A resident job is stepped with the same kernels and tiles as when it is
 // stepped alone with the same workgroup size, so its result does not depend
-// on which other jobs share its rounds.  A round returns once its steps are
-// queued: the next round's admissions overlap with them on the device.
+// on which other jobs share its rounds.
+//
+// Threads: run_round (the caller's thread) only hands waiting jobs that fit
+// the budget to the admission threads, hands finished jobs to a collection
+// thread, updates the batch members and queues the next round; it waits for
+// the device only to keep at most kRoundsInFlight rounds queued.  The
+// admission threads build new jobs' device state, several at a time (each
+// set-up waits for its copies and reductions on the device); the collection
+// thread downloads a finished job's state and frees it.  A context belongs
+// to one thread at a time: created by an admission thread, stepped by
+// run_round, released by the collection thread.
 //
 // Streams: each of the two batches runs on a stream of its own, the
-// resident jobs share one (the batch orders it after each advance),
-// admissions use another and finished jobs a last one, so neither setting up
-// a new job nor collecting a finished one waits behind the queued steps.
+// resident jobs share one (the batch orders it after each advance), each
+// admission thread has one and finished jobs rotate over kRetireStreams more,
+// so neither setting up a new job nor collecting a finished one waits behind
+// the queued steps when the streams map to different hardware queues.  HIP
+// maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+// default); streams that share one run in order, so a set-up can wait behind
+// queued rounds, which is why several set-ups run at a time.
+
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -87,7 +104,9 @@ double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::d
 
 }  // namespace
 
-constexpr int kAdmitPerRound = 8;
+constexpr int kAdmitPerRound = 8;   // jobs handed to the set-up thread per round, at most
+constexpr int kRoundsInFlight = 2;  // rounds queued on the device ahead of the host, at most
+constexpr int kAdmitThreads = 4;    // jobs set up in parallel (each waits on the device in turns)
 
 struct rbhip_server {
     int device = 0;
@@ -97,39 +116,87 @@ struct rbhip_server {
     int block = 256;
     int exactKind = -1;                 // rotation kind of the jobs: 1 exact, 0 NO-SQUISH
     long long nextId = 1;
-    size_t inUse = 0;
     long long bodySteps = 0;
     long long rounds = 0;
-    double phase[4] = {0.0, 0.0, 0.0, 0.0};   // host seconds: admit, collect, members, launch
-    std::deque<std::unique_ptr<Job>> queue;
-    std::vector<std::unique_ptr<Job>> resident;     // batch members
-    std::vector<std::unique_ptr<Job>> pending;      // admitted this round, members from the next launch
+    double phase[4] = {0.0, 0.0, 0.0, 0.0};   // host seconds: admit, collect (under mtx), members, launch
+
+    // Shared with the worker threads (under mtx): jobs chosen for admission,
+    // jobs whose contexts are ready, finished jobs to collect, completed jobs,
+    // the device bytes held, and the first worker error.  kAdmitThreads
+    // threads set up new jobs, one more collects finished ones, so an
+    // admission never waits behind a download.
+    std::mutex mtx;
+    std::condition_variable wake;       // the worker threads wait for work
+    std::condition_variable ready;      // run_round waits for an admission
+    std::deque<std::unique_ptr<Job>> queue;          // waiting (run_round thread only)
+    std::deque<std::unique_ptr<Job>> toAdmit;
+    std::vector<std::unique_ptr<Job>> admitted;
+    std::deque<std::unique_ptr<Job>> toCollect;
     std::map<long long, std::unique_ptr<Job>> completed;
+    size_t inUse = 0;
+    int busy[2] = {0, 0};               // jobs being admitted, being collected
+    bool stop = false;
+    int workerStatus = RBHIP_OK;
+    std::string workerError;
+    std::thread workers[kAdmitThreads + 1];   // admission threads, then the collection thread
+
+    // run_round thread only
+    std::vector<std::unique_ptr<Job>> resident;     // batch members
     // two batches: one runs the queued round while the other takes a new
     // member list; members[b] is the list batch b was last given
     rbhip_batch* batches[2] = {nullptr, nullptr};
     std::vector<rbhip_context*> members[2];
     int cur = 0;
     hipStream_t batchStreams[2] = {nullptr, nullptr};
-    hipStream_t memberStream = nullptr, admitStream = nullptr, retireStream = nullptr;
+    hipStream_t memberStream = nullptr;
+    hipStream_t admitStreams[kAdmitThreads] = {};
+    // finished jobs move to retire stream (round index mod kRetireStreams):
+    // a download then waits only for the rounds queued on that stream, not
+    // for the jobs that finished after it
+    static constexpr int kRetireStreams = 4;
+    hipStream_t retireStreams[kRetireStreams] = {nullptr, nullptr, nullptr, nullptr};
+    long long queuedRounds = 0;
+    std::deque<hipEvent_t> inFlight;    // one event per queued round, oldest first
+    std::vector<hipEvent_t> spareEvents;
+
+    std::vector<hipStream_t> allStreams() const {
+        std::vector<hipStream_t> v = {batchStreams[0], batchStreams[1], memberStream};
+        v.insert(v.end(), admitStreams, admitStreams + kAdmitThreads);
+        v.insert(v.end(), retireStreams, retireStreams + kRetireStreams);
+        return v;
+    }
 
     ~rbhip_server() {
-        for (hipStream_t st : {batchStreams[0], batchStreams[1], memberStream, admitStream, retireStream})
+        {
+            std::lock_guard<std::mutex> lock(mtx);
+            stop = true;
+        }
+        wake.notify_all();
+        for (std::thread& t : workers)
+            if (t.joinable()) t.join();
+        for (hipStream_t st : allStreams())
             if (st) (void)hipStreamSynchronize(st);
         for (rbhip_batch* b : batches)
             if (b) (void)rbhip_batch_destroy(b);
         for (auto& j : resident)
             if (j->ctx) (void)rbhip_destroy(j->ctx);
-        for (auto& j : pending)
+        for (auto& j : admitted)
             if (j->ctx) (void)rbhip_destroy(j->ctx);
-        for (hipStream_t st : {batchStreams[0], batchStreams[1], memberStream, admitStream, retireStream})
+        for (auto& j : toCollect)
+            if (j->ctx) (void)rbhip_destroy(j->ctx);
+        for (hipEvent_t e : inFlight) (void)hipEventDestroy(e);
+        for (hipEvent_t e : spareEvents) (void)hipEventDestroy(e);
+        for (hipStream_t st : allStreams())
             if (st) {
                 (void)hipStreamSynchronize(st);
                 (void)hipStreamDestroy(st);
             }
     }
 
-    void admit(Job& j) {
+    // ---- worker threads ----
+    // creates a job's context on an admission stream and moves it to the
+    // resident jobs' stream
+    void admit(Job& j, hipStream_t admitStream) {
         rbhip_context* c = nullptr;
         call(rbhip_create(j.numAtoms, j.masses.data(), j.bodyIndices.data(), j.stepSize, j.rotationMode, precision,
                           device, &c));
@@ -139,9 +206,6 @@ struct rbhip_server {
             if (!j.tetherK.empty()) call(rbhip_set_tether(c, j.tetherK.data(), j.tetherR0.data()));
             call(rbhip_set_positions(c, j.positions.data()));
             if (!j.velocities.empty()) call(rbhip_set_velocities(c, j.velocities.data()));
-            size_t b = 0;
-            call(rbhip_device_bytes(c, &b));
-            j.bytes = std::max(j.bytes, b);
             call(rbhip_set_stream(c, memberStream));
         } catch (...) {
             (void)rbhip_destroy(c);
@@ -158,36 +222,129 @@ struct rbhip_server {
         std::vector<int>().swap(j.bodyIndices);
     }
 
+    // downloads a finished job's final state (on the retire stream, after the
+    // round it finished in) and frees its device state
+    void collect(Job& j) {
+        j.outPos.resize(3 * (size_t)j.numAtoms);
+        j.outVel.resize(3 * (size_t)j.numAtoms);
+        call(rbhip_get_positions(j.ctx, j.outPos.data()));
+        call(rbhip_get_velocities(j.ctx, j.outVel.data()));
+        call(rbhip_kinetic_energies(j.ctx, j.ke));
+        call(rbhip_destroy(j.ctx));
+        j.ctx = nullptr;
+        j.finished = Clock::now();
+    }
+
+    // an admission thread (slot < kAdmitThreads) or the collection thread
+    void work(int slot) {
+        const bool collecting = slot == kAdmitThreads;
+        std::unique_lock<std::mutex> lock(mtx);
+        if (hipSetDevice(device) != hipSuccess) {
+            workerStatus = RBHIP_E_HIP;
+            workerError = "worker thread: hipSetDevice failed";
+            ready.notify_all();
+            return;
+        }
+        std::deque<std::unique_ptr<Job>>& in = collecting ? toCollect : toAdmit;
+        for (;;) {
+            wake.wait(lock, [&] { return stop || !in.empty(); });
+            if (stop) return;
+            std::unique_ptr<Job> j = std::move(in.front());
+            in.pop_front();
+            busy[collecting]++;
+            lock.unlock();
+            Clock::time_point t0 = Clock::now();
+            int status = RBHIP_OK;
+            std::string error;
+            try {
+                if (collecting) collect(*j);
+                else admit(*j, admitStreams[slot]);
+            } catch (const ServerError& e) {
+                status = e.code;
+                error = e.what();
+            } catch (const std::exception& e) {
+                status = RBHIP_E_ARG;
+                error = e.what();
+            }
+            double dt = seconds(t0, Clock::now());
+            lock.lock();
+            busy[collecting]--;
+            phase[collecting ? 1 : 0] += dt;
+            if (status != RBHIP_OK) {
+                if (workerStatus == RBHIP_OK) {
+                    workerStatus = status;
+                    workerError = (collecting ? "collecting job " : "admitting job ") + std::to_string(j->id) +
+                                  ": " + error;
+                }
+                if (j->ctx) (void)rbhip_destroy(j->ctx);
+                inUse -= std::min(inUse, j->bytes);
+            } else if (collecting) {
+                inUse -= std::min(inUse, j->bytes);
+                long long id = j->id;
+                completed[id] = std::move(j);
+            } else {
+                admitted.push_back(std::move(j));
+            }
+            ready.notify_all();
+        }
+    }
+
+    // ---- run_round thread ----
+    void checkWorker() {
+        if (workerStatus != RBHIP_OK) throw ServerError(workerStatus, workerError);
+    }
+
+    // waits for the oldest queued round to finish
+    void retireOldestRound() {
+        hipEvent_t e = inFlight.front();
+        inFlight.pop_front();
+        if (hipEventSynchronize(e) != hipSuccess) throw ServerError(RBHIP_E_HIP, "round event failed");
+        spareEvents.push_back(e);
+    }
+
     // One round.
-    //  1. Admission: waiting jobs are set up while the round queued by the
-    //     previous call runs on the device.
-    //  2. Jobs that reached their step count move to the retire stream,
-    //     which then continues after the round they finished in.
-    //  3. The next round is queued: on the same batch if the member list is
-    //     unchanged, else on the other batch (idle, so its tables can be
-    //     rebuilt while the queued round runs); it starts when that round ends.
-    //  4. The finished jobs are collected (their downloads wait only for the
-    //     round they finished in) and their device memory is released.
+    //  1. Jobs whose contexts the set-up thread has finished join the
+    //     resident jobs; waiting jobs that fit the budget go to the set-up
+    //     thread (at most kAdmitPerRound per round).  With nothing resident,
+    //     the round waits for the first admission.
+    //  2. Jobs that reached their step count move to the retire stream, which
+    //     continues after the round they finished in, and go to the set-up
+    //     thread for collection.
+    //  3. With kRoundsInFlight rounds queued, the oldest is waited for.  The
+    //     next round is queued on the same batch if the member list is
+    //     unchanged, else on the other batch (idle: its tables are rebuilt
+    //     while the queued round runs); it starts when that round ends.
     int round() {
-        rounds++;
-        Clock::time_point t0 = Clock::now();
-        int admitted = 0;
-        while (!queue.empty() && inUse + queue.front()->bytes <= budget && admitted < kAdmitPerRound) {
-            admitted++;
-            std::unique_ptr<Job> j = std::move(queue.front());
-            queue.pop_front();
-            admit(*j);
-            inUse += j->bytes;
-            pending.push_back(std::move(j));
+        std::vector<std::unique_ptr<Job>> done, next;
+        for (auto& j : resident) (j->done >= j->steps ? done : next).push_back(std::move(j));
+        resident.clear();
+        for (auto& j : done) call(rbhip_set_stream(j->ctx, retireStreams[queuedRounds % kRetireStreams]));
+        int finished = 0;
+        {
+            std::unique_lock<std::mutex> lock(mtx);
+            checkWorker();
+            for (auto& j : done) toCollect.push_back(std::move(j));
+            size_t before = completed.size();
+            int handed = 0;
+            while (!queue.empty() && inUse + queue.front()->bytes <= budget && handed < kAdmitPerRound) {
+                inUse += queue.front()->bytes;
+                toAdmit.push_back(std::move(queue.front()));
+                queue.pop_front();
+                handed++;
+            }
+            if (handed || !toCollect.empty()) wake.notify_all();
+            if (next.empty() && admitted.empty() && (!toAdmit.empty() || busy[0] > 0))
+                ready.wait(lock, [&] { return !admitted.empty() || workerStatus != RBHIP_OK ||
+                                              (toAdmit.empty() && busy[0] == 0); });
+            checkWorker();
+            for (auto& j : admitted) next.push_back(std::move(j));
+            admitted.clear();
+            finished = (int)(completed.size() - before);
         }
         Clock::time_point t1 = Clock::now();
-        std::vector<std::unique_ptr<Job>> next, done;
-        for (auto& j : resident) (j->done >= j->steps ? done : next).push_back(std::move(j));
-        for (auto& j : pending) next.push_back(std::move(j));
-        pending.clear();
-        for (auto& j : done) call(rbhip_set_stream(j->ctx, retireStream));
         std::vector<rbhip_context*> m;
         for (auto& j : next) m.push_back(j->ctx);
+        if (!m.empty() && (int)inFlight.size() >= kRoundsInFlight) retireOldestRound();
         if (!m.empty() && m != members[cur]) {
             cur = 1 - cur;
             if (m != members[cur]) {
@@ -200,34 +357,29 @@ struct rbhip_server {
         for (auto& j : next) n = std::min(n, j->steps - j->done);
         if (!next.empty() && n > 0) {
             call(rbhip_batch_execute(batches[cur], (int)n));
+            hipEvent_t e = nullptr;
+            if (!spareEvents.empty()) {
+                e = spareEvents.back();
+                spareEvents.pop_back();
+            } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                throw ServerError(RBHIP_E_HIP, "event creation failed");
+            }
+            if (hipEventRecord(e, batchStreams[cur]) != hipSuccess) throw ServerError(RBHIP_E_HIP, "event record failed");
+            inFlight.push_back(e);
+            queuedRounds++;
             for (auto& j : next) {
                 j->done += n;
                 bodySteps += j->bodies * n;
             }
         }
         resident.swap(next);
-        Clock::time_point t3 = Clock::now();
-        int finished = 0;
-        for (auto& j : done) {
-            j->outPos.resize(3 * (size_t)j->numAtoms);
-            j->outVel.resize(3 * (size_t)j->numAtoms);
-            call(rbhip_get_positions(j->ctx, j->outPos.data()));
-            call(rbhip_get_velocities(j->ctx, j->outVel.data()));
-            call(rbhip_kinetic_energies(j->ctx, j->ke));
-            j->finished = Clock::now();
-            call(rbhip_destroy(j->ctx));
-            j->ctx = nullptr;
-            inUse -= std::min(inUse, j->bytes);
-            completed[j->id] = std::move(j);
-            finished++;
-        }
         // a batch whose member list held finished jobs forgets it (their
         // handles are gone)
         for (int b = 0; b < 2; b++)
             if (b != cur || resident.empty()) members[b].clear();
-        Clock::time_point t4 = Clock::now();
-        phase[0] += seconds(t0, t1);
-        phase[1] += seconds(t3, t4);
+        Clock::time_point t3 = Clock::now();
+        std::lock_guard<std::mutex> lock(mtx);
+        rounds = queuedRounds;
         phase[2] += seconds(t1, t2);
         phase[3] += seconds(t2, t3);
         return finished;
@@ -253,8 +405,10 @@ int rbhip_server_create(int device, size_t hbmBudget, int stepsPerRound, int pre
         s->stepsPerRound = stepsPerRound;
         s->precision = precision;
         s->block = blockSize;
-        for (hipStream_t* st : {&s->batchStreams[0], &s->batchStreams[1], &s->memberStream, &s->admitStream,
-                                &s->retireStream})
+        std::vector<hipStream_t*> streams = {&s->batchStreams[0], &s->batchStreams[1], &s->memberStream};
+        for (hipStream_t& st : s->admitStreams) streams.push_back(&st);
+        for (hipStream_t& st : s->retireStreams) streams.push_back(&st);
+        for (hipStream_t* st : streams)
             if (hipStreamCreateWithFlags(st, hipStreamNonBlocking) != hipSuccess)
                 throw ServerError(RBHIP_E_HIP, "stream creation failed");
         for (int b = 0; b < 2; b++) {
@@ -262,6 +416,8 @@ int rbhip_server_create(int device, size_t hbmBudget, int stepsPerRound, int pre
             call(rbhip_batch_set_stream(s->batches[b], s->batchStreams[b]));
             call(rbhip_batch_set_launch(s->batches[b], blockSize, 0));
         }
+        rbhip_server* p = s.get();
+        for (int w = 0; w <= kAdmitThreads; w++) s->workers[w] = std::thread([p, w] { p->work(w); });
         *out = s.release();
     });
 }
@@ -322,9 +478,12 @@ int rbhip_server_run_round(rbhip_server* s, int* finished) {
 int rbhip_server_status(rbhip_server* s, long long* counts3, size_t* bytesInUse, long long* bodySteps) {
     return guard([&] {
         need(s != nullptr, "null server");
+        std::lock_guard<std::mutex> lock(s->mtx);
         if (counts3) {
+            // jobs in set-up or collection hold device memory: resident
             counts3[0] = (long long)s->queue.size();
-            counts3[1] = (long long)s->resident.size();
+            counts3[1] = (long long)(s->resident.size() + s->toAdmit.size() + s->admitted.size() +
+                                     s->toCollect.size() + s->busy[0] + s->busy[1]);
             counts3[2] = (long long)s->completed.size();
         }
         if (bytesInUse) *bytesInUse = s->inUse;
@@ -335,6 +494,7 @@ int rbhip_server_status(rbhip_server* s, long long* counts3, size_t* bytesInUse,
 int rbhip_server_profile(rbhip_server* s, long long* rounds, double* seconds4) {
     return guard([&] {
         need(s != nullptr, "null server");
+        std::lock_guard<std::mutex> lock(s->mtx);
         if (rounds) *rounds = s->rounds;
         if (seconds4)
             for (int i = 0; i < 4; i++) seconds4[i] = s->phase[i];
@@ -344,6 +504,7 @@ int rbhip_server_profile(rbhip_server* s, long long* rounds, double* seconds4) {
 int rbhip_server_completed(rbhip_server* s, long long* ids, int max, int* count) {
     return guard([&] {
         need(s && count && (max == 0 || ids), "bad arguments");
+        std::lock_guard<std::mutex> lock(s->mtx);
         int n = 0;
         for (auto& kv : s->completed) {
             if (n >= max) break;
@@ -357,9 +518,15 @@ int rbhip_server_take(rbhip_server* s, long long jobId, double* positions, doubl
                       double* times3) {
     return guard([&] {
         need(s != nullptr, "null server");
-        auto it = s->completed.find(jobId);
-        need(it != s->completed.end(), "no completed job with id " + std::to_string(jobId), RBHIP_E_STATE);
-        Job& j = *it->second;
+        std::unique_ptr<Job> owned;
+        {
+            std::lock_guard<std::mutex> lock(s->mtx);
+            auto it = s->completed.find(jobId);
+            need(it != s->completed.end(), "no completed job with id " + std::to_string(jobId), RBHIP_E_STATE);
+            owned = std::move(it->second);
+            s->completed.erase(it);
+        }
+        Job& j = *owned;
         if (positions) std::copy(j.outPos.begin(), j.outPos.end(), positions);
         if (velocities) std::copy(j.outVel.begin(), j.outVel.end(), velocities);
         if (ke2) {
@@ -371,7 +538,6 @@ int rbhip_server_take(rbhip_server* s, long long jobId, double* positions, doubl
             times3[1] = seconds(j.admitted, j.finished);     // resident
             times3[2] = seconds(j.submitted, j.finished);    // completion latency
         }
-        s->completed.erase(it);
     });
 }
 
