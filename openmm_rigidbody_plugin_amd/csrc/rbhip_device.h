@@ -420,14 +420,15 @@ __device__ inline void linear_rotor_exact(Q4<double>& q, V3<double> L, V3<double
     q = q * r;
 }
 
-// exact free-rotor rotation of (q, L) over dt (reference RigidBody.cpp:241-308),
-// FP64; L is the body-frame angular momentum (the reference's B(q)^T pi / 2).
-// The reference's two branches on l1 < l3 (:286-300) differ only in their
-// parameters, so they are one computation with selected parameters here (the
-// lanes of a wave then do not run both); K(m) comes from the AGM that the
-// Jacobi functions need anyway, and F(s0 | m) (:270) and Omega(s0) (:289,
-// :297) share one duplication sequence.
-__device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> I, V3<double> invI, double dt) {
+// exact free-rotor rotation of (q, L) over dt by the reference's closed form
+// (RigidBody.cpp:241-308), FP64; L is the body-frame angular momentum (the
+// reference's B(q)^T pi / 2).  The reference's two branches on l1 < l3
+// (:286-300) differ only in their parameters, so they are one computation
+// with selected parameters here (the lanes of a wave then do not run both);
+// K(m) comes from the AGM that the Jacobi functions need anyway, and
+// F(s0 | m) (:270) and Omega(s0) (:289, :297) share one duplication sequence.
+__device__ inline void exact_rotation_closed(Q4<double>& q, V3<double>& Lb, V3<double> I, V3<double> invI,
+                                                   double dt) {
     const double eps = 2.220446049250313e-16;
     const double PI = 3.14159265358979323846;
     V3<double> Iw = Lb;
@@ -497,6 +498,171 @@ __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> 
     double r = 1.0 / sqrt(dot(q, q));
     q = q * r;
     Lb = Iw;
+}
+
+// Taylor coefficients of the Jacobi functions in u, each a polynomial in m
+// (generated exactly by gen_jacobi_series.py from sn' = cn dn, cn' = -sn dn,
+// dn' = -m sn cn): odd powers 1..17 of sn, even powers 0..16 of cn and dn.
+// The truncated series are exact to 2e-16 for |u| <= 0.2 and any 0 <= m < 1.
+__device__ constexpr double kSnCoef[9][9] = {
+    {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {-0.16666666666666666, -0.16666666666666666, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {0.008333333333333333, 0.11666666666666667, 0.008333333333333333, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {-0.0001984126984126984, -0.026785714285714284, -0.026785714285714284, -0.0001984126984126984, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {2.7557319223985893e-06, 0.0033840388007054674, 0.015095899470899472, 0.0033840388007054674, 2.7557319223985893e-06, 0.0, 0.0, 0.0, 0.0},
+    {-2.505210838544172e-08, -0.0002773017877184544, -0.004154290925124258, -0.004154290925124258, -0.0002773017877184544, -2.505210838544172e-08, 0.0, 0.0, 0.0},
+    {1.6059043836821613e-10, 1.6001552459885793e-05, 0.0007217497972706306, 0.0021166250159305715, 0.0007217497972706306, 1.6001552459885793e-05, 1.6059043836821613e-10, 0.0, 0.0},
+    {-7.647163731819816e-13, -6.857999376187207e-07, -8.893244055694453e-05, -0.0006382989522663795, -0.0006382989522663795, -8.893244055694453e-05, -6.857999376187207e-07, -7.647163731819816e-13, 0.0},
+    {2.8114572543455206e-15, 2.269199123287981e-08, 8.293700555544529e-06, 0.00013256618181982426, 0.0003082622922067598, 0.00013256618181982426, 8.293700555544529e-06, 2.269199123287981e-08, 2.8114572543455206e-15},
+};
+__device__ constexpr double kCnCoef[9][9] = {
+    {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {-0.5, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {0.041666666666666664, 0.16666666666666666, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {-0.001388888888888889, -0.06111111111111111, -0.022222222222222223, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {2.48015873015873e-05, 0.01011904761904762, 0.02261904761904762, 0.0015873015873015873, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {-2.755731922398589e-07, -0.0010163139329805997, -0.008478835978835978, -0.004356261022927689, -7.054673721340387e-05, 0.0, 0.0, 0.0, 0.0},
+    {2.08767569878681e-09, 6.933588530810753e-05, 0.0018176139703917481, 0.0032120143231254344, 0.0005413927636149858, 2.1377799155576935e-06, 0.0, 0.0, 0.0},
+    {-1.1470745597729725e-11, -3.428972923020542e-06, -0.00026319690010166203, -0.0012264866233120202, -0.0007457005076052695, -4.79590955781432e-05, -4.698417396830095e-08, 0.0, 0.0},
+    {4.779477332387385e-14, 1.285878228669234e-07, 2.8037847110267745e-05, 0.00030290726934245453, 0.0004712798102348367, 0.00012125411331760538, 3.2045164320296596e-06, 7.830695661383492e-10, 0.0},
+};
+__device__ constexpr double kDnCoef[9][9] = {
+    {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {0.0, -0.5, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {0.0, 0.16666666666666666, 0.041666666666666664, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {0.0, -0.022222222222222223, -0.06111111111111111, -0.001388888888888889, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {0.0, 0.0015873015873015873, 0.02261904761904762, 0.01011904761904762, 2.48015873015873e-05, 0.0, 0.0, 0.0, 0.0},
+    {0.0, -7.054673721340387e-05, -0.004356261022927689, -0.008478835978835978, -0.0010163139329805997, -2.755731922398589e-07, 0.0, 0.0, 0.0},
+    {0.0, 2.1377799155576935e-06, 0.0005413927636149858, 0.0032120143231254344, 0.0018176139703917481, 6.933588530810753e-05, 2.08767569878681e-09, 0.0, 0.0},
+    {0.0, -4.698417396830095e-08, -4.79590955781432e-05, -0.0007457005076052695, -0.0012264866233120202, -0.00026319690010166203, -3.428972923020542e-06, -1.1470745597729725e-11, 0.0},
+    {0.0, 7.830695661383492e-10, 3.2045164320296596e-06, 0.00012125411331760538, 0.0004712798102348367, 0.00030290726934245453, 2.8037847110267745e-05, 1.285878228669234e-07, 4.779477332387385e-14},
+};
+
+struct JacobiSeries { double s[9], c[9], d[9]; };   // the coefficients at one m
+
+__device__ inline void jacobi_series_at(double m, JacobiSeries& J) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        double ps = kSnCoef[k][8], pc = kCnCoef[k][8], pd = kDnCoef[k][8];
+#pragma unroll
+        for (int j = 7; j >= 0; j--) {
+            ps = fma(ps, m, kSnCoef[k][j]);
+            pc = fma(pc, m, kCnCoef[k][j]);
+            pd = fma(pd, m, kDnCoef[k][j]);
+        }
+        J.s[k] = ps;
+        J.c[k] = pc;
+        J.d[k] = pd;
+    }
+}
+
+__device__ inline void jacobi_series(double u, const JacobiSeries& J, double& sn, double& cn, double& dn) {
+    double u2 = u * u;
+    double ps = J.s[8], pc = J.c[8], pd = J.d[8];
+#pragma unroll
+    for (int k = 7; k >= 0; k--) {
+        ps = fma(ps, u2, J.s[k]);
+        pc = fma(pc, u2, J.c[k]);
+        pd = fma(pd, u2, J.d[k]);
+    }
+    sn = u * ps;
+    cn = pc;
+    dn = pd;
+}
+
+// exact free-rotor rotation of (q, L) over dt, FP64.  The reference's solution
+// is L(t) = I a (cn, sn, dn)(u0 + wp t) (or (dn, sn, cn)) and
+// q(t + dt) = q (x) z0^* (x) z(theta) (RigidBody.cpp:251, 302-305).  When the
+// elliptic argument advances little in one step, |wp dt| <= 0.2 (at 300 K and
+// dt <= 2 fs it stays below 0.09), the same solution is evaluated without
+// elliptic integrals:
+//  - the Jacobi functions at u0 + x follow from their values at u0, which are
+//    the angular velocity's components over a (s0, c0, d0), and their values
+//    at x (the series above) by the addition theorem (DLMF 22.8.1-3);
+//  - theta satisfies 2 dtheta/dt = (|L| + L1) X - L1 / I1 with
+//    X = (L2^2/I2 + L3^2/I3) / (L2^2 + L3^2): q (x) n(t)^* stays constant for
+//    the unit frame quaternion n of z(theta), and Euler's equations give its
+//    rate; five-point Gauss-Legendre quadrature integrates it over the step.
+// Against a tight ODE integration of Euler's equations this is exact to
+// ~1e-15, where the closed form loses up to ~1e-11 near the axis-1 pole and
+// the separatrix.  Longer steps take the closed form.
+__device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> I, V3<double> invI, double dt) {
+    const double eps = 2.220446049250313e-16;
+    // the frame quaternion z(L) is singular for L along +e1, and so is the
+    // rate of theta in the complex neighbourhood of a step that passes near
+    // it; a body with L1 > 0 is turned by pi about e3 first (q -> P3 q,
+    // L -> (-L1, -L2, L3): the inertia tensor is unchanged) and back after
+    const bool flip = Lb.x > 0.0;
+    V3<double> Iw = flip ? V3<double>{-Lb.x, -Lb.y, Lb.z} : Lb;
+    V3<double> w0 = hadamard(invI, Iw);
+    double Lsq = Iw.y * Iw.y + Iw.z * Iw.z;
+    if (Lsq < eps) {
+        exact_rotation_closed(q, Lb, I, invI, dt);
+        return;
+    }
+    Lsq += Iw.x * Iw.x;
+    double L = sqrt(Lsq);
+    double twoKr = dot(Iw, w0);
+    double r1 = Lsq - twoKr * I.z;
+    double r3 = twoKr * I.x - Lsq;
+    double l1 = r1 * invI.y / (I.y - I.z);
+    double l3 = r3 * invI.y / (I.x - I.y);
+    const bool low = l1 < l3;
+    double lmin = fmin(l1, l3);
+    double c13 = 1.0 / (I.x - I.z);
+    V3<double> a = {sign_of(w0.x) * sqrt(r1 * invI.x * c13), sqrt(lmin), sign_of(w0.z) * sqrt(r3 * invI.z * c13)};
+    double m = lmin / fmax(l1, l3);
+    double s0 = w0.y / a.y;
+    const bool inside = fabs(s0) < 1.0;
+    if (!inside) {
+        a.y = fabs(w0.y);
+        s0 = sign_of(s0);
+    }
+    double wp = -invI.y * a.x * a.z / (a.y * c13);
+    double delta = wp * dt;
+    if (!(fabs(delta) <= 0.2) || !(m >= 0.0 && m < 1.0)) {
+        exact_rotation_closed(q, Lb, I, invI, dt);
+        return;
+    }
+    double c0 = inside ? (low ? w0.x / a.x : w0.z / a.z) : 0.0;
+    double d0 = inside ? (low ? w0.z / a.z : w0.x / a.x) : sqrt(1.0 - m);
+    V3<double> Ia = hadamard(I, a);
+    JacobiSeries J;
+    jacobi_series_at(m, J);
+    const double ms0 = m * s0, c0d0 = c0 * d0;
+    // body-frame L at u0 + x
+    auto body_L = [&](double x) {
+        double sx, cx, dx;
+        jacobi_series(x, J, sx, cx, dx);
+        double inv = 1.0 / (1.0 - ms0 * s0 * sx * sx);
+        double sn = (s0 * cx * dx + sx * c0d0) * inv;
+        double cn = (c0 * cx - s0 * sx * d0 * dx) * inv;
+        double dn = (d0 * dx - ms0 * sx * c0 * cx) * inv;
+        return low ? V3<double>{Ia.x * cn, Ia.y * sn, Ia.z * dn} : V3<double>{Ia.x * dn, Ia.y * sn, Ia.z * cn};
+    };
+    // 2 dtheta/dt at body-frame L
+    auto rate = [&](V3<double> Lt) {
+        double y2 = Lt.y * Lt.y, z2 = Lt.z * Lt.z;
+        return (L + Lt.x) * (y2 * invI.y + z2 * invI.z) / (y2 + z2) - Lt.x * invI.x;
+    };
+    // Gauss-Legendre nodes and weights on [-1, 1] (5 points)
+    const double gx1 = 0.5384693101056831, gx2 = 0.9061798459386640;
+    const double gw0 = 0.5688888888888889, gw1 = 0.4786286704993665, gw2 = 0.2369268850561891;
+    const double h = 0.5 * delta;
+    double sum = gw0 * rate(body_L(h)) + gw1 * (rate(body_L(h * (1.0 - gx1))) + rate(body_L(h * (1.0 + gx1)))) +
+                 gw2 * (rate(body_L(h * (1.0 - gx2))) + rate(body_L(h * (1.0 + gx2))));
+    double theta = 0.25 * dt * sum;
+    V3<double> Iw1 = body_L(delta);
+    Q4<double> z0 = {Iw.z, Iw.y, L - Iw.x, 0.0};
+    double st, ct;
+    sincos(theta, &st, &ct);
+    Q4<double> z = Q4<double>{Iw1.z, Iw1.y, L - Iw1.x, 0.0} * ct + Q4<double>{-Iw1.y, Iw1.z, 0.0, L - Iw1.x} * st;
+    Q4<double> qf = flip ? perm<3>(q) : q;
+    qf = z * dot(z0, qf) + Cmul(z, Ctmul(z0, qf));
+    double r = 1.0 / sqrt(dot(qf, qf));
+    // P3 P3 q = -q: the turn back is -P3
+    q = flip ? perm<3>(qf) * -r : qf * r;
+    Lb = flip ? V3<double>{-Iw1.x, -Iw1.y, Iw1.z} : Iw1;
 }
 
 // ---------------------------------------------------------------------------

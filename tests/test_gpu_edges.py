@@ -106,3 +106,44 @@ def test_start_at_rest(hip, oracle, mode):
     spec = clusters([3] * 30 + [2] * 20, 20, seed=6, mode=mode)
     spec.tether_r0 = spec.tether_r0 + 0.05     # displaced anchors: the bodies start to move
     compare(oracle, hip, spec, "mixed", start_at_rest=True)
+
+
+def rotation_arg_step(I, L, dt):
+    """|wp dt|: how far the exact free-rotor solution advances the Jacobi
+    argument in one step (the exact rotation evaluates short advances, up to
+    0.2, by series and quadrature and longer ones by the closed form)."""
+    invI = 1.0 / I
+    w0 = invI * L
+    Lsq, twoKr = L @ L, L @ w0
+    r1, r3 = Lsq - twoKr * I[2], twoKr * I[0] - Lsq
+    l1, l3 = r1 * invI[1] / (I[1] - I[2]), r3 * invI[1] / (I[0] - I[1])
+    c13 = 1.0 / (I[0] - I[2])
+    a = np.sqrt([r1 * invI[0] * c13, min(l1, l3), r3 * invI[2] * c13])
+    if abs(w0[1] / a[1]) >= 1:
+        a[1] = abs(w0[1])
+    return abs(invI[1] * a[0] * a[2] / (a[1] * c13)) * dt
+
+
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_fast_rotors_take_both_exact_paths(hip, oracle, precision):
+    spec = clusters([5] * 40 + [9] * 24, 10, seed=9, mode=0)
+    rng = np.random.default_rng(9)
+    # spin the bodies: rigid rotation at |w| up to 600 rad/ps on top of 300 K
+    v = spec.velocities.copy()
+    for b in range(1, 65):
+        sel = spec.body_indices == b
+        m = spec.masses[sel]
+        com = (m[:, None] * spec.positions[sel]).sum(0) / m.sum()
+        w = rng.normal(size=3)
+        w *= rng.uniform(5.0, 600.0) / np.linalg.norm(w)
+        v[sel] += np.cross(w, spec.positions[sel] - com)
+    spec.velocities = v
+    o = oracle.OracleSim.from_spec(spec)
+    steps = []
+    for st in o.body_states():
+        a, b_, c, d = st["q"]
+        Bq = np.array([[-b_, -c, -d], [a, -d, c], [d, a, -b_], [-c, b_, a]])
+        steps.append(rotation_arg_step(st["I"], 0.5 * Bq.T @ st["pi"], spec.dt))
+    steps = np.array(steps)
+    assert (steps <= 0.2).sum() >= 8 and (steps > 0.2).sum() >= 8, steps
+    compare(oracle, hip, spec, precision)
