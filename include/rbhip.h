@@ -268,8 +268,18 @@ const char* rbhip_server_last_error(void);
 int rbhip_server_create(int device, size_t hbmBudget, int stepsPerRound, int precision, int blockSize,
                         rbhip_server** out);
 int rbhip_server_destroy(rbhip_server* server);
-/* queues a job (the inputs are copied; velocities and the tether may be NULL);
-   rejects a job larger than the whole budget */
+/* the order in which waiting jobs are admitted when the budget has room:
+   RBHIP_ADMIT_FIFO (the default) in submission order, or
+   RBHIP_ADMIT_LONGEST_FIRST by step count, most steps first (submission
+   order among equal counts), which keeps long jobs from running alone at
+   the end of a backlog.  Either way admission stops at the first waiting
+   job that does not fit.  Applies to the jobs already waiting too. */
+enum { RBHIP_ADMIT_FIFO = 0, RBHIP_ADMIT_LONGEST_FIRST = 1 };
+int rbhip_server_set_admission_order(rbhip_server* server, int order);
+/* queues a job; velocities and the tether may be NULL.  The arrays are not
+   copied: the server reads them when it sets the job up, so they must stay
+   valid and unchanged until rbhip_server_take has returned the job (or the
+   server is destroyed).  Rejects a job larger than the whole budget. */
 int rbhip_server_submit(rbhip_server* server, int numAtoms, const double* masses, const int* bodyIndices,
                         const double* positions, const double* velocities, const double* tetherK,
                         const double* tetherR0, double stepSize, int rotationMode, long long steps,
@@ -288,12 +298,13 @@ int rbhip_server_run_round(rbhip_server* server, int* finished);
    bytes of the resident jobs; body-steps done so far.  Any output may be
    NULL. */
 int rbhip_server_status(rbhip_server* server, long long* counts3, size_t* bytesInUse, long long* bodySteps);
-/* rounds queued so far and the host seconds spent in their phases:
-   seconds4 = {admitting (admission thread), collecting finished jobs
-   (collection thread, includes waiting for the device), updating the batch
-   members (includes waiting for the oldest queued round), queueing the
-   steps} */
-int rbhip_server_profile(rbhip_server* server, long long* rounds, double* seconds4);
+/* rounds2 = {rounds queued so far, those cut below steps-per-round by a job
+   reaching its step count}; seconds6 = {admitting (summed over the admission
+   threads), collecting finished jobs (collection thread, includes waiting
+   for the device), waiting for the oldest queued round, updating the batch
+   members, queueing the steps (these three on the run_round thread), the
+   retired rounds' duration on the device}.  Either output may be NULL. */
+int rbhip_server_profile(rbhip_server* server, long long* rounds2, double* seconds6);
 /* ids of completed jobs not yet taken, up to max */
 int rbhip_server_completed(rbhip_server* server, long long* ids, int max, int* count);
 /* final positions and velocities (numAtoms x 3, original order), {KE_t, KE_r}

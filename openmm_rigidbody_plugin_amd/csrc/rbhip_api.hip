@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1101,17 +1102,40 @@ int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, si
     return guard([&] {
         require(numAtoms > 0 && bodyIndices && bytes, "bad arguments");
         require(precision == RBHIP_PRECISION_MIXED || precision == RBHIP_PRECISION_DOUBLE, "unknown precision");
-        std::vector<int> bi = clean_body_indices(bodyIndices, numAtoms);
-        int numBodies = 0, numFree = 0;
-        for (int v : bi) numBodies = std::max(numBodies, v);
-        std::vector<int> n(numBodies, 0);
-        for (int v : bi) {
-            if (v == 0) numFree++;
-            else n[v - 1]++;
+        // the body and free-atom counts and the distinct body sizes that
+        // clean_body_indices gives; labels spanning at most 4n + 1024 values
+        // are counted directly, wider ones through the renumbering
+        int lo = INT_MAX, hi = 0, numFree = 0;
+        for (int i = 0; i < numAtoms; i++) {
+            int v = bodyIndices[i];
+            if (v <= 0) {
+                numFree++;
+            } else {
+                lo = std::min(lo, v);
+                hi = std::max(hi, v);
+            }
         }
-        std::vector<int> sizes(n);
-        std::sort(sizes.begin(), sizes.end());
-        int classes = (int)(std::unique(sizes.begin(), sizes.end()) - sizes.begin());
+        std::vector<int> n;
+        if (numFree < numAtoms && (long long)hi - lo <= 4LL * numAtoms + 1024) {
+            n.assign((size_t)(hi - lo) + 1, 0);
+            for (int i = 0; i < numAtoms; i++)
+                if (bodyIndices[i] > 0) n[bodyIndices[i] - lo]++;
+        } else if (numFree < numAtoms) {
+            std::vector<int> bi = clean_body_indices(bodyIndices, numAtoms);
+            n.assign(*std::max_element(bi.begin(), bi.end()), 0);
+            for (int v : bi)
+                if (v > 0) n[v - 1]++;
+        }
+        int numBodies = 0, classes = 0;
+        std::vector<char> seen((size_t)numAtoms + 1, 0);
+        for (int c : n) {
+            if (c == 0) continue;
+            numBodies++;
+            if (!seen[c]) {
+                seen[c] = 1;
+                classes++;
+            }
+        }
         size_t R = precision == RBHIP_PRECISION_MIXED ? 4 : 8;
         *bytes = context_arena_bytes(numAtoms, numBodies, numFree, numAtoms - numFree, classes, R) +
                  tile_bytes_bound(numAtoms, numBodies, numFree, classes);

@@ -56,8 +56,11 @@ using Clock = std::chrono::steady_clock;
 struct Job {
     long long id = 0;
     int numAtoms = 0;
-    std::vector<double> masses, positions, velocities, tetherK, tetherR0;
-    std::vector<int> bodyIndices;
+    // the caller's arrays, read once at admission (velocities and the tether
+    // may be null)
+    const double *masses = nullptr, *positions = nullptr, *velocities = nullptr;
+    const double *tetherK = nullptr, *tetherR0 = nullptr;
+    const int* bodyIndices = nullptr;
     double stepSize = 0.0;
     int rotationMode = 0;
     long long steps = 0, done = 0;
@@ -115,10 +118,15 @@ struct rbhip_server {
     int precision = RBHIP_PRECISION_MIXED;
     int block = 256;
     int exactKind = -1;                 // rotation kind of the jobs: 1 exact, 0 NO-SQUISH
+    int admissionOrder = RBHIP_ADMIT_FIFO;
     long long nextId = 1;
     long long bodySteps = 0;
     long long rounds = 0;
-    double phase[4] = {0.0, 0.0, 0.0, 0.0};   // host seconds: admit, collect (under mtx), members, launch
+    long long shortRounds = 0;          // rounds cut below stepsPerRound by a finishing job
+    // seconds: admit and collect (worker threads), waiting for a queued
+    // round, rebuilding a member list, queueing a round (run_round thread),
+    // and the rounds' own duration on the device (events)
+    double phase[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 
     // Shared with the worker threads (under mtx): jobs chosen for admission,
     // jobs whose contexts are ready, finished jobs to collect, completed jobs,
@@ -156,8 +164,10 @@ struct rbhip_server {
     static constexpr int kRetireStreams = 4;
     hipStream_t retireStreams[kRetireStreams] = {nullptr, nullptr, nullptr, nullptr};
     long long queuedRounds = 0;
-    std::deque<hipEvent_t> inFlight;    // one event per queued round, oldest first
-    std::vector<hipEvent_t> spareEvents;
+    struct RoundEvents { hipEvent_t start, end; };
+    std::deque<RoundEvents> inFlight;   // the queued rounds, oldest first
+    std::vector<RoundEvents> spareEvents;
+    double deviceSeconds = 0.0;         // sum of the retired rounds' durations
 
     std::vector<hipStream_t> allStreams() const {
         std::vector<hipStream_t> v = {batchStreams[0], batchStreams[1], memberStream};
@@ -184,8 +194,14 @@ struct rbhip_server {
             if (j->ctx) (void)rbhip_destroy(j->ctx);
         for (auto& j : toCollect)
             if (j->ctx) (void)rbhip_destroy(j->ctx);
-        for (hipEvent_t e : inFlight) (void)hipEventDestroy(e);
-        for (hipEvent_t e : spareEvents) (void)hipEventDestroy(e);
+        for (RoundEvents e : inFlight) {
+            (void)hipEventDestroy(e.start);
+            (void)hipEventDestroy(e.end);
+        }
+        for (RoundEvents e : spareEvents) {
+            (void)hipEventDestroy(e.start);
+            (void)hipEventDestroy(e.end);
+        }
         for (hipStream_t st : allStreams())
             if (st) {
                 (void)hipStreamSynchronize(st);
@@ -198,14 +214,13 @@ struct rbhip_server {
     // resident jobs' stream
     void admit(Job& j, hipStream_t admitStream) {
         rbhip_context* c = nullptr;
-        call(rbhip_create(j.numAtoms, j.masses.data(), j.bodyIndices.data(), j.stepSize, j.rotationMode, precision,
-                          device, &c));
+        call(rbhip_create(j.numAtoms, j.masses, j.bodyIndices, j.stepSize, j.rotationMode, precision, device, &c));
         try {
             call(rbhip_set_stream(c, admitStream));
             call(rbhip_set_launch(c, block, 0));
-            if (!j.tetherK.empty()) call(rbhip_set_tether(c, j.tetherK.data(), j.tetherR0.data()));
-            call(rbhip_set_positions(c, j.positions.data()));
-            if (!j.velocities.empty()) call(rbhip_set_velocities(c, j.velocities.data()));
+            if (j.tetherK) call(rbhip_set_tether(c, j.tetherK, j.tetherR0));
+            call(rbhip_set_positions(c, j.positions));
+            if (j.velocities) call(rbhip_set_velocities(c, j.velocities));
             call(rbhip_set_stream(c, memberStream));
         } catch (...) {
             (void)rbhip_destroy(c);
@@ -213,13 +228,9 @@ struct rbhip_server {
         }
         j.ctx = c;
         j.admitted = Clock::now();
-        // host inputs are no longer needed
-        std::vector<double>().swap(j.masses);
-        std::vector<double>().swap(j.positions);
-        std::vector<double>().swap(j.velocities);
-        std::vector<double>().swap(j.tetherK);
-        std::vector<double>().swap(j.tetherR0);
-        std::vector<int>().swap(j.bodyIndices);
+        // the context holds its own copies now
+        j.masses = j.positions = j.velocities = j.tetherK = j.tetherR0 = nullptr;
+        j.bodyIndices = nullptr;
     }
 
     // downloads a finished job's final state (on the retire stream, after the
@@ -296,9 +307,12 @@ struct rbhip_server {
 
     // waits for the oldest queued round to finish
     void retireOldestRound() {
-        hipEvent_t e = inFlight.front();
+        RoundEvents e = inFlight.front();
         inFlight.pop_front();
-        if (hipEventSynchronize(e) != hipSuccess) throw ServerError(RBHIP_E_HIP, "round event failed");
+        float ms = 0.0f;
+        if (hipEventSynchronize(e.end) != hipSuccess || hipEventElapsedTime(&ms, e.start, e.end) != hipSuccess)
+            throw ServerError(RBHIP_E_HIP, "round event failed");
+        deviceSeconds += 1e-3 * ms;
         spareEvents.push_back(e);
     }
 
@@ -341,10 +355,11 @@ struct rbhip_server {
             admitted.clear();
             finished = (int)(completed.size() - before);
         }
-        Clock::time_point t1 = Clock::now();
+        Clock::time_point t0 = Clock::now();
         std::vector<rbhip_context*> m;
         for (auto& j : next) m.push_back(j->ctx);
         if (!m.empty() && (int)inFlight.size() >= kRoundsInFlight) retireOldestRound();
+        Clock::time_point t1 = Clock::now();
         if (!m.empty() && m != members[cur]) {
             cur = 1 - cur;
             if (m != members[cur]) {
@@ -356,17 +371,24 @@ struct rbhip_server {
         long long n = stepsPerRound;
         for (auto& j : next) n = std::min(n, j->steps - j->done);
         if (!next.empty() && n > 0) {
-            call(rbhip_batch_execute(batches[cur], (int)n));
-            hipEvent_t e = nullptr;
+            RoundEvents e{nullptr, nullptr};
             if (!spareEvents.empty()) {
                 e = spareEvents.back();
                 spareEvents.pop_back();
-            } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            } else if (hipEventCreate(&e.start) != hipSuccess || hipEventCreate(&e.end) != hipSuccess) {
                 throw ServerError(RBHIP_E_HIP, "event creation failed");
             }
-            if (hipEventRecord(e, batchStreams[cur]) != hipSuccess) throw ServerError(RBHIP_E_HIP, "event record failed");
+            // rounds run one after another (consecutive rounds share members);
+            // waiting here makes the start event mark this round's own start
+            if (!inFlight.empty() && hipStreamWaitEvent(batchStreams[cur], inFlight.back().end, 0) != hipSuccess)
+                throw ServerError(RBHIP_E_HIP, "stream wait failed");
+            if (hipEventRecord(e.start, batchStreams[cur]) != hipSuccess)
+                throw ServerError(RBHIP_E_HIP, "event record failed");
+            call(rbhip_batch_execute(batches[cur], (int)n));
+            if (hipEventRecord(e.end, batchStreams[cur]) != hipSuccess) throw ServerError(RBHIP_E_HIP, "event record failed");
             inFlight.push_back(e);
             queuedRounds++;
+            if (n < stepsPerRound) shortRounds++;
             for (auto& j : next) {
                 j->done += n;
                 bodySteps += j->bodies * n;
@@ -380,8 +402,10 @@ struct rbhip_server {
         Clock::time_point t3 = Clock::now();
         std::lock_guard<std::mutex> lock(mtx);
         rounds = queuedRounds;
-        phase[2] += seconds(t1, t2);
-        phase[3] += seconds(t2, t3);
+        phase[2] += seconds(t0, t1);
+        phase[3] += seconds(t1, t2);
+        phase[4] += seconds(t2, t3);
+        phase[5] = deviceSeconds;
         return finished;
     }
 };
@@ -426,6 +450,19 @@ int rbhip_server_destroy(rbhip_server* s) {
     return guard([&] { delete s; });
 }
 
+int rbhip_server_set_admission_order(rbhip_server* s, int order) {
+    return guard([&] {
+        need(s != nullptr, "null server");
+        need(order == RBHIP_ADMIT_FIFO || order == RBHIP_ADMIT_LONGEST_FIRST, "unknown admission order");
+        s->admissionOrder = order;
+        // job ids grow with submission: FIFO is ascending id
+        std::stable_sort(s->queue.begin(), s->queue.end(),
+                         [&](const std::unique_ptr<Job>& a, const std::unique_ptr<Job>& b) {
+                             return order == RBHIP_ADMIT_FIFO ? a->id < b->id : a->steps > b->steps;
+                         });
+    });
+}
+
 int rbhip_server_submit(rbhip_server* s, int numAtoms, const double* masses, const int* bodyIndices,
                         const double* positions, const double* velocities, const double* tetherK,
                         const double* tetherR0, double stepSize, int rotationMode, long long steps,
@@ -447,14 +484,12 @@ int rbhip_server_submit(rbhip_server* s, int numAtoms, const double* masses, con
         std::unique_ptr<Job> j(new Job());
         j->id = s->nextId++;
         j->numAtoms = numAtoms;
-        j->masses.assign(masses, masses + numAtoms);
-        j->bodyIndices.assign(bodyIndices, bodyIndices + numAtoms);
-        j->positions.assign(positions, positions + 3 * (size_t)numAtoms);
-        if (velocities) j->velocities.assign(velocities, velocities + 3 * (size_t)numAtoms);
-        if (tetherK) {
-            j->tetherK.assign(tetherK, tetherK + numAtoms);
-            j->tetherR0.assign(tetherR0, tetherR0 + 3 * (size_t)numAtoms);
-        }
+        j->masses = masses;
+        j->bodyIndices = bodyIndices;
+        j->positions = positions;
+        j->velocities = velocities;
+        j->tetherK = tetherK;
+        j->tetherR0 = tetherK ? tetherR0 : nullptr;
         j->bodies = numBodies;
         j->stepSize = stepSize;
         j->rotationMode = rotationMode;
@@ -463,7 +498,11 @@ int rbhip_server_submit(rbhip_server* s, int numAtoms, const double* masses, con
         j->submitted = Clock::now();
         s->exactKind = kind;
         *jobId = j->id;
-        s->queue.push_back(std::move(j));
+        auto at = s->queue.end();
+        if (s->admissionOrder == RBHIP_ADMIT_LONGEST_FIRST)
+            at = std::find_if(s->queue.begin(), s->queue.end(),
+                              [&](const std::unique_ptr<Job>& w) { return w->steps < j->steps; });
+        s->queue.insert(at, std::move(j));
     });
 }
 
@@ -491,13 +530,16 @@ int rbhip_server_status(rbhip_server* s, long long* counts3, size_t* bytesInUse,
     });
 }
 
-int rbhip_server_profile(rbhip_server* s, long long* rounds, double* seconds4) {
+int rbhip_server_profile(rbhip_server* s, long long* rounds2, double* seconds6) {
     return guard([&] {
         need(s != nullptr, "null server");
         std::lock_guard<std::mutex> lock(s->mtx);
-        if (rounds) *rounds = s->rounds;
-        if (seconds4)
-            for (int i = 0; i < 4; i++) seconds4[i] = s->phase[i];
+        if (rounds2) {
+            rounds2[0] = s->rounds;
+            rounds2[1] = s->shortRounds;
+        }
+        if (seconds6)
+            for (int i = 0; i < 6; i++) seconds6[i] = s->phase[i];
     });
 }
 

@@ -99,6 +99,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_server_last_error": (ctypes.c_char_p, []),
             "rbhip_server_create": (_I, [_I, ctypes.c_size_t, _I, _I, _I, ctypes.POINTER(_P)]),
             "rbhip_server_destroy": (_I, [_P]),
+            "rbhip_server_set_admission_order": (_I, [_P, _I]),
             "rbhip_server_submit": (_I, [_P, _I, _DP, _IP, _DP, _DP, _DP, _DP, _D, _I, ctypes.c_longlong, _LLP]),
             "rbhip_server_run_round": (_I, [_P, _IP]),
             "rbhip_server_status": (_I, [_P, _LLP, _SIZEP, _LLP]),
@@ -424,16 +425,25 @@ def _scheck(status):
 class Server:
     """Job server: admission by HBM budget, continuous batching (rbhip_server_*)."""
 
-    def __init__(self, hbm_budget, steps_per_round=100, precision="mixed", block=256, device=0):
+    ADMISSION_ORDERS = {"fifo": 0, "longest_first": 1}
+
+    def __init__(self, hbm_budget, steps_per_round=100, precision="mixed", block=256, device=0,
+                 admission="fifo"):
         self._h = _P()
         _scheck(lib().rbhip_server_create(int(device), int(hbm_budget), int(steps_per_round),
                                           PRECISIONS[precision], int(block), ctypes.byref(self._h)))
         self.num_atoms = {}
+        if admission != "fifo":
+            self.set_admission_order(admission)
+        # the server reads a job's arrays when it sets the job up: they are
+        # kept alive here until the job is taken or the server closed
+        self._inputs = {}
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().rbhip_server_destroy(self._h)
             self._h = _P()
+        self._inputs = {}
 
     def __del__(self):
         try:
@@ -441,8 +451,16 @@ class Server:
         except Exception:
             pass
 
+    def set_admission_order(self, order):
+        """'fifo' (submission order) or 'longest_first' (most steps first)."""
+        if order not in self.ADMISSION_ORDERS:
+            raise ValueError(f"unknown admission order {order!r}")
+        _scheck(lib().rbhip_server_set_admission_order(self._h, self.ADMISSION_ORDERS[order]))
+
     def submit(self, spec, steps):
-        """Queues a SystemSpec for `steps` steps; returns the job id."""
+        """Queues a SystemSpec for `steps` steps; returns the job id.  The
+        spec's arrays are read when the job is set up, not copied here: do not
+        modify them in place until the job has been taken."""
         n = spec.num_atoms
         m = np.ascontiguousarray(spec.masses, dtype=np.float64)
         bi = np.ascontiguousarray(spec.body_indices, dtype=np.int32)
@@ -454,6 +472,7 @@ class Server:
         _scheck(lib().rbhip_server_submit(self._h, n, _dp(m), _ip(bi), _dp(x), _dp(v), _dp(k), _dp(r0),
                                           float(spec.dt), int(spec.rotation_mode), int(steps), ctypes.byref(jid)))
         self.num_atoms[jid.value] = n
+        self._inputs[jid.value] = (m, bi, x, v, k, r0)
         return jid.value
 
     def run_round(self):
@@ -469,11 +488,13 @@ class Server:
                 "body_steps": bs.value}
 
     def profile(self):
-        """Rounds run and host seconds per phase (admit, collect, members, launch)."""
-        r = ctypes.c_longlong()
-        t = np.empty(4)
-        _scheck(lib().rbhip_server_profile(self._h, ctypes.byref(r), _dp(t)))
-        return {"rounds": r.value, "admit_s": t[0], "collect_s": t[1], "members_s": t[2], "launch_s": t[3]}
+        """Rounds queued (and those cut short by a finishing job), host seconds
+        per phase and the rounds' summed duration on the device."""
+        r = (ctypes.c_longlong * 2)()
+        t = np.empty(6)
+        _scheck(lib().rbhip_server_profile(self._h, r, _dp(t)))
+        return {"rounds": r[0], "short_rounds": r[1], "admit_s": float(t[0]), "collect_s": float(t[1]),
+                "wait_s": float(t[2]), "members_s": float(t[3]), "launch_s": float(t[4]), "device_s": float(t[5])}
 
     def completed(self, max_ids=4096):
         ids = (ctypes.c_longlong * max_ids)()
@@ -487,4 +508,5 @@ class Server:
         n = self.num_atoms.pop(job_id)
         x, v, ke, t = np.empty((n, 3)), np.empty((n, 3)), np.empty(2), np.empty(3)
         _scheck(lib().rbhip_server_take(self._h, int(job_id), _dp(x), _dp(v), _dp(ke), _dp(t)))
+        self._inputs.pop(job_id, None)
         return {"positions": x, "velocities": v, "kinetic": ke, "wait": t[0], "resident": t[1], "latency": t[2]}

@@ -5,6 +5,7 @@ import glob
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -55,3 +56,35 @@ def test_no_gpu_means_loud_failure():
     with pytest.raises(rbhip.RbhipError):
         rbhip.Context([1.0, 1.0], [1, 1], 0.002)
     assert rbhip.lib().rbhip_version() == 1
+
+
+def test_estimate_counts_bodies_whatever_the_labels():
+    # host-only: the body count and the byte estimate depend on which atoms
+    # share a positive label, not on the labels' values (labels <= 0 are
+    # free atoms); compact labels are counted directly, labels spread wider
+    # than 4 n + 1024 go through the renumbering
+    from openmm_rigidbody_plugin_amd import rbhip
+    lib = rbhip.lib()
+
+    def estimate(labels):
+        bi = np.ascontiguousarray(labels, dtype=np.int32)
+        n, nb = ctypes.c_size_t(), ctypes.c_int()
+        assert lib.rbhip_estimate_bytes(bi.size, bi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 0,
+                                        ctypes.byref(n), ctypes.byref(nb)) == 0
+        return n.value, nb.value
+
+    rng = np.random.default_rng(3)
+    sizes = rng.choice([1, 2, 3, 6, 20], size=300)
+    labels = np.repeat(np.arange(1, 301), sizes)
+    labels = np.concatenate([labels, np.zeros(150, int), -np.arange(1, 40)])
+    labels = labels[rng.permutation(labels.size)]
+    base = estimate(labels)
+    assert base[1] == 300
+    body = labels > 0
+    for relabel in (lambda v: v + 1000, lambda v: v * 7_000_000 // 300 + 5, lambda v: 2_000_000_000 - v):
+        other = labels.copy()
+        other[body] = relabel(labels[body])
+        assert estimate(other) == base
+    # every atom free, and one body holding every atom
+    assert estimate(np.zeros(10, int))[1] == 0
+    assert estimate(np.full(10, 123456789))[1] == 1

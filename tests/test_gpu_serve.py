@@ -6,6 +6,8 @@ server's workgroup size its bodies go through the same kernels and tiles as
 when stepped alone, so every result equals the job stepped alone, bit for
 bit, whatever the mix.  A short job is also checked against the FP64 oracle
 (mixed tolerance)."""
+import time
+
 import numpy as np
 import pytest
 
@@ -29,9 +31,19 @@ def alone(hip, spec, steps, block):
     return c
 
 
-def drain(server, max_rounds=1000):
+def rounds_until(seconds=60.0):
+    """Yields until a deadline: once every job has finished stepping, a round
+    returns at once while the collection thread still downloads the last
+    jobs (the first download of a process takes longest), so loops over
+    rounds are bounded in time, not in count."""
+    deadline = time.monotonic() + seconds
+    while time.monotonic() < deadline:
+        yield
+
+
+def drain(server):
     results = {}
-    for _ in range(max_rounds):
+    for _ in rounds_until():
         server.run_round()
         for jid in server.completed():
             results[jid] = server.take(jid)
@@ -42,7 +54,7 @@ def drain(server, max_rounds=1000):
             for jid in server.completed():
                 results[jid] = server.take(jid)
             return results
-    raise AssertionError("server did not drain")
+    raise AssertionError(f"server did not drain: {server.status()} {server.profile()} taken {sorted(results)}")
 
 
 @pytest.mark.parametrize("mode,block", [(0, 64), (10, 256)])
@@ -68,6 +80,60 @@ def test_server_results_equal_alone(hip, mode, block):
         assert r["latency"] >= r["resident"] >= 0.0
 
 
+def test_submitted_arrays_outlive_the_caller(hip):
+    # the server reads a job's arrays when it sets the job up; the wrapper
+    # keeps them alive, including the converted copies of float32 and
+    # non-contiguous inputs, after the caller has dropped every reference
+    import copy
+    import gc
+    specs, steps = jobs(0)
+    server = hip.Server(hbm_budget=2 << 20, steps_per_round=5, block=64)   # most jobs wait for admission
+    ids = []
+    for s, n in zip(specs, steps):
+        t = copy.deepcopy(s)
+        t.positions = np.asfortranarray(t.positions)
+        t.velocities = t.velocities.astype(np.float32).astype(np.float64)[:, :]
+        t.masses = t.masses.astype(np.float32)
+        ids.append(server.submit(t, n))
+        del t
+    gc.collect()
+    results = drain(server)
+    for jid, spec, n in zip(ids, specs, steps):
+        ref = copy.deepcopy(spec)
+        ref.velocities = ref.velocities.astype(np.float32).astype(np.float64)
+        ref.masses = ref.masses.astype(np.float32).astype(np.float64)
+        np.testing.assert_array_equal(results[jid]["positions"], alone(hip, ref, n, 64).positions())
+    assert server._inputs == {}
+
+
+def test_longest_first_admission(hip):
+    # equal-size jobs and a budget for one of them: jobs are admitted by step
+    # count, most steps first, so they complete in that order; results equal
+    # alone
+    specs = [systems.heterogeneous(700, 250, seed=300 + i, rotation_mode=0, name=f"same{i}") for i in range(4)]
+    steps = [12, 40, 25, 33]
+    budget = max(hip.estimate_bytes(s.body_indices) for s in specs)
+    server = hip.Server(hbm_budget=budget, steps_per_round=8, block=64)
+    server.set_admission_order("longest_first")
+    ids = [server.submit(s, n) for s, n in zip(specs, steps)]
+    order = []
+    for _ in rounds_until():
+        server.run_round()
+        st = server.status()
+        assert st["resident"] <= 1
+        for jid in server.completed():
+            order.append(jid)
+            r = server.take(jid)
+            i = ids.index(jid)
+            np.testing.assert_array_equal(r["positions"], alone(hip, specs[i], steps[i], 64).positions())
+        if st["waiting"] == 0 and st["resident"] == 0 and len(order) == len(ids):
+            break
+    assert order == [ids[1], ids[3], ids[2], ids[0]]
+    assert hip.lib().rbhip_server_set_admission_order(server._h, 7) != 0
+    with pytest.raises(ValueError):
+        server.set_admission_order("shortest_first")
+
+
 def test_server_respects_budget(hip):
     specs, steps = jobs(0)
     est = [hip.estimate_bytes(s.body_indices) for s in specs]
@@ -76,7 +142,7 @@ def test_server_respects_budget(hip):
     ids = [server.submit(s, n) for s, n in zip(specs, steps)]
     results = {}
     saw_waiting = False
-    for _ in range(1000):
+    for _ in rounds_until():
         server.run_round()
         st = server.status()
         assert st["bytes_in_use"] <= budget
