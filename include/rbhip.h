@@ -77,6 +77,27 @@ int rbhip_device_synchronize(void);
  * documented limits (massless atoms, single-atom bodies). */
 int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, double stepSize,
                  int rotationMode, int precision, int device, rbhip_context** out);
+/* rbhip_create for a System with distance constraints (System::addConstraint):
+   constraintAtoms holds two atom indices per constraint, constraintDistances
+   one length each.  As the reference kernel does with OpenMM's constraints
+   (ReferenceRigidBodyKernels.cpp:92-104), every step projects the free atoms'
+   positions after part 1 and their velocities after part 2; constraints on
+   body atoms are refused with the reference's message
+   (RigidBodySystem.cpp:107-112) and each constraint removes a degree of
+   freedom (RigidBodySystem.cpp:130).  The solver is SHAKE and RATTLE's
+   velocity stage over each connected cluster of constraints, in FP64; a
+   cluster may join at most 16 atoms.  Contexts with constraints cannot join
+   a batch and do not compute refined energies. */
+int rbhip_create_constrained(int numAtoms, const double* masses, const int* bodyIndices, int numConstraints,
+                             const int* constraintAtoms, const double* constraintDistances, double stepSize,
+                             int rotationMode, int precision, int device, rbhip_context** out);
+/* Integrator::setConstraintTolerance (default 1e-5): positions converge when
+   every constrained distance is within (1 +- tolerance) of its length,
+   velocities when every bond's relative velocity is within tolerance of
+   orthogonal to the bond */
+int rbhip_set_constraint_tolerance(rbhip_context* ctx, double tolerance);
+/* cluster projections that did not converge within 1000 sweeps, so far */
+int rbhip_constraint_failures(rbhip_context* ctx, long long* count);
 int rbhip_destroy(rbhip_context* ctx);
 /* launch stream for all work of this context (hipStream_t; NULL restores the
    stream the library assigned, one of a per-device pool of non-blocking

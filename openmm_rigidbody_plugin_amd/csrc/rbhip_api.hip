@@ -331,13 +331,14 @@ void time_steps(hipStream_t stream, int steps, P1 p1, F f, P2 p2, double* avgMs,
 
 // device bytes rbhip_create allocates for a system (the arena; tiles and
 // bonds come on top), kept in step with rbhip_create by a check there
-static size_t context_arena_bytes(size_t na, int numBodies, int numFree, int numBodyAtoms, int numClasses, size_t R) {
+static size_t context_arena_bytes(size_t na, int numBodies, int numFree, int numBodyAtoms, int numClasses, size_t R,
+                                  int clusterSlots = 0) {
     auto a = [](size_t n) { return n == 0 ? (size_t)256 : n; };
     const size_t nb = std::max(numBodies, 1);
     const size_t bodyStride = (nb + 63) / 64 * 64, atomStride = (na + 63) / 64 * 64;
     const size_t dStride = (std::max<size_t>(numBodyAtoms, 1) + 63) / 64 * 64;
     const bool mixed = R == 4;
-    const size_t maxKe = ((size_t)numFree + 63) / 64 + 1 + numBodies + numClasses + 1;
+    const size_t maxKe = ((size_t)numFree + 63) / 64 + 1 + numBodies + numClasses + 1 + clusterSlots;
     const size_t maxPe = (na + 63) / 64 + 1;
     size_t b = 0;
     b += a(4 * R * nb) + a(3 * R * nb) + a(3 * R * nb) + a(4 * R * nb);           // q, L, pcm, iI
@@ -371,6 +372,9 @@ struct rbhip_context {
     hipStream_t stream = nullptr;
 
     int numAtoms = 0, numFree = 0, numBodies = 0, numBodyAtoms = 0, numDOF = 0;
+    // distance constraints between free atoms: the atoms of each cluster are
+    // contiguous and follow the numFree - numClustered unconstrained ones
+    int numConstraints = 0, numClusters = 0, numClustered = 0;
     std::vector<int> bodyIndex, atomIndex;
     std::vector<int> refBodyN, refBodyLoc;       // per reference body
     std::vector<int> internalBody;               // internal body -> reference body
@@ -383,12 +387,19 @@ struct rbhip_context {
 
     DeviceArena arena;
     DeviceArena tileArena;
+    DeviceArena clusterArena;
     DevSys h{};
     DevSys* dSys = nullptr;
     Tile* dPartTiles = nullptr;
     Tile* dForceTiles = nullptr;
     Tile* dSetupTiles = nullptr;
     int nPartTiles = 0, nForceTiles = 0, nSetupTiles = 0;
+    // cluster tiles, 64 clusters each: KE slots after the part tiles, and a
+    // copy whose slots follow the set-up tiles
+    Tile* dClusterTiles = nullptr;
+    Tile* dClusterSetupTiles = nullptr;
+    int nClusterTiles = 0;
+    static constexpr int kClusterBlock = 64;
     int maxKeSlots = 0, maxPeSlots = 0;
     int keSlots = 0;                             // slots written by the last KE producer
     int* dSlotCounts = nullptr;                  // [0] ke, [1] pe, [2] refined ke, [3] refined pe
@@ -478,8 +489,9 @@ struct rbhip_context {
     std::vector<Tile> partTiles(int blk, const std::map<int, int>& lanesByN) const {
         std::vector<Tile> t;
         const int freeTile = FREE_PER_THREAD * blk;
-        for (int f = 0; f < numFree; f += freeTile)
-            t.push_back(Tile{0, TILE_FREE, f, std::min(freeTile, numFree - f), 0, 0, 1, 0});
+        const int numUnconstrained = numFree - numClustered;
+        for (int f = 0; f < numUnconstrained; f += freeTile)
+            t.push_back(Tile{0, TILE_FREE, f, std::min(freeTile, numUnconstrained - f), 0, 0, 1, 0});
         for (const Class& c : classes) {
             auto it = lanesByN.find(c.N);
             int L = it == lanesByN.end() ? 1 : it->second;
@@ -507,7 +519,7 @@ struct rbhip_context {
             require(t.slot >= 0 && t.slot < maxSlots, "tile slot out of range", RBHIP_E_STATE);
             require(t.count > 0, "empty tile", RBHIP_E_STATE);
             if (t.kind == TILE_FREE) {
-                require(t.count <= FREE_PER_THREAD * blk && t.first >= 0 && t.first + t.count <= numFree,
+                require(t.count <= FREE_PER_THREAD * blk && t.first >= 0 && t.first + t.count <= numFree - numClustered,
                         "free tile out of range",
                         RBHIP_E_STATE);
             } else if (t.kind == TILE_ATOMS) {
@@ -526,6 +538,17 @@ struct rbhip_context {
                             "body tile does not match the body layout", RBHIP_E_STATE);
             }
         }
+    }
+
+    // tiles of up to kClusterBlock clusters (one lane each), KE slots from slot0
+    std::vector<Tile> clusterTiles(int slot0) const {
+        std::vector<Tile> t;
+        for (int c = 0; c < numClusters; c += kClusterBlock) {
+            Tile x{0, TILE_CLUSTERS, c, std::min(kClusterBlock, numClusters - c), 0, 0, 1, slot0 + (int)t.size()};
+            require(x.slot < maxKeSlots, "cluster tile slot out of range", RBHIP_E_STATE);
+            t.push_back(x);
+        }
+        return t;
     }
 
     // this context's validated tiles for workgroup size `blk` and its lanes per
@@ -570,6 +593,14 @@ struct rbhip_context {
         dPartTiles = tileArena.allocT<Tile>(pt.size() + 1);
         dForceTiles = tileArena.allocT<Tile>(ft.size() + 1);
         dSetupTiles = tileArena.allocT<Tile>(st.size() + 1);
+        std::vector<Tile> ct = clusterTiles((int)pt.size()), cst = clusterTiles((int)st.size());
+        dClusterTiles = tileArena.allocT<Tile>(ct.size() + 1);
+        dClusterSetupTiles = tileArena.allocT<Tile>(cst.size() + 1);
+        if (!ct.empty()) {
+            HIPCHK(hipMemcpyAsync(dClusterTiles, ct.data(), ct.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(dClusterSetupTiles, cst.data(), cst.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                                  stream));
+        }
         if (!pt.empty())
             HIPCHK(hipMemcpyAsync(dPartTiles, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
         if (!ft.empty())
@@ -580,6 +611,7 @@ struct rbhip_context {
         nPartTiles = (int)pt.size();
         nForceTiles = (int)ft.size();
         nSetupTiles = (int)st.size();
+        nClusterTiles = (int)ct.size();
         graphs.reset();
     }
 
@@ -592,15 +624,23 @@ struct rbhip_context {
     void launchForces() {
         HIPCHK(launch_forces(precision, dSys, dForceTiles, nForceTiles, block, maxBlocks, stream));
     }
+    void launchClusters(int phase) {
+        if (nClusterTiles > 0)
+            HIPCHK(launch_clusters(precision, dSys, dClusterTiles, nClusterTiles, phase, kClusterBlock, stream));
+    }
     void launchPart1() {
         HIPCHK(launch_part1(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+        launchClusters(CLUSTER_PART1);
     }
     void launchPart2() {
         HIPCHK(launch_part2(precision, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+        launchClusters(CLUSTER_PART2);
     }
     void launchFused() {
         HIPCHK(launch_fused(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+        launchClusters(CLUSTER_FUSED);
     }
+    int partKeSlots() const { return nPartTiles + nClusterTiles; }
     void launchRefineBegin(bool folded) {
         HIPCHK(launch_refine_begin(precision, rotationMode, dSys, dPartTiles, nPartTiles, folded ? 1 : 0, block,
                                    maxBlocks, stream));
@@ -624,13 +664,17 @@ struct rbhip_context {
         h.useStageV = stageVFresh ? 1 : 0;
         uploadSys();
         HIPCHK(launch_setup(precision, dSys, dSetupTiles, nSetupTiles, geometry, setupBlock(), stream));
-        keSlots = nSetupTiles;
+        if (nClusterTiles > 0)
+            HIPCHK(launch_clusters(precision, dSys, dClusterSetupTiles, nClusterTiles, CLUSTER_KE, kClusterBlock,
+                                   stream));
+        keSlots = nSetupTiles + nClusterTiles;
         kickDt = dt;
         built = true;
         refinedValid = false;
         double r[4];
         reduce(0, r);
-        if (geometry) numDOF = numFree + (int)std::lround(r[2]);
+        // RigidBodySystem.cpp:130: each constraint removes a degree of freedom
+        if (geometry) numDOF = numFree - numConstraints + (int)std::lround(r[2]);
     }
 
     // original-order FP64 xyz -> internal order
@@ -678,7 +722,7 @@ struct rbhip_context {
             refKeSlots = nPartTiles;
             refinedValid = true;
         }
-        keSlots = nPartTiles;
+        keSlots = partKeSlots();
         stageVFresh = false;
         positionsDerived = true;
         time += dt * steps;
@@ -689,6 +733,8 @@ struct rbhip_context {
     // or off; either way the accumulators start empty.  Kernels see the
     // buffers only while it is on.
     void setRefined(bool on) {
+        if (on && numConstraints > 0)
+            throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with constraints");
         if (on && !refBuf[0]) {
             const size_t R = realSize();
             refArena.stream = stream;
@@ -711,6 +757,7 @@ struct rbhip_context {
         graphs.reset();
         tileArena.release();
         refArena.release();
+        clusterArena.release();
         arena.release();
         if (handoff) (void)hipEventDestroy(handoff);
     }
@@ -878,8 +925,9 @@ int rbhip_device_synchronize(void) {
     return guard([&] { HIPCHK(hipDeviceSynchronize()); });
 }
 
-int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, double stepSize, int rotationMode,
-                 int precision, int device, rbhip_context** out) {
+static int create_context(int numAtoms, const double* masses, const int* bodyIndices, int numConstraints,
+                          const int* constraintAtoms, const double* constraintDistances, double stepSize,
+                          int rotationMode, int precision, int device, rbhip_context** out) {
     return guard([&] {
         require(out != nullptr, "null output handle");
         *out = nullptr;
@@ -929,6 +977,59 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
             if (b > 0) C.atomIndex[C.numFree + C.refBodyLoc[b - 1] + fill[b - 1]++] = i;
         }
 
+        // ---- constraints between free atoms (RigidBodySystem.cpp:107-112):
+        // connected clusters, in the order of their first constraint, go
+        // after the unconstrained free atoms, each cluster's atoms together
+        std::vector<int> freeOrder(C.atomIndex.begin(), C.atomIndex.begin() + C.numFree);
+        std::vector<std::vector<int>> clusterAtoms, clusterCons;
+        require(numConstraints >= 0 && (numConstraints == 0 || (constraintAtoms && constraintDistances)),
+                "bad constraint arrays");
+        if (numConstraints > 0) {
+            std::vector<int> root(numAtoms);
+            for (int i = 0; i < numAtoms; i++) root[i] = i;
+            auto find = [&](int x) {
+                while (root[x] != x) x = root[x] = root[root[x]];
+                return x;
+            };
+            for (int k = 0; k < numConstraints; k++) {
+                int a = constraintAtoms[2 * k], b = constraintAtoms[2 * k + 1];
+                double d = constraintDistances[k];
+                require(a >= 0 && a < numAtoms && b >= 0 && b < numAtoms && a != b,
+                        "constraint " + std::to_string(k) + ": bad atom indices");
+                require(d > 0.0 && std::isfinite(d), "constraint " + std::to_string(k) + ": bad distance");
+                if (C.bodyIndex[a] != 0 || C.bodyIndex[b] != 0)
+                    throw RbError(RBHIP_E_ARG, "Constraints involving rigid-body atoms are not allowed");
+                root[find(a)] = find(b);
+            }
+            std::vector<int> id(numAtoms, -1);
+            for (int k = 0; k < numConstraints; k++) {
+                int r = find(constraintAtoms[2 * k]);
+                if (id[r] < 0) {
+                    id[r] = (int)clusterCons.size();
+                    clusterCons.emplace_back();
+                    clusterAtoms.emplace_back();
+                }
+                clusterCons[id[r]].push_back(k);
+            }
+            std::vector<int> unconstrained;
+            for (int i : freeOrder) {
+                int r = find(i);
+                if (id[r] < 0) unconstrained.push_back(i);
+                else clusterAtoms[id[r]].push_back(i);
+            }
+            freeOrder = unconstrained;
+            for (const auto& ca : clusterAtoms) {
+                if ((int)ca.size() > CLUSTER_MAX_ATOMS)
+                    throw RbError(RBHIP_E_UNSUPPORTED, "a cluster of constraints joins " + std::to_string(ca.size()) +
+                                                           " atoms; at most " + std::to_string(CLUSTER_MAX_ATOMS) +
+                                                           " are supported");
+                freeOrder.insert(freeOrder.end(), ca.begin(), ca.end());
+                C.numClustered += (int)ca.size();
+            }
+            C.numConstraints = numConstraints;
+            C.numClusters = (int)clusterAtoms.size();
+        }
+
         // ---- internal order: bodies grouped by size class ----
         std::vector<int> order(C.numBodies);
         for (int b = 0; b < C.numBodies; b++) order[b] = b;
@@ -938,7 +1039,7 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         C.bodyN.resize(C.numBodies);
         C.bodyAtom0.resize(C.numBodies);
         C.atomOrig.resize(numAtoms);
-        for (int k = 0; k < C.numFree; k++) C.atomOrig[k] = C.atomIndex[k];
+        for (int k = 0; k < C.numFree; k++) C.atomOrig[k] = freeOrder[k];
         int a = C.numFree;
         for (int ib = 0; ib < C.numBodies; ib++) {
             int r = order[ib];
@@ -1003,7 +1104,8 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         h.refBody = h.refFree = nullptr;
         h.refKePart = h.refPePart = nullptr;
         size_t nFreeTilesMax = (C.numFree + 63) / 64 + 1;
-        C.maxKeSlots = (int)(nFreeTilesMax + C.numBodies + C.classes.size() + 1);
+        const int clusterSlots = C.numClusters > 0 ? (C.numClusters + 63) / 64 + 1 : 0;
+        C.maxKeSlots = (int)(nFreeTilesMax + C.numBodies + C.classes.size() + 1) + clusterSlots;
         C.maxPeSlots = (int)((na + 63) / 64 + 1);
         h.kePart = C.arena.allocT<double>(4 * (size_t)C.maxKeSlots);
         h.pePart = C.arena.allocT<double>(4 * (size_t)C.maxPeSlots);
@@ -1015,7 +1117,7 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
         C.dTmp = C.arena.allocT<double>(C.tmpCount);
         C.dRed = C.arena.allocT<double>(4);
         require(C.arena.bytes == context_arena_bytes(na, C.numBodies, C.numFree, C.numBodyAtoms,
-                                                     (int)C.classes.size(), R),
+                                                     (int)C.classes.size(), R, clusterSlots),
                 "internal: device size estimate out of step with the allocation", RBHIP_E_STATE);
 
         // zero state, constants (queued on the context's stream; the host
@@ -1052,10 +1154,77 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
             HIPCHK(hipMemcpyAsync(C.dBodyAtom0, C.bodyAtom0.data(), C.numBodies * sizeof(int), hipMemcpyHostToDevice,
                                   st));
         }
+        if (C.numClusters > 0) {
+            // cluster tables in internal atom numbering
+            std::vector<int> internalOf(numAtoms, -1);
+            for (int k = 0; k < C.numFree; k++) internalOf[C.atomOrig[k]] = k;
+            std::vector<int> cs(1, C.numFree - C.numClustered), ks(1, 0), ca, cb;
+            std::vector<double> cd;
+            for (int c2 = 0; c2 < C.numClusters; c2++) {
+                cs.push_back(cs.back() + (int)clusterAtoms[c2].size());
+                for (int k : clusterCons[c2]) {
+                    ca.push_back(internalOf[constraintAtoms[2 * k]]);
+                    cb.push_back(internalOf[constraintAtoms[2 * k + 1]]);
+                    cd.push_back(constraintDistances[k]);
+                }
+                ks.push_back((int)ca.size());
+            }
+            C.clusterArena.stream = st;
+            h.numClusters = C.numClusters;
+            h.clusterStart = C.clusterArena.allocT<int>(cs.size());
+            h.consStart = C.clusterArena.allocT<int>(ks.size());
+            h.consA = C.clusterArena.allocT<int>(ca.size());
+            h.consB = C.clusterArena.allocT<int>(cb.size());
+            h.consDist = C.clusterArena.allocT<double>(cd.size());
+            h.constraintFail = C.clusterArena.allocT<unsigned>(1);
+            h.constraintTol = 1e-5;
+            HIPCHK(hipMemcpyAsync(h.clusterStart, cs.data(), cs.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(h.consStart, ks.data(), ks.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(h.consA, ca.data(), ca.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(h.consB, cb.data(), cb.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(h.consDist, cd.data(), cd.size() * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemsetAsync(h.constraintFail, 0, sizeof(unsigned), st));
+        }
         C.uploadSys();
         HIPCHK(hipStreamSynchronize(st));
         C.rebuildTiles();
         *out = c.release();
+    });
+}
+
+int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, double stepSize, int rotationMode,
+                 int precision, int device, rbhip_context** out) {
+    return create_context(numAtoms, masses, bodyIndices, 0, nullptr, nullptr, stepSize, rotationMode, precision,
+                          device, out);
+}
+
+int rbhip_create_constrained(int numAtoms, const double* masses, const int* bodyIndices, int numConstraints,
+                             const int* constraintAtoms, const double* constraintDistances, double stepSize,
+                             int rotationMode, int precision, int device, rbhip_context** out) {
+    return create_context(numAtoms, masses, bodyIndices, numConstraints, constraintAtoms, constraintDistances,
+                          stepSize, rotationMode, precision, device, out);
+}
+
+int rbhip_set_constraint_tolerance(rbhip_context* ctx, double tolerance) {
+    return guard([&] {
+        require(ctx, "null context");
+        require(tolerance > 0.0 && std::isfinite(tolerance), "constraint tolerance must be positive");
+        ctx->setDevice();
+        ctx->h.constraintTol = tolerance;
+        ctx->uploadSys();
+    });
+}
+
+int rbhip_constraint_failures(rbhip_context* ctx, long long* count) {
+    return guard([&] {
+        require(ctx && count, "null argument");
+        *count = 0;
+        if (ctx->numClusters == 0) return;
+        ctx->setDevice();
+        unsigned n = 0;
+        HIPCHK(hipMemcpyAsync(&n, ctx->h.constraintFail, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *count = n;
     });
 }
 
@@ -1087,14 +1256,14 @@ int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream) {
             HIPCHK(hipStreamWaitEvent(next, ctx->handoff, 0));
         }
         ctx->stream = next;
-        ctx->arena.stream = ctx->tileArena.stream = ctx->refArena.stream = ctx->stream;
+        ctx->arena.stream = ctx->tileArena.stream = ctx->refArena.stream = ctx->clusterArena.stream = ctx->stream;
     });
 }
 
 int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes) {
     return guard([&] {
         require(ctx && bytes, "null argument");
-        *bytes = ctx->arena.bytes + ctx->tileArena.bytes + ctx->refArena.bytes;
+        *bytes = ctx->arena.bytes + ctx->tileArena.bytes + ctx->refArena.bytes + ctx->clusterArena.bytes;
     });
 }
 
@@ -1351,7 +1520,7 @@ int rbhip_part2(rbhip_context* ctx) {
         require(ctx->built, "positions must be set before stepping", RBHIP_E_STATE);
         ctx->setDevice();
         ctx->launchPart2();
-        ctx->keSlots = ctx->nPartTiles;
+        ctx->keSlots = ctx->partKeSlots();
         ctx->stageVFresh = false;
         if (ctx->refined) {
             ctx->launchRefineEnd();
@@ -1562,7 +1731,7 @@ int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* tot
         time_phases(C.stream, steps, {[&] { C.launchFused(); }, [&] { C.launchForces(); }}, fusedMs, nullptr);
         C.launchPart2();
         avgMs[RBHIP_KERNEL_FUSED] = fusedMs[0];
-        C.keSlots = C.nPartTiles;
+        C.keSlots = C.partKeSlots();
         C.stageVFresh = false;
         C.time += C.dt * (2 * steps + 1);
         C.stepCount += 2 * steps + 1;
@@ -1711,6 +1880,7 @@ int rbhip_batch_set_members(rbhip_batch* batch, int count, rbhip_context* const*
             require(m[i]->device == batch->device, "member on another device");
             require(m[i]->built, "member body state not built", RBHIP_E_STATE);
             require(m[i]->precision == m[0]->precision, "members must share the precision");
+            require(m[i]->numConstraints == 0, "members with constraints cannot be batched", RBHIP_E_UNSUPPORTED);
             require((m[i]->rotationMode == 0) == (m[0]->rotationMode == 0), "members must share the rotation kind");
             for (auto& kv : m[i]->lanes)
                 require(batch->block % kv.second == 0, "member lanes per body must divide the batch block size");

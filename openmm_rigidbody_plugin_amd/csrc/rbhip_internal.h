@@ -31,7 +31,7 @@
 
 namespace rbhip {
 
-enum TileKind : int { TILE_FREE = 0, TILE_BODIES = 1, TILE_ATOMS = 2 };
+enum TileKind : int { TILE_FREE = 0, TILE_BODIES = 1, TILE_ATOMS = 2, TILE_CLUSTERS = 3 };
 
 // free-atom tiles cover this many atoms per workgroup thread (the integrator
 // kernels loop over them), so the per-tile work of the kinetic-energy
@@ -98,6 +98,18 @@ struct DevSys {
     void* refFree;      // Real[3 * numFree] packed: a_v per free atom, likewise
     double* refKePart;  // per part tile: 6 dt KE_t refined, 6 dt KE_r refined
     double* refPePart;  // per part tile: sum |F|^2/m + tau.(I^-1 tau)
+    // distance constraints between free atoms (numClusters = 0 without): the
+    // atoms of each connected cluster of constraints are contiguous, after
+    // the unconstrained free atoms
+    int numClusters;
+    int pad1;
+    int* clusterStart;  // [numClusters + 1] first internal atom of each cluster
+    int* consStart;     // [numClusters + 1] first constraint of each cluster
+    int* consA;         // internal atom indices of each constraint, in the order given
+    int* consB;
+    double* consDist;
+    double constraintTol;
+    unsigned* constraintFail;   // count of cluster iterations that did not converge
 };
 
 }  // namespace rbhip

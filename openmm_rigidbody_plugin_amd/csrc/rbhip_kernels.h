@@ -27,6 +27,14 @@ hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* ti
                             int maxBlocks, hipStream_t stream);
 hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
                          int maxBlocks, hipStream_t stream);
+// clusters of constrained free atoms (TILE_CLUSTERS tiles, one lane per
+// cluster, block <= 256): part 1 (kick, drift, SHAKE), part 2 (kick, RATTLE,
+// KE partials), both at a fused boundary, or the KE partials after the set-up
+enum ClusterPhase : int { CLUSTER_PART1 = 1, CLUSTER_PART2 = 2, CLUSTER_FUSED = 3, CLUSTER_KE = 4 };
+constexpr int CLUSTER_MAX_ATOMS = 16;
+constexpr int CLUSTER_MAX_ITERATIONS = 1000;
+hipError_t launch_clusters(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int phase,
+                           int block, hipStream_t stream);
 // which: 0 kePart, 1 pePart, 2 refKePart, 3 refPePart
 hipError_t launch_reduce(const DevSys* systems, const int* nslots, int numSystems, int which, double* out,
                          hipStream_t stream);

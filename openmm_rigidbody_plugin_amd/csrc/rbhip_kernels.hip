@@ -676,6 +676,148 @@ __global__ __launch_bounds__(512) void k_refine_pe(const DevSys* __restrict__ sy
 }
 
 // ---------------------------------------------------------------------------
+// Distance constraints between free atoms.  The reference applies OpenMM's
+// constraints to the positions after part 1 (the positions before the step
+// give the bond directions) and to the velocities after part 2
+// (ReferenceRigidBodyKernels.cpp:92-104); part 2's (r - saved)/dt term
+// (RigidBodySystem.cpp:193-204) carries the position correction into the
+// velocities.  One lane per cluster of constrained atoms: SHAKE (Ryckaert,
+// Ciccotti and Berendsen 1977) and RATTLE's velocity stage (Andersen 1983),
+// Gauss-Seidel over the cluster's constraints in the order given, in FP64.
+// Positions are converged when every |r_ij| is within (1 +- tol) d_ij,
+// velocities when every |r_ij . v_ij| <= tol |r_ij| |v_ij|.  The kicks and
+// the kinetic energy are those of the unconstrained free atoms; the
+// correction (r - r_drift)/dt is added right after SHAKE rather than in
+// part 2 (nothing reads the velocities in between).
+// ---------------------------------------------------------------------------
+template <class Real>
+__device__ bool shake(const DevSys& S, int k0, int k1, int a0, V3<double>* r, const V3<double>* ro,
+                      const double* w) {
+    const double lo = (1.0 - S.constraintTol) * (1.0 - S.constraintTol);
+    const double hi = (1.0 + S.constraintTol) * (1.0 + S.constraintTol);
+    for (int it = 0; it < CLUSTER_MAX_ITERATIONS; it++) {
+        bool done = true;
+        for (int k = k0; k < k1; k++) {
+            const int i = S.consA[k] - a0, j = S.consB[k] - a0;
+            const double d2 = S.consDist[k] * S.consDist[k];
+            V3<double> s = r[i] - r[j];
+            double s2 = dot(s, s);
+            if (s2 >= lo * d2 && s2 <= hi * d2) continue;
+            done = false;
+            V3<double> b = ro[i] - ro[j];
+            double g = (d2 - s2) / (2.0 * (w[i] + w[j]) * dot(s, b));
+            r[i] = r[i] + b * (g * w[i]);
+            r[j] = r[j] - b * (g * w[j]);
+        }
+        if (done) return true;
+    }
+    return false;
+}
+
+template <class Real>
+__device__ bool rattle(const DevSys& S, int k0, int k1, int a0, const V3<double>* r, V3<double>* v,
+                       const double* w) {
+    const double tol2 = S.constraintTol * S.constraintTol;
+    for (int it = 0; it < CLUSTER_MAX_ITERATIONS; it++) {
+        bool done = true;
+        for (int k = k0; k < k1; k++) {
+            const int i = S.consA[k] - a0, j = S.consB[k] - a0;
+            V3<double> s = r[i] - r[j];
+            V3<double> dv = v[i] - v[j];
+            double rv = dot(s, dv), s2 = dot(s, s);
+            if (rv * rv <= tol2 * s2 * dot(dv, dv)) continue;
+            done = false;
+            double g = -rv / ((w[i] + w[j]) * s2);
+            v[i] = v[i] + s * (g * w[i]);
+            v[j] = v[j] - s * (g * w[j]);
+        }
+        if (done) return true;
+    }
+    return false;
+}
+
+template <class Real>
+__device__ void cluster_phase(const DevSys& S, int c, int phase, double (&ke)[2]) {
+    const int a0 = S.clusterStart[c], n = S.clusterStart[c + 1] - a0;
+    const int k0 = S.consStart[c], k1 = S.consStart[c + 1];
+    const Real* invm = reinterpret_cast<const Real*>(S.freeInvM);
+    if (phase == CLUSTER_KE) {
+        // after the set-up: as k_setup's free atoms (staged FP64 velocities)
+        for (int j = 0; j < n; j++) {
+            const int a = a0 + j;
+            V3<double> v = S.useStageV ? V3<double>{S.stageV[3 * (size_t)a], S.stageV[3 * (size_t)a + 1],
+                                                    S.stageV[3 * (size_t)a + 2]}
+                                       : cvt<double>(ld3<Real>(S.vel, S.atomStride, a));
+            ke[0] += dot(v, v) / (double)invm[a];
+        }
+        return;
+    }
+    const double dt = S.dt;
+    const Real halfDt = (Real)(0.5 * dt);
+    V3<Real> v[CLUSTER_MAX_ATOMS];
+    V3<double> r[CLUSTER_MAX_ATOMS], vd[CLUSTER_MAX_ATOMS];
+    double w[CLUSTER_MAX_ATOMS];
+    for (int j = 0; j < n; j++) {
+        v[j] = ld3<Real>(S.vel, S.atomStride, a0 + j);
+        r[j] = load_pos<Real>(S, a0 + j);
+        w[j] = 1.0 / S.mass[a0 + j];
+    }
+    bool ok = true;
+    if (phase == CLUSTER_PART2 || phase == CLUSTER_FUSED) {
+        for (int j = 0; j < n; j++) {
+            V3<Real> f = ld3<Real>(S.force, S.atomStride, a0 + j);
+            vd[j] = cvt<double>(v[j] + f * invm[a0 + j] * halfDt);
+        }
+        ok = rattle<Real>(S, k0, k1, a0, r, vd, w) && ok;
+        for (int j = 0; j < n; j++) v[j] = cvt<Real>(vd[j]);
+        if (phase == CLUSTER_PART2)
+            for (int j = 0; j < n; j++) {
+                V3<double> u = cvt<double>(v[j]);
+                ke[0] += dot(u, u) / (double)invm[a0 + j];
+            }
+    }
+    if (phase == CLUSTER_PART1 || phase == CLUSTER_FUSED) {
+        V3<double> ro[CLUSTER_MAX_ATOMS], ru[CLUSTER_MAX_ATOMS];   // before the step, after the drift
+        for (int j = 0; j < n; j++) {
+            V3<Real> f = ld3<Real>(S.force, S.atomStride, a0 + j);
+            v[j] = v[j] + f * invm[a0 + j] * halfDt;
+            ro[j] = r[j];
+            r[j] = r[j] + cvt<double>(v[j]) * dt;
+            ru[j] = r[j];
+        }
+        ok = shake<Real>(S, k0, k1, a0, r, ro, w) && ok;
+        const double invDt = 1.0 / dt;
+        for (int j = 0; j < n; j++) {
+            v[j] = cvt<Real>(cvt<double>(v[j]) + (r[j] - ru[j]) * invDt);
+            store_pos<Real>(S, a0 + j, r[j]);
+        }
+    }
+    for (int j = 0; j < n; j++) st3<Real>(S.vel, S.atomStride, a0 + j, v[j]);
+    if (!ok) atomicAdd(S.constraintFail, 1u);
+}
+
+template <class Real>
+__global__ __launch_bounds__(256) void k_clusters(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
+                                                   int numTiles, int phase) {
+    for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const DevSys& S = systems[tile.sys];
+        double ke[3] = {0.0, 0.0, 0.0};
+        if ((int)threadIdx.x < tile.count) {
+            double k2[2] = {0.0, 0.0};
+            cluster_phase<Real>(S, tile.first + threadIdx.x, phase, k2);
+            ke[0] = k2[0];
+        }
+        // set-up partials carry a dof entry (the bodies'), zero here
+        if (phase == CLUSTER_KE) block_reduce_store<3>(ke, S.kePart, tile.slot);
+        else if (phase == CLUSTER_PART2) {
+            double k2[2] = {ke[0], 0.0};
+            block_reduce_store<2>(k2, S.kePart, tile.slot);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // force field: harmonic tether + harmonic bonds, one lane per atom.  Body
 // atoms are read at their fp32 positions (as a mixed-precision force field
 // reads fp32 coordinates), free atoms at hi + lo.
@@ -800,6 +942,15 @@ hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles,
     if (grid == 0) return hipSuccess;
     if (precision == 0) k_part2<float><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
     else k_part2<double><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_clusters(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int phase,
+                           int block, hipStream_t stream) {
+    if (numTiles <= 0) return hipSuccess;
+    if (block > 256) return hipErrorInvalidValue;
+    if (precision == 0) k_clusters<float><<<numTiles, block, 0, stream>>>(systems, tiles, numTiles, phase);
+    else k_clusters<double><<<numTiles, block, 0, stream>>>(systems, tiles, numTiles, phase);
     return hipGetLastError();
 }
 

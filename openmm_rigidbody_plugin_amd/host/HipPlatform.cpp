@@ -40,8 +40,13 @@ void HipPlatform::PlatformData::createSimulation(const System& system, const std
     const int n = system.getNumParticles();
     std::vector<double> masses(n);
     for (int i = 0; i < n; i++) masses[i] = system.getParticleMass(i);
+    const int nc = system.getNumConstraints();
+    std::vector<int> pairs(2 * (size_t)nc);
+    std::vector<double> dist(nc);
+    for (int c = 0; c < nc; c++) system.getConstraintParameters(c, pairs[2 * c], pairs[2 * c + 1], dist[c]);
     rbhip_context* s = nullptr;
-    rbhip_check(rbhip_create(n, masses.data(), bodyIndices.data(), stepSize, rotationMode, precision, device, &s));
+    rbhip_check(rbhip_create_constrained(n, masses.data(), bodyIndices.data(), nc, pairs.data(), dist.data(),
+                                         stepSize, rotationMode, precision, device, &s));
     try {
         for (int f = 0; f < system.getNumForces(); f++) {
             const Force& force = system.getForce(f);

@@ -7,10 +7,10 @@ using rbhost::Exception;
 using rbhost::rbhip_check;
 
 void HipIntegrateRigidBodyStepKernel::initialize(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator) {
-    if (context.getSystem().getNumConstraints() > 0)
-        throw Exception("The HIP platform does not support constraints between free atoms");
     stepSize = integrator.getStepSize();
     data.createSimulation(context.getSystem(), integrator.getBodyIndices(), stepSize, integrator.getRotationMode());
+    tolerance = -1.0;
+    syncTolerance(integrator);
     if (integrator.getComputeRefinedEnergies()) rbhip_check(rbhip_set_refined_energies(data.require(), 1));
 }
 
@@ -30,8 +30,17 @@ void HipIntegrateRigidBodyStepKernel::execute(rbhost::ContextImpl& context, cons
     execute(context, integrator, 1);
 }
 
+void HipIntegrateRigidBodyStepKernel::syncTolerance(const RigidBodyIntegrator& integrator) {
+    double tol = integrator.getConstraintTolerance();
+    if (tol != tolerance) {
+        rbhip_check(rbhip_set_constraint_tolerance(data.require(), tol));
+        tolerance = tol;
+    }
+}
+
 void HipIntegrateRigidBodyStepKernel::execute(rbhost::ContextImpl&, const RigidBodyIntegrator& integrator, int steps) {
     syncStepSize(integrator);
+    syncTolerance(integrator);
     rbhip_check(rbhip_execute(data.require(), steps));
 }
 

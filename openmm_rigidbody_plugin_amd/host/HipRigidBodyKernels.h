@@ -34,8 +34,11 @@ public:
 
 private:
     void syncStepSize(const RigidBodyIntegrator& integrator);
+    // the reference reads the tolerance at every step (ReferenceRigidBodyKernels.cpp:85)
+    void syncTolerance(const RigidBodyIntegrator& integrator);
     rbhost::HipPlatform::PlatformData& data;
     double stepSize = 0.0;
+    double tolerance = -1.0;
 };
 
 class HipRigidBodyKernelFactory : public rbhost::KernelFactory {

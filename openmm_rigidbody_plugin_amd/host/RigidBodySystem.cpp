@@ -71,8 +71,9 @@ void RigidBodySystem::update(rbhost::ContextImpl&, bool geometry, bool) {
     if (!geometry) return;
     int counts[5];
     rbhost::rbhip_check(rbhip_get_counts(require(), counts));
-    // numDOF = numFree - numConstraints + sum of body dof (RigidBodySystem.cpp:130-134)
-    numDOF = counts[4] - numConstraints;
+    // numDOF = numFree - numConstraints + sum of body dof (RigidBodySystem.cpp:130-134),
+    // as the device context counts it
+    numDOF = counts[4];
 }
 
 Vec3 RigidBodySystem::getBodyFixedPosition(int i) const {

@@ -49,6 +49,9 @@ def lib() -> ctypes.CDLL:
             "rbhip_device_count": (_I, [_IP]),
             "rbhip_device_synchronize": (_I, []),
             "rbhip_create": (_I, [_I, _DP, _IP, _D, _I, _I, _I, ctypes.POINTER(_P)]),
+            "rbhip_create_constrained": (_I, [_I, _DP, _IP, _I, _IP, _DP, _D, _I, _I, _I, ctypes.POINTER(_P)]),
+            "rbhip_set_constraint_tolerance": (_I, [_P, _D]),
+            "rbhip_constraint_failures": (_I, [_P, _LLP]),
             "rbhip_destroy": (_I, [_P]),
             "rbhip_set_stream": (_I, [_P, _P]),
             "rbhip_device_bytes": (_I, [_P, _SIZEP]),
@@ -160,7 +163,9 @@ def copy_calibration(nbytes: int, reps: int = 20) -> float:
 class Context:
     """One rigid-body system on one GPU: RigidBodyIntegrator + its HIP step kernel."""
 
-    def __init__(self, masses, body_indices, dt, rotation_mode=10, precision="mixed", device=0):
+    def __init__(self, masses, body_indices, dt, rotation_mode=10, precision="mixed", device=0, constraints=None):
+        """`constraints`: (atom1, atom2, distance) triples between free atoms
+        (System::addConstraint), or None."""
         m = np.ascontiguousarray(masses, dtype=np.float64)
         b = np.ascontiguousarray(body_indices, dtype=np.int32)
         if m.shape != b.shape or m.ndim != 1:
@@ -168,13 +173,21 @@ class Context:
         self.n = m.size
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self._h = _P()
-        _check(lib().rbhip_create(self.n, _dp(m), _ip(b), float(dt), int(rotation_mode), self.precision,
-                                  int(device), ctypes.byref(self._h)))
+        if constraints is not None and len(constraints) > 0:
+            pairs = np.ascontiguousarray([[int(a), int(c)] for a, c, _ in constraints], dtype=np.int32)
+            dist = np.ascontiguousarray([float(d) for _, _, d in constraints], dtype=np.float64)
+            _check(lib().rbhip_create_constrained(self.n, _dp(m), _ip(b), len(dist), _ip(pairs), _dp(dist),
+                                                  float(dt), int(rotation_mode), self.precision, int(device),
+                                                  ctypes.byref(self._h)))
+        else:
+            _check(lib().rbhip_create(self.n, _dp(m), _ip(b), float(dt), int(rotation_mode), self.precision,
+                                      int(device), ctypes.byref(self._h)))
 
     @classmethod
-    def from_spec(cls, spec, precision="mixed", rotation_mode=None, dt=None, device=0, set_state=True):
+    def from_spec(cls, spec, precision="mixed", rotation_mode=None, dt=None, device=0, set_state=True,
+                  constraints=None):
         c = cls(spec.masses, spec.body_indices, spec.dt if dt is None else dt,
-                spec.rotation_mode if rotation_mode is None else rotation_mode, precision, device)
+                spec.rotation_mode if rotation_mode is None else rotation_mode, precision, device, constraints)
         c.set_tether(spec.tether_k, spec.tether_r0)
         if set_state:
             c.set_positions(spec.positions)
@@ -195,6 +208,15 @@ class Context:
     @property
     def handle(self):
         return self._h
+
+    # ---- constraints ----
+    def set_constraint_tolerance(self, tol):
+        _check(lib().rbhip_set_constraint_tolerance(self._h, float(tol)))
+
+    def constraint_failures(self):
+        n = ctypes.c_longlong()
+        _check(lib().rbhip_constraint_failures(self._h, ctypes.byref(n)))
+        return n.value
 
     # ---- bookkeeping ----
     def counts(self):

@@ -43,6 +43,9 @@ def lib():
             "rbref_destroy": (None, [_P]),
             "rbref_set_tether": (_I, [_P, _DP, _DP]),
             "rbref_add_bond": (_I, [_P, _I, _I, _D, _D]),
+            "rbref_add_constraint": (_I, [_P, _I, _I, _D]),
+            "rbref_set_constraint_tolerance": (_I, [_P, _D]),
+            "rbref_constraint_failures": (ctypes.c_long, [_P]),
             "rbref_set_positions": (_I, [_P, _DP]),
             "rbref_set_velocities": (_I, [_P, _DP]),
             "rbref_set_step_size": (_I, [_P, _D]),
@@ -139,6 +142,15 @@ class OracleSim:
 
     def add_bond(self, a, b, length, k):
         _check(lib().rbref_add_bond(self._h, a, b, length, k))
+
+    def add_constraint(self, a, b, distance):
+        _check(lib().rbref_add_constraint(self._h, int(a), int(b), float(distance)))
+
+    def set_constraint_tolerance(self, tol):
+        _check(lib().rbref_set_constraint_tolerance(self._h, float(tol)))
+
+    def constraint_failures(self):
+        return int(lib().rbref_constraint_failures(self._h))
 
     def set_positions(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -50,6 +50,16 @@
  *                           → test_oracle_refined: exact at constant velocity,
  *                             third order for free rotors, shadow-energy
  *                             fluctuation O(dt^3) vs O(dt^2), closed-form U_ref
+ *   free-atom constraints   the call sites ReferenceRigidBodyKernels.cpp:92-104,
+ *                           the body-atom refusal RigidBodySystem.cpp:107-112
+ *                           and the DOF count :130; the solver (OpenMM's, not
+ *                           in the reference tree) as SHAKE + RATTLE
+ *                           → test_oracle_constraints: the projection equals a
+ *                             Newton solve of the constraint equations, RATTLE
+ *                             velocities orthogonal to the bonds, constrained
+ *                             free atoms converge to the rigid-body trajectory
+ *                             as dt -> 0, energy conservation, the reference's
+ *                             (commented-out) constraint tests
  *
  * Parity unpinned (no check fixes them beyond the restatement itself): the
  * bit-level FP64 output of the original exactRotation composition and of the
@@ -688,6 +698,7 @@ struct BodySystem {
     std::vector<Vec3> d;          /* bodyFixedPositions */
     std::vector<FreeAtom> freeAtom;
     int numFree = 0, numBodies = 0, numActualAtoms = 0, numBodyAtoms = 0, numDOF = 0;
+    int numConstraints = 0;
     double transKE = 0.0, rotKE = 0.0;
 
     int atomOf(const Body& b, int j) const { return atomIndex[numFree + b.loc + j]; }
@@ -812,7 +823,7 @@ struct BodySystem {
     /* RigidBodySystem::update, RigidBodySystem.cpp:120-142 */
     void update(const SystemState& ctx, bool geometry, bool velocities) {
         if (geometry) {
-            numDOF = numFree;   /* minus constraints: none in this system model */
+            numDOF = numFree - numConstraints;
             for (Body& b : body) { buildGeometry(b, ctx.R, ctx.F, ctx.mass); numDOF += b.dof; }
         }
         if (velocities)
@@ -879,6 +890,92 @@ struct BodySystem {
     }
 };
 
+/* ------------------------------------------------------------------------ */
+/* Distance constraints between free atoms.  The reference hands them to      */
+/* OpenMM's ReferenceConstraints: positions after integratePart1, with the    */
+/* positions before the step as the reference directions, and velocities     */
+/* after integratePart2 (ReferenceRigidBodyKernels.cpp:92-104); constraints   */
+/* touching body atoms are refused (RigidBodySystem.cpp:107-112) and each one */
+/* removes a degree of freedom (RigidBodySystem.cpp:130).  OpenMM is not part */
+/* of the reference tree, so its constraint solver is not restated: this is  */
+/* SHAKE for positions (Ryckaert, Ciccotti and Berendsen, J. Comput. Phys.    */
+/* 23, 327 (1977)) and RATTLE's velocity stage (Andersen, J. Comput. Phys.    */
+/* 52, 24 (1983)), Gauss-Seidel over each connected cluster of constraints in */
+/* the order they were added.  Positions are converged when every |r_ij| is   */
+/* within (1 +- tol) d_ij, OpenMM's criterion; velocities when every          */
+/* |r_ij . v_ij| <= tol |r_ij| |v_ij|.  The converged results are fixed by    */
+/* the constraint equations, which the tests check (parity of the iteration   */
+/* itself with OpenMM is unpinned).                                           */
+/* ------------------------------------------------------------------------ */
+struct Constraint { int a, b; double d; };
+
+struct Constraints {
+    static constexpr int kMaxIterations = 1000;
+    std::vector<Constraint> list;
+    std::vector<std::vector<int>> clusters;   /* constraint indices, in order added */
+    double tol = 1e-5;
+    long failures = 0;
+
+    /* connected components of the constraint graph */
+    void build(int n) {
+        std::vector<int> root(n);
+        for (int i = 0; i < n; i++) root[i] = i;
+        auto find = [&](int x) { while (root[x] != x) x = root[x] = root[root[x]]; return x; };
+        for (const Constraint& c : list) root[find(c.a)] = find(c.b);
+        std::vector<int> id(n, -1);
+        clusters.clear();
+        for (size_t k = 0; k < list.size(); k++) {
+            int r = find(list[k].a);
+            if (id[r] < 0) { id[r] = (int)clusters.size(); clusters.emplace_back(); }
+            clusters[id[r]].push_back((int)k);
+        }
+    }
+
+    void applyPositions(const std::vector<Vec3>& Rold, std::vector<Vec3>& R, const std::vector<double>& invMass) {
+        const double lo = (1.0 - tol) * (1.0 - tol), hi = (1.0 + tol) * (1.0 + tol);
+        for (const std::vector<int>& cl : clusters) {
+            bool done = false;
+            for (int it = 0; it < kMaxIterations && !done; it++) {
+                done = true;
+                for (int k : cl) {
+                    const Constraint& c = list[k];
+                    Vec3 s = R[c.a] - R[c.b];
+                    double s2 = s.dot(s), d2 = c.d * c.d;
+                    if (s2 >= lo * d2 && s2 <= hi * d2) continue;
+                    done = false;
+                    Vec3 ro = Rold[c.a] - Rold[c.b];
+                    double g = (d2 - s2) / (2.0 * (invMass[c.a] + invMass[c.b]) * s.dot(ro));
+                    R[c.a] += ro * (g * invMass[c.a]);
+                    R[c.b] -= ro * (g * invMass[c.b]);
+                }
+            }
+            if (!done) failures++;
+        }
+    }
+
+    void applyVelocities(const std::vector<Vec3>& R, std::vector<Vec3>& V, const std::vector<double>& invMass) {
+        const double tol2 = tol * tol;
+        for (const std::vector<int>& cl : clusters) {
+            bool done = false;
+            for (int it = 0; it < kMaxIterations && !done; it++) {
+                done = true;
+                for (int k : cl) {
+                    const Constraint& c = list[k];
+                    Vec3 s = R[c.a] - R[c.b];
+                    Vec3 dv = V[c.a] - V[c.b];
+                    double rv = s.dot(dv), s2 = s.dot(s);
+                    if (rv * rv <= tol2 * s2 * dv.dot(dv)) continue;
+                    done = false;
+                    double g = -rv / ((invMass[c.a] + invMass[c.b]) * s2);
+                    V[c.a] += s * (g * invMass[c.a]);
+                    V[c.b] -= s * (g * invMass[c.b]);
+                }
+            }
+            if (!done) failures++;
+        }
+    }
+};
+
 }  // namespace
 
 /* ------------------------------------------------------------------------ */
@@ -894,6 +991,8 @@ struct rbref_sim {
     BodySystem integratorSystem;   /* RigidBodyIntegrator::bodySystem */
     BodySystem kernelSystem;       /* ReferenceIntegrateRigidBodyStepKernel::bodySystem */
     std::vector<Vec3> oldPos;
+    Constraints constraints;
+    std::vector<double> invMass;   /* 1/m per atom, for the constraints */
 
     /* RigidBodyIntegrator::stateChanged, RigidBodyIntegrator.cpp:63-74, with
        uploadBodySystem = RigidBodySystem::copy (ReferenceRigidBodyKernels.cpp:78-80) */
@@ -924,9 +1023,13 @@ struct rbref_sim {
             oldPos[i] = ctx.R[i];
         }
         kernelSystem.integratePart1(dt, ctx.F, ctx.V, ctx.R);
-        /* constraints / virtual sites: none in this system model */
+        if (kernelSystem.numFree != 0 && !constraints.list.empty())
+            constraints.applyPositions(oldPos, ctx.R, invMass);
+        /* virtual sites: none in this system model */
         ctx.calcForces();
         kernelSystem.integratePart2(dt, ctx.R, ctx.F, ctx.V);
+        if (kernelSystem.numFree != 0 && !constraints.list.empty())
+            constraints.applyVelocities(ctx.R, ctx.V, invMass);
         if (refined) refineEnd();
         ctx.time += dt;
         ctx.stepCount++;
@@ -1062,6 +1165,8 @@ rbref_sim* rbref_create(int numAtoms, const double* masses, const int* bodyIndic
         s->rotationMode = rotationMode;
         s->flags = flags;
         s->oldPos.assign(numAtoms, Vec3());
+        s->invMass.resize(numAtoms);
+        for (int i = 0; i < numAtoms; i++) s->invMass[i] = 1.0 / masses[i];
         s->integratorSystem.initialize(s->ctx, s->bodyIndices, rotationMode, flags);
         s->kernelSystem = s->integratorSystem;
     });
@@ -1085,6 +1190,30 @@ int rbref_add_bond(rbref_sim* s, int a, int b, double length, double k) {
         s->ctx.bonds.push_back(Bond{a, b, length, k});
     });
 }
+
+int rbref_add_constraint(rbref_sim* s, int a, int b, double distance) {
+    return guarded([&] {
+        if (a < 0 || b < 0 || a >= s->ctx.n || b >= s->ctx.n || a == b)
+            throw std::runtime_error("constraint atoms out of range");
+        if (!(distance > 0.0)) throw std::runtime_error("constraint distance must be positive");
+        /* RigidBodySystem::initialize, RigidBodySystem.cpp:107-112 */
+        if (s->integratorSystem.bodyIndex[a] != 0 || s->integratorSystem.bodyIndex[b] != 0)
+            throw std::runtime_error("Constraints involving rigid-body atoms are not allowed");
+        s->constraints.list.push_back(Constraint{a, b, distance});
+        s->constraints.build(s->ctx.n);
+        int nc = (int)s->constraints.list.size();
+        s->integratorSystem.numConstraints = s->kernelSystem.numConstraints = nc;
+    });
+}
+
+int rbref_set_constraint_tolerance(rbref_sim* s, double tol) {
+    return guarded([&] {
+        if (!(tol > 0.0)) throw std::runtime_error("constraint tolerance must be positive");
+        s->constraints.tol = tol;
+    });
+}
+
+long rbref_constraint_failures(rbref_sim* s) { return s->constraints.failures; }
 
 int rbref_set_positions(rbref_sim* s, const double* xyz) {
     return guarded([&] { loadVec(s->ctx.R, xyz, s->ctx.n); s->stateChanged(true); });

@@ -70,6 +70,12 @@ void rbref_destroy(rbref_sim* s);
 int rbref_set_tether(rbref_sim* s, const double* k, const double* r0);
 int rbref_add_bond(rbref_sim* s, int a, int b, double length, double k);
 /* Context::setPositions / setVelocities → stateChanged (xyz interleaved, original order) */
+/* a distance constraint between two free atoms (System::addConstraint);
+   refused for body atoms, as RigidBodySystem.cpp:107-112 does.  Takes effect
+   in the next step; the DOF count follows at the next state update. */
+int rbref_add_constraint(rbref_sim* s, int a, int b, double distance);
+int rbref_set_constraint_tolerance(rbref_sim* s, double tol);
+long rbref_constraint_failures(rbref_sim* s);   /* clusters not converged, so far */
 int rbref_set_positions(rbref_sim* s, const double* xyz);
 int rbref_set_velocities(rbref_sim* s, const double* xyz);
 int rbref_set_step_size(rbref_sim* s, double dt);

@@ -16,12 +16,18 @@
 //                        oscillator: refined kinetic energy + potential +
 //                        potential refinement is the velocity-Verlet shadow
 //                        energy, conserved far better than the plain energy
+//   testConstraints / testConstrainedClusters  the reference's constraint
+//                        tests (tests/TestRigidBodyIntegrator.h:85-197,
+//                        commented out there) with tethers standing in for
+//                        NonbondedForce: distances held to 1e-4 / 2e-5,
+//                        energy to 1% over 1000 steps, one DOF per constraint
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "HipRigidBodyKernels.h"
@@ -182,6 +188,73 @@ void testSingleBondRefined(Platform& platform) {
     }
 }
 
+// deterministic uniform numbers in [-0.5, 0.5) (a 64-bit LCG)
+struct Uniform {
+    unsigned long long x;
+    explicit Uniform(unsigned long long seed) : x(seed * 2862933555777941757ULL + 3037000493ULL) {}
+    double operator()() {
+        x = x * 6364136223846793005ULL + 1442695040888963407ULL;
+        return (double)(x >> 11) / 9007199254740992.0 - 0.5;
+    }
+};
+
+static void runConstrained(Platform& platform, const std::string& precision, const std::vector<double>& masses,
+                           const std::vector<Vec3>& positions, const std::vector<std::tuple<int, int, double>>& cons,
+                           double distTol, unsigned long long seed) {
+    const int n = (int)masses.size();
+    System system;
+    for (double m : masses) system.addParticle(m);
+    for (const auto& c : cons) system.addConstraint(std::get<0>(c), std::get<1>(c), std::get<2>(c));
+    Uniform u(seed);
+    TetherForce* tether = new TetherForce();
+    for (int i = 0; i < n; i++) tether->addParticle(i, 50.0, positions[i] + Vec3(0.4 * u(), 0.4 * u(), 0.4 * u()));
+    system.addForce(tether);
+    RigidBodyIntegrator integrator(0.001, std::vector<int>(n, 0));
+    integrator.setConstraintTolerance(1e-5);
+    Context context(system, integrator, platform, props(precision));
+    context.setPositions(positions);
+    std::vector<Vec3> v(n);
+    for (int i = 0; i < n; i++) v[i] = Vec3(u(), u(), u());
+    context.setVelocities(v);
+    // RigidBodySystem.cpp:130: numFree - numConstraints (a free atom counts one)
+    if (integrator.getRigidBodySystem().getNumDOF() != n - (int)cons.size()) throw Exception("numDOF");
+    double e0 = 0.0;
+    for (int i = 0; i < 1000; i++) {
+        State st = context.getState(State::Positions | State::Energy);
+        const std::vector<Vec3>& R = st.getPositions();
+        for (const auto& c : cons) {
+            Vec3 d = R[std::get<0>(c)] - R[std::get<1>(c)];
+            CHECK_TOL(std::get<2>(c), std::sqrt(d.dot(d)), distTol);
+        }
+        double e = st.getPotentialEnergy() + st.getKineticEnergy();
+        if (i == 1) e0 = e;
+        else if (i > 1 && !(std::fabs(e - e0) <= 0.01 * std::fabs(e0)))
+            throw Exception("energy drifted from " + std::to_string(e0) + " to " + std::to_string(e));
+        integrator.step(1);
+    }
+}
+
+void testConstraints(Platform& platform, const std::string& precision) {
+    std::vector<double> masses;
+    std::vector<Vec3> positions;
+    for (int i = 0; i < 8; i++) {
+        masses.push_back(i % 2 == 0 ? 5.0 : 10.0);
+        positions.push_back(Vec3(i / 2, (i + 1) / 2, 0));
+    }
+    runConstrained(platform, precision, masses, positions,
+                   {{0, 1, 1.0}, {1, 2, 1.0}, {2, 3, 1.0}, {4, 5, 1.0}, {6, 7, 1.0}}, 1e-4, 0);
+}
+
+void testConstrainedClusters(Platform& platform, const std::string& precision) {
+    const double r2 = std::sqrt(2.0);
+    runConstrained(platform, precision, {10.0, 10.0, 1.0, 1.0, 1.0, 1.0, 1.0},
+                   {Vec3(0, 0, 0), Vec3(1, 0, 0), Vec3(-1, 0, 0), Vec3(0, 1, 0), Vec3(0, 0, 1), Vec3(2, 0, 0),
+                    Vec3(1, 1, 0)},
+                   {{0, 1, 1.0}, {0, 2, 1.0}, {0, 3, 1.0}, {0, 4, 1.0}, {1, 5, 1.0}, {1, 6, 1.0}, {2, 3, r2},
+                    {2, 4, r2}, {3, 4, r2}, {5, 6, r2}},
+                   2e-5, 1);
+}
+
 void testApiErrors(Platform& platform) {
     RigidBodyIntegrator a(0.001, std::vector<int>{1, 1});
     CHECK_THROWS(a.setRotationMode(-1));
@@ -277,6 +350,10 @@ int main() {
         {"testApiErrors", [&] { testApiErrors(platform); }},
         {"testBookkeeping", [&] { testBookkeeping(platform); }},
         {"testSingleBondRefined", [&] { testSingleBondRefined(platform); }},
+        {"testConstraints mixed", [&] { testConstraints(platform, "mixed"); }},
+        {"testConstraints double", [&] { testConstraints(platform, "double"); }},
+        {"testConstrainedClusters mixed", [&] { testConstrainedClusters(platform, "mixed"); }},
+        {"testConstrainedClusters double", [&] { testConstrainedClusters(platform, "double"); }},
     };
     for (auto& t : tests) {
         std::printf("%s ...\n", t.first.c_str());
