@@ -375,6 +375,8 @@ struct rbhip_context {
     // distance constraints between free atoms: the atoms of each cluster are
     // contiguous and follow the numFree - numClustered unconstrained ones
     int numConstraints = 0, numClusters = 0, numClustered = 0;
+    std::vector<int> clusterN;                   // atoms per cluster (small classes first)
+    std::vector<int> clusterK;                   // constraints per cluster
     std::vector<int> bodyIndex, atomIndex;
     std::vector<int> refBodyN, refBodyLoc;       // per reference body
     std::vector<int> internalBody;               // internal body -> reference body
@@ -399,6 +401,7 @@ struct rbhip_context {
     Tile* dClusterTiles = nullptr;
     Tile* dClusterSetupTiles = nullptr;
     int nClusterTiles = 0;
+    int clusterClassTiles[4] = {0, 0, 0, 0};      // tiles of 2, 3, 4 atoms, then larger
     static constexpr int kClusterBlock = 64;
     int maxKeSlots = 0, maxPeSlots = 0;
     int keSlots = 0;                             // slots written by the last KE producer
@@ -540,13 +543,22 @@ struct rbhip_context {
         }
     }
 
-    // tiles of up to kClusterBlock clusters (one lane each), KE slots from slot0
+    // tiles of up to kClusterBlock clusters (one lane each) of one size class
+    // (N = 2, 3 or 4 atoms with at most N (N - 1) / 2 constraints, held in
+    // registers, or N = 0 for the others), KE slots from slot0
     std::vector<Tile> clusterTiles(int slot0) const {
         std::vector<Tile> t;
-        for (int c = 0; c < numClusters; c += kClusterBlock) {
-            Tile x{0, TILE_CLUSTERS, c, std::min(kClusterBlock, numClusters - c), 0, 0, 1, slot0 + (int)t.size()};
+        auto cls = [&](int c) {
+            int n = clusterN[c];
+            return n <= 4 && clusterK[c] <= n * (n - 1) / 2 ? n : 0;
+        };
+        for (int c = 0; c < numClusters;) {
+            int e = c;
+            while (e < numClusters && e - c < kClusterBlock && cls(e) == cls(c)) e++;
+            Tile x{0, TILE_CLUSTERS, c, e - c, cls(c), 0, 1, slot0 + (int)t.size()};
             require(x.slot < maxKeSlots, "cluster tile slot out of range", RBHIP_E_STATE);
             t.push_back(x);
+            c = e;
         }
         return t;
     }
@@ -612,6 +624,15 @@ struct rbhip_context {
         nForceTiles = (int)ft.size();
         nSetupTiles = (int)st.size();
         nClusterTiles = (int)ct.size();
+        // classes in order 2, 3, 4, larger (N = 0), as the clusters are sorted
+        int prev = 0;
+        for (int i = 0; i < 4; i++) clusterClassTiles[i] = 0;
+        for (const Tile& x : ct) {
+            int cls = x.N == 0 ? 3 : x.N - 2;
+            require(cls >= prev && cls >= 0 && cls < 4, "cluster tiles out of class order", RBHIP_E_STATE);
+            clusterClassTiles[cls]++;
+            prev = cls;
+        }
         graphs.reset();
     }
 
@@ -626,7 +647,7 @@ struct rbhip_context {
     }
     void launchClusters(int phase) {
         if (nClusterTiles > 0)
-            HIPCHK(launch_clusters(precision, dSys, dClusterTiles, nClusterTiles, phase, kClusterBlock, stream));
+            HIPCHK(launch_clusters(precision, dSys, dClusterTiles, clusterClassTiles, phase, kClusterBlock, stream));
     }
     void launchPart1() {
         HIPCHK(launch_part1(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
@@ -665,7 +686,7 @@ struct rbhip_context {
         uploadSys();
         HIPCHK(launch_setup(precision, dSys, dSetupTiles, nSetupTiles, geometry, setupBlock(), stream));
         if (nClusterTiles > 0)
-            HIPCHK(launch_clusters(precision, dSys, dClusterSetupTiles, nClusterTiles, CLUSTER_KE, kClusterBlock,
+            HIPCHK(launch_clusters(precision, dSys, dClusterSetupTiles, clusterClassTiles, CLUSTER_KE, kClusterBlock,
                                    stream));
         keSlots = nSetupTiles + nClusterTiles;
         kickDt = dt;
@@ -1018,6 +1039,24 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
                 else clusterAtoms[id[r]].push_back(i);
             }
             freeOrder = unconstrained;
+            // clusters of n = 2-4 atoms with at most n (n - 1) / 2
+            // constraints first, by size, then the others: each tile holds
+            // one class
+            auto key = [&](int c2) {
+                size_t n = clusterAtoms[c2].size();
+                return n <= 4 && clusterCons[c2].size() <= n * (n - 1) / 2 ? n : 100 + n;
+            };
+            std::vector<int> bySize(clusterAtoms.size());
+            for (size_t c2 = 0; c2 < bySize.size(); c2++) bySize[c2] = (int)c2;
+            std::stable_sort(bySize.begin(), bySize.end(), [&](int x, int y) { return key(x) < key(y); });
+            std::vector<std::vector<int>> sortedAtoms, sortedCons;
+            for (int c2 : bySize) {
+                sortedAtoms.push_back(std::move(clusterAtoms[c2]));
+                sortedCons.push_back(std::move(clusterCons[c2]));
+            }
+            clusterAtoms.swap(sortedAtoms);
+            clusterCons.swap(sortedCons);
+            for (const auto& cc : clusterCons) C.clusterK.push_back((int)cc.size());
             for (const auto& ca : clusterAtoms) {
                 if ((int)ca.size() > CLUSTER_MAX_ATOMS)
                     throw RbError(RBHIP_E_UNSUPPORTED, "a cluster of constraints joins " + std::to_string(ca.size()) +
@@ -1025,6 +1064,7 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
                                                            " are supported");
                 freeOrder.insert(freeOrder.end(), ca.begin(), ca.end());
                 C.numClustered += (int)ca.size();
+                C.clusterN.push_back((int)ca.size());
             }
             C.numConstraints = numConstraints;
             C.numClusters = (int)clusterAtoms.size();
@@ -1104,7 +1144,8 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
         h.refBody = h.refFree = nullptr;
         h.refKePart = h.refPePart = nullptr;
         size_t nFreeTilesMax = (C.numFree + 63) / 64 + 1;
-        const int clusterSlots = C.numClusters > 0 ? (C.numClusters + 63) / 64 + 1 : 0;
+        // one partial tile per size class (2, 3, 4, larger) at most
+        const int clusterSlots = C.numClusters > 0 ? (C.numClusters + 63) / 64 + 4 : 0;
         C.maxKeSlots = (int)(nFreeTilesMax + C.numBodies + C.classes.size() + 1) + clusterSlots;
         C.maxPeSlots = (int)((na + 63) / 64 + 1);
         h.kePart = C.arena.allocT<double>(4 * (size_t)C.maxKeSlots);

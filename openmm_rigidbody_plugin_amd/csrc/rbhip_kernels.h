@@ -33,7 +33,9 @@ hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles
 enum ClusterPhase : int { CLUSTER_PART1 = 1, CLUSTER_PART2 = 2, CLUSTER_FUSED = 3, CLUSTER_KE = 4 };
 constexpr int CLUSTER_MAX_ATOMS = 16;
 constexpr int CLUSTER_MAX_ITERATIONS = 1000;
-hipError_t launch_clusters(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int phase,
+// (counts: tiles of clusters of n = 2, 3 and 4 atoms with at most
+// n (n - 1) / 2 constraints, then of the other clusters, in that order)
+hipError_t launch_clusters(int precision, const DevSys* systems, const Tile* tiles, const int* counts, int phase,
                            int block, hipStream_t stream);
 // which: 0 kePart, 1 pePart, 2 refKePart, 3 refPePart
 hipError_t launch_reduce(const DevSys* systems, const int* nslots, int numSystems, int which, double* out,

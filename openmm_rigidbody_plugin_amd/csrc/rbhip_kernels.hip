@@ -22,6 +22,8 @@
 // covers any number of systems (the batch path) and any grid cap.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "rbhip_access.h"
 #include "rbhip_kernels.h"
 
@@ -690,56 +692,132 @@ __global__ __launch_bounds__(512) void k_refine_pe(const DevSys* __restrict__ sy
 // correction (r - r_drift)/dt is added right after SHAKE rather than in
 // part 2 (nothing reads the velocities in between).
 // ---------------------------------------------------------------------------
-template <class Real>
-__device__ bool shake(const DevSys& S, int k0, int k1, int a0, V3<double>* r, const V3<double>* ro,
-                      const double* w) {
+// Per-atom values of one cluster.  Clusters of NA <= 4 atoms (every tile
+// holds clusters of one size) keep them in registers and reach atom i of a
+// constraint by selects; larger ones index private memory.
+template <int NA> struct ClusterSlots {
+    V3<double> v[NA];
+    __device__ __forceinline__ V3<double> get(int i) const {
+        if constexpr (NA <= 4) {
+            V3<double> o = v[0];
+#pragma unroll
+            for (int k = 1; k < NA; k++)
+                if (i == k) o = v[k];
+            return o;
+        } else {
+            return v[i];
+        }
+    }
+    __device__ __forceinline__ void add(int i, V3<double> d) {
+        if constexpr (NA <= 4) {
+#pragma unroll
+            for (int k = 0; k < NA; k++)
+                if (i == k) v[k] = v[k] + d;
+        } else {
+            v[i] = v[i] + d;
+        }
+    }
+};
+
+// The constraints of one cluster: for NA <= 4 at most NA (NA - 1) / 2 (the
+// distinct pairs), held in registers and read once per step; larger
+// clusters read them from memory in every sweep.
+template <int NA> struct ClusterCons {
+    static constexpr int kMax = NA <= 4 ? NA * (NA - 1) / 2 : 1;
+    int n = 0, k0 = 0, a0 = 0;
+    int ci[kMax], cj[kMax];
+    double cd[kMax];
+    const DevSys* S = nullptr;
+    __device__ __forceinline__ void load(const DevSys& sys, int first, int last, int atom0) {
+        S = &sys;
+        n = last - first;
+        k0 = first;
+        a0 = atom0;
+        if constexpr (NA <= 4) {
+#pragma unroll
+            for (int k = 0; k < kMax; k++)
+                if (k < n) {
+                    ci[k] = sys.consA[first + k] - atom0;
+                    cj[k] = sys.consB[first + k] - atom0;
+                    cd[k] = sys.consDist[first + k];
+                }
+        }
+    }
+    // atoms and length of constraint k (0 <= k < n)
+    __device__ __forceinline__ void get(int k, int& i, int& j, double& d) const {
+        if constexpr (NA <= 4) {
+            i = ci[k];
+            j = cj[k];
+            d = cd[k];
+        } else {
+            i = S->consA[k0 + k] - a0;
+            j = S->consB[k0 + k] - a0;
+            d = S->consDist[k0 + k];
+        }
+    }
+};
+
+template <int NA>
+__device__ __forceinline__ bool shake(const DevSys& S, const ClusterCons<NA>& C, ClusterSlots<NA>& r, const ClusterSlots<NA>& ro,
+                      const ClusterSlots<NA>& w) {
     const double lo = (1.0 - S.constraintTol) * (1.0 - S.constraintTol);
     const double hi = (1.0 + S.constraintTol) * (1.0 + S.constraintTol);
     for (int it = 0; it < CLUSTER_MAX_ITERATIONS; it++) {
         bool done = true;
-        for (int k = k0; k < k1; k++) {
-            const int i = S.consA[k] - a0, j = S.consB[k] - a0;
-            const double d2 = S.consDist[k] * S.consDist[k];
-            V3<double> s = r[i] - r[j];
+#pragma unroll
+        for (int k = 0; k < ClusterCons<NA>::kMax || NA > 4; k++) {
+            if (k >= C.n) break;
+            int i, j;
+            double dist;
+            C.get(k, i, j, dist);
+            const double d2 = dist * dist;
+            V3<double> s = r.get(i) - r.get(j);
             double s2 = dot(s, s);
             if (s2 >= lo * d2 && s2 <= hi * d2) continue;
             done = false;
-            V3<double> b = ro[i] - ro[j];
-            double g = (d2 - s2) / (2.0 * (w[i] + w[j]) * dot(s, b));
-            r[i] = r[i] + b * (g * w[i]);
-            r[j] = r[j] - b * (g * w[j]);
+            const double wi = w.get(i).x, wj = w.get(j).x;
+            V3<double> b = ro.get(i) - ro.get(j);
+            double g = (d2 - s2) / (2.0 * (wi + wj) * dot(s, b));
+            r.add(i, b * (g * wi));
+            r.add(j, b * (-(g * wj)));
         }
         if (done) return true;
     }
     return false;
 }
 
-template <class Real>
-__device__ bool rattle(const DevSys& S, int k0, int k1, int a0, const V3<double>* r, V3<double>* v,
-                       const double* w) {
+template <int NA>
+__device__ __forceinline__ bool rattle(const DevSys& S, const ClusterCons<NA>& C, const ClusterSlots<NA>& r, ClusterSlots<NA>& v,
+                       const ClusterSlots<NA>& w) {
     const double tol2 = S.constraintTol * S.constraintTol;
     for (int it = 0; it < CLUSTER_MAX_ITERATIONS; it++) {
         bool done = true;
-        for (int k = k0; k < k1; k++) {
-            const int i = S.consA[k] - a0, j = S.consB[k] - a0;
-            V3<double> s = r[i] - r[j];
-            V3<double> dv = v[i] - v[j];
+#pragma unroll
+        for (int k = 0; k < ClusterCons<NA>::kMax || NA > 4; k++) {
+            if (k >= C.n) break;
+            int i, j;
+            double dist;
+            C.get(k, i, j, dist);
+            V3<double> s = r.get(i) - r.get(j);
+            V3<double> dv = v.get(i) - v.get(j);
             double rv = dot(s, dv), s2 = dot(s, s);
             if (rv * rv <= tol2 * s2 * dot(dv, dv)) continue;
             done = false;
-            double g = -rv / ((w[i] + w[j]) * s2);
-            v[i] = v[i] + s * (g * w[i]);
-            v[j] = v[j] - s * (g * w[j]);
+            const double wi = w.get(i).x, wj = w.get(j).x;
+            double g = -rv / ((wi + wj) * s2);
+            v.add(i, s * (g * wi));
+            v.add(j, s * (-(g * wj)));
         }
         if (done) return true;
     }
     return false;
 }
 
-template <class Real>
-__device__ void cluster_phase(const DevSys& S, int c, int phase, double (&ke)[2]) {
-    const int a0 = S.clusterStart[c], n = S.clusterStart[c + 1] - a0;
-    const int k0 = S.consStart[c], k1 = S.consStart[c + 1];
+// one cluster of n atoms (n == NA when NA <= 4, n <= NA otherwise)
+template <class Real, int NA>
+__device__ __forceinline__ void cluster_phase(const DevSys& S, int c, int phase, double (&ke)[2]) {
+    const int a0 = S.clusterStart[c];
+    const int n = NA <= 4 ? NA : S.clusterStart[c + 1] - a0;
     const Real* invm = reinterpret_cast<const Real*>(S.freeInvM);
     if (phase == CLUSTER_KE) {
         // after the set-up: as k_setup's free atoms (staged FP64 velocities)
@@ -754,49 +832,242 @@ __device__ void cluster_phase(const DevSys& S, int c, int phase, double (&ke)[2]
     }
     const double dt = S.dt;
     const Real halfDt = (Real)(0.5 * dt);
-    V3<Real> v[CLUSTER_MAX_ATOMS];
-    V3<double> r[CLUSTER_MAX_ATOMS], vd[CLUSTER_MAX_ATOMS];
-    double w[CLUSTER_MAX_ATOMS];
-    for (int j = 0; j < n; j++) {
+    V3<Real> v[NA];
+    ClusterSlots<NA> r, vd, w;   // w holds 1/m in x
+    ClusterCons<NA> C;
+    C.load(S, S.consStart[c], S.consStart[c + 1], a0);
+#pragma unroll
+    for (int j = 0; j < NA; j++) {
+        if (NA > 4 && j >= n) break;
         v[j] = ld3<Real>(S.vel, S.atomStride, a0 + j);
-        r[j] = load_pos<Real>(S, a0 + j);
-        w[j] = 1.0 / S.mass[a0 + j];
+        r.v[j] = load_pos<Real>(S, a0 + j);
+        w.v[j] = V3<double>{1.0 / S.mass[a0 + j], 0.0, 0.0};
     }
     bool ok = true;
     if (phase == CLUSTER_PART2 || phase == CLUSTER_FUSED) {
-        for (int j = 0; j < n; j++) {
+#pragma unroll
+        for (int j = 0; j < NA; j++) {
+            if (NA > 4 && j >= n) break;
             V3<Real> f = ld3<Real>(S.force, S.atomStride, a0 + j);
-            vd[j] = cvt<double>(v[j] + f * invm[a0 + j] * halfDt);
+            vd.v[j] = cvt<double>(v[j] + f * invm[a0 + j] * halfDt);
         }
-        ok = rattle<Real>(S, k0, k1, a0, r, vd, w) && ok;
-        for (int j = 0; j < n; j++) v[j] = cvt<Real>(vd[j]);
-        if (phase == CLUSTER_PART2)
-            for (int j = 0; j < n; j++) {
+        ok = rattle<NA>(S, C, r, vd, w) && ok;
+#pragma unroll
+        for (int j = 0; j < NA; j++) {
+            if (NA > 4 && j >= n) break;
+            v[j] = cvt<Real>(vd.v[j]);
+            if (phase == CLUSTER_PART2) {
                 V3<double> u = cvt<double>(v[j]);
                 ke[0] += dot(u, u) / (double)invm[a0 + j];
             }
+        }
     }
     if (phase == CLUSTER_PART1 || phase == CLUSTER_FUSED) {
-        V3<double> ro[CLUSTER_MAX_ATOMS], ru[CLUSTER_MAX_ATOMS];   // before the step, after the drift
-        for (int j = 0; j < n; j++) {
+        ClusterSlots<NA> ro, ru;   // before the step, after the drift
+#pragma unroll
+        for (int j = 0; j < NA; j++) {
+            if (NA > 4 && j >= n) break;
             V3<Real> f = ld3<Real>(S.force, S.atomStride, a0 + j);
             v[j] = v[j] + f * invm[a0 + j] * halfDt;
-            ro[j] = r[j];
-            r[j] = r[j] + cvt<double>(v[j]) * dt;
-            ru[j] = r[j];
+            ro.v[j] = r.v[j];
+            r.v[j] = r.v[j] + cvt<double>(v[j]) * dt;
+            ru.v[j] = r.v[j];
         }
-        ok = shake<Real>(S, k0, k1, a0, r, ro, w) && ok;
+        ok = shake<NA>(S, C, r, ro, w) && ok;
         const double invDt = 1.0 / dt;
-        for (int j = 0; j < n; j++) {
-            v[j] = cvt<Real>(cvt<double>(v[j]) + (r[j] - ru[j]) * invDt);
-            store_pos<Real>(S, a0 + j, r[j]);
+#pragma unroll
+        for (int j = 0; j < NA; j++) {
+            if (NA > 4 && j >= n) break;
+            v[j] = cvt<Real>(cvt<double>(v[j]) + (r.v[j] - ru.v[j]) * invDt);
+            store_pos<Real>(S, a0 + j, r.v[j]);
         }
     }
-    for (int j = 0; j < n; j++) st3<Real>(S.vel, S.atomStride, a0 + j, v[j]);
+#pragma unroll
+    for (int j = 0; j < NA; j++) {
+        if (NA > 4 && j >= n) break;
+        st3<Real>(S.vel, S.atomStride, a0 + j, v[j]);
+    }
     if (!ok) atomicAdd(S.constraintFail, 1u);
 }
 
-template <class Real>
+// ---- clusters of NA = 2..4 atoms: every per-atom and per-constraint value
+// is a named register reached by a compile-time index (an array indexed by
+// the atom numbers of a constraint would go to private memory)
+template <int... I, class F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, I...>, F&& f) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+template <class T> struct Four {
+    T e0, e1, e2, e3;
+    template <int J> __device__ __forceinline__ T& at() {
+        if constexpr (J == 0) return e0;
+        else if constexpr (J == 1) return e1;
+        else if constexpr (J == 2) return e2;
+        else return e3;
+    }
+    // element i (0 <= i < NA) as a sum of the elements weighted by exact 0/1
+    // factors (selects would be turned back into an indexed load)
+    template <int NA> __device__ __forceinline__ T get(int i) const {
+        T o = e0 * (double)(i == 0);
+        if (NA > 1) o = o + e1 * (double)(i == 1);
+        if (NA > 2) o = o + e2 * (double)(i == 2);
+        if (NA > 3) o = o + e3 * (double)(i == 3);
+        return o;
+    }
+    template <int NA> __device__ __forceinline__ void add(int i, T d) {
+        e0 = e0 + d * (double)(i == 0);
+        if (NA > 1) e1 = e1 + d * (double)(i == 1);
+        if (NA > 2) e2 = e2 + d * (double)(i == 2);
+        if (NA > 3) e3 = e3 + d * (double)(i == 3);
+    }
+};
+
+// the constraints of a small cluster, at most NA (NA - 1) / 2
+template <int NA> struct SmallCons {
+    static constexpr int kMax = NA * (NA - 1) / 2;
+    int n;
+    int ci[6], cj[6];
+    double cd[6];
+};
+
+template <int NA>
+__device__ __forceinline__ bool shake_small(const DevSys& S, const SmallCons<NA>& C, Four<V3<double>>& r,
+                                            const Four<V3<double>>& ro, const Four<double>& w) {
+    const double lo = (1.0 - S.constraintTol) * (1.0 - S.constraintTol);
+    const double hi = (1.0 + S.constraintTol) * (1.0 + S.constraintTol);
+    for (int it = 0; it < CLUSTER_MAX_ITERATIONS; it++) {
+        bool done = true;
+        static_for<SmallCons<NA>::kMax>([&](auto K) {
+            constexpr int k = K;
+            if (k >= C.n) return;
+            const int i = C.ci[k], j = C.cj[k];
+            const double d2 = C.cd[k] * C.cd[k];
+            V3<double> s = r.template get<NA>(i) - r.template get<NA>(j);
+            double s2 = dot(s, s);
+            if (s2 >= lo * d2 && s2 <= hi * d2) return;
+            done = false;
+            const double wi = w.template get<NA>(i), wj = w.template get<NA>(j);
+            V3<double> b = ro.template get<NA>(i) - ro.template get<NA>(j);
+            double g = (d2 - s2) / (2.0 * (wi + wj) * dot(s, b));
+            r.template add<NA>(i, b * (g * wi));
+            r.template add<NA>(j, b * (-(g * wj)));
+        });
+        if (done) return true;
+    }
+    return false;
+}
+
+template <int NA>
+__device__ __forceinline__ bool rattle_small(const DevSys& S, const SmallCons<NA>& C, const Four<V3<double>>& r,
+                                             Four<V3<double>>& v, const Four<double>& w) {
+    const double tol2 = S.constraintTol * S.constraintTol;
+    for (int it = 0; it < CLUSTER_MAX_ITERATIONS; it++) {
+        bool done = true;
+        static_for<SmallCons<NA>::kMax>([&](auto K) {
+            constexpr int k = K;
+            if (k >= C.n) return;
+            const int i = C.ci[k], j = C.cj[k];
+            V3<double> s = r.template get<NA>(i) - r.template get<NA>(j);
+            V3<double> dv = v.template get<NA>(i) - v.template get<NA>(j);
+            double rv = dot(s, dv), s2 = dot(s, s);
+            if (rv * rv <= tol2 * s2 * dot(dv, dv)) return;
+            done = false;
+            const double wi = w.template get<NA>(i), wj = w.template get<NA>(j);
+            double g = -rv / ((wi + wj) * s2);
+            v.template add<NA>(i, s * (g * wi));
+            v.template add<NA>(j, s * (-(g * wj)));
+        });
+        if (done) return true;
+    }
+    return false;
+}
+
+// the same steps as cluster_phase, for a cluster of exactly NA atoms
+template <class Real, int NA>
+__device__ __forceinline__ void cluster_phase_small(const DevSys& S, int c, int phase, double (&ke)[2]) {
+    const int a0 = S.clusterStart[c];
+    const Real* invm = reinterpret_cast<const Real*>(S.freeInvM);
+    if (phase == CLUSTER_KE) {
+        static_for<NA>([&](auto J) {
+            const int a = a0 + (int)J;
+            V3<double> v = S.useStageV ? V3<double>{S.stageV[3 * (size_t)a], S.stageV[3 * (size_t)a + 1],
+                                                    S.stageV[3 * (size_t)a + 2]}
+                                       : cvt<double>(ld3<Real>(S.vel, S.atomStride, a));
+            ke[0] += dot(v, v) / (double)invm[a];
+        });
+        return;
+    }
+    SmallCons<NA> C;
+    const int k0 = S.consStart[c];
+    C.n = S.consStart[c + 1] - k0;
+    static_for<SmallCons<NA>::kMax>([&](auto K) {
+        constexpr int k = K;
+        if (k < C.n) {
+            C.ci[k] = S.consA[k0 + k] - a0;
+            C.cj[k] = S.consB[k0 + k] - a0;
+            C.cd[k] = S.consDist[k0 + k];
+        }
+    });
+    const double dt = S.dt;
+    const Real halfDt = (Real)(0.5 * dt);
+    Four<V3<Real>> v;
+    Four<V3<double>> r, vd;
+    Four<double> w;
+    static_for<NA>([&](auto J) {
+        constexpr int j = J;
+        v.template at<j>() = ld3<Real>(S.vel, S.atomStride, a0 + j);
+        r.template at<j>() = load_pos<Real>(S, a0 + j);
+        w.template at<j>() = 1.0 / S.mass[a0 + j];
+    });
+    bool ok = true;
+    if (phase == CLUSTER_PART2 || phase == CLUSTER_FUSED) {
+        static_for<NA>([&](auto J) {
+            constexpr int j = J;
+            V3<Real> f = ld3<Real>(S.force, S.atomStride, a0 + j);
+            vd.template at<j>() = cvt<double>(v.template at<j>() + f * invm[a0 + j] * halfDt);
+        });
+        ok = rattle_small<NA>(S, C, r, vd, w) && ok;
+        static_for<NA>([&](auto J) {
+            constexpr int j = J;
+            v.template at<j>() = cvt<Real>(vd.template at<j>());
+            if (phase == CLUSTER_PART2) {
+                V3<double> u = cvt<double>(v.template at<j>());
+                ke[0] += dot(u, u) / (double)invm[a0 + j];
+            }
+        });
+    }
+    if (phase == CLUSTER_PART1 || phase == CLUSTER_FUSED) {
+        Four<V3<double>> ro, ru;   // before the step, after the drift
+        static_for<NA>([&](auto J) {
+            constexpr int j = J;
+            V3<Real> f = ld3<Real>(S.force, S.atomStride, a0 + j);
+            v.template at<j>() = v.template at<j>() + f * invm[a0 + j] * halfDt;
+            ro.template at<j>() = r.template at<j>();
+            r.template at<j>() = r.template at<j>() + cvt<double>(v.template at<j>()) * dt;
+            ru.template at<j>() = r.template at<j>();
+        });
+        ok = shake_small<NA>(S, C, r, ro, w) && ok;
+        const double invDt = 1.0 / dt;
+        static_for<NA>([&](auto J) {
+            constexpr int j = J;
+            v.template at<j>() = cvt<Real>(cvt<double>(v.template at<j>()) + (r.template at<j>() - ru.template at<j>()) * invDt);
+            store_pos<Real>(S, a0 + j, r.template at<j>());
+        });
+    }
+    static_for<NA>([&](auto J) {
+        constexpr int j = J;
+        st3<Real>(S.vel, S.atomStride, a0 + j, v.template at<j>());
+    });
+    if (!ok) atomicAdd(S.constraintFail, 1u);
+}
+
+// tiles of clusters of NA = 2, 3 or 4 atoms (registers only), or of larger
+// clusters (NA = CLUSTER_MAX_ATOMS, private memory)
+template <class Real, int NA>
 __global__ __launch_bounds__(256) void k_clusters(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
                                                    int numTiles, int phase) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
@@ -805,7 +1076,8 @@ __global__ __launch_bounds__(256) void k_clusters(const DevSys* __restrict__ sys
         double ke[3] = {0.0, 0.0, 0.0};
         if ((int)threadIdx.x < tile.count) {
             double k2[2] = {0.0, 0.0};
-            cluster_phase<Real>(S, tile.first + threadIdx.x, phase, k2);
+            if constexpr (NA <= 4) cluster_phase_small<Real, NA>(S, tile.first + threadIdx.x, phase, k2);
+            else cluster_phase<Real, NA>(S, tile.first + threadIdx.x, phase, k2);
             ke[0] = k2[0];
         }
         // set-up partials carry a dof entry (the bodies'), zero here
@@ -945,12 +1217,26 @@ hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles,
     return hipGetLastError();
 }
 
-hipError_t launch_clusters(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int phase,
+template <class Real>
+static void launch_cluster_classes(const DevSys* systems, const Tile* tiles, const int* counts, int phase, int block,
+                                   hipStream_t stream) {
+    if (counts[0]) k_clusters<Real, 2><<<counts[0], block, 0, stream>>>(systems, tiles, counts[0], phase);
+    tiles += counts[0];
+    if (counts[1]) k_clusters<Real, 3><<<counts[1], block, 0, stream>>>(systems, tiles, counts[1], phase);
+    tiles += counts[1];
+    if (counts[2]) k_clusters<Real, 4><<<counts[2], block, 0, stream>>>(systems, tiles, counts[2], phase);
+    tiles += counts[2];
+    if (counts[3])
+        k_clusters<Real, CLUSTER_MAX_ATOMS><<<counts[3], block, 0, stream>>>(systems, tiles, counts[3], phase);
+}
+
+hipError_t launch_clusters(int precision, const DevSys* systems, const Tile* tiles, const int* counts, int phase,
                            int block, hipStream_t stream) {
-    if (numTiles <= 0) return hipSuccess;
     if (block > 256) return hipErrorInvalidValue;
-    if (precision == 0) k_clusters<float><<<numTiles, block, 0, stream>>>(systems, tiles, numTiles, phase);
-    else k_clusters<double><<<numTiles, block, 0, stream>>>(systems, tiles, numTiles, phase);
+    for (int i = 0; i < 4; i++)
+        if (counts[i] < 0) return hipErrorInvalidValue;
+    if (precision == 0) launch_cluster_classes<float>(systems, tiles, counts, phase, block, stream);
+    else launch_cluster_classes<double>(systems, tiles, counts, phase, block, stream);
     return hipGetLastError();
 }
 

@@ -21,23 +21,31 @@ TOL = {"mixed": dict(pos=1e-6, vel=1e-4), "double": dict(pos=1e-10, vel=1e-8)}
 
 
 def constrained_spec(mode, seed=5):
-    """Waters as rigid bodies plus 60 free atoms, 45 of them in clusters:
-    chains of 4, triangles, branched 5-atom clusters and pairs; the
-    constraint lengths are the starting distances."""
+    """Waters as rigid bodies plus 60 free atoms, 43 of them in clusters:
+    chains of 4, triangles, branched 5-atom clusters, pairs and a
+    tetrahedron; the constraint lengths are the starting distances."""
     spec = systems.water_box(27, 0.5, num_free=60, seed=seed, rotation_mode=mode)
     rng = np.random.default_rng(seed)
     spec.tether_r0 = spec.tether_r0 + 0.03 * rng.normal(size=spec.tether_r0.shape)
     free = np.flatnonzero(spec.body_indices == 0)
     x = spec.positions
     # pull each cluster's atoms close together so the bonds are short
+    # (the last: a tetrahedron with one edge given twice, 7 constraints on 4
+    # atoms, which takes the path for larger clusters)
     shapes = [[(0, 1), (1, 2), (2, 3)]] * 3 + [[(0, 1), (1, 2), (0, 2)]] * 3 + \
-             [[(0, 1), (0, 2), (0, 3), (2, 3), (3, 4)]] * 2 + [[(0, 1)]] * 4
+             [[(0, 1), (0, 2), (0, 3), (2, 3), (3, 4)]] * 2 + [[(0, 1)]] * 4 + \
+             [[(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (0, 1)]]
     cons, at = [], 0
     for sh in shapes:
         n = max(max(p) for p in sh) + 1
         atoms = free[at:at + n]
         at += n
-        x[atoms[1:]] = x[atoms[0]] + rng.normal(size=(n - 1, 3)) * 0.08
+        if len(sh) == 7:
+            # a regular tetrahedron (a flat one makes SHAKE converge slowly)
+            tet = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float) * 0.05
+            x[atoms] = x[atoms[0]] + tet - tet[0]
+        else:
+            x[atoms[1:]] = x[atoms[0]] + rng.normal(size=(n - 1, 3)) * 0.08
         for a, b in sh:
             cons.append((int(atoms[a]), int(atoms[b]), float(np.linalg.norm(x[atoms[a]] - x[atoms[b]]))))
     spec.positions = x
