@@ -74,7 +74,7 @@ int rbhip_device_synchronize(void);
  * labels one rigid body).  rotationMode: 0 exact, n > 0 NO-SQUISH with n
  * substeps (RigidBodyIntegrator.h:58-63).  Errors mirror the reference's
  * exceptions (negative mode, constraints on body atoms) plus this platform's
- * documented limits (massless atoms, single-atom bodies). */
+ * documented limits (single-atom bodies, massless particles in a body). */
 int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, double stepSize,
                  int rotationMode, int precision, int device, rbhip_context** out);
 /* rbhip_create for a System with distance constraints (System::addConstraint):
@@ -91,6 +91,20 @@ int rbhip_create(int numAtoms, const double* masses, const int* bodyIndices, dou
 int rbhip_create_constrained(int numAtoms, const double* masses, const int* bodyIndices, int numConstraints,
                              const int* constraintAtoms, const double* constraintDistances, double stepSize,
                              int rotationMode, int precision, int device, rbhip_context** out);
+/* OpenMM virtual sites on massless particles (rbhip_create accepts massless
+   particles labelled free: like the reference, RigidBodySystem.cpp:66-76, it
+   integrates neither them nor sites).  Each site is recomputed from its
+   parents after part 1 and the constraints, as ReferenceRigidBodyKernels.cpp
+   :100 does, and after every force evaluation its force moves to its parents
+   (the transpose of d r_site / d r_parent) and is cleared.  Per site: a type,
+   the site particle, 3 parents (the third unused for two-particle averages)
+   and 3 weights: {w1, w2, -} two-particle average, {w1, w2, w3} three-particle
+   average, {w12, w13, wCross} out of plane (r = r1 + w12 r12 + w13 r13 +
+   wCross r12 x r13).  Parents must be massive.  Set once, before stepping;
+   contexts with sites cannot join a batch or compute refined energies. */
+enum { RBHIP_SITE_TWO_AVERAGE = 1, RBHIP_SITE_THREE_AVERAGE = 2, RBHIP_SITE_OUT_OF_PLANE = 3 };
+int rbhip_set_virtual_sites(rbhip_context* ctx, int count, const int* types, const int* siteAtoms,
+                            const int* parents, const double* weights);
 /* Integrator::setConstraintTolerance (default 1e-5): positions converge when
    every constrained distance is within (1 +- tolerance) of its length,
    velocities when every bond's relative velocity is within tolerance of

@@ -375,6 +375,9 @@ struct rbhip_context {
     // distance constraints between free atoms: the atoms of each cluster are
     // contiguous and follow the numFree - numClustered unconstrained ones
     int numConstraints = 0, numClusters = 0, numClustered = 0;
+    // massless particles, after the body atoms; virtual sites among them
+    int numMassless = 0, numSites = 0, numSiteParents = 0;
+    DeviceArena siteArena;
     std::vector<int> clusterN;                   // atoms per cluster (small classes first)
     std::vector<int> clusterK;                   // constraints per cluster
     std::vector<int> bodyIndex, atomIndex;
@@ -644,14 +647,19 @@ struct rbhip_context {
 
     void launchForces() {
         HIPCHK(launch_forces(precision, dSys, dForceTiles, nForceTiles, block, maxBlocks, stream));
+        if (numSites > 0) HIPCHK(launch_site_forces(precision, dSys, numSites, numSiteParents, stream));
     }
     void launchClusters(int phase) {
         if (nClusterTiles > 0)
             HIPCHK(launch_clusters(precision, dSys, dClusterTiles, clusterClassTiles, phase, kClusterBlock, stream));
     }
+    void launchSites() {
+        if (numSites > 0) HIPCHK(launch_site_positions(precision, dSys, numSites, stream));
+    }
     void launchPart1() {
         HIPCHK(launch_part1(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
         launchClusters(CLUSTER_PART1);
+        launchSites();
     }
     void launchPart2() {
         HIPCHK(launch_part2(precision, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
@@ -660,6 +668,7 @@ struct rbhip_context {
     void launchFused() {
         HIPCHK(launch_fused(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
         launchClusters(CLUSTER_FUSED);
+        launchSites();
     }
     int partKeSlots() const { return nPartTiles + nClusterTiles; }
     void launchRefineBegin(bool folded) {
@@ -756,6 +765,8 @@ struct rbhip_context {
     void setRefined(bool on) {
         if (on && numConstraints > 0)
             throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with constraints");
+        if (on && numSites > 0)
+            throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with virtual sites");
         if (on && !refBuf[0]) {
             const size_t R = realSize();
             refArena.stream = stream;
@@ -779,6 +790,7 @@ struct rbhip_context {
         tileArena.release();
         refArena.release();
         clusterArena.release();
+        siteArena.release();
         arena.release();
         if (handoff) (void)hipEventDestroy(handoff);
     }
@@ -957,9 +969,8 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
         require(precision == RBHIP_PRECISION_MIXED || precision == RBHIP_PRECISION_DOUBLE, "unknown precision");
         require(stepSize > 0.0, "step size must be positive");
         for (int i = 0; i < numAtoms; i++)
-            if (!(masses[i] > 0.0))
-                throw RbError(RBHIP_E_UNSUPPORTED, "massless or negative-mass particles are not supported (particle " +
-                                                       std::to_string(i) + ")");
+            if (!(masses[i] >= 0.0))
+                throw RbError(RBHIP_E_ARG, "negative or invalid mass (particle " + std::to_string(i) + ")");
         int ndev = 0;
         HIPCHK(hipGetDeviceCount(&ndev));
         require(device >= 0 && device < ndev, "no such device");
@@ -978,11 +989,23 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
         for (int v : C.bodyIndex) C.numBodies = std::max(C.numBodies, v);
         C.refBodyN.assign(C.numBodies, 0);
         C.atomIndex.assign(numAtoms, 0);
+        // massless particles (virtual sites among them) are neither free nor
+        // body atoms (RigidBodySystem.cpp:66-76); they must be labelled free,
+        // because the reference's body-atom loop (:99-104) does not skip them
+        std::vector<int> massless;
         for (int i = 0; i < numAtoms; i++) {
+            if (masses[i] == 0.0) {
+                if (C.bodyIndex[i] != 0)
+                    throw RbError(RBHIP_E_ARG, "massless particles cannot belong to rigid bodies (particle " +
+                                                   std::to_string(i) + ")");
+                massless.push_back(i);
+                continue;
+            }
             if (C.bodyIndex[i] == 0) C.atomIndex[C.numFree++] = i;
             else C.refBodyN[C.bodyIndex[i] - 1]++;
         }
-        C.numBodyAtoms = numAtoms - C.numFree;
+        C.numMassless = (int)massless.size();
+        C.numBodyAtoms = numAtoms - C.numFree - C.numMassless;
         C.refBodyLoc.assign(C.numBodies, 0);
         int loc = 0;
         for (int b = 0; b < C.numBodies; b++) {
@@ -1020,6 +1043,8 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
                 require(d > 0.0 && std::isfinite(d), "constraint " + std::to_string(k) + ": bad distance");
                 if (C.bodyIndex[a] != 0 || C.bodyIndex[b] != 0)
                     throw RbError(RBHIP_E_ARG, "Constraints involving rigid-body atoms are not allowed");
+                if (masses[a] == 0.0 || masses[b] == 0.0)
+                    throw RbError(RBHIP_E_ARG, "A constraint cannot involve a massless particle");
                 root[find(a)] = find(b);
             }
             std::vector<int> id(numAtoms, -1);
@@ -1092,6 +1117,7 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
             C.classes.back().count++;
             a += N;
         }
+        for (int i : massless) C.atomOrig[a++] = i;
 
         // lanes per body: one lane per body up to 7 atoms, then a power of
         // two that leaves each lane 4-7 atoms (shorter serial loops)
@@ -1246,6 +1272,82 @@ int rbhip_create_constrained(int numAtoms, const double* masses, const int* body
                           stepSize, rotationMode, precision, device, out);
 }
 
+int rbhip_set_virtual_sites(rbhip_context* ctx, int count, const int* types, const int* siteAtoms,
+                            const int* parents, const double* weights) {
+    return guard([&] {
+        require(ctx && count >= 0 && (count == 0 || (types && siteAtoms && parents && weights)), "bad arguments");
+        rbhip_context& C = *ctx;
+        require(C.numSites == 0, "virtual sites are already set", RBHIP_E_STATE);
+        require(!C.refined, "refined energies are not supported with virtual sites", RBHIP_E_UNSUPPORTED);
+        if (count == 0) return;
+        std::vector<int> internalOf(C.numAtoms);
+        for (int k = 0; k < C.numAtoms; k++) internalOf[C.atomOrig[k]] = k;
+        const int firstMassless = C.numAtoms - C.numMassless;
+        std::vector<char> isSite(C.numAtoms, 0);
+        std::vector<int> sa(count), st(count), sp(3 * (size_t)count, 0);
+        std::vector<double> sw(3 * (size_t)count, 0.0);
+        for (int v = 0; v < count; v++) {
+            const int t = types[v], site = siteAtoms[v];
+            require(t == RBHIP_SITE_TWO_AVERAGE || t == RBHIP_SITE_THREE_AVERAGE || t == RBHIP_SITE_OUT_OF_PLANE,
+                    "unknown virtual site type");
+            require(site >= 0 && site < C.numAtoms && internalOf[site] >= firstMassless,
+                    "a virtual site must be a massless particle");
+            require(!isSite[site], "particle " + std::to_string(site) + " is given two virtual sites");
+            isSite[site] = 1;
+            const int np = t == RBHIP_SITE_TWO_AVERAGE ? 2 : 3;
+            for (int k = 0; k < np; k++) {
+                int pa = parents[3 * v + k];
+                require(pa >= 0 && pa < C.numAtoms && pa != site && internalOf[pa] < firstMassless,
+                        "a virtual site's parents must be massive particles");
+                sp[3 * v + k] = internalOf[pa];
+            }
+            for (int k = 0; k < 3; k++) sw[3 * v + k] = np == 2 && k == 2 ? 0.0 : weights[3 * v + k];
+            sa[v] = internalOf[site];
+            st[v] = t;
+        }
+        // each parent's (site, role) entries in site order
+        std::map<int, std::vector<std::pair<int, int>>> byParent;
+        for (int v = 0; v < count; v++)
+            for (int k = 0; k < (st[v] == RBHIP_SITE_TWO_AVERAGE ? 2 : 3); k++) byParent[sp[3 * v + k]].push_back({v, k});
+        std::vector<int> pa, ps(1, 0), pv, pr;
+        for (auto& kv : byParent) {
+            pa.push_back(kv.first);
+            for (auto& e : kv.second) {
+                pv.push_back(e.first);
+                pr.push_back(e.second);
+            }
+            ps.push_back((int)pv.size());
+        }
+        C.setDevice();
+        hipStream_t s = C.stream;
+        C.siteArena.stream = s;
+        DevSys& h = C.h;
+        h.siteAtom = C.siteArena.allocT<int>(sa.size());
+        h.siteType = C.siteArena.allocT<int>(st.size());
+        h.siteParent = C.siteArena.allocT<int>(sp.size());
+        h.siteW = C.siteArena.allocT<double>(sw.size());
+        h.parentAtom = C.siteArena.allocT<int>(pa.size());
+        h.parentStart = C.siteArena.allocT<int>(ps.size());
+        h.parentSite = C.siteArena.allocT<int>(pv.size());
+        h.parentRole = C.siteArena.allocT<int>(pr.size());
+        HIPCHK(hipMemcpyAsync(h.siteAtom, sa.data(), sa.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.siteType, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.siteParent, sp.data(), sp.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.siteW, sw.data(), sw.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.parentAtom, pa.data(), pa.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.parentStart, ps.data(), ps.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.parentSite, pv.data(), pv.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.parentRole, pr.data(), pr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        h.numSites = count;
+        h.numSiteParents = (int)pa.size();
+        C.numSites = count;
+        C.numSiteParents = (int)pa.size();
+        C.uploadSys();
+        C.graphs.reset();   // the step sequence gains the site kernels
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
 int rbhip_set_constraint_tolerance(rbhip_context* ctx, double tolerance) {
     return guard([&] {
         require(ctx, "null context");
@@ -1297,14 +1399,16 @@ int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream) {
             HIPCHK(hipStreamWaitEvent(next, ctx->handoff, 0));
         }
         ctx->stream = next;
-        ctx->arena.stream = ctx->tileArena.stream = ctx->refArena.stream = ctx->clusterArena.stream = ctx->stream;
+        ctx->arena.stream = ctx->tileArena.stream = ctx->refArena.stream = ctx->clusterArena.stream =
+            ctx->siteArena.stream = ctx->stream;
     });
 }
 
 int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes) {
     return guard([&] {
         require(ctx && bytes, "null argument");
-        *bytes = ctx->arena.bytes + ctx->tileArena.bytes + ctx->refArena.bytes + ctx->clusterArena.bytes;
+        *bytes = ctx->arena.bytes + ctx->tileArena.bytes + ctx->refArena.bytes + ctx->clusterArena.bytes +
+                 ctx->siteArena.bytes;
     });
 }
 
@@ -1356,10 +1460,14 @@ int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, si
 int rbhip_get_counts(rbhip_context* ctx, int* out5) {
     return guard([&] {
         require(ctx && out5, "null argument");
+        // RigidBodySystem.cpp:64-79: virtual sites are not actual atoms, and
+        // numBodyAtoms = numActualAtoms - numFree counts massless non-site
+        // particles too
+        const int actual = ctx->numAtoms - ctx->numSites;
         out5[0] = ctx->numFree;
         out5[1] = ctx->numBodies;
-        out5[2] = ctx->numAtoms;
-        out5[3] = ctx->numBodyAtoms;
+        out5[2] = actual;
+        out5[3] = actual - ctx->numFree;
         out5[4] = ctx->numDOF;
     });
 }
@@ -1367,7 +1475,7 @@ int rbhip_get_counts(rbhip_context* ctx, int* out5) {
 int rbhip_get_atom_index(rbhip_context* ctx, int* out) {
     return guard([&] {
         require(ctx && out, "null argument");
-        std::copy(ctx->atomIndex.begin(), ctx->atomIndex.end(), out);
+        std::copy(ctx->atomIndex.begin(), ctx->atomIndex.begin() + (ctx->numAtoms - ctx->numSites), out);
     });
 }
 
@@ -1922,6 +2030,7 @@ int rbhip_batch_set_members(rbhip_batch* batch, int count, rbhip_context* const*
             require(m[i]->built, "member body state not built", RBHIP_E_STATE);
             require(m[i]->precision == m[0]->precision, "members must share the precision");
             require(m[i]->numConstraints == 0, "members with constraints cannot be batched", RBHIP_E_UNSUPPORTED);
+            require(m[i]->numSites == 0, "members with virtual sites cannot be batched", RBHIP_E_UNSUPPORTED);
             require((m[i]->rotationMode == 0) == (m[0]->rotationMode == 0), "members must share the rotation kind");
             for (auto& kv : m[i]->lanes)
                 require(batch->block % kv.second == 0, "member lanes per body must divide the batch block size");

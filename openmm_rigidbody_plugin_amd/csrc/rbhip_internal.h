@@ -31,6 +31,9 @@
 
 namespace rbhip {
 
+// virtual-site kinds (the values of include/rbhip.h's RBHIP_SITE_*)
+enum SiteKind : int { SITE_TWO_AVERAGE = 1, SITE_THREE_AVERAGE = 2, SITE_OUT_OF_PLANE = 3 };
+
 enum TileKind : int { TILE_FREE = 0, TILE_BODIES = 1, TILE_ATOMS = 2, TILE_CLUSTERS = 3 };
 
 // free-atom tiles cover this many atoms per workgroup thread (the integrator
@@ -110,6 +113,21 @@ struct DevSys {
     double* consDist;
     double constraintTol;
     unsigned* constraintFail;   // count of cluster iterations that did not converge
+    // virtual sites on massless particles (numSites = 0 without).  Massless
+    // particles follow the body atoms in internal order and are never
+    // integrated.  Each parent atom of a site gathers the force of its sites
+    // in a fixed order: parentAtom[g], and the (site, role) entries
+    // [parentStart[g], parentStart[g + 1]) of parentSite / parentRole.
+    int numSites;
+    int numSiteParents;
+    int* siteAtom;      // internal index of each site
+    int* siteType;      // RBHIP_SITE_*
+    int* siteParent;    // 3 internal parent indices per site
+    double* siteW;      // 3 weights per site
+    int* parentAtom;
+    int* parentStart;
+    int* parentSite;
+    int* parentRole;    // 0, 1, 2: which parent of the site
 };
 
 }  // namespace rbhip

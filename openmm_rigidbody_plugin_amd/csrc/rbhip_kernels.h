@@ -27,6 +27,10 @@ hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* ti
                             int maxBlocks, hipStream_t stream);
 hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
                          int maxBlocks, hipStream_t stream);
+// virtual sites of one system: positions from the parents; after the forces,
+// the sites' forces moved to their parents and cleared
+hipError_t launch_site_positions(int precision, const DevSys* sys, int numSites, hipStream_t stream);
+hipError_t launch_site_forces(int precision, const DevSys* sys, int numSites, int numParents, hipStream_t stream);
 // clusters of constrained free atoms (TILE_CLUSTERS tiles, one lane per
 // cluster, block <= 256): part 1 (kick, drift, SHAKE), part 2 (kick, RATTLE,
 // KE partials), both at a fused boundary, or the KE partials after the set-up

@@ -1089,15 +1089,81 @@ __global__ __launch_bounds__(256) void k_clusters(const DevSys* __restrict__ sys
     }
 }
 
+// positions as the force field reads them: body atoms (and the massless
+// particles after them) at their fp32 positions in mixed precision, free
+// atoms at hi + lo
+template <class Real> __device__ __forceinline__ V3<double> force_pos(const DevSys& S, int i) {
+    if (sizeof(Real) == 4 && i >= S.numFree) return cvt<double>(ld3<Real>(S.pos, S.atomStride, i));
+    return load_pos<Real>(S, i);
+}
+
+// ---------------------------------------------------------------------------
+// Virtual sites (OpenMM's, placed after part 1 and the constraints,
+// ReferenceRigidBodyKernels.cpp:100): two- and three-particle averages and
+// out-of-plane sites from their parents' positions; after the forces, each
+// parent gathers the force of its sites (the transpose of d r_site /
+// d r_parent) in a fixed order, and the sites' forces are cleared.  One lane
+// per site or per parent.
+// ---------------------------------------------------------------------------
+template <class Real>
+__global__ __launch_bounds__(256) void k_site_positions(const DevSys* __restrict__ sys) {
+    const DevSys& S = *sys;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S.numSites) return;
+    const int* p = S.siteParent + 3 * (size_t)s;
+    const double* w = S.siteW + 3 * (size_t)s;
+    V3<double> r1 = force_pos<Real>(S, p[0]), r2 = force_pos<Real>(S, p[1]), x;
+    const int type = S.siteType[s];
+    if (type == SITE_TWO_AVERAGE) {
+        x = r1 * w[0] + r2 * w[1];
+    } else if (type == SITE_THREE_AVERAGE) {
+        x = r1 * w[0] + r2 * w[1] + force_pos<Real>(S, p[2]) * w[2];
+    } else {
+        V3<double> r12 = r2 - r1, r13 = force_pos<Real>(S, p[2]) - r1;
+        x = r1 + r12 * w[0] + r13 * w[1] + cross(r12, r13) * w[2];
+    }
+    store_pos<Real>(S, S.siteAtom[s], x);
+}
+
+template <class Real>
+__global__ __launch_bounds__(256) void k_site_forces(const DevSys* __restrict__ sys) {
+    const DevSys& S = *sys;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S.numSiteParents) return;
+    const int a = S.parentAtom[g];
+    V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, a));
+    for (int e = S.parentStart[g]; e < S.parentStart[g + 1]; e++) {
+        const int s = S.parentSite[e], role = S.parentRole[e];
+        const int* p = S.siteParent + 3 * (size_t)s;
+        const double* w = S.siteW + 3 * (size_t)s;
+        V3<double> fs = cvt<double>(ld3<Real>(S.force, S.atomStride, S.siteAtom[s]));
+        const int type = S.siteType[s];
+        if (type != SITE_OUT_OF_PLANE) {
+            f = f + fs * w[role];
+        } else {
+            V3<double> r1 = force_pos<Real>(S, p[0]);
+            V3<double> r12 = force_pos<Real>(S, p[1]) - r1, r13 = force_pos<Real>(S, p[2]) - r1;
+            V3<double> f2 = fs * w[0] + cross(r13, fs) * w[2];
+            V3<double> f3 = fs * w[1] + cross(fs, r12) * w[2];
+            f = f + (role == 0 ? fs - f2 - f3 : role == 1 ? f2 : f3);
+        }
+    }
+    st3<Real>(S.force, S.atomStride, a, cvt<Real>(f));
+}
+
+template <class Real>
+__global__ __launch_bounds__(256) void k_site_clear(const DevSys* __restrict__ sys) {
+    const DevSys& S = *sys;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S.numSites) return;
+    st3<Real>(S.force, S.atomStride, S.siteAtom[s], V3<Real>{(Real)0, (Real)0, (Real)0});
+}
+
 // ---------------------------------------------------------------------------
 // force field: harmonic tether + harmonic bonds, one lane per atom.  Body
 // atoms are read at their fp32 positions (as a mixed-precision force field
 // reads fp32 coordinates), free atoms at hi + lo.
 // ---------------------------------------------------------------------------
-template <class Real> __device__ __forceinline__ V3<double> force_pos(const DevSys& S, int i) {
-    if (sizeof(Real) == 4 && i >= S.numFree) return cvt<double>(ld3<Real>(S.pos, S.atomStride, i));
-    return load_pos<Real>(S, i);
-}
 
 template <class Real>
 __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
@@ -1237,6 +1303,27 @@ hipError_t launch_clusters(int precision, const DevSys* systems, const Tile* til
         if (counts[i] < 0) return hipErrorInvalidValue;
     if (precision == 0) launch_cluster_classes<float>(systems, tiles, counts, phase, block, stream);
     else launch_cluster_classes<double>(systems, tiles, counts, phase, block, stream);
+    return hipGetLastError();
+}
+
+hipError_t launch_site_positions(int precision, const DevSys* sys, int numSites, hipStream_t stream) {
+    if (numSites <= 0) return hipSuccess;
+    const int grid = (numSites + 255) / 256;
+    if (precision == 0) k_site_positions<float><<<grid, 256, 0, stream>>>(sys);
+    else k_site_positions<double><<<grid, 256, 0, stream>>>(sys);
+    return hipGetLastError();
+}
+
+hipError_t launch_site_forces(int precision, const DevSys* sys, int numSites, int numParents, hipStream_t stream) {
+    if (numSites <= 0) return hipSuccess;
+    const int gp = (numParents + 255) / 256, gs = (numSites + 255) / 256;
+    if (precision == 0) {
+        if (gp) k_site_forces<float><<<gp, 256, 0, stream>>>(sys);
+        k_site_clear<float><<<gs, 256, 0, stream>>>(sys);
+    } else {
+        if (gp) k_site_forces<double><<<gp, 256, 0, stream>>>(sys);
+        k_site_clear<double><<<gs, 256, 0, stream>>>(sys);
+    }
     return hipGetLastError();
 }
 

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 
+#include "rbhip.h"
+
 namespace rbhost {
 
 namespace {
@@ -48,9 +50,71 @@ void TetherForce::getParticleParameters(int index, int& particle, double& k, Vec
 }
 
 // ---------------------------------------------------------------------------
+int VirtualSite::getParticle(int index) const {
+    check_index(index, (int)particles.size(), "Virtual site particle");
+    return particles[index];
+}
+
+TwoParticleAverageSite::TwoParticleAverageSite(int particle1, int particle2, double weight1, double weight2)
+    : w1(weight1), w2(weight2) {
+    particles = {particle1, particle2};
+}
+double TwoParticleAverageSite::getWeight(int particle) const {
+    check_index(particle, 2, "Virtual site particle");
+    return particle == 0 ? w1 : w2;
+}
+int TwoParticleAverageSite::type() const { return RBHIP_SITE_TWO_AVERAGE; }
+void TwoParticleAverageSite::weights(double w[3]) const {
+    w[0] = w1;
+    w[1] = w2;
+    w[2] = 0.0;
+}
+
+ThreeParticleAverageSite::ThreeParticleAverageSite(int particle1, int particle2, int particle3, double weight1,
+                                                   double weight2, double weight3)
+    : w{weight1, weight2, weight3} {
+    particles = {particle1, particle2, particle3};
+}
+double ThreeParticleAverageSite::getWeight(int particle) const {
+    check_index(particle, 3, "Virtual site particle");
+    return w[particle];
+}
+int ThreeParticleAverageSite::type() const { return RBHIP_SITE_THREE_AVERAGE; }
+void ThreeParticleAverageSite::weights(double out[3]) const {
+    for (int k = 0; k < 3; k++) out[k] = w[k];
+}
+
+OutOfPlaneSite::OutOfPlaneSite(int particle1, int particle2, int particle3, double weight12, double weight13,
+                               double weightCross)
+    : w12(weight12), w13(weight13), wc(weightCross) {
+    particles = {particle1, particle2, particle3};
+}
+int OutOfPlaneSite::type() const { return RBHIP_SITE_OUT_OF_PLANE; }
+void OutOfPlaneSite::weights(double w[3]) const {
+    w[0] = w12;
+    w[1] = w13;
+    w[2] = wc;
+}
+
 int System::addParticle(double mass) {
     masses.push_back(mass);
+    sites.emplace_back();
     return (int)masses.size() - 1;
+}
+
+void System::setVirtualSite(int index, VirtualSite* site) {
+    check_index(index, (int)masses.size(), "Particle");
+    sites[index].reset(site);
+}
+
+bool System::isVirtualSite(int index) const {
+    check_index(index, (int)masses.size(), "Particle");
+    return sites[index] != nullptr;
+}
+
+const VirtualSite& System::getVirtualSite(int index) const {
+    if (!isVirtualSite(index)) throw Exception("This particle is not a virtual site");
+    return *sites[index];
 }
 
 double System::getParticleMass(int index) const {

@@ -73,6 +73,57 @@ private:
 // ---------------------------------------------------------------------------
 // System
 // ---------------------------------------------------------------------------
+// OpenMM's virtual sites of the three kinds this platform computes; a site
+// particle must be massless.  type() is the RBHIP_SITE_* value.
+class VirtualSite {
+public:
+    virtual ~VirtualSite() = default;
+    int getNumParticles() const { return (int)particles.size(); }
+    int getParticle(int index) const;
+    virtual int type() const = 0;
+    // {w1, w2, 0}, {w1, w2, w3} or {w12, w13, wCross}
+    virtual void weights(double w[3]) const = 0;
+
+protected:
+    std::vector<int> particles;
+};
+
+class TwoParticleAverageSite : public VirtualSite {
+public:
+    TwoParticleAverageSite(int particle1, int particle2, double weight1, double weight2);
+    double getWeight(int particle) const;
+    int type() const override;
+    void weights(double w[3]) const override;
+
+private:
+    double w1, w2;
+};
+
+class ThreeParticleAverageSite : public VirtualSite {
+public:
+    ThreeParticleAverageSite(int particle1, int particle2, int particle3, double weight1, double weight2,
+                             double weight3);
+    double getWeight(int particle) const;
+    int type() const override;
+    void weights(double w[3]) const override;
+
+private:
+    double w[3];
+};
+
+class OutOfPlaneSite : public VirtualSite {
+public:
+    OutOfPlaneSite(int particle1, int particle2, int particle3, double weight12, double weight13, double weightCross);
+    double getWeight12() const { return w12; }
+    double getWeight13() const { return w13; }
+    double getWeightCross() const { return wc; }
+    int type() const override;
+    void weights(double w[3]) const override;
+
+private:
+    double w12, w13, wc;
+};
+
 class System {
 public:
     System() = default;
@@ -82,8 +133,10 @@ public:
     int getNumParticles() const { return (int)masses.size(); }
     double getParticleMass(int index) const;
     void setParticleMass(int index, double mass);
-    // virtual sites are not supported by this host: every particle is real
-    bool isVirtualSite(int) const { return false; }
+    // takes ownership; the particle should be massless
+    void setVirtualSite(int index, VirtualSite* site);
+    bool isVirtualSite(int index) const;
+    const VirtualSite& getVirtualSite(int index) const;
     int addConstraint(int particle1, int particle2, double distance);
     int getNumConstraints() const { return (int)c1.size(); }
     void getConstraintParameters(int index, int& particle1, int& particle2, double& distance) const;
@@ -95,6 +148,7 @@ public:
 
 private:
     std::vector<double> masses;
+    std::vector<std::unique_ptr<VirtualSite>> sites;   // per particle, or null
     std::vector<int> c1, c2;
     std::vector<double> cd;
     std::vector<std::unique_ptr<Force>> forces;

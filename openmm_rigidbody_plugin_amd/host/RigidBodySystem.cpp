@@ -30,11 +30,16 @@ void RigidBodySystem::initialize(rbhost::ContextImpl& context, const std::vector
     rotationMode = mode;
     numBodies = 0;
     for (int b : bodyIndex) numBodies = std::max(numBodies, b);
-    numActualAtoms = numAtoms;   // no virtual sites on this host
+    // RigidBodySystem.cpp:64-79: virtual sites are not actual atoms, and
+    // neither they nor massless particles are free or body atoms
+    numActualAtoms = numAtoms;
+    for (int i = 0; i < numAtoms; i++)
+        if (system.isVirtualSite(i)) numActualAtoms--;
     atomIndex.assign(numActualAtoms, 0);
     bodyN.assign(numBodies, 0);
     numFree = 0;
     for (int i = 0; i < numAtoms; i++) {
+        if (system.isVirtualSite(i) || system.getParticleMass(i) == 0.0) continue;
         if (bodyIndex[i] == 0) atomIndex[numFree++] = i;
         else bodyN[bodyIndex[i] - 1]++;
     }

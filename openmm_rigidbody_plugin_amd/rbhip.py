@@ -51,6 +51,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_create": (_I, [_I, _DP, _IP, _D, _I, _I, _I, ctypes.POINTER(_P)]),
             "rbhip_create_constrained": (_I, [_I, _DP, _IP, _I, _IP, _DP, _D, _I, _I, _I, ctypes.POINTER(_P)]),
             "rbhip_set_constraint_tolerance": (_I, [_P, _D]),
+            "rbhip_set_virtual_sites": (_I, [_P, _I, _IP, _IP, _IP, _DP]),
             "rbhip_constraint_failures": (_I, [_P, _LLP]),
             "rbhip_destroy": (_I, [_P]),
             "rbhip_set_stream": (_I, [_P, _P]),
@@ -208,6 +209,23 @@ class Context:
     @property
     def handle(self):
         return self._h
+
+    # ---- virtual sites ----
+    SITE_TWO_AVERAGE, SITE_THREE_AVERAGE, SITE_OUT_OF_PLANE = 1, 2, 3
+
+    def set_virtual_sites(self, sites):
+        """sites: (kind, site, parents, weights) tuples; kind SITE_TWO_AVERAGE
+        (2 parents, weights w1 w2), SITE_THREE_AVERAGE (3, w1 w2 w3) or
+        SITE_OUT_OF_PLANE (3, w12 w13 wCross)."""
+        n = len(sites)
+        t = np.ascontiguousarray([int(k) for k, _, _, _ in sites], dtype=np.int32)
+        a = np.ascontiguousarray([int(x) for _, x, _, _ in sites], dtype=np.int32)
+        p = np.zeros((n, 3), dtype=np.int32)
+        w = np.zeros((n, 3))
+        for i, (_, _, par, wt) in enumerate(sites):
+            p[i, :len(par)] = par
+            w[i, :len(wt)] = wt
+        _check(lib().rbhip_set_virtual_sites(self._h, n, _ip(t), _ip(a), _ip(p), _dp(w)))
 
     # ---- constraints ----
     def set_constraint_tolerance(self, tol):

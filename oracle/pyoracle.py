@@ -21,6 +21,9 @@ FIX_STALE_SYSTEM = 8
 FIX_ORTHONORMAL = 16
 FIX_REFINED_FREE = 32
 FIX_ALL = 63
+SITE_TWO_AVERAGE = 1
+SITE_THREE_AVERAGE = 2
+SITE_OUT_OF_PLANE = 3
 
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -44,6 +47,7 @@ def lib():
             "rbref_set_tether": (_I, [_P, _DP, _DP]),
             "rbref_add_bond": (_I, [_P, _I, _I, _D, _D]),
             "rbref_add_constraint": (_I, [_P, _I, _I, _D]),
+            "rbref_add_virtual_site": (_I, [_P, _I, _I, _IP, _DP]),
             "rbref_set_constraint_tolerance": (_I, [_P, _D]),
             "rbref_constraint_failures": (ctypes.c_long, [_P]),
             "rbref_set_positions": (_I, [_P, _DP]),
@@ -145,6 +149,14 @@ class OracleSim:
 
     def add_constraint(self, a, b, distance):
         _check(lib().rbref_add_constraint(self._h, int(a), int(b), float(distance)))
+
+    def add_virtual_site(self, kind, site, parents, weights):
+        """kind: SITE_TWO_AVERAGE, SITE_THREE_AVERAGE or SITE_OUT_OF_PLANE."""
+        p = np.zeros(3, dtype=np.int32)
+        p[:len(parents)] = parents
+        w = np.zeros(3)
+        w[:len(weights)] = weights
+        _check(lib().rbref_add_virtual_site(self._h, int(kind), int(site), _ip(p), _dp(w)))
 
     def set_constraint_tolerance(self, tol):
         _check(lib().rbref_set_constraint_tolerance(self._h, float(tol)))

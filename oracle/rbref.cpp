@@ -652,6 +652,8 @@ std::vector<int> cleanBodyIndices(const std::vector<int>& in) {
 /* ------------------------------------------------------------------------ */
 struct Bond { int a, b; double length, k; };
 
+struct VirtualSite { int type; int site; int p[3]; double w[3]; };
+
 struct SystemState {
     int n = 0;
     std::vector<double> mass;
@@ -659,6 +661,7 @@ struct SystemState {
     std::vector<double> tetherK;
     std::vector<Vec3> tetherR0;
     std::vector<Bond> bonds;
+    std::vector<VirtualSite> sites;
     double pe = 0.0;
     double time = 0.0;
     long stepCount = 0;
@@ -681,6 +684,49 @@ struct SystemState {
             Vec3 f = dr * (dEdR / r);
             F[b.a] += f;
             F[b.b] -= f;
+        }
+        distributeSiteForces();
+    }
+
+    /* OpenMM's virtual sites (ReferenceVirtualSites, called at
+       ReferenceRigidBodyKernels.cpp:100), written from their published
+       definitions: TwoParticleAverage r = w1 r1 + w2 r2, ThreeParticleAverage
+       r = w1 r1 + w2 r2 + w3 r3, OutOfPlane r = r1 + w12 r12 + w13 r13 +
+       wc r12 x r13 (r1j = rj - r1) */
+    void computeSitePositions() {
+        for (const VirtualSite& v : sites) {
+            const Vec3 &r1 = R[v.p[0]], &r2 = R[v.p[1]];
+            if (v.type == RBREF_SITE_TWO_AVERAGE) {
+                R[v.site] = r1 * v.w[0] + r2 * v.w[1];
+            } else if (v.type == RBREF_SITE_THREE_AVERAGE) {
+                R[v.site] = r1 * v.w[0] + r2 * v.w[1] + R[v.p[2]] * v.w[2];
+            } else {
+                Vec3 r12 = r2 - r1, r13 = R[v.p[2]] - r1;
+                R[v.site] = r1 + r12 * v.w[0] + r13 * v.w[1] + r12.cross(r13) * v.w[2];
+            }
+        }
+    }
+    /* the force on a site moved to its parents by the chain rule (the
+       transpose of d r_site / d r_parent), the site's own force cleared */
+    void distributeSiteForces() {
+        for (const VirtualSite& v : sites) {
+            Vec3 f = F[v.site];
+            F[v.site] = Vec3();
+            if (v.type == RBREF_SITE_TWO_AVERAGE) {
+                F[v.p[0]] += f * v.w[0];
+                F[v.p[1]] += f * v.w[1];
+            } else if (v.type == RBREF_SITE_THREE_AVERAGE) {
+                F[v.p[0]] += f * v.w[0];
+                F[v.p[1]] += f * v.w[1];
+                F[v.p[2]] += f * v.w[2];
+            } else {
+                Vec3 r12 = R[v.p[1]] - R[v.p[0]], r13 = R[v.p[2]] - R[v.p[0]];
+                Vec3 f2 = f * v.w[0] + r13.cross(f) * v.w[2];
+                Vec3 f3 = f * v.w[1] + f.cross(r12) * v.w[2];
+                F[v.p[0]] += f - f2 - f3;
+                F[v.p[1]] += f2;
+                F[v.p[2]] += f3;
+            }
         }
     }
 };
@@ -715,17 +761,22 @@ struct BodySystem {
         atomIndex.assign(numActualAtoms, 0);
         numFree = 0;
         body.assign(numBodies, Body());
+        /* massless particles (virtual sites are massless) are neither free
+           nor body atoms (RigidBodySystem.cpp:66-76); they must be labelled
+           free, because the reference's body-atom loop (:99-104) does not
+           skip them and would overwrite the next body's entries */
         for (int i = 0; i < numAtoms; i++) {
             if (ctx.mass[i] == 0.0) {
                 if (bodyIndex[i] != 0)
-                    throw std::runtime_error("massless atoms cannot belong to rigid bodies");
-                throw std::runtime_error("massless atoms are not supported");
+                    throw std::runtime_error("massless particles cannot belong to rigid bodies");
+                continue;
             }
             if (bodyIndex[i] == 0) atomIndex[numFree++] = i;
             else body[bodyIndex[i] - 1].N++;
         }
         numBodyAtoms = numActualAtoms - numFree;
         d.assign(numBodyAtoms, Vec3());
+        if (numActualAtoms < 0) throw std::runtime_error("internal: atom counts");
         freeAtom.resize(numFree);
         for (int k = 0; k < numFree; k++) {
             freeAtom[k].index = atomIndex[k];
@@ -1025,7 +1076,7 @@ struct rbref_sim {
         kernelSystem.integratePart1(dt, ctx.F, ctx.V, ctx.R);
         if (kernelSystem.numFree != 0 && !constraints.list.empty())
             constraints.applyPositions(oldPos, ctx.R, invMass);
-        /* virtual sites: none in this system model */
+        ctx.computeSitePositions();
         ctx.calcForces();
         kernelSystem.integratePart2(dt, ctx.R, ctx.F, ctx.V);
         if (kernelSystem.numFree != 0 && !constraints.list.empty())
@@ -1199,10 +1250,39 @@ int rbref_add_constraint(rbref_sim* s, int a, int b, double distance) {
         /* RigidBodySystem::initialize, RigidBodySystem.cpp:107-112 */
         if (s->integratorSystem.bodyIndex[a] != 0 || s->integratorSystem.bodyIndex[b] != 0)
             throw std::runtime_error("Constraints involving rigid-body atoms are not allowed");
+        if (s->ctx.mass[a] == 0.0 || s->ctx.mass[b] == 0.0)
+            throw std::runtime_error("A constraint cannot involve a massless particle");
         s->constraints.list.push_back(Constraint{a, b, distance});
         s->constraints.build(s->ctx.n);
         int nc = (int)s->constraints.list.size();
         s->integratorSystem.numConstraints = s->kernelSystem.numConstraints = nc;
+    });
+}
+
+int rbref_add_virtual_site(rbref_sim* s, int type, int site, const int* parents, const double* weights) {
+    return guarded([&] {
+        int n = s->ctx.n;
+        int np = type == RBREF_SITE_TWO_AVERAGE ? 2 : 3;
+        if (type < RBREF_SITE_TWO_AVERAGE || type > RBREF_SITE_OUT_OF_PLANE) throw std::runtime_error("unknown site type");
+        if (site < 0 || site >= n) throw std::runtime_error("site out of range");
+        if (s->ctx.mass[site] != 0.0) throw std::runtime_error("a virtual site must be massless");
+        VirtualSite v{type, site, {0, 0, 0}, {0.0, 0.0, 0.0}};
+        for (int k = 0; k < np; k++) {
+            int p = parents[k];
+            if (p < 0 || p >= n || p == site || s->ctx.mass[p] == 0.0)
+                throw std::runtime_error("a site's parents must be massive particles other than the site");
+            v.p[k] = p;
+        }
+        for (int k = 0; k < 3; k++) v.w[k] = weights[k];
+        for (const VirtualSite& o : s->ctx.sites)
+            if (o.site == site) throw std::runtime_error("particle is already a virtual site");
+        s->ctx.sites.push_back(v);
+        /* RigidBodySystem.cpp:65-68: virtual sites are not actual atoms */
+        for (BodySystem* b : {&s->integratorSystem, &s->kernelSystem}) {
+            b->numActualAtoms--;
+            b->numBodyAtoms = b->numActualAtoms - b->numFree;
+            b->atomIndex.resize(b->numActualAtoms);
+        }
     });
 }
 

@@ -16,6 +16,7 @@
 //                        oscillator: refined kinetic energy + potential +
 //                        potential refinement is the velocity-Verlet shadow
 //                        energy, conserved far better than the plain energy
+//   testVirtualSites     a rigid water with a massless tethered M site
 //   testConstraints / testConstrainedClusters  the reference's constraint
 //                        tests (tests/TestRigidBodyIntegrator.h:85-197,
 //                        commented out there) with tethers standing in for
@@ -255,6 +256,42 @@ void testConstrainedClusters(Platform& platform, const std::string& precision) {
                    2e-5, 1);
 }
 
+// a rigid water with a massless fourth site (TIP4P's M site as a
+// three-particle average) that carries a tether: the site follows its
+// definition, the energy is conserved, and sites are not actual atoms
+void testVirtualSites(Platform& platform, const std::string& precision) {
+    System system;
+    system.addParticle(15.9994);
+    system.addParticle(1.008);
+    system.addParticle(1.008);
+    system.addParticle(0.0);
+    const double a = 0.128, b = 0.128, c = 1.0 - a - b;
+    system.setVirtualSite(3, new ThreeParticleAverageSite(0, 1, 2, c, a, b));
+    const double th = 104.52 * M_PI / 180.0, d = 0.09572;
+    std::vector<Vec3> x = {Vec3(0, 0, 0), Vec3(d, 0, 0), Vec3(d * std::cos(th), d * std::sin(th), 0)};
+    x.push_back(x[0] * c + x[1] * a + x[2] * b);
+    TetherForce* tether = new TetherForce();
+    for (int i = 0; i < 4; i++) tether->addParticle(i, 400.0, x[i] + Vec3(0.01 * (i + 1), -0.005 * i, 0.008));
+    system.addForce(tether);
+    RigidBodyIntegrator integrator(0.001, std::vector<int>{1, 1, 1, 0});
+    Context context(system, integrator, platform, props(precision));
+    context.setPositions(x);
+    context.setVelocities({Vec3(0.2, 0.1, 0.0), Vec3(-0.5, 0.8, 0.3), Vec3(0.4, -0.6, -0.2), Vec3()});
+    const RigidBodySystem& bs = integrator.getRigidBodySystem();
+    if (bs.getNumActualAtoms() != 3 || bs.getNumBodies() != 1 || bs.getNumFree() != 0) throw Exception("counts");
+    double e0 = 0.0;
+    for (int i = 0; i < 1000; i++) {
+        State st = context.getState(State::Positions | State::Energy);
+        const std::vector<Vec3>& R = st.getPositions();
+        if (i > 0) CHECK_VEC(R[0] * c + R[1] * a + R[2] * b, R[3], 2e-6);
+        double e = st.getPotentialEnergy() + st.getKineticEnergy();
+        if (i == 1) e0 = e;
+        else if (i > 1 && !(std::fabs(e - e0) <= 0.01 * std::fabs(e0)))
+            throw Exception("energy drifted from " + std::to_string(e0) + " to " + std::to_string(e));
+        integrator.step(1);
+    }
+}
+
 void testApiErrors(Platform& platform) {
     RigidBodyIntegrator a(0.001, std::vector<int>{1, 1});
     CHECK_THROWS(a.setRotationMode(-1));
@@ -284,7 +321,7 @@ void testApiErrors(Platform& platform) {
         CHECK_THROWS(Context(system, integrator, platform));
     }
     {
-        // this platform's limits: single-atom bodies, massless particles
+        // this platform's limits: single-atom bodies, massless particles in a body
         System system;
         system.addParticle(1.0);
         system.addParticle(1.0);
@@ -292,9 +329,17 @@ void testApiErrors(Platform& platform) {
         CHECK_THROWS(Context(system, integrator, platform));
         System massless;
         massless.addParticle(1.0);
+        massless.addParticle(1.0);
         massless.addParticle(0.0);
-        RigidBodyIntegrator integrator2(0.001, std::vector<int>{0, 0});
+        RigidBodyIntegrator integrator2(0.001, std::vector<int>{1, 1, 1});
         CHECK_THROWS(Context(massless, integrator2, platform));
+        // a constraint cannot involve a massless particle (testConstrainedMasslessParticles)
+        System cm;
+        cm.addParticle(0.0);
+        cm.addParticle(1.0);
+        cm.addConstraint(0, 1, 1.5);
+        RigidBodyIntegrator integrator3(0.01, std::vector<int>{0, 0});
+        CHECK_THROWS(Context(cm, integrator3, platform));
     }
     {
         System system;
@@ -354,6 +399,8 @@ int main() {
         {"testConstraints double", [&] { testConstraints(platform, "double"); }},
         {"testConstrainedClusters mixed", [&] { testConstrainedClusters(platform, "mixed"); }},
         {"testConstrainedClusters double", [&] { testConstrainedClusters(platform, "double"); }},
+        {"testVirtualSites mixed", [&] { testVirtualSites(platform, "mixed"); }},
+        {"testVirtualSites double", [&] { testVirtualSites(platform, "double"); }},
     };
     for (auto& t : tests) {
         std::printf("%s ...\n", t.first.c_str());
