@@ -98,11 +98,16 @@ int rbhip_create_constrained(int numAtoms, const double* masses, const int* body
    :100 does, and after every force evaluation its force moves to its parents
    (the transpose of d r_site / d r_parent) and is cleared.  Per site: a type,
    the site particle, 3 parents (the third unused for two-particle averages)
-   and 3 weights: {w1, w2, -} two-particle average, {w1, w2, w3} three-particle
-   average, {w12, w13, wCross} out of plane (r = r1 + w12 r12 + w13 r13 +
-   wCross r12 x r13).  Parents must be massive.  Set once, before stepping;
-   contexts with sites cannot join a batch or compute refined energies. */
-enum { RBHIP_SITE_TWO_AVERAGE = 1, RBHIP_SITE_THREE_AVERAGE = 2, RBHIP_SITE_OUT_OF_PLANE = 3 };
+   and 12 weights, of which a site uses {w1, w2} (two-particle average),
+   {w1, w2, w3} (three-particle average), {w12, w13, wCross} (out of plane:
+   r = r1 + w12 r12 + w13 r13 + wCross r12 x r13) or all 12 (local
+   coordinates: origin, x and y weights of the 3 parents, then the local
+   position; r = o + p_x e_x + p_y e_y + p_z e_z with e_x along the x-weighted
+   sum, e_z along (x-sum) x (y-sum), e_y = e_z x e_x).  Parents must be
+   massive.  Set once, before stepping; contexts with sites cannot join a
+   batch or compute refined energies. */
+enum { RBHIP_SITE_TWO_AVERAGE = 1, RBHIP_SITE_THREE_AVERAGE = 2, RBHIP_SITE_OUT_OF_PLANE = 3,
+       RBHIP_SITE_LOCAL_COORDINATES = 4 };
 int rbhip_set_virtual_sites(rbhip_context* ctx, int count, const int* types, const int* siteAtoms,
                             const int* parents, const double* weights);
 /* Integrator::setConstraintTolerance (default 1e-5): positions converge when

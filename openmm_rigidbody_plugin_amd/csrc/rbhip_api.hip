@@ -1285,10 +1285,11 @@ int rbhip_set_virtual_sites(rbhip_context* ctx, int count, const int* types, con
         const int firstMassless = C.numAtoms - C.numMassless;
         std::vector<char> isSite(C.numAtoms, 0);
         std::vector<int> sa(count), st(count), sp(3 * (size_t)count, 0);
-        std::vector<double> sw(3 * (size_t)count, 0.0);
+        std::vector<double> sw(12 * (size_t)count, 0.0);
         for (int v = 0; v < count; v++) {
             const int t = types[v], site = siteAtoms[v];
-            require(t == RBHIP_SITE_TWO_AVERAGE || t == RBHIP_SITE_THREE_AVERAGE || t == RBHIP_SITE_OUT_OF_PLANE,
+            require(t == RBHIP_SITE_TWO_AVERAGE || t == RBHIP_SITE_THREE_AVERAGE || t == RBHIP_SITE_OUT_OF_PLANE ||
+                        t == RBHIP_SITE_LOCAL_COORDINATES,
                     "unknown virtual site type");
             require(site >= 0 && site < C.numAtoms && internalOf[site] >= firstMassless,
                     "a virtual site must be a massless particle");
@@ -1301,7 +1302,8 @@ int rbhip_set_virtual_sites(rbhip_context* ctx, int count, const int* types, con
                         "a virtual site's parents must be massive particles");
                 sp[3 * v + k] = internalOf[pa];
             }
-            for (int k = 0; k < 3; k++) sw[3 * v + k] = np == 2 && k == 2 ? 0.0 : weights[3 * v + k];
+            const int nw = t == RBHIP_SITE_LOCAL_COORDINATES ? 12 : np;
+            for (int k = 0; k < nw; k++) sw[12 * (size_t)v + k] = weights[12 * (size_t)v + k];
             sa[v] = internalOf[site];
             st[v] = t;
         }

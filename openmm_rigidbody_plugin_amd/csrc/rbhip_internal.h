@@ -32,7 +32,7 @@
 namespace rbhip {
 
 // virtual-site kinds (the values of include/rbhip.h's RBHIP_SITE_*)
-enum SiteKind : int { SITE_TWO_AVERAGE = 1, SITE_THREE_AVERAGE = 2, SITE_OUT_OF_PLANE = 3 };
+enum SiteKind : int { SITE_TWO_AVERAGE = 1, SITE_THREE_AVERAGE = 2, SITE_OUT_OF_PLANE = 3, SITE_LOCAL_COORDINATES = 4 };
 
 enum TileKind : int { TILE_FREE = 0, TILE_BODIES = 1, TILE_ATOMS = 2, TILE_CLUSTERS = 3 };
 
@@ -123,7 +123,7 @@ struct DevSys {
     int* siteAtom;      // internal index of each site
     int* siteType;      // RBHIP_SITE_*
     int* siteParent;    // 3 internal parent indices per site
-    double* siteW;      // 3 weights per site
+    double* siteW;      // 12 weights per site (3 used but by local-coordinates sites)
     int* parentAtom;
     int* parentStart;
     int* parentSite;

@@ -1105,16 +1105,40 @@ template <class Real> __device__ __forceinline__ V3<double> force_pos(const DevS
 // d r_parent) in a fixed order, and the sites' forces are cleared.  One lane
 // per site or per parent.
 // ---------------------------------------------------------------------------
+// the local frame of a local-coordinates site: x and y are the x- and
+// y-weighted sums of the parents, e_x = x/|x|, e_z = (x × y)/|x × y|,
+// e_y = e_z × e_x
+struct SiteFrame { V3<double> x, y, ex, ey, ez; double nx, nz; };
+
+template <class Real>
+__device__ __forceinline__ SiteFrame site_frame(const DevSys& S, const int* p, const double* w) {
+    SiteFrame f;
+    V3<double> r0 = force_pos<Real>(S, p[0]), r1 = force_pos<Real>(S, p[1]), r2 = force_pos<Real>(S, p[2]);
+    f.x = r0 * w[3] + r1 * w[4] + r2 * w[5];
+    f.y = r0 * w[6] + r1 * w[7] + r2 * w[8];
+    V3<double> z = cross(f.x, f.y);
+    f.nx = sqrt(dot(f.x, f.x));
+    f.nz = sqrt(dot(z, z));
+    f.ex = f.x * (1.0 / f.nx);
+    f.ez = z * (1.0 / f.nz);
+    f.ey = cross(f.ez, f.ex);
+    return f;
+}
+
 template <class Real>
 __global__ __launch_bounds__(256) void k_site_positions(const DevSys* __restrict__ sys) {
     const DevSys& S = *sys;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S.numSites) return;
     const int* p = S.siteParent + 3 * (size_t)s;
-    const double* w = S.siteW + 3 * (size_t)s;
+    const double* w = S.siteW + 12 * (size_t)s;
     V3<double> r1 = force_pos<Real>(S, p[0]), r2 = force_pos<Real>(S, p[1]), x;
     const int type = S.siteType[s];
-    if (type == SITE_TWO_AVERAGE) {
+    if (type == SITE_LOCAL_COORDINATES) {
+        SiteFrame f = site_frame<Real>(S, p, w);
+        V3<double> o = r1 * w[0] + r2 * w[1] + force_pos<Real>(S, p[2]) * w[2];
+        x = o + f.ex * w[9] + f.ey * w[10] + f.ez * w[11];
+    } else if (type == SITE_TWO_AVERAGE) {
         x = r1 * w[0] + r2 * w[1];
     } else if (type == SITE_THREE_AVERAGE) {
         x = r1 * w[0] + r2 * w[1] + force_pos<Real>(S, p[2]) * w[2];
@@ -1135,11 +1159,21 @@ __global__ __launch_bounds__(256) void k_site_forces(const DevSys* __restrict__ 
     for (int e = S.parentStart[g]; e < S.parentStart[g + 1]; e++) {
         const int s = S.parentSite[e], role = S.parentRole[e];
         const int* p = S.siteParent + 3 * (size_t)s;
-        const double* w = S.siteW + 3 * (size_t)s;
+        const double* w = S.siteW + 12 * (size_t)s;
         V3<double> fs = cvt<double>(ld3<Real>(S.force, S.atomStride, S.siteAtom[s]));
         const int type = S.siteType[s];
-        if (type != SITE_OUT_OF_PLANE) {
+        if (type == SITE_TWO_AVERAGE || type == SITE_THREE_AVERAGE) {
             f = f + fs * w[role];
+        } else if (type == SITE_LOCAL_COORDINATES) {
+            // ow F + xw Gx + yw Gy, Gx and Gy the transposed derivatives of
+            // the site with respect to x and y
+            SiteFrame fr = site_frame<Real>(S, p, w);
+            auto Jx = [&](V3<double> a) { return (a - fr.ex * dot(fr.ex, a)) * (1.0 / fr.nx); };
+            auto Jz = [&](V3<double> a) { return (a - fr.ez * dot(fr.ez, a)) * (1.0 / fr.nz); };
+            V3<double> jzexf = Jz(cross(fr.ex, fs)), jzf = Jz(fs);
+            V3<double> gx = Jx(fs) * w[9] + (cross(fr.y, jzexf) - Jx(cross(fr.ez, fs))) * w[10] + cross(fr.y, jzf) * w[11];
+            V3<double> gy = cross(fr.x, jzexf) * (-w[10]) - cross(fr.x, jzf) * w[11];
+            f = f + (fs * w[role] + gx * w[3 + role] + gy * w[6 + role]);
         } else {
             V3<double> r1 = force_pos<Real>(S, p[0]);
             V3<double> r12 = force_pos<Real>(S, p[1]) - r1, r13 = force_pos<Real>(S, p[2]) - r1;

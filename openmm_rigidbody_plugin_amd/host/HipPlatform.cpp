@@ -77,14 +77,12 @@ void HipPlatform::PlatformData::createSimulation(const System& system, const std
         for (int i = 0; i < n; i++) {
             if (!system.isVirtualSite(i)) continue;
             const VirtualSite& v = system.getVirtualSite(i);
-            double w[3];
+            double w[12];
             v.weights(w);
             types.push_back(v.type());
             siteAtoms.push_back(i);
-            for (int k = 0; k < 3; k++) {
-                parents.push_back(k < v.getNumParticles() ? v.getParticle(k) : 0);
-                weights.push_back(w[k]);
-            }
+            for (int k = 0; k < 3; k++) parents.push_back(k < v.getNumParticles() ? v.getParticle(k) : 0);
+            weights.insert(weights.end(), w, w + 12);
         }
         if (!types.empty())
             rbhip_check(rbhip_set_virtual_sites(s, (int)types.size(), types.data(), siteAtoms.data(), parents.data(),

@@ -64,10 +64,10 @@ double TwoParticleAverageSite::getWeight(int particle) const {
     return particle == 0 ? w1 : w2;
 }
 int TwoParticleAverageSite::type() const { return RBHIP_SITE_TWO_AVERAGE; }
-void TwoParticleAverageSite::weights(double w[3]) const {
+void TwoParticleAverageSite::weights(double w[12]) const {
+    for (int k = 0; k < 12; k++) w[k] = 0.0;
     w[0] = w1;
     w[1] = w2;
-    w[2] = 0.0;
 }
 
 ThreeParticleAverageSite::ThreeParticleAverageSite(int particle1, int particle2, int particle3, double weight1,
@@ -80,8 +80,8 @@ double ThreeParticleAverageSite::getWeight(int particle) const {
     return w[particle];
 }
 int ThreeParticleAverageSite::type() const { return RBHIP_SITE_THREE_AVERAGE; }
-void ThreeParticleAverageSite::weights(double out[3]) const {
-    for (int k = 0; k < 3; k++) out[k] = w[k];
+void ThreeParticleAverageSite::weights(double out[12]) const {
+    for (int k = 0; k < 12; k++) out[k] = k < 3 ? w[k] : 0.0;
 }
 
 OutOfPlaneSite::OutOfPlaneSite(int particle1, int particle2, int particle3, double weight12, double weight13,
@@ -90,10 +90,26 @@ OutOfPlaneSite::OutOfPlaneSite(int particle1, int particle2, int particle3, doub
     particles = {particle1, particle2, particle3};
 }
 int OutOfPlaneSite::type() const { return RBHIP_SITE_OUT_OF_PLANE; }
-void OutOfPlaneSite::weights(double w[3]) const {
+void OutOfPlaneSite::weights(double w[12]) const {
+    for (int k = 0; k < 12; k++) w[k] = 0.0;
     w[0] = w12;
     w[1] = w13;
     w[2] = wc;
+}
+
+LocalCoordinatesSite::LocalCoordinatesSite(int particle1, int particle2, int particle3, const Vec3& originWeights,
+                                           const Vec3& xWeights, const Vec3& yWeights, const Vec3& localPosition)
+    : ow(originWeights), xw(xWeights), yw(yWeights), lp(localPosition) {
+    particles = {particle1, particle2, particle3};
+}
+int LocalCoordinatesSite::type() const { return RBHIP_SITE_LOCAL_COORDINATES; }
+void LocalCoordinatesSite::weights(double w[12]) const {
+    for (int k = 0; k < 3; k++) {
+        w[k] = ow[k];
+        w[3 + k] = xw[k];
+        w[6 + k] = yw[k];
+        w[9 + k] = lp[k];
+    }
 }
 
 int System::addParticle(double mass) {

@@ -36,6 +36,9 @@ struct Vec3 {
     Vec3 operator-(const Vec3& o) const { return Vec3(v[0] - o[0], v[1] - o[1], v[2] - o[2]); }
     Vec3 operator*(double s) const { return Vec3(v[0] * s, v[1] * s, v[2] * s); }
     double dot(const Vec3& o) const { return v[0] * o[0] + v[1] * o[1] + v[2] * o[2]; }
+    Vec3 cross(const Vec3& o) const {
+        return Vec3(v[1] * o[2] - v[2] * o[1], v[2] * o[0] - v[0] * o[2], v[0] * o[1] - v[1] * o[0]);
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -73,7 +76,7 @@ private:
 // ---------------------------------------------------------------------------
 // System
 // ---------------------------------------------------------------------------
-// OpenMM's virtual sites of the three kinds this platform computes; a site
+// OpenMM's virtual sites of the four kinds this platform computes; a site
 // particle must be massless.  type() is the RBHIP_SITE_* value.
 class VirtualSite {
 public:
@@ -81,8 +84,9 @@ public:
     int getNumParticles() const { return (int)particles.size(); }
     int getParticle(int index) const;
     virtual int type() const = 0;
-    // {w1, w2, 0}, {w1, w2, w3} or {w12, w13, wCross}
-    virtual void weights(double w[3]) const = 0;
+    // 12 values, as rbhip_set_virtual_sites takes them: {w1, w2}, {w1, w2, w3},
+    // {w12, w13, wCross}, or origin, x, y weights and the local position
+    virtual void weights(double w[12]) const = 0;
 
 protected:
     std::vector<int> particles;
@@ -93,7 +97,7 @@ public:
     TwoParticleAverageSite(int particle1, int particle2, double weight1, double weight2);
     double getWeight(int particle) const;
     int type() const override;
-    void weights(double w[3]) const override;
+    void weights(double w[12]) const override;
 
 private:
     double w1, w2;
@@ -105,7 +109,7 @@ public:
                              double weight3);
     double getWeight(int particle) const;
     int type() const override;
-    void weights(double w[3]) const override;
+    void weights(double w[12]) const override;
 
 private:
     double w[3];
@@ -118,10 +122,25 @@ public:
     double getWeight13() const { return w13; }
     double getWeightCross() const { return wc; }
     int type() const override;
-    void weights(double w[3]) const override;
+    void weights(double w[12]) const override;
 
 private:
     double w12, w13, wc;
+};
+
+class LocalCoordinatesSite : public VirtualSite {
+public:
+    LocalCoordinatesSite(int particle1, int particle2, int particle3, const Vec3& originWeights, const Vec3& xWeights,
+                         const Vec3& yWeights, const Vec3& localPosition);
+    Vec3 getOriginWeights() const { return ow; }
+    Vec3 getXWeights() const { return xw; }
+    Vec3 getYWeights() const { return yw; }
+    Vec3 getLocalPosition() const { return lp; }
+    int type() const override;
+    void weights(double w[12]) const override;
+
+private:
+    Vec3 ow, xw, yw, lp;
 };
 
 class System {

@@ -211,17 +211,18 @@ class Context:
         return self._h
 
     # ---- virtual sites ----
-    SITE_TWO_AVERAGE, SITE_THREE_AVERAGE, SITE_OUT_OF_PLANE = 1, 2, 3
+    SITE_TWO_AVERAGE, SITE_THREE_AVERAGE, SITE_OUT_OF_PLANE, SITE_LOCAL_COORDINATES = 1, 2, 3, 4
 
     def set_virtual_sites(self, sites):
         """sites: (kind, site, parents, weights) tuples; kind SITE_TWO_AVERAGE
         (2 parents, weights w1 w2), SITE_THREE_AVERAGE (3, w1 w2 w3) or
-        SITE_OUT_OF_PLANE (3, w12 w13 wCross)."""
+        SITE_OUT_OF_PLANE (3, w12 w13 wCross) or SITE_LOCAL_COORDINATES (3,
+        origin, x and y weights and the local position: 12 values)."""
         n = len(sites)
         t = np.ascontiguousarray([int(k) for k, _, _, _ in sites], dtype=np.int32)
         a = np.ascontiguousarray([int(x) for _, x, _, _ in sites], dtype=np.int32)
         p = np.zeros((n, 3), dtype=np.int32)
-        w = np.zeros((n, 3))
+        w = np.zeros((n, 12))
         for i, (_, _, par, wt) in enumerate(sites):
             p[i, :len(par)] = par
             w[i, :len(wt)] = wt

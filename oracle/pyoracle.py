@@ -24,6 +24,7 @@ FIX_ALL = 63
 SITE_TWO_AVERAGE = 1
 SITE_THREE_AVERAGE = 2
 SITE_OUT_OF_PLANE = 3
+SITE_LOCAL_COORDINATES = 4
 
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -151,10 +152,11 @@ class OracleSim:
         _check(lib().rbref_add_constraint(self._h, int(a), int(b), float(distance)))
 
     def add_virtual_site(self, kind, site, parents, weights):
-        """kind: SITE_TWO_AVERAGE, SITE_THREE_AVERAGE or SITE_OUT_OF_PLANE."""
+        """kind: SITE_TWO_AVERAGE, SITE_THREE_AVERAGE, SITE_OUT_OF_PLANE or
+        SITE_LOCAL_COORDINATES (12 weights: origin, x, y, local position)."""
         p = np.zeros(3, dtype=np.int32)
         p[:len(parents)] = parents
-        w = np.zeros(3)
+        w = np.zeros(12)
         w[:len(weights)] = weights
         _check(lib().rbref_add_virtual_site(self._h, int(kind), int(site), _ip(p), _dp(w)))
 

@@ -652,7 +652,7 @@ std::vector<int> cleanBodyIndices(const std::vector<int>& in) {
 /* ------------------------------------------------------------------------ */
 struct Bond { int a, b; double length, k; };
 
-struct VirtualSite { int type; int site; int p[3]; double w[3]; };
+struct VirtualSite { int type; int site; int p[3]; double w[12]; };
 
 struct SystemState {
     int n = 0;
@@ -692,11 +692,33 @@ struct SystemState {
        ReferenceRigidBodyKernels.cpp:100), written from their published
        definitions: TwoParticleAverage r = w1 r1 + w2 r2, ThreeParticleAverage
        r = w1 r1 + w2 r2 + w3 r3, OutOfPlane r = r1 + w12 r12 + w13 r13 +
-       wc r12 x r13 (r1j = rj - r1) */
+       wc r12 x r13 (r1j = rj - r1), LocalCoordinates r = o + p_x e_x +
+       p_y e_y + p_z e_z with o, x, y the origin-, x- and y-weighted sums of
+       the parents, e_x = x/|x|, e_z = (x × y)/|x × y|, e_y = e_z × e_x
+       (weights: origin 0-2, x 3-5, y 6-8, local position 9-11) */
+    struct LocalFrame { Vec3 x, y, ex, ey, ez; double nx, nz; };
+    LocalFrame localFrame(const VirtualSite& v) const {
+        LocalFrame f;
+        for (int k = 0; k < 3; k++) {
+            f.x += R[v.p[k]] * v.w[3 + k];
+            f.y += R[v.p[k]] * v.w[6 + k];
+        }
+        Vec3 z = f.x.cross(f.y);
+        f.nx = std::sqrt(f.x.dot(f.x));
+        f.nz = std::sqrt(z.dot(z));
+        f.ex = f.x / f.nx;
+        f.ez = z / f.nz;
+        f.ey = f.ez.cross(f.ex);
+        return f;
+    }
     void computeSitePositions() {
         for (const VirtualSite& v : sites) {
             const Vec3 &r1 = R[v.p[0]], &r2 = R[v.p[1]];
-            if (v.type == RBREF_SITE_TWO_AVERAGE) {
+            if (v.type == RBREF_SITE_LOCAL_COORDINATES) {
+                LocalFrame f = localFrame(v);
+                Vec3 o = R[v.p[0]] * v.w[0] + R[v.p[1]] * v.w[1] + R[v.p[2]] * v.w[2];
+                R[v.site] = o + f.ex * v.w[9] + f.ey * v.w[10] + f.ez * v.w[11];
+            } else if (v.type == RBREF_SITE_TWO_AVERAGE) {
                 R[v.site] = r1 * v.w[0] + r2 * v.w[1];
             } else if (v.type == RBREF_SITE_THREE_AVERAGE) {
                 R[v.site] = r1 * v.w[0] + r2 * v.w[1] + R[v.p[2]] * v.w[2];
@@ -719,6 +741,18 @@ struct SystemState {
                 F[v.p[0]] += f * v.w[0];
                 F[v.p[1]] += f * v.w[1];
                 F[v.p[2]] += f * v.w[2];
+            } else if (v.type == RBREF_SITE_LOCAL_COORDINATES) {
+                /* F_i = ow_i F + xw_i Gx + yw_i Gy with Gx, Gy the transposed
+                   derivatives of r with respect to x and y: Jx, Jz project
+                   out e_x, e_z and divide by |x|, |x × y| */
+                LocalFrame fr = localFrame(v);
+                auto Jx = [&](Vec3 a) { return (a - fr.ex * fr.ex.dot(a)) / fr.nx; };
+                auto Jz = [&](Vec3 a) { return (a - fr.ez * fr.ez.dot(a)) / fr.nz; };
+                const double px = v.w[9], py = v.w[10], pz = v.w[11];
+                Vec3 jzexf = Jz(fr.ex.cross(f)), jzf = Jz(f);
+                Vec3 gx = Jx(f) * px + (fr.y.cross(jzexf) - Jx(fr.ez.cross(f))) * py + fr.y.cross(jzf) * pz;
+                Vec3 gy = fr.x.cross(jzexf) * (-py) - fr.x.cross(jzf) * pz;
+                for (int k = 0; k < 3; k++) F[v.p[k]] += f * v.w[k] + gx * v.w[3 + k] + gy * v.w[6 + k];
             } else {
                 Vec3 r12 = R[v.p[1]] - R[v.p[0]], r13 = R[v.p[2]] - R[v.p[0]];
                 Vec3 f2 = f * v.w[0] + r13.cross(f) * v.w[2];
@@ -1263,17 +1297,18 @@ int rbref_add_virtual_site(rbref_sim* s, int type, int site, const int* parents,
     return guarded([&] {
         int n = s->ctx.n;
         int np = type == RBREF_SITE_TWO_AVERAGE ? 2 : 3;
-        if (type < RBREF_SITE_TWO_AVERAGE || type > RBREF_SITE_OUT_OF_PLANE) throw std::runtime_error("unknown site type");
+        if (type < RBREF_SITE_TWO_AVERAGE || type > RBREF_SITE_LOCAL_COORDINATES)
+            throw std::runtime_error("unknown site type");
         if (site < 0 || site >= n) throw std::runtime_error("site out of range");
         if (s->ctx.mass[site] != 0.0) throw std::runtime_error("a virtual site must be massless");
-        VirtualSite v{type, site, {0, 0, 0}, {0.0, 0.0, 0.0}};
+        VirtualSite v{type, site, {0, 0, 0}, {}};
         for (int k = 0; k < np; k++) {
             int p = parents[k];
             if (p < 0 || p >= n || p == site || s->ctx.mass[p] == 0.0)
                 throw std::runtime_error("a site's parents must be massive particles other than the site");
             v.p[k] = p;
         }
-        for (int k = 0; k < 3; k++) v.w[k] = weights[k];
+        for (int k = 0; k < (type == RBREF_SITE_LOCAL_COORDINATES ? 12 : 3); k++) v.w[k] = weights[k];
         for (const VirtualSite& o : s->ctx.sites)
             if (o.site == site) throw std::runtime_error("particle is already a virtual site");
         s->ctx.sites.push_back(v);

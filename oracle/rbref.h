@@ -78,10 +78,13 @@ int rbref_set_constraint_tolerance(rbref_sim* s, double tol);
 /* an OpenMM virtual site on a massless particle: positions after part 1 and
    the constraints, forces moved to the parents after each force evaluation.
    weights: {w1, w2} (two-particle average), {w1, w2, w3} (three-particle
-   average), {w12, w13, wCross} (out of plane) */
-#define RBREF_SITE_TWO_AVERAGE   1
-#define RBREF_SITE_THREE_AVERAGE 2
-#define RBREF_SITE_OUT_OF_PLANE  3
+   average), {w12, w13, wCross} (out of plane), or 12 values for local
+   coordinates: origin weights, x weights, y weights (3 each) and the local
+   position */
+#define RBREF_SITE_TWO_AVERAGE       1
+#define RBREF_SITE_THREE_AVERAGE     2
+#define RBREF_SITE_OUT_OF_PLANE      3
+#define RBREF_SITE_LOCAL_COORDINATES 4
 int rbref_add_virtual_site(rbref_sim* s, int type, int site, const int* parents, const double* weights);
 long rbref_constraint_failures(rbref_sim* s);   /* clusters not converged, so far */
 int rbref_set_positions(rbref_sim* s, const double* xyz);

@@ -16,7 +16,8 @@
 //                        oscillator: refined kinetic energy + potential +
 //                        potential refinement is the velocity-Verlet shadow
 //                        energy, conserved far better than the plain energy
-//   testVirtualSites     a rigid water with a massless tethered M site
+//   testVirtualSites     a rigid water with two massless tethered sites (an
+//                        M site and a local-coordinates site)
 //   testConstraints / testConstrainedClusters  the reference's constraint
 //                        tests (tests/TestRigidBodyIntegrator.h:85-197,
 //                        commented out there) with tethers standing in for
@@ -265,25 +266,39 @@ void testVirtualSites(Platform& platform, const std::string& precision) {
     system.addParticle(1.008);
     system.addParticle(1.008);
     system.addParticle(0.0);
+    system.addParticle(0.0);
     const double a = 0.128, b = 0.128, c = 1.0 - a - b;
     system.setVirtualSite(3, new ThreeParticleAverageSite(0, 1, 2, c, a, b));
+    // a second site in the water's own frame
+    const Vec3 lp(0.01, 0.02, 0.015);
+    system.setVirtualSite(4, new LocalCoordinatesSite(0, 1, 2, Vec3(1, 0, 0), Vec3(-1, 1, 0), Vec3(-1, 0, 1), lp));
     const double th = 104.52 * M_PI / 180.0, d = 0.09572;
     std::vector<Vec3> x = {Vec3(0, 0, 0), Vec3(d, 0, 0), Vec3(d * std::cos(th), d * std::sin(th), 0)};
     x.push_back(x[0] * c + x[1] * a + x[2] * b);
+    auto local = [&](const std::vector<Vec3>& R) {
+        Vec3 xd = R[1] - R[0], yd = R[2] - R[0];
+        Vec3 ex = xd * (1.0 / std::sqrt(xd.dot(xd))), z = xd.cross(yd);
+        Vec3 ez = z * (1.0 / std::sqrt(z.dot(z))), ey = ez.cross(ex);
+        return R[0] + ex * lp[0] + ey * lp[1] + ez * lp[2];
+    };
+    x.push_back(local(x));
     TetherForce* tether = new TetherForce();
-    for (int i = 0; i < 4; i++) tether->addParticle(i, 400.0, x[i] + Vec3(0.01 * (i + 1), -0.005 * i, 0.008));
+    for (int i = 0; i < 5; i++) tether->addParticle(i, 400.0, x[i] + Vec3(0.01 * (i + 1), -0.005 * i, 0.008));
     system.addForce(tether);
-    RigidBodyIntegrator integrator(0.001, std::vector<int>{1, 1, 1, 0});
+    RigidBodyIntegrator integrator(0.001, std::vector<int>{1, 1, 1, 0, 0});
     Context context(system, integrator, platform, props(precision));
     context.setPositions(x);
-    context.setVelocities({Vec3(0.2, 0.1, 0.0), Vec3(-0.5, 0.8, 0.3), Vec3(0.4, -0.6, -0.2), Vec3()});
+    context.setVelocities({Vec3(0.2, 0.1, 0.0), Vec3(-0.5, 0.8, 0.3), Vec3(0.4, -0.6, -0.2), Vec3(), Vec3()});
     const RigidBodySystem& bs = integrator.getRigidBodySystem();
     if (bs.getNumActualAtoms() != 3 || bs.getNumBodies() != 1 || bs.getNumFree() != 0) throw Exception("counts");
     double e0 = 0.0;
     for (int i = 0; i < 1000; i++) {
         State st = context.getState(State::Positions | State::Energy);
         const std::vector<Vec3>& R = st.getPositions();
-        if (i > 0) CHECK_VEC(R[0] * c + R[1] * a + R[2] * b, R[3], 2e-6);
+        if (i > 0) {
+            CHECK_VEC(R[0] * c + R[1] * a + R[2] * b, R[3], 2e-6);
+            CHECK_VEC(local(R), R[4], 2e-6);
+        }
         double e = st.getPotentialEnergy() + st.getKineticEnergy();
         if (i == 1) e0 = e;
         else if (i > 1 && !(std::fabs(e - e0) <= 0.01 * std::fabs(e0)))

@@ -1,35 +1,25 @@
 """Virtual sites and massless particles on the GPU (rbhip_set_virtual_sites)
 against the oracle, whose sites are pinned in test_oracle_sites.py: rigid
 waters carrying a tethered massless site of each kind (two- and
-three-particle averages, out of plane), free atoms with sites, a massless
-particle that is no site, with and without constraints; both rotation modes
-and precisions, fused and two-pass steps, graphs on and off."""
+three-particle averages, out of plane, local coordinates), free atoms with
+sites, a massless particle that is no site, with and without constraints;
+both rotation modes and precisions, fused and two-pass steps, graphs on and
+off."""
 import numpy as np
 import pytest
 
 from openmm_rigidbody_plugin_amd import systems
+from test_oracle_sites import SITES as KINDS, site_position   # the four site kinds
 
 pytestmark = pytest.mark.gpu
 
 TOL = {"mixed": dict(pos=2e-6, vel=1e-4), "double": dict(pos=1e-10, vel=1e-8)}
 
 
-def site_position(kind, r, w):
-    if kind == 1:
-        return w[0] * r[0] + w[1] * r[1]
-    if kind == 2:
-        return w[0] * r[0] + w[1] * r[1] + w[2] * r[2]
-    r12, r13 = r[1] - r[0], r[2] - r[0]
-    return r[0] + w[0] * r12 + w[1] * r13 + w[2] * np.cross(r12, r13)
-
-
-KINDS = [(1, [0.3, 0.7]), (2, [0.6, 0.25, 0.15]), (3, [0.2, 0.35, 2.5])]
-
-
 def site_spec(mode, seed=3, free_sites=True):
-    """27 waters and 12 argon atoms; one massless site per water (kinds in
-    turn), sites on triples of free atoms, and one massless particle that is
-    no site.  Returns the spec and the site list."""
+    """27 waters and 12 argon atoms; one massless site per water (the four
+    kinds in turn), sites on triples of free atoms, and one massless particle
+    that is no site.  Returns the spec and the site list."""
     base = systems.water_box(27, 0.5, num_free=12, seed=seed, rotation_mode=mode)
     rng = np.random.default_rng(seed)
     x = list(base.positions)
@@ -41,7 +31,8 @@ def site_spec(mode, seed=3, free_sites=True):
     if free_sites:
         owners += [(None, free[i:i + 3]) for i in range(0, 12, 3)]
     for i, (_, par) in enumerate(owners):
-        kind, w = KINDS[i % 3]
+        # local-coordinates sites on the waters only (the argon triples are collinear)
+        kind, w = KINDS[i % len(KINDS)] if i < 27 else KINDS[i % 3]
         par = par[:2] if kind == 1 else par
         site = len(x)
         x.append(site_position(kind, [base.positions[p] for p in par], w))

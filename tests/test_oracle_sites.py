@@ -4,8 +4,8 @@ The reference places OpenMM's virtual sites after part 1 and the constraints
 (ReferenceRigidBodyKernels.cpp:100) and leaves massless particles and sites
 out of the free atoms and the bodies (RigidBodySystem.cpp:65-76); OpenMM
 moves a site's force to its parents.  Pinned by:
-- the site definitions (two- and three-particle averages, out of plane)
-  after every step;
+- the site definitions (two- and three-particle averages, out of plane,
+  local coordinates) after every step;
 - the redistributed forces against central differences of the potential
   energy with the site following its parents (a wrong sign or a transposed
   cross product fails);
@@ -23,20 +23,32 @@ O_MASS, H_MASS = 15.9994, 1.008
 
 
 def site_position(kind, r, w):
+    r = np.asarray(r)
     if kind == 1:
         return w[0] * r[0] + w[1] * r[1]
     if kind == 2:
         return w[0] * r[0] + w[1] * r[1] + w[2] * r[2]
-    r12, r13 = r[1] - r[0], r[2] - r[0]
-    return r[0] + w[0] * r12 + w[1] * r13 + w[2] * np.cross(r12, r13)
+    if kind == 3:
+        r12, r13 = r[1] - r[0], r[2] - r[0]
+        return r[0] + w[0] * r12 + w[1] * r13 + w[2] * np.cross(r12, r13)
+    w = np.asarray(w)
+    o, x, y = w[0:3] @ r, w[3:6] @ r, w[6:9] @ r
+    ex = x / np.linalg.norm(x)
+    z = np.cross(x, y)
+    ez = z / np.linalg.norm(z)
+    ey = np.cross(ez, ex)
+    return o + w[9] * ex + w[10] * ey + w[11] * ez
 
 
-SITES = [(1, [0.3, 0.7]), (2, [0.6, 0.25, 0.15]), (3, [0.2, 0.35, 2.5])]
+# two- and three-particle averages, out of plane, local coordinates
+SITES = [(1, [0.3, 0.7]), (2, [0.6, 0.25, 0.15]), (3, [0.2, 0.35, 2.5]),
+         (4, [0.5, 0.25, 0.25, -1.0, 1.0, 0.0, -1.0, 0.0, 1.0, 0.02, 0.03, 0.015])]
+NS = len(SITES)
 
 
 def water_with_sites(oracle, rigid=True, tether_site=True, dt=0.001, seed=1):
     """Two waters (O, H, H) plus one site of each kind on the first water
-    and a massless particle that is not a site."""
+    (particles 6 ..) and a massless particle that is not a site (the last)."""
     rng = np.random.default_rng(seed)
     t = water_template()
     x = np.concatenate([t, t + [0.4, 0.0, 0.0]])
@@ -55,7 +67,7 @@ def water_with_sites(oracle, rigid=True, tether_site=True, dt=0.001, seed=1):
         o.add_virtual_site(kind, 6 + s, [0, 1, 2][:2 if kind == 1 else 3], w)
     k = np.full(n, 200.0)
     if not tether_site:
-        k[6:9] = 0.0
+        k[6:6 + NS] = 0.0
     o.set_tether(k, x + rng.normal(size=x.shape) * 0.02)
     o.set_positions(x)
     v = rng.normal(size=x.shape) * 0.3
@@ -71,7 +83,7 @@ def test_sites_follow_their_definitions(oracle):
         x = o.positions()
         for s, (kind, w) in enumerate(SITES):
             np.testing.assert_allclose(x[6 + s], site_position(kind, x[:3], w), rtol=0, atol=1e-14)
-        np.testing.assert_array_equal(x[9], [1.0, 1.0, 1.0])
+        np.testing.assert_array_equal(x[-1], [1.0, 1.0, 1.0])
 
 
 @pytest.mark.parametrize("kind,w", SITES)
@@ -118,7 +130,7 @@ def test_energy_conservation_with_a_tethered_site(oracle):
 def test_site_bookkeeping(oracle):
     o, _ = water_with_sites(oracle)
     c = o.counts()
-    assert c["numActualAtoms"] == 10 - 3
+    assert c["numActualAtoms"] == (6 + NS + 1) - NS     # every particle but the sites
     assert c["numFree"] == 0 and c["numBodies"] == 2
     free, _ = water_with_sites(oracle, rigid=False)
     assert free.counts()["numFree"] == 6
