@@ -321,7 +321,7 @@ def main():
                     help="only stream MIB MiB through the plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the integrator has no cross-GPU exchange to measure)", file=sys.stderr)
         sys.exit(2)
 

@@ -1,7 +1,7 @@
 /*
  * rbref.cpp — CPU FP64 restatement of the reference RigidBodyIntegrator path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Built with g++ -ffp-contract=off so
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Built with g++ -ffp-contract=off so
  * every operation is an IEEE double op in the order the reference writes it.
  *
  * Function → reference source it follows → how it is pinned (tests/):

@@ -1,7 +1,7 @@
 /*
  * rbref — CPU FP64 restatement of the reference RigidBodyIntegrator path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle for the HIP product
  * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
  * may load it.  The product (openmm_rigidbody_plugin_amd) never links or calls
  * it.

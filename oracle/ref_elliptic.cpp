@@ -1,8 +1,8 @@
 /*
  * Exports the reference's own elliptic routines for cross-checking the
  * oracle's independently written ones.  This file contains no reference code:
- * it includes the reference header in place (built with
- * -I /root/reference/openmmapi/include by oracle/Makefile, output only into
+ * it includes the reference header in place (oracle/Makefile puts the
+ * reference's openmmapi/include on the include path and writes only into
  * oracle/_ref/).  The header needs only <math.h>, <limits> and <algorithm>,
  * so it compiles from its own source with no stand-ins.
  * Test infrastructure only.
