@@ -507,7 +507,31 @@ struct rbhip_context {
                                  c.firstAtom + b * c.N, L, 0});
         }
         for (size_t i = 0; i < t.size(); i++) t[i].slot = (int)i;
-        return t;
+        return spreadGroups(t);
+    }
+
+    // launch order: the free-atom tiles and each body class spread evenly
+    // over the list (group g's i-th of n tiles at (i + 1/2)/n), so that the
+    // memory-only free tiles and the rotation-heavy body tiles share the CUs
+    // for the whole pass instead of running one after the other.  Each tile
+    // keeps its slot, so the partial sums are reduced in the same order.
+    static std::vector<Tile> spreadGroups(const std::vector<Tile>& t) {
+        std::vector<size_t> start;   // first tile of each run of one kind and N
+        for (size_t i = 0; i < t.size(); i++)
+            if (i == 0 || t[i].kind != t[i - 1].kind || t[i].N != t[i - 1].N) start.push_back(i);
+        start.push_back(t.size());
+        std::vector<std::pair<double, size_t>> key;
+        key.reserve(t.size());
+        for (size_t g = 0; g + 1 < start.size(); g++) {
+            const double n = (double)(start[g + 1] - start[g]);
+            for (size_t i = start[g]; i < start[g + 1]; i++) key.push_back({((double)(i - start[g]) + 0.5) / n, i});
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<double, size_t>& a, const std::pair<double, size_t>& b) { return a.first < b.first; });
+        std::vector<Tile> out;
+        out.reserve(t.size());
+        for (const auto& k : key) out.push_back(t[k.second]);
+        return out;
     }
 
     std::vector<Tile> forceTiles(int blk) const {
