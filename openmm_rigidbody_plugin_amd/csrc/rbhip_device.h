@@ -540,27 +540,38 @@ __device__ constexpr double kDnCoef[9][9] = {
 
 struct JacobiSeries { double s[9], c[9], d[9]; };   // the coefficients at one m
 
+// The first TERMS coefficients at m.  Row k of each table is a polynomial of
+// degree k in m (k - 1 for cn), so Horner starts at that degree: the higher
+// steps would be fma(0, m, 0), which is +0 for every finite m, so the values
+// are the same bit for bit, and the compiler cannot drop those steps itself
+// (0 * m is not 0 for m = inf or NaN).
+template <int TERMS>
 __device__ inline void jacobi_series_at(double m, JacobiSeries& J) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        double ps = kSnCoef[k][8], pc = kCnCoef[k][8], pd = kDnCoef[k][8];
+    for (int k = 0; k < TERMS; k++) {
+        double ps = kSnCoef[k][k], pd = kDnCoef[k][k];
 #pragma unroll
-        for (int j = 7; j >= 0; j--) {
+        for (int j = k - 1; j >= 0; j--) {
             ps = fma(ps, m, kSnCoef[k][j]);
-            pc = fma(pc, m, kCnCoef[k][j]);
             pd = fma(pd, m, kDnCoef[k][j]);
         }
+        const int kc = k > 0 ? k - 1 : 0;
+        double pc = kCnCoef[k][kc];
+#pragma unroll
+        for (int j = kc - 1; j >= 0; j--) pc = fma(pc, m, kCnCoef[k][j]);
         J.s[k] = ps;
         J.c[k] = pc;
         J.d[k] = pd;
     }
 }
 
+// the series truncated after TERMS terms: order 2 TERMS - 1 for sn
+template <int TERMS>
 __device__ inline void jacobi_series(double u, const JacobiSeries& J, double& sn, double& cn, double& dn) {
     double u2 = u * u;
-    double ps = J.s[8], pc = J.c[8], pd = J.d[8];
+    double ps = J.s[TERMS - 1], pc = J.c[TERMS - 1], pd = J.d[TERMS - 1];
 #pragma unroll
-    for (int k = 7; k >= 0; k--) {
+    for (int k = TERMS - 2; k >= 0; k--) {
         ps = fma(ps, u2, J.s[k]);
         pc = fma(pc, u2, J.c[k]);
         pd = fma(pd, u2, J.d[k]);
@@ -586,7 +597,13 @@ __device__ inline void jacobi_series(double u, const JacobiSeries& J, double& sn
 // Against a tight ODE integration of Euler's equations this is exact to
 // ~1e-15, where the closed form loses up to ~1e-11 near the axis-1 pole and
 // the separatrix.  Longer steps take the closed form.
+// FP32_STATE: the caller keeps q and L in fp32 (relative rounding 6e-8), so
+// the series stops at order 11 and the quadrature takes three points; the
+// result stays within 2e-13 of the full evaluation for |wp dt| <= 0.09 and
+// 1e-10 up to 0.2, far below the rounding of the state it is stored in.
+template <bool FP32_STATE = false>
 __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> I, V3<double> invI, double dt) {
+    constexpr int TERMS = FP32_STATE ? 6 : 9;
     const double eps = 2.220446049250313e-16;
     // the frame quaternion z(L) is singular for L along +e1, and so is the
     // rate of theta in the complex neighbourhood of a step that passes near
@@ -628,12 +645,12 @@ __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> 
     double d0 = inside ? (low ? w0.z / a.z : w0.x / a.x) : sqrt(1.0 - m);
     V3<double> Ia = hadamard(I, a);
     JacobiSeries J;
-    jacobi_series_at(m, J);
+    jacobi_series_at<TERMS>(m, J);
     const double ms0 = m * s0, c0d0 = c0 * d0;
     // body-frame L at u0 + x
     auto body_L = [&](double x) {
         double sx, cx, dx;
-        jacobi_series(x, J, sx, cx, dx);
+        jacobi_series<TERMS>(x, J, sx, cx, dx);
         double inv = 1.0 / (1.0 - ms0 * s0 * sx * sx);
         double sn = (s0 * cx * dx + sx * c0d0) * inv;
         double cn = (c0 * cx - s0 * sx * d0 * dx) * inv;
@@ -645,12 +662,19 @@ __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> 
         double y2 = Lt.y * Lt.y, z2 = Lt.z * Lt.z;
         return (L + Lt.x) * (y2 * invI.y + z2 * invI.z) / (y2 + z2) - Lt.x * invI.x;
     };
-    // Gauss-Legendre nodes and weights on [-1, 1] (5 points)
-    const double gx1 = 0.5384693101056831, gx2 = 0.9061798459386640;
-    const double gw0 = 0.5688888888888889, gw1 = 0.4786286704993665, gw2 = 0.2369268850561891;
+    // Gauss-Legendre nodes and weights on [-1, 1]: 5 points, or 3 for fp32 state
     const double h = 0.5 * delta;
-    double sum = gw0 * rate(body_L(h)) + gw1 * (rate(body_L(h * (1.0 - gx1))) + rate(body_L(h * (1.0 + gx1)))) +
-                 gw2 * (rate(body_L(h * (1.0 - gx2))) + rate(body_L(h * (1.0 + gx2))));
+    double sum;
+    if constexpr (FP32_STATE) {
+        const double gx1 = 0.7745966692414834;
+        const double gw0 = 0.8888888888888888, gw1 = 0.5555555555555556;
+        sum = gw0 * rate(body_L(h)) + gw1 * (rate(body_L(h * (1.0 - gx1))) + rate(body_L(h * (1.0 + gx1))));
+    } else {
+        const double gx1 = 0.5384693101056831, gx2 = 0.9061798459386640;
+        const double gw0 = 0.5688888888888889, gw1 = 0.4786286704993665, gw2 = 0.2369268850561891;
+        sum = gw0 * rate(body_L(h)) + gw1 * (rate(body_L(h * (1.0 - gx1))) + rate(body_L(h * (1.0 + gx1)))) +
+              gw2 * (rate(body_L(h * (1.0 - gx2))) + rate(body_L(h * (1.0 + gx2))));
+    }
     double theta = 0.25 * dt * sum;
     V3<double> Iw1 = body_L(delta);
     Q4<double> z0 = {Iw.z, Iw.y, L - Iw.x, 0.0};

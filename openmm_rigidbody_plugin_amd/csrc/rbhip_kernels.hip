@@ -206,7 +206,7 @@ __device__ __forceinline__ void free_rotation(Q4<Real>& q, V3<Real>& L, V3<Real>
             linear_rotor_exact(qd, Ld, invId, h);
         } else {
             V3<double> I = {1.0 / invId.x, 1.0 / invId.y, 1.0 / invId.z};
-            exact_rotation(qd, Ld, I, invId, h);
+            exact_rotation<sizeof(Real) == 4>(qd, Ld, I, invId, h);
         }
         q = cvt<Real>(qd);
         L = cvt<Real>(Ld);

@@ -7,6 +7,8 @@ scipy's DOP853 at rtol 2.5e-14.
   agree with the integration to 2e-13 in every class of body below, including
   L within 1e-5 of a principal axis, near-symmetric tops and orbits near the
   separatrix;
+- the kernels' evaluation for fp32 state (order-11 series, three-point
+  quadrature) must agree to 2.5e-13 for |wp dt| <= 0.09 and 2e-10 up to 0.2;
 - the closed form (the reference's algorithm, RigidBody.cpp:241-305) must
   agree with the oracle's restatement of it to 1e-12 away from the poles;
   near the axis-1 pole both lose accuracy to the cancellation in L - L1;
@@ -117,14 +119,31 @@ def test_short_steps_match_the_integration(rot, kind):
     assert worst <= 2e-13, worst
 
 
+@pytest.mark.parametrize("kind", KINDS)
+def test_fp32_state_evaluation_matches_the_integration(rot, kind):
+    # the kernels' evaluation for fp32 state (order-11 series, 3-point
+    # quadrature): within 2.5e-13 of the integration for |wp dt| <= 0.09 (every
+    # body at 300 K and dt <= 2 fs) and 2e-10 up to 0.2, against the 6e-8
+    # rounding of the fp32 state
+    rng = np.random.default_rng(KINDS.index(kind) + 60)
+    for lo, hi, tol in ((0.005, 0.09, 2.5e-13), (0.09, 0.2, 2e-10)):
+        worst = 0.0
+        for _ in range(15):
+            I, q, L = body(kind, rng)
+            dt = min(rng.uniform(lo, hi) / arg_step(I, L, 1.0), 5.0)
+            worst = max(worst, err(*rot(2, I, q, L, dt), *integrate(q, L, I, dt)))
+        assert worst <= tol, (lo, hi, worst)
+
+
 def test_long_steps_take_the_closed_form(rot, oracle):
     rng = np.random.default_rng(7)
     for _ in range(40):
         I, q, L = body("general", rng)
         dt = rng.uniform(0.25, 3.0) / arg_step(I, L, 1.0)
-        fast, closed = rot(0, I, q, L, dt), rot(1, I, q, L, dt)
-        np.testing.assert_array_equal(fast[0], closed[0])
-        np.testing.assert_array_equal(fast[1], closed[1])
+        fast, fp32, closed = rot(0, I, q, L, dt), rot(2, I, q, L, dt), rot(1, I, q, L, dt)
+        for r in (fast, fp32):
+            np.testing.assert_array_equal(r[0], closed[0])
+            np.testing.assert_array_equal(r[1], closed[1])
 
 
 def test_closed_form_matches_the_oracle(rot, oracle):
