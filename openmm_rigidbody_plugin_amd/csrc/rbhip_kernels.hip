@@ -724,6 +724,9 @@ template <int NA> struct ClusterSlots {
 // clusters read them from memory in every sweep.
 template <int NA> struct ClusterCons {
     static constexpr int kMax = NA <= 4 ? NA * (NA - 1) / 2 : 1;
+    // sweeps over the constraints unroll fully for the register classes; the
+    // general class loops over its count
+    static constexpr int kUnroll = NA <= 4 ? kMax : 1;
     int n = 0, k0 = 0, a0 = 0;
     int ci[kMax], cj[kMax];
     double cd[kMax];
@@ -764,7 +767,7 @@ __device__ __forceinline__ bool shake(const DevSys& S, const ClusterCons<NA>& C,
     const double hi = (1.0 + S.constraintTol) * (1.0 + S.constraintTol);
     for (int it = 0; it < CLUSTER_MAX_ITERATIONS; it++) {
         bool done = true;
-#pragma unroll
+#pragma unroll ClusterCons<NA>::kUnroll
         for (int k = 0; k < ClusterCons<NA>::kMax || NA > 4; k++) {
             if (k >= C.n) break;
             int i, j;
@@ -792,7 +795,7 @@ __device__ __forceinline__ bool rattle(const DevSys& S, const ClusterCons<NA>& C
     const double tol2 = S.constraintTol * S.constraintTol;
     for (int it = 0; it < CLUSTER_MAX_ITERATIONS; it++) {
         bool done = true;
-#pragma unroll
+#pragma unroll ClusterCons<NA>::kUnroll
         for (int k = 0; k < ClusterCons<NA>::kMax || NA > 4; k++) {
             if (k >= C.n) break;
             int i, j;
