@@ -25,7 +25,10 @@ from . import plugin
 
 class RigidBodySystem:
     """The body bookkeeping and kinetic energies of a bound integrator at the
-    time it was asked for (RigidBodySystem's getters)."""
+    time it was asked for (RigidBodySystem's getters).  The reference hands out
+    the copy it built when the state was last set, which goes stale as the
+    integrator steps (its TODO.md:5 lists downloading it first); here the
+    values are read from the device when asked for."""
 
     def __init__(self, num_dof, kinetic):
         self._dof = int(num_dof)
