@@ -21,6 +21,8 @@ FIX_STALE_SYSTEM = 8
 FIX_ORTHONORMAL = 16
 FIX_REFINED_FREE = 32
 FIX_ALL = 63
+# planted mistake (tests only): body-atom velocities rebuilt as delta x w
+PLANT_VELOCITY_SIGN = 1024
 SITE_TWO_AVERAGE = 1
 SITE_THREE_AVERAGE = 2
 SITE_OUT_OF_PLANE = 3

@@ -36,7 +36,11 @@
  *                           tight step), L and KE conservation, NO-SQUISH
  *                           n->large convergence
  *   updateAtomicPositions   RigidBody.cpp:148-153 → rigidity
- *   updateAtomicVelocities  RigidBody.cpp:159-168 → V_j - v_cm = w x delta
+ *   updateAtomicVelocities  RigidBody.cpp:159-168 → test_oracle_velocities:
+ *                             V_j = central difference of r_j(t), V_j - v_cm =
+ *                             w e_z x (r_j - c) for a spinning dimer, constrained
+ *                             triangle velocities -> rigid-body velocities as
+ *                             dt -> 0; each fails with the planted delta x w
  *   forceAndTorque          RigidBody.cpp:174-183 → brute-force sums
  *   integratePart1/2        RigidBodySystem.cpp:170-204
  *   execute                 ReferenceRigidBodyKernels.cpp:82-108
@@ -929,7 +933,8 @@ struct BodySystem {
         Vec3 omega = diagMul(b.invI, L);
         Vec3 ws = b.q.At(omega);
         Vec3 vcm = b.pcm * b.invMass;
-        for (int j = 0; j < b.N; j++) V[atomOf(b, j)] = vcm + ws.cross(b.delta[j]);
+        for (int j = 0; j < b.N; j++)
+            V[atomOf(b, j)] = vcm + ((flags & RBREF_PLANT_VELOCITY_SIGN) ? b.delta[j].cross(ws) : ws.cross(b.delta[j]));
         b.twoKt = b.pcm.dot(vcm);
         b.twoKr = L.dot(omega);
     }

@@ -54,6 +54,10 @@ extern "C" {
    energy at constant velocity) */
 #define RBREF_FIX_REFINED_FREE   32u
 #define RBREF_FIX_ALL            63u
+/* planted mistake, for tests that show a check has teeth: the body-atom
+   velocities are rebuilt as delta_j x w_s instead of w_s x delta_j
+   (RigidBody.cpp:166); never part of a policy */
+#define RBREF_PLANT_VELOCITY_SIGN 1024u
 
 typedef struct rbref_sim rbref_sim;
 

@@ -147,3 +147,24 @@ def test_fast_rotors_take_both_exact_paths(hip, oracle, precision):
     steps = np.array(steps)
     assert (steps <= 0.2).sum() >= 8 and (steps > 0.2).sum() >= 8, steps
     compare(oracle, hip, spec, precision)
+
+
+@pytest.mark.parametrize("mode", [0, 10])
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_spinning_dimer_atom_velocities(hip, mode, precision):
+    """Rigid-body kinematics alone, no oracle: a dimer spun about z at w keeps
+    every atom at v_cm + w e_z x (r_j - c) (RigidBody.cpp:159-168), which a
+    sign slip (delta x w) reverses."""
+    m = np.array([3.0, 1.0])
+    x = np.array([[1.00, 2.00, 3.00], [1.20, 2.00, 3.00]])
+    c0 = (m[:, None] * x).sum(0) / m.sum()
+    vcm, w, ez = np.array([0.1, -0.2, 0.3]), 5.0, np.array([0.0, 0.0, 1.0])
+    g = hip.Context(m, np.array([1, 1], np.int32), 0.002, mode, precision)
+    g.set_positions(x)
+    g.set_velocities(vcm + w * np.cross(ez, x - c0))
+    g.step(7)
+    xg, vg = g.positions(), g.velocities()
+    c = (m[:, None] * xg).sum(0) / m.sum()
+    tol = 1e-5 if precision == "mixed" else 1e-9
+    np.testing.assert_allclose(vg, vcm + w * np.cross(ez, xg - c), rtol=0, atol=tol)
+    np.testing.assert_allclose(c, c0 + vcm * 0.014, rtol=0, atol=10 * tol)

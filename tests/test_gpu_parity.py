@@ -2,9 +2,14 @@
 
 Tolerances (after 10 steps):
   mixed precision (fp32 state, FP64 accumulation): positions <= 1e-6 nm;
-    per-body momentum and space-frame angular momentum <= 1e-5 of their RMS;
-    kinetic energies <= 1e-5 relative
-  double precision: positions <= 1e-10 nm; momenta <= 1e-9 of RMS; KE <= 1e-11
+    each body's momentum and space-frame angular momentum within 1e-5 of its
+    own magnitude, above an absolute floor of 1e-6 of the RMS over the
+    bodies (the error fp32 positions put into the forces over 10 steps,
+    k ulp(x) 10 dt, is about 1e-6 of a 300 K water's momentum, so the
+    relative bound applies to every body down to a tenth of the RMS);
+    free-atom velocities likewise per atom; kinetic energies <= 1e-5 relative
+  double precision: positions <= 1e-10 nm; momenta <= 1e-9 relative above a
+    floor of 1e-10 of the RMS; KE <= 1e-11
 Integer bookkeeping (body numbering, atom order, counts, DOF) is bit-exact.
 """
 import numpy as np
@@ -15,7 +20,16 @@ from subsample import body_frames
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"mixed": dict(x=1e-6, p=1e-5, ke=1e-5), "double": dict(x=1e-10, p=1e-9, ke=1e-11)}
+TOL = {"mixed": dict(x=1e-6, p=1e-5, floor=1e-6, ke=1e-5), "double": dict(x=1e-10, p=1e-9, floor=1e-10, ke=1e-11)}
+
+
+def per_item_within(ref, got, rel, floor):
+    """|got_i - ref_i| <= rel |ref_i| + floor * RMS_i |ref_i| for every row i;
+    returns the largest ratio of the error to its bound."""
+    mag = np.linalg.norm(ref, axis=1)
+    bound = rel * mag + floor * np.sqrt((mag ** 2).mean())
+    ratio = np.linalg.norm(got - ref, axis=1) / bound
+    return ratio.max()
 
 
 def configs():
@@ -36,14 +50,15 @@ def compare_states(spec, o, g, tol):
     fg = body_frames(spec.masses, spec.body_indices, xg, vg)
     P = np.array([fo[b][1] for b in fo]); Pg = np.array([fg[b][1] for b in fo])
     L = np.array([fo[b][2] for b in fo]); Lg = np.array([fg[b][2] for b in fo])
-    rms_p = np.sqrt((P ** 2).sum(1).mean())
-    rms_l = np.sqrt((L ** 2).sum(1).mean())
-    assert np.abs(P - Pg).max() <= tol["p"] * rms_p
-    assert np.abs(L - Lg).max() <= tol["p"] * rms_l
+    rp = per_item_within(P, Pg, tol["p"], tol["floor"])
+    rl = per_item_within(L, Lg, tol["p"], tol["floor"])
+    assert rp <= 1.0, f"momentum error {rp:.3g} x its per-body bound"
+    assert rl <= 1.0, f"angular momentum error {rl:.3g} x its per-body bound"
     free = spec.body_indices <= 0
     if free.any():
-        vscale = np.sqrt((vo[free] ** 2).sum(1).mean())
-        assert np.abs(vo[free] - vg[free]).max() <= tol["p"] * vscale
+        rv = per_item_within(vo[free], vg[free], tol["p"], tol["floor"])
+        assert rv <= 1.0, f"free-atom velocity error {rv:.3g} x its per-atom bound"
+    print(f"per-body bound ratios: p {rp:.3g}, L {rl:.3g}")
     ko, kg = o.kinetic_energies(), g.kinetic_energies()
     assert np.allclose(kg, ko, rtol=tol["ke"], atol=0)
 

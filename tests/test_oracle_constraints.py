@@ -111,17 +111,18 @@ def test_one_step_equals_the_projection(oracle, n_atoms, cons):
     assert o.constraint_failures() == 0
 
 
-def triangle_runs(oracle, dt, t_total=0.02):
+def triangle_runs(oracle, dt, t_total=0.02, flags=None):
     """One 3-atom molecule in a tether field: as a rigid body (NO-SQUISH) and as
-    three constrained free atoms."""
+    three constrained free atoms.  Returns the largest difference of the final
+    positions and of the final atom velocities between the two."""
     spec = water_box(1, 0.5, seed=4)
     x = spec.positions
     k = np.full(3, 300.0)
     r0 = x + np.array([[0.02, -0.01, 0.015], [-0.01, 0.02, 0.0], [0.005, 0.0, -0.02]])
-    out = []
+    pos, vel = [], []
     for rigid in (True, False):
         bi = np.ones(3, np.int32) if rigid else np.zeros(3, np.int32)
-        o = oracle.OracleSim(spec.masses, bi, dt, 200)
+        o = oracle.OracleSim(spec.masses, bi, dt, 200, **({} if flags is None else {"flags": flags}))
         o.set_constraint_tolerance(1e-13)
         o.set_tether(k, r0)
         if not rigid:
@@ -130,14 +131,27 @@ def triangle_runs(oracle, dt, t_total=0.02):
         o.set_positions(x)
         o.set_velocities(spec.velocities)
         o.step(int(round(t_total / dt)))
-        out.append(o.positions())
-    return np.abs(out[0] - out[1]).max()
+        pos.append(o.positions())
+        vel.append(o.velocities())
+    return np.abs(pos[0] - pos[1]).max(), np.abs(vel[0] - vel[1]).max()
 
 
 def test_constrained_triangle_follows_the_rigid_body(oracle):
-    e = [triangle_runs(oracle, dt) for dt in (0.002, 0.001, 0.0005)]
-    assert e[0] > 1e-9
-    assert 3.0 < e[0] / e[1] < 5.0 and 3.0 < e[1] / e[2] < 5.0, e
+    """Positions and atom velocities (the body atoms' rebuilt v_cm + w x delta
+    against RATTLE's free-atom velocities) converge at O(dt^2)."""
+    runs = [triangle_runs(oracle, dt) for dt in (0.002, 0.001, 0.0005)]
+    for e in zip(*runs):
+        assert e[0] > 1e-9
+        assert 3.0 < e[0] / e[1] < 5.0 and 3.0 < e[1] / e[2] < 5.0, e
+
+
+def test_constrained_triangle_catches_a_planted_velocity_sign(oracle):
+    """The same comparison with the body-atom velocities rebuilt as delta x w
+    (planted in the oracle) does not converge."""
+    ex, ev = triangle_runs(oracle, 0.0005)
+    px, pv = triangle_runs(oracle, 0.0005, flags=oracle.FIX_ALL | oracle.PLANT_VELOCITY_SIGN)
+    assert px == ex          # the positions never read the atom velocities
+    assert pv > 1e3 * ev and pv > 0.1
 
 
 def reference_style(oracle, masses, positions, cons, dist_tol, seed):
