@@ -37,30 +37,60 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kernel, n_atoms_per_body, num_bodies, num_free):
-    """Algorithmic bytes per launch, mixed precision (fp32 state, FP64 centre
-    of mass), the next step's first half-kick folded into part2:
-      part1: per body reads q (16), L, p (24), 1/I, 1/M (16), r_cm (24), d (12N);
-             writes q, L (28), r_cm (24), positions (12N): 132 + 24N;
-             per free atom reads v, F, 1/m, r hi/lo (52), writes v, r hi/lo (36): 88
-      part2: per body reads q, L, p, 1/I, 1/M (56), d (12N), forces (12N);
-             writes L, p (24), velocities (12N): 80 + 36N;
-             per free atom reads v, F, 1/m (28), writes v (12): 40
-      fused (part2 of step n + part1 of step n+1): per body reads q, L, p,
-             1/I, 1/M, r_cm (80), d (12N), forces (12N); writes q, L, p, r_cm
-             (64), positions (12N): 144 + 36N; per free atom 88
-      forces (tether): per atom reads position (12, +12 low part for free
-             atoms), r0 hi/lo (24), writes the force (12): 48
-    The fused pass moves 252 B per 3-site body, the two-pass step 392 B.
-    """
+def algorithmic_bytes(kernel, n_atoms_per_body, num_bodies, num_free, indexed=False):
+    """Algorithmic bytes per launch, counted as the reference's canonical
+    drop-in step moves them: fp32 4-byte I/O, the quaternion momentum pi as
+    4 values, the next step's first half-kick folded into part 2.  Per body
+    of N atoms:
+      part1  reads r, p, q, pi (56), 1/M, 1/I (16), d (12N); writes r, q, pi
+             (44), positions (12N): 116 + 24N
+      part2  reads p, q, pi (44), 1/M, 1/I (16), d (12N), forces (12N);
+             writes p, pi (28), velocities (12N): 88 + 36N
+      (part1 + part2 = 204 + 60N = 384 B for a 3-site body)
+      fused  (part2 of step n + part1 of step n+1) reads r, p, q, pi, 1/M,
+             1/I (72), d (12N), forces (12N); writes r, p, q, pi (56),
+             positions (12N): 128 + 36N = 236 B for a 3-site body
+    Per free atom: part1 reads r, v, F, 1/m (40), writes r, v (24): 64;
+    part2 reads v, F, 1/m (28), writes v (12): 40; fused 64.
+    forces (the benchmark tether) read r and r0, write F: 36 B per atom.
+    Atoms not stored body by body (`indexed`) add an int32 index read per
+    atom and pass (+4 B).  What the implementation moves beyond these bytes
+    (the FP64 centre of mass, the free atoms' low position parts) is
+    precision overhead and is never counted here."""
+    N = n_atoms_per_body
+    idx = 4 if indexed else 0
+    atoms = num_bodies * N + num_free
+    if kernel == "part1":
+        b = num_bodies * (116 + 24 * N) + num_free * 64
+    elif kernel == "part2":
+        b = num_bodies * (88 + 36 * N) + num_free * 40
+    elif kernel == "fused":
+        b = num_bodies * (128 + 36 * N) + num_free * 64
+    elif kernel == "forces":
+        return atoms * 36
+    else:
+        raise ValueError(kernel)
+    return b + idx * atoms
+
+
+def implementation_bytes(kernel, n_atoms_per_body, num_bodies, num_free):
+    """Bytes the mixed-precision kernels move by design: body state as q (16),
+    body-frame L and p (12 each), 1/I and 1/M (16), r_cm in FP64 (24); free
+    atoms' positions as fp32 hi + lo (24); the tether's anchor in fp32.
+      part1: 132 + 24N per body, 88 per free atom
+      part2: 80 + 36N per body, 40 per free atom
+      fused: 144 + 36N per body, 88 per free atom
+      forces: 36 per atom, + 12 per free atom (low position part)"""
     N = n_atoms_per_body
     if kernel == "part1":
         return num_bodies * (132 + 24 * N) + num_free * 88
+    if kernel == "part2":
+        return num_bodies * (80 + 36 * N) + num_free * 40
     if kernel == "fused":
         return num_bodies * (144 + 36 * N) + num_free * 88
     if kernel == "forces":
-        return (num_bodies * N + num_free) * 48 + num_free * 12
-    return num_bodies * (80 + 36 * N) + num_free * 40
+        return (num_bodies * N + num_free) * 36 + num_free * 12
+    raise ValueError(kernel)
 
 
 def make_workload(name):
@@ -81,11 +111,40 @@ def make_workload(name):
     raise SystemExit(f"unknown config {name}")
 
 
+def native_oracle():
+    """Path of the oracle built for this host's CPU (-O3 -march=native,
+    oracle/Makefile `native`), or the portable build if that fails."""
+    import subprocess
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native"], capture_output=True, text=True)
+    path = os.path.join(ROOT, "oracle", "_native", "librbref_native.so")
+    if r.returncode == 0 and os.path.exists(path):
+        return path, "g++ -O3 -march=native -ffp-contract=off"
+    log(f"native oracle build failed, timing the portable build: {r.stderr[-300:]}")
+    return os.path.join(ROOT, "oracle", "librbref.so"), "g++ -O3 -ffp-contract=off"
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(spec, budget_s=15.0, sample_bodies=100_000):
-    """The FP64 CPU restatement (single thread) on the first `sample_bodies`
-    bodies of the same workload; steps chosen so the run takes ~budget_s."""
+    """The FP64 CPU restatement of the reference platform (single thread, as
+    the reference loops over bodies serially), built for this host: the
+    first `sample_bodies` bodies of the headline workload for ~budget_s
+    (value, body-steps/s), and the reference's own small case (512 waters +
+    64 argon atoms) for all 1000 steps in both rotation modes."""
+    lib_path, compiler = native_oracle()
+    os.environ["RBREF_LIB"] = lib_path
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
+    from openmm_rigidbody_plugin_amd import systems
     from openmm_rigidbody_plugin_amd.systems import SystemSpec
     bi = spec.body_indices
     keep = np.nonzero((bi > 0) & (bi <= sample_bodies))[0]
@@ -101,21 +160,29 @@ def cpu_baseline(spec, budget_s=15.0, sample_bodies=100_000):
     t = time.perf_counter()
     o.step(steps)
     el = time.perf_counter() - t
-    cpu = platform.processor() or ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
+    small = {}
+    for mode in (10, 0):
+        c1 = systems.water_c1(rotation_mode=mode)
+        oc = pyoracle.OracleSim.from_spec(c1)
+        oc.step(10)
+        t1 = time.perf_counter()
+        oc.step(1000)
+        s1 = time.perf_counter() - t1
+        small["nosquish10" if mode else "exact"] = {
+            "steps": 1000, "seconds": round(s1, 3), "body_steps_per_s": 512 * 1000 / s1,
+            "atom_steps_per_s_free": 64 * 1000 / s1}
+    cpu = cpu_model()
     return {"value": nb * steps / el, "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": f"first {nb} bodies of the same workload, {steps} steps, FP64 oracle "
-                      f"(single thread, g++ -O3) on {cpu}", "seconds": round(el, 2)}
+            "sample": f"first {nb} bodies of the same workload, {steps} steps, FP64 oracle (single thread, "
+                      f"{compiler}) on {cpu}", "seconds": round(el, 2), "cpu_model": cpu,
+            "nproc": os.cpu_count(), "compiler": compiler,
+            "reference_small_case": {"workload": "512 water bodies + 64 argon atoms, tether k=1000, dt 2 fs, FP64",
+                                     **small}}
 
 
 def valu_from_profiles(config, kernel):
+    """VALU issue fraction recorded in profiles/ (rocprofv3 counters of an
+    earlier run of this kernel; not measured by this run)."""
     path = os.path.join(ROOT, "profiles", f"valu_{kernel}_{config}.json")
     if not os.path.exists(path):
         return None
@@ -124,6 +191,9 @@ def valu_from_profiles(config, kernel):
 
 
 def traffic_from_profiles(config, kernel):
+    """HBM bytes per launch recorded in profiles/ (two rocprofv3 --pmc runs,
+    FETCH_SIZE and WRITE_SIZE scaled by a copy kernel's; not measured by this
+    run)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{config}.json")
     if not os.path.exists(path):
         return None
@@ -287,6 +357,8 @@ def served(num_jobs=96, rate=25.0, budget=1 << 30, steps_per_round=250, seed=11)
     log(f"  serial plain, step(K): {serial:.2f}s")
     same = all(np.array_equal(x, lat_s[i]["positions"]) and np.array_equal(x, lat[i]["positions"])
                for (i, _), x in alone.items())
+    # the jobs with the fewest steps, kept for the FP64 check of the CPU leg
+    check = [(specs[int(i)], int(steps[int(i)]), lat_s[int(i)]["positions"], int(i)) for i in np.argsort(steps)[:3]]
     return {
         "workload": f"{num_jobs} jobs, bodies LogUniform(500, 1e5) (30% 2-atom, 40% 3-site, 20% 6-ring, "
                     "10% 20-atom) + 2.5 free atoms per body, steps LogUniform(100, 1e4), exact rotation, "
@@ -302,7 +374,103 @@ def served(num_jobs=96, rate=25.0, budget=1 << 30, steps_per_round=250, seed=11)
         "saturated_speedup_vs_serial_plain_per_step": serial_ps / makespan_s,
         "saturated_speedup_vs_serial_plain_step_k": serial / makespan_s,
         "sampled_jobs_equal_alone": bool(same), "sampled_jobs": sorted({i for i, _ in alone}),
+        "_fp64_check_jobs": check,
     }
+
+
+def served_fp64_check(check, bodies=64, free=64, tol_nm=1e-5):
+    """Per-job agreement of served results with the FP64 oracle on a sample
+    (part of the CPU leg): for each (spec, steps, served positions, job) in
+    `check`, `bodies` bodies and `free` free atoms stepped alone by the
+    oracle (every force is a per-atom tether, so a sample evolves as it
+    does in the whole job), compared atom by atom with the served result."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import pyoracle
+    from subsample import sample_atoms, subsystem
+    out = []
+    for spec, steps, positions, job in check:
+        atoms = sample_atoms(spec, bodies, free, seed=job)
+        o = pyoracle.OracleSim.from_spec(subsystem(spec, atoms))
+        o.step(steps)
+        dx = float(np.abs(o.positions() - positions[atoms]).max())
+        out.append({"job": job, "steps": steps, "atoms": int(atoms.size), "max_dx_nm": dx})
+        log(f"  fp64 check: job {job}, {steps} steps, {atoms.size} atoms, max |dx| {dx:.2e} nm")
+    return {"tolerance_nm": tol_nm, "jobs": out, "ok": all(j["max_dx_nm"] <= tol_nm for j in out)}
+
+
+def kernel_roofline(kms, natoms_body, nb, nf, config):
+    """The HBM roofline of the dominant kernel of step(K) (the fused pass)
+    from its HIP-event duration and its algorithmic bytes; the two-pass step
+    (part1 + part2, what one execute() with forces computed outside the
+    library runs) and the full step with the tether beside it."""
+    alg = {k: algorithmic_bytes(k, natoms_body, nb, nf) for k in ("part1", "part2", "fused", "forces")}
+    impl = {k: implementation_bytes(k, natoms_body, nb, nf) for k in ("part1", "part2", "fused", "forces")}
+    achieved = {k: alg[k] / (kms[k] * 1e-3) / 1e9 for k in alg}
+    dominant = "fused"
+    two_ms = kms["part1"] + kms["part2"]
+    two_alg = alg["part1"] + alg["part2"]
+    return {
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved[dominant],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved[dominant] / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(config, dominant),
+            "traffic_source": "recorded: profiles/traffic_fused_" + config + ".json (rocprofv3 --pmc, not this run)",
+            "algorithmic_bytes_per_launch": alg[dominant],
+            "implementation_bytes_per_launch": impl[dominant],
+            "precision_overhead": impl[dominant] / alg[dominant],
+            "avg_launch_ms": kms[dominant],
+            "valu_issue_frac_recorded": valu_from_profiles(config, dominant),
+        },
+        "integrator": {
+            "note": "step(K) runs one fused pass and one force evaluation per step; the two-pass step "
+                    "(part1 + part2, one execute() with the forces computed outside the library) beside it",
+            "fused_ms_per_step": kms["fused"],
+            "fused_body_steps_per_s": nb / (kms["fused"] * 1e-3),
+            "two_pass_ms_per_step": two_ms,
+            "two_pass_algorithmic_bytes": two_alg,
+            "two_pass_frac_hbm": two_alg / (two_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "two_pass_body_steps_per_s": nb / (two_ms * 1e-3),
+            "forces_ms": kms["forces"],
+            "forces_frac_hbm": achieved["forces"] / HBM_PEAK_GBS,
+            "kernel_achieved_GBs": achieved,
+            "kernel_algorithmic_bytes": alg,
+            "kernel_implementation_bytes": impl,
+        },
+    }
+
+
+def hetero_leg(steps, warmup):
+    """The heterogeneous configuration timed by the same clock: 200k bodies
+    (30% 2-atom rotors, 40% 3-site, 20% 6-rings, 10% 20-atom clusters, random
+    masses) + 500k free atoms, atom order permuted, exact rotation, dt 1 fs,
+    mixed precision."""
+    from openmm_rigidbody_plugin_amd import rbhip
+    spec, cfg = make_workload("hetero200k")
+    ctx = rbhip.Context.from_spec(spec, precision="mixed")
+    counts = ctx.counts()
+    nb, nf = counts["numBodies"], counts["numFree"]
+    ctx.autotune(reps=3)
+    ctx.step(warmup)
+    rbhip.synchronize()
+    t = time.perf_counter()
+    ctx.step(steps)
+    rbhip.synchronize()
+    el = time.perf_counter() - t
+    kms, _ = ctx.time_kernels(100)
+    roof = kernel_roofline(kms, counts["numBodyAtoms"] / max(nb, 1), nb, nf, "hetero200k")
+    ke = ctx.kinetic_energies()
+    assert np.all(np.isfinite(ke)), "non-finite kinetic energy"
+    log(f"  hetero200k: {el / steps * 1e3:.4f} ms/step, fused {kms['fused']:.4f} ms, "
+        f"frac {roof['roofline']['frac']:.3f}")
+    return {"workload": cfg["workload"], "bodies": nb, "free_atoms": nf, "steps": steps,
+            "ms_per_step": el / steps * 1e3, "value": nb * steps / el, "unit": "body-steps/s",
+            "free_atom_steps_per_s": nf * steps / el, "launch": ctx.describe(), "kernel_ms": kms,
+            "roofline": roof["roofline"], "integrator": roof["integrator"]}
 
 
 def main():
@@ -317,6 +485,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica batched measurement")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true", help="skip the served job-stream measurement")
+    ap.add_argument("--no-hetero", action="store_true",
+                    help="skip the heterogeneous 200k-body exact-rotation measurement")
+    ap.add_argument("--hetero-steps", type=int, default=1000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only stream MIB MiB through the plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -359,14 +530,7 @@ def main():
     kms, span = ctx.time_kernels(args.kernel_steps)
     log("kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in kms.items()))
     natoms_body = counts["numBodyAtoms"] / max(nb, 1)
-    achieved = {}
-    for k in ("part1", "part2", "fused", "forces"):
-        achieved[k] = algorithmic_bytes(k, natoms_body, nb, nf) / (kms[k] * 1e-3) / 1e9
-    # the step(K) path runs one fused pass per step: the dominant kernel
-    dominant = "fused"
-    ach = achieved[dominant]
-    two_pass_bytes = (algorithmic_bytes("part1", natoms_body, nb, nf)
-                      + algorithmic_bytes("part2", natoms_body, nb, nf))
+    roof = kernel_roofline(kms, natoms_body, nb, nf, args.config)
     ke = ctx.kinetic_energies()
     assert np.all(np.isfinite(ke)), "non-finite kinetic energy"
 
@@ -384,40 +548,30 @@ def main():
         "dtype": "f32 state, f64 accumulation (mixed)",
         "data": "synthetic (seeded lattice, Maxwell-Boltzmann 300 K, harmonic tethers)",
         "config": {**cfg, "parallelism": "single GPU", "launch": ctx.describe()},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dominant,
-            "achieved": ach,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": ach / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(args.config, dominant),
-            "valu_issue_frac": valu_from_profiles(args.config, dominant),
-            "algorithmic_bytes_per_launch": algorithmic_bytes(dominant, natoms_body, nb, nf),
-            "avg_launch_ms": kms[dominant],
-        },
+        "roofline": roof["roofline"],
         "kernel_ms": kms,
-        "integrator": {
-            "note": "step(K) runs one fused pass and one force evaluation per step; the two-pass "
-                    "step (part1 + part2 launches, as RigidBodyIntegrator::step(1)) is shown beside it",
-            "fused_ms_per_step": kms["fused"],
-            "fused_body_steps_per_s": nb / (kms["fused"] * 1e-3),
-            "two_pass_ms_per_step": kms["part1"] + kms["part2"],
-            "two_pass_frac_hbm": two_pass_bytes / ((kms["part1"] + kms["part2"]) * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "kernel_achieved_GBs": achieved,
-        },
+        "integrator": roof["integrator"],
     }
-    if not args.no_batch and args.config == "water1M":
+    if args.config == "water1M" and not args.no_hetero:
         del ctx
+        log("heterogeneous 200k bodies + 500k free atoms, exact rotation ...")
+        result["hetero200k"] = hetero_leg(args.hetero_steps, args.warmup)
+    if not args.no_batch and args.config == "water1M":
+        ctx = None
         log("batched replicas ...")
         result["batched"] = batched(args.batch_steps, args.warmup)
     if not args.no_serve and args.config == "water1M":
         log("served job stream ...")
         result["served"] = served()
+    check = result.get("served", {}).pop("_fp64_check_jobs", None)
     if not args.no_cpu_baseline:
+        # the CPU leg: the oracle timed on this host, and the served jobs'
+        # FP64 agreement on a sample
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(spec)
         log(f"cpu baseline: {result['cpu_baseline']}")
+        if check:
+            result["served"]["fp64_check"] = served_fp64_check(check)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

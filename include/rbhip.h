@@ -139,6 +139,21 @@ int rbhip_get_counts(rbhip_context* ctx, int* out5);
 int rbhip_get_atom_index(rbhip_context* ctx, int* out);
 /* cleanBodyIndices result (RigidBodySystem.cpp:28-49); out has numAtoms ints */
 int rbhip_get_body_index(rbhip_context* ctx, int* out);
+/* The same bookkeeping on the device, for labels and masses already in
+   device memory: cleanBodyIndices (RigidBodySystem.cpp:28-49) and the atom
+   order and body layout of RigidBodySystem::initialize (:55-105).  Inputs
+   bodyIndices[numAtoms] and masses[numAtoms] and outputs are device arrays:
+   bodyIndexOut[numAtoms] (0 free, 1..B by ascending label); atomIndexOut[F +
+   body atoms] (free atoms ascending, then body 1's atoms ascending, body
+   2's, ...); bodyAtomStartOut[B + 1] (room for numAtoms + 1), body b's first
+   atom among the body atoms (its loc) and the body-atom count last.
+   counts4 (host) = numFree, numBodies, numMassless, numBodyAtoms.  Massless
+   particles are neither free nor body atoms (RigidBodySystem.cpp:66-76) and
+   are refused with a body label, as rbhip_create refuses them.  Work is
+   queued on `stream` (NULL: a library stream of `device`), which the call
+   waits for; its scratch is allocated and freed in stream order. */
+int rbhip_body_bookkeeping(int numAtoms, const int* bodyIndices, const double* masses, int* bodyIndexOut,
+                           int* atomIndexOut, int* bodyAtomStartOut, int* counts4, int device, rbhip_stream stream);
 
 /* ---------------------------------------------------------------------- */
 /* Force field slot (ContextImpl::calcForcesAndEnergy,                      */
@@ -151,6 +166,54 @@ int rbhip_add_bonds(rbhip_context* ctx, int count, const int* atomPairs, const d
                     const double* k);
 /* evaluates forces (and, if energy != NULL, the potential energy) */
 int rbhip_compute_forces(rbhip_context* ctx, double* energy);
+/* Forces computed outside the library (FP64, xyz per atom, original order),
+   as the context's force evaluation leaves them for integratePart2
+   (ReferenceRigidBodyKernels.cpp:101-102): a caller with its own force field
+   steps with rbhip_part1, then its forces at the new positions through this
+   call, then rbhip_part2.  Set after a state change or rbhip_part2 (when the
+   bodies' momenta already carry the next step's first half-kick, made with
+   the forces held before), the half-kick is redone with the new forces.
+   Forces on virtual sites move to their parents.  rbhip_set_positions and
+   rbhip_execute evaluate the library's own force field and replace them.
+   Not for contexts with adopted arrays (their forces are in the array). */
+int rbhip_set_forces(rbhip_context* ctx, const double* xyz);
+
+/* ---------------------------------------------------------------------- */
+/* Adopted atom arrays (OpenMM's GPU layout)                               */
+/* ---------------------------------------------------------------------- */
+/* The context keeps its atoms in a caller's device arrays in the layout of
+   OpenMM's CUDA/HIP platforms, which the reference's CUDA kernels step in
+   place (CudaRigidBodyKernels.cpp:377-444, rigidbodyintegrator.cu:270-396):
+     posq            Real4[paddedNumAtoms]: x, y, z, charge (float4 in mixed
+                     precision, double4 in double)
+     posqCorrection  float4[paddedNumAtoms], low parts of x, y, z (mixed
+                     precision; NULL in double)
+     velm            double4[paddedNumAtoms]: vx, vy, vz, 1/m
+     force           long long[3 * paddedNumAtoms], component-major, fixed
+                     point with 2^32 per kJ/mol/nm
+   atomOrder[s] (orderLength entries, a permutation of 0 .. orderLength-1)
+   is the original atom held at slot s: OpenMM's getAtomIndex().  The step
+   kernels read and write the arrays through the atom -> slot map (the
+   reference's atomLocation, CudaRigidBodyKernels.cpp:258-265); charges and
+   inverse masses (w) are never written.  Afterwards the state is read from
+   the arrays at rbhip_state_changed; rbhip_part1 / rbhip_part2 step them,
+   with the caller's force evaluation in between, and rbhip_execute steps
+   them with the library's force field.  The get/set calls reach the arrays.
+   Not with constraints, virtual sites, refined energies or batches.  The
+   arrays must outlive the context or a later rbhip_destroy. */
+int rbhip_adopt_arrays(rbhip_context* ctx, void* posq, void* posqCorrection, void* velm, long long* force,
+                       int paddedNumAtoms, const int* atomOrder, int orderLength);
+/* the arrays' atom order changed (ReorderListener::execute,
+   CudaRigidBodyKernels.cpp:69-113): positions and velocities were permuted
+   by their owner; the forces of the actual atoms move to the new slots
+   here, as the reference's listener moves them */
+int rbhip_set_atom_order(rbhip_context* ctx, const int* atomOrder, int orderLength);
+/* RigidBodyIntegrator::stateChanged (RigidBodyIntegrator.cpp:63-74) for
+   adopted arrays: after the caller wrote positions (and evaluated the
+   forces there) or velocities into them, rebuilds the body state from the
+   arrays (RigidBodySystem::update, RigidBodySystem.cpp:120-142) */
+enum { RBHIP_STATE_POSITIONS = 1, RBHIP_STATE_VELOCITIES = 2 };
+int rbhip_state_changed(rbhip_context* ctx, int what);
 
 /* ---------------------------------------------------------------------- */
 /* Context state                                                           */

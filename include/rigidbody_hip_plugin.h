@@ -46,6 +46,19 @@ int rbplugin_create_ex(int numAtoms, const double* masses, const int* bodyIndice
                        int rotationMode, int computeRefinedEnergies, int precision, int device, int numBonds,
                        const int* bondPairs, const double* bondLength, const double* bondK, const double* tetherK,
                        const double* tetherR0, rbplugin_sim** out);
+/* A force field evaluated on the host (rbhost::HostForce): at each force
+   evaluation the callback receives the positions of all atoms (xyz, FP64,
+   original order), adds its forces to `forces` (zero on entry) and returns
+   its potential energy.  It is called between part 1 and part 2 of every
+   step (RigidBodyIntegrator::step then runs one execute() per step,
+   ReferenceRigidBodyKernels.cpp:98-104) and at state changes. */
+typedef double (*rbplugin_force_fn)(void* user, int numAtoms, const double* positions, double* forces);
+/* as rbplugin_create_ex, with hostForce (may be NULL) added to the System */
+int rbplugin_create_with_force(int numAtoms, const double* masses, const int* bodyIndices, double stepSize,
+                               int rotationMode, int computeRefinedEnergies, int precision, int device,
+                               int numBonds, const int* bondPairs, const double* bondLength, const double* bondK,
+                               const double* tetherK, const double* tetherR0, rbplugin_force_fn hostForce,
+                               void* user, rbplugin_sim** out);
 int rbplugin_destroy(rbplugin_sim* sim);
 /* Context::setPositions / setVelocities (-> RigidBodyIntegrator::stateChanged) */
 int rbplugin_set_positions(rbplugin_sim* sim, const double* xyz);

@@ -44,6 +44,8 @@ UNITS = [
     # FP64 body-state build: unfused operations, in the reference's order
     ("rbhip_setup.hip", ["-ffp-contract=off"]),
     ("rbhip_api.hip", []),
+    # device bookkeeping (rocPRIM sort and scan)
+    ("rbhip_bookkeeping.hip", []),
     # job server: host code over the C ABI
     ("rbhip_server.cpp", []),
 ]

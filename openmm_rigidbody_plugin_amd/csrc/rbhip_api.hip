@@ -25,6 +25,8 @@
 #include <vector>
 
 #include "rbhip.h"
+#include "rbhip_bookkeeping.h"
+#include "rbhip_graph_cache.h"
 #include "rbhip_internal.h"
 #include "rbhip_kernels.h"
 
@@ -172,31 +174,41 @@ int pow2_at_least(int v) {
     return p;
 }
 
-struct GraphCache {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    void reset() {
+// HIP handles for KeyedGraph: a graph is destroyed only after the stream it
+// was launched on has drained
+struct HipGraphOps {
+    void wait(const void* stream) { (void)hipStreamSynchronize(static_cast<hipStream_t>(const_cast<void*>(stream))); }
+    void destroy(hipGraphExec_t exec, hipGraph_t graph) {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
     }
 };
+using GraphCache = KeyedGraph<hipGraphExec_t, hipGraph_t>;
 
-// captures the launches `launches` makes on `stream` into g, once
-template <class F> void ensure_graph(GraphCache& g, hipStream_t stream, F&& launches) {
-    if (g.exec) return;
-    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    try {
-        launches();
-    } catch (...) {
-        hipGraph_t tmp = nullptr;
-        (void)hipStreamEndCapture(stream, &tmp);
-        if (tmp) (void)hipGraphDestroy(tmp);
-        throw;
-    }
-    HIPCHK(hipStreamEndCapture(stream, &g.graph));
-    HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+// the graph of the launches `launches` makes on key.stream, captured once
+// per key
+template <class F> hipGraphExec_t ensure_graph(GraphCache& g, const GraphKey& key, F&& launches) {
+    HipGraphOps ops;
+    hipStream_t stream = static_cast<hipStream_t>(const_cast<void*>(key.stream));
+    return g.get(key, ops, [&](hipGraphExec_t& exec, hipGraph_t& graph) {
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            launches();
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(stream, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(stream, &graph));
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            graph = nullptr;
+            exec = nullptr;
+            throw RbError(RBHIP_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    });
 }
 
 // The repeated launch sequences of a call, as HIP graphs: the two-pass step
@@ -206,13 +218,16 @@ template <class F> void ensure_graph(GraphCache& g, hipStream_t stream, F&& laun
 // the chunk.
 struct StepGraphs {
     static constexpr int kFusedChunk = 16;
+    enum Kind { TWO_PASS = 1, FUSED = 2, CHUNK = 3 };
     GraphCache twoPass, fused, chunk;
+    // drops every graph (each after its stream has drained)
     void reset() {
-        twoPass.reset();
-        fused.reset();
-        chunk.reset();
+        HipGraphOps ops;
+        twoPass.drop(ops);
+        fused.drop(ops);
+        chunk.drop(ops);
     }
-    bool any() const { return twoPass.exec || fused.exec || chunk.exec; }
+    bool any() const { return twoPass.valid() || fused.valid() || chunk.valid(); }
 };
 
 // RigidBodyIntegrator::step(steps) on a context or a batch (`o` provides the
@@ -229,21 +244,21 @@ template <class O> void run_steps(O& o, int steps, bool refine) {
         int interior = steps - 1 - tail;
         if (o.useGraph) {
             if (interior >= StepGraphs::kFusedChunk) {
-                ensure_graph(o.graphs.chunk, o.stream, [&] {
+                hipGraphExec_t g = ensure_graph(o.graphs.chunk, o.graphKey(StepGraphs::CHUNK), [&] {
                     for (int i = 0; i < StepGraphs::kFusedChunk; i++) {
                         o.launchFused();
                         o.launchForces();
                     }
                 });
                 for (; interior >= StepGraphs::kFusedChunk; interior -= StepGraphs::kFusedChunk)
-                    HIPCHK(hipGraphLaunch(o.graphs.chunk.exec, o.stream));
+                    HIPCHK(hipGraphLaunch(g, o.stream));
             }
             if (interior > 0) {
-                ensure_graph(o.graphs.fused, o.stream, [&] {
+                hipGraphExec_t g = ensure_graph(o.graphs.fused, o.graphKey(StepGraphs::FUSED), [&] {
                     o.launchFused();
                     o.launchForces();
                 });
-                for (; interior > 0; interior--) HIPCHK(hipGraphLaunch(o.graphs.fused.exec, o.stream));
+                for (; interior > 0; interior--) HIPCHK(hipGraphLaunch(g, o.stream));
             }
         } else {
             for (; interior > 0; interior--) {
@@ -259,12 +274,12 @@ template <class O> void run_steps(O& o, int steps, bool refine) {
         o.launchPart2();
     } else {
         if (o.useGraph) {
-            ensure_graph(o.graphs.twoPass, o.stream, [&] {
+            hipGraphExec_t g = ensure_graph(o.graphs.twoPass, o.graphKey(StepGraphs::TWO_PASS), [&] {
                 o.launchPart1();
                 o.launchForces();
                 o.launchPart2();
             });
-            for (int s = 0; s < steps - tail; s++) HIPCHK(hipGraphLaunch(o.graphs.twoPass.exec, o.stream));
+            for (int s = 0; s < steps - tail; s++) HIPCHK(hipGraphLaunch(g, o.stream));
         } else {
             for (int s = 0; s < steps - tail; s++) {
                 o.launchPart1();
@@ -347,7 +362,7 @@ static size_t context_arena_bytes(size_t na, int numBodies, int numFree, int num
     b += a(3 * R * atomStride) + a(3 * R * atomStride);                            // vel, force
     b += a(R * std::max(numFree, 1)) + a(sizeof(double) * na);                    // freeInvM, mass
     b += a(3 * sizeof(double) * na) * 2;                                           // stageR, stageV
-    b += a(R * na) + a(3 * R * atomStride) + (mixed ? a(3 * sizeof(float) * atomStride) : 0);  // tether
+    b += a(R * na) + a(3 * R * atomStride);                                       // tether k, r0
     b += a(4 * sizeof(double) * maxKe) + a(4 * sizeof(double) * maxPe);            // partials
     b += a(sizeof(DevSys)) + a(4 * sizeof(int)) + a(sizeof(int) * nb) * 2;
     b += a(sizeof(double) * std::max(3 * na, 36 * nb)) + a(4 * sizeof(double));  // staging, reduction
@@ -429,6 +444,16 @@ struct rbhip_context {
     bool built = false;
     bool stageVFresh = false;
     bool positionsDerived = false;               // body-atom positions hold fp32 r_cm + A(q)^T d only
+    // the stored body momenta carry the next step's first half-kick, made
+    // with the forces in the force array (after the set-up and part 2; not
+    // between part 1 and part 2)
+    bool kickFolded = false;
+    // atom state adopted from a caller in OpenMM's layout (h.omm*): two slot
+    // maps, the live one and the next (a new atom order is moved to it)
+    bool adopted = false;
+    int* dSlot[2] = {nullptr, nullptr};
+    int liveSlot = 0;
+    DeviceArena adoptArena;
     bool useGraph = true;
     bool useFusion = true;
     StepGraphs graphs;
@@ -470,6 +495,13 @@ struct rbhip_context {
             {h.pePart, 4 * sizeof(double) * (size_t)maxPeSlots}};
         if (h.posLo) spans.push_back({h.posLo, 3 * sizeof(float) * as});
         if (h.refKePart) spans.push_back({h.refKePart, 4 * sizeof(double) * (size_t)maxKeSlots});
+        if (adopted) {
+            const size_t ps = (size_t)h.ommStride;
+            spans.push_back({h.ommPosq, 4 * R * ps});
+            if (h.ommPosqCorr) spans.push_back({h.ommPosqCorr, 4 * sizeof(float) * ps});
+            spans.push_back({h.ommVelm, 4 * sizeof(double) * ps});
+            spans.push_back({h.ommForce, 3 * sizeof(long long) * ps});
+        }
         for (auto& sp : spans) {
             void* c = s.arena.alloc(sp.second);
             HIPCHK(hipMemcpyAsync(c, sp.first, sp.second, hipMemcpyDeviceToDevice, stream));
@@ -669,8 +701,27 @@ struct rbhip_context {
         HIPCHK(hipStreamSynchronize(stream));
     }
 
+    int layout() const { return adopted ? 1 : 0; }
+    // everything a captured step sequence of this context reads
+    GraphKey graphKey(int kind) const {
+        GraphKey k;
+        k.stream = stream;
+        k.systems = dSys;
+        k.partTiles = dPartTiles;
+        k.forceTiles = dForceTiles;
+        k.clusterTiles = dClusterTiles;
+        k.numPart = nPartTiles;
+        k.numForce = nForceTiles;
+        k.numClusterTiles = nClusterTiles;
+        k.numSites = numSites;
+        k.block = block;
+        k.maxBlocks = maxBlocks;
+        k.layout = layout();
+        k.kind = kind;
+        return k;
+    }
     void launchForces() {
-        HIPCHK(launch_forces(precision, dSys, dForceTiles, nForceTiles, block, maxBlocks, stream));
+        HIPCHK(launch_forces(precision, dSys, dForceTiles, nForceTiles, block, maxBlocks, stream, layout()));
         if (numSites > 0) HIPCHK(launch_site_forces(precision, dSys, numSites, numSiteParents, stream));
     }
     void launchClusters(int phase) {
@@ -681,16 +732,16 @@ struct rbhip_context {
         if (numSites > 0) HIPCHK(launch_site_positions(precision, dSys, numSites, stream));
     }
     void launchPart1() {
-        HIPCHK(launch_part1(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+        HIPCHK(launch_part1(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream, layout()));
         launchClusters(CLUSTER_PART1);
         launchSites();
     }
     void launchPart2() {
-        HIPCHK(launch_part2(precision, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+        HIPCHK(launch_part2(precision, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream, layout()));
         launchClusters(CLUSTER_PART2);
     }
     void launchFused() {
-        HIPCHK(launch_fused(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+        HIPCHK(launch_fused(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream, layout()));
         launchClusters(CLUSTER_FUSED);
         launchSites();
     }
@@ -724,6 +775,7 @@ struct rbhip_context {
         keSlots = nSetupTiles + nClusterTiles;
         kickDt = dt;
         built = true;
+        kickFolded = true;
         refinedValid = false;
         double r[4];
         reduce(0, r);
@@ -744,8 +796,8 @@ struct rbhip_context {
     }
 
     void download(int what, double* xyz) {
-        HIPCHK(launch_stage_out(precision, dSys, dTmp, numAtoms, what, stream));
-        if (what == 0 && positionsDerived && precision == RBHIP_PRECISION_MIXED)
+        HIPCHK(launch_stage_out(precision, dSys, dTmp, numAtoms, what, stream, layout()));
+        if (what == 0 && positionsDerived && precision == RBHIP_PRECISION_MIXED && !adopted)
             HIPCHK(launch_body_atoms_out(precision, dSys, dBodyN, dBodyAtom0, numBodies, dTmp, stream));
         std::vector<double> tmp(3 * (size_t)numAtoms);
         HIPCHK(hipMemcpyAsync(tmp.data(), dTmp, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -758,9 +810,62 @@ struct rbhip_context {
         }
     }
 
+    // Forces replaced outside a step: momenta that carry the next step's
+    // half-kick (made with the forces the force array held) get it again
+    // with the new ones.  beginForceChange keeps a copy of the old forces
+    // (in a buffer allocated at the first such change) when the kick is
+    // folded; endForceChange redoes the kick from the difference.
+    DeviceArena prevArena;
+    void* dForcePrev = nullptr;
+    bool beginForceChange() {
+        const bool refold = built && kickFolded && numBodies > 0;
+        if (!refold) return false;
+        fixKick();
+        adoptedForcesIn();
+        const size_t bytes = 3 * realSize() * (size_t)h.atomStride;
+        if (!dForcePrev) {
+            prevArena.stream = stream;
+            dForcePrev = prevArena.alloc(bytes);
+        }
+        HIPCHK(hipMemcpyAsync(dForcePrev, h.force, bytes, hipMemcpyDeviceToDevice, stream));
+        return true;
+    }
+    void endForceChange(bool refold) {
+        if (!refold) return;
+        adoptedForcesIn();
+        HIPCHK(launch_refold(precision, dSys, dBodyN, dBodyAtom0, numBodies, dForcePrev, stream));
+    }
+    // evaluates the library's force field outside a step
+    void refreshForces() {
+        const bool refold = beginForceChange();
+        launchForces();
+        endForceChange(refold);
+    }
+
+    // the library's force array takes the adopted forces (the body-state
+    // build, the re-kick and the body-state download read it there)
+    void adoptedForcesIn() {
+        if (adopted) HIPCHK(launch_adopted_forces_in(precision, dSys, numAtoms, stream));
+    }
+    // FP64 copies of the adopted positions and velocities, as asked, for
+    // the body-state build, and the adopted forces in the library's force
+    // array
+    void adoptedStateIn(bool positions, bool velocities) {
+        if (!adopted) return;
+        if (positions) HIPCHK(launch_stage_out(precision, dSys, h.stageR, numAtoms, 0, stream, 1));
+        if (velocities) {
+            HIPCHK(launch_stage_out(precision, dSys, h.stageV, numAtoms, 1, stream, 1));
+            stageVFresh = true;
+        }
+        adoptedForcesIn();
+    }
+
     void fixKick() {
         if (built && kickDt != dt) {
-            HIPCHK(launch_rekick(precision, dSys, dBodyN, dBodyAtom0, numBodies, dt - kickDt, stream));
+            if (kickFolded) {
+                adoptedForcesIn();
+                HIPCHK(launch_rekick(precision, dSys, dBodyN, dBodyAtom0, numBodies, dt - kickDt, stream));
+            }
             kickDt = dt;
         }
     }
@@ -779,6 +884,7 @@ struct rbhip_context {
         keSlots = partKeSlots();
         stageVFresh = false;
         positionsDerived = true;
+        kickFolded = true;
         time += dt * steps;
         stepCount += steps;
     }
@@ -791,6 +897,8 @@ struct rbhip_context {
             throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with constraints");
         if (on && numSites > 0)
             throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with virtual sites");
+        if (on && adopted)
+            throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with adopted atom arrays");
         if (on && !refBuf[0]) {
             const size_t R = realSize();
             refArena.stream = stream;
@@ -811,6 +919,8 @@ struct rbhip_context {
 
     ~rbhip_context() {
         graphs.reset();
+        adoptArena.release();
+        prevArena.release();
         tileArena.release();
         refArena.release();
         clusterArena.release();
@@ -898,17 +1008,31 @@ struct rbhip_batch {
         if (copied) HIPCHK(hipStreamSynchronize(stream));
     }
 
+    GraphKey graphKey(int kind) const {
+        GraphKey k;
+        k.stream = stream;
+        k.systems = dSystems;
+        k.partTiles = dPartTiles;
+        k.forceTiles = dForceTiles;
+        k.numPart = nPart;
+        k.numForce = nForce;
+        k.block = block;
+        k.maxBlocks = maxBlocks;
+        k.kind = kind;
+        return k;
+    }
+    // (members hold their atoms in the library's own arrays)
     void launchPart1() {
-        HIPCHK(launch_part1(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
+        HIPCHK(launch_part1(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream, 0));
     }
     void launchForces() {
-        HIPCHK(launch_forces(precision, dSystems, dForceTiles, nForce, block, maxBlocks, stream));
+        HIPCHK(launch_forces(precision, dSystems, dForceTiles, nForce, block, maxBlocks, stream, 0));
     }
     void launchPart2() {
-        HIPCHK(launch_part2(precision, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
+        HIPCHK(launch_part2(precision, dSystems, dPartTiles, nPart, block, maxBlocks, stream, 0));
     }
     void launchFused() {
-        HIPCHK(launch_fused(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
+        HIPCHK(launch_fused(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream, 0));
     }
     void launchRefineBegin(bool folded) {
         HIPCHK(launch_refine_begin(precision, rotationMode, dSystems, dPartTiles, nPart, folded ? 1 : 0, block,
@@ -934,7 +1058,10 @@ struct rbhip_batch {
             }
             c->keSlots = memberPartTiles[m];
             c->stageVFresh = false;
-            if (steps > 0) c->positionsDerived = true;
+            if (steps > 0) {
+                c->positionsDerived = true;
+                c->kickFolded = true;
+            }
             c->time += c->dt * steps;
             c->stepCount += steps;
         }
@@ -1186,7 +1313,6 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
         h.stageV = C.arena.allocT<double>(3 * na);
         h.tetherK = C.arena.alloc(R * na);
         h.r0 = C.arena.alloc(3 * R * as);
-        h.r0Lo = precision == RBHIP_PRECISION_MIXED ? C.arena.allocT<float>(3 * as) : nullptr;
         h.bondStart = nullptr;
         h.bondOther = nullptr;
         h.bondLen = nullptr;
@@ -1221,7 +1347,6 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
         HIPCHK(hipMemsetAsync(h.d, 0, 3 * R * (size_t)h.dStride, st));
         HIPCHK(hipMemsetAsync(h.tetherK, 0, R * na, st));
         HIPCHK(hipMemsetAsync(h.r0, 0, 3 * R * as, st));
-        if (h.r0Lo) HIPCHK(hipMemsetAsync(h.r0Lo, 0, 3 * sizeof(float) * as, st));
         HIPCHK(hipMemsetAsync(h.kePart, 0, 4 * sizeof(double) * C.maxKeSlots, st));
         HIPCHK(hipMemsetAsync(h.pePart, 0, 4 * sizeof(double) * C.maxPeSlots, st));
         std::vector<double> m(na);
@@ -1426,7 +1551,7 @@ int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream) {
         }
         ctx->stream = next;
         ctx->arena.stream = ctx->tileArena.stream = ctx->refArena.stream = ctx->clusterArena.stream =
-            ctx->siteArena.stream = ctx->stream;
+            ctx->siteArena.stream = ctx->adoptArena.stream = ctx->prevArena.stream = ctx->stream;
     });
 }
 
@@ -1434,7 +1559,7 @@ int rbhip_device_bytes(rbhip_context* ctx, size_t* bytes) {
     return guard([&] {
         require(ctx && bytes, "null argument");
         *bytes = ctx->arena.bytes + ctx->tileArena.bytes + ctx->refArena.bytes + ctx->clusterArena.bytes +
-                 ctx->siteArena.bytes;
+                 ctx->siteArena.bytes + ctx->adoptArena.bytes + ctx->prevArena.bytes;
     });
 }
 
@@ -1483,6 +1608,37 @@ int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, si
     });
 }
 
+// the checks of rbhip_create on the device bookkeeping's status flags
+static void bookkeeping_status(const int* small) {
+    if (small[0] & 2) throw RbError(RBHIP_E_ARG, "negative or invalid mass");
+    if (small[0] & 1) throw RbError(RBHIP_E_ARG, "massless particles cannot belong to rigid bodies");
+}
+
+int rbhip_body_bookkeeping(int numAtoms, const int* bodyIndices, const double* masses, int* bodyIndexOut,
+                           int* atomIndexOut, int* bodyAtomStartOut, int* counts4, int device, rbhip_stream stream) {
+    return guard([&] {
+        require(numAtoms > 0 && bodyIndices && masses && bodyIndexOut && atomIndexOut && bodyAtomStartOut && counts4,
+                "bad arguments");
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        require(device >= 0 && device < ndev, "no such device");
+        HIPCHK(hipSetDevice(device));
+        hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : pooled_stream(device);
+        size_t bytes = 0;
+        HIPCHK(bookkeeping_scratch_bytes(numAtoms, st, &bytes));
+        DeviceArena arena;
+        arena.stream = st;
+        void* scratch = arena.alloc(bytes);
+        int small[5] = {0, 0, 0, 0, 0};
+        hipError_t e = launch_bookkeeping(numAtoms, bodyIndices, masses, bodyIndexOut, atomIndexOut, bodyAtomStartOut,
+                                          scratch, bytes, st, small);
+        arena.release();
+        HIPCHK(e);
+        bookkeeping_status(small);
+        for (int k = 0; k < 4; k++) counts4[k] = small[k + 1];
+    });
+}
+
 int rbhip_get_counts(rbhip_context* ctx, int* out5) {
     return guard([&] {
         require(ctx && out5, "null argument");
@@ -1520,17 +1676,13 @@ int rbhip_set_tether(rbhip_context* ctx, const double* k, const double* r0) {
         size_t na = C.numAtoms, as = C.h.atomStride;
         std::vector<double> r = C.toInternal(r0);
         if (C.precision == RBHIP_PRECISION_MIXED) {
-            std::vector<float> kk(na), hi(3 * as, 0.f), lo(3 * as, 0.f);
+            // spring constants and anchors at the state precision
+            std::vector<float> kk(na), rr(3 * as, 0.f);
             for (size_t i = 0; i < na; i++) kk[i] = (float)k[C.atomOrig[i]];
             for (size_t i = 0; i < na; i++)
-                for (int c = 0; c < 3; c++) {
-                    double v = r[3 * i + c];
-                    hi[c * as + i] = (float)v;
-                    lo[c * as + i] = (float)(v - (double)hi[c * as + i]);
-                }
+                for (int c = 0; c < 3; c++) rr[c * as + i] = (float)r[3 * i + c];
             HIPCHK(hipMemcpyAsync(C.h.tetherK, kk.data(), na * 4, hipMemcpyHostToDevice, C.stream));
-            HIPCHK(hipMemcpyAsync(C.h.r0, hi.data(), 3 * as * 4, hipMemcpyHostToDevice, C.stream));
-            HIPCHK(hipMemcpyAsync(C.h.r0Lo, lo.data(), 3 * as * 4, hipMemcpyHostToDevice, C.stream));
+            HIPCHK(hipMemcpyAsync(C.h.r0, rr.data(), 3 * as * 4, hipMemcpyHostToDevice, C.stream));
             HIPCHK(hipStreamSynchronize(C.stream));
         } else {
             std::vector<double> kk(na), rr(3 * as, 0.0);
@@ -1600,12 +1752,124 @@ int rbhip_compute_forces(rbhip_context* ctx, double* energy) {
     return guard([&] {
         require(ctx, "null context");
         ctx->setDevice();
-        ctx->launchForces();
+        ctx->refreshForces();
         if (energy) {
             double r[4];
             ctx->reduce(1, r);
             *energy = r[0];
         }
+    });
+}
+
+int rbhip_set_forces(rbhip_context* ctx, const double* xyz) {
+    return guard([&] {
+        require(ctx && xyz, "null argument");
+        rbhip_context& C = *ctx;
+        require(!C.adopted, "the forces of this context live in its adopted force array", RBHIP_E_STATE);
+        C.setDevice();
+        std::vector<double> f = C.toInternal(xyz);
+        HIPCHK(hipMemcpyAsync(C.dTmp, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice, C.stream));
+        const bool refold = C.beginForceChange();
+        HIPCHK(launch_stage_in(C.precision, C.dSys, C.dTmp, C.numAtoms, 2, C.stream, 0));
+        // forces on virtual sites move to their parents, as a force
+        // evaluation leaves them
+        if (C.numSites > 0) HIPCHK(launch_site_forces(C.precision, C.dSys, C.numSites, C.numSiteParents, C.stream));
+        C.endForceChange(refold);
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+// internal atom -> slot in arrays of OpenMM order `order` (order[s] = the
+// original atom held at slot s; a permutation of 0 .. length - 1)
+static std::vector<int> slots_for_order(const rbhip_context& C, const int* order, int length) {
+    require(order != nullptr && length >= C.numAtoms, "atom order shorter than the system");
+    std::vector<int> inv(length, -1);
+    for (int s = 0; s < length; s++) {
+        int o = order[s];
+        require(o >= 0 && o < length && inv[o] < 0, "atom order is not a permutation");
+        inv[o] = s;
+    }
+    std::vector<int> slot(C.numAtoms);
+    for (int i = 0; i < C.numAtoms; i++) slot[i] = inv[C.atomOrig[i]];
+    return slot;
+}
+
+int rbhip_adopt_arrays(rbhip_context* ctx, void* posq, void* posqCorrection, void* velm, long long* force,
+                       int paddedNumAtoms, const int* atomOrder, int orderLength) {
+    return guard([&] {
+        require(ctx, "null context");
+        rbhip_context& C = *ctx;
+        require(!C.adopted, "the context already holds adopted arrays", RBHIP_E_STATE);
+        require(posq && velm && force, "null array");
+        require((C.precision == RBHIP_PRECISION_MIXED) == (posqCorrection != nullptr),
+                "posqCorrection is required in mixed precision and absent in double precision");
+        require(paddedNumAtoms >= C.numAtoms && orderLength <= paddedNumAtoms, "arrays shorter than the system");
+        require(C.numConstraints == 0, "adopted arrays are not supported with constraints", RBHIP_E_UNSUPPORTED);
+        require(C.numSites == 0, "adopted arrays are not supported with virtual sites", RBHIP_E_UNSUPPORTED);
+        require(!C.refined, "adopted arrays are not supported with refined energies", RBHIP_E_UNSUPPORTED);
+        for (void* p : {posq, posqCorrection, velm, (void*)force})
+            require(((uintptr_t)p & 15) == 0, "adopted arrays must be 16-byte aligned");
+        std::vector<int> slot = slots_for_order(C, atomOrder, orderLength);
+        C.setDevice();
+        HIPCHK(hipStreamSynchronize(C.stream));
+        C.adoptArena.stream = C.stream;
+        C.dSlot[0] = C.adoptArena.allocT<int>(C.numAtoms);
+        C.dSlot[1] = C.adoptArena.allocT<int>(C.numAtoms);
+        C.liveSlot = 0;
+        HIPCHK(hipMemcpyAsync(C.dSlot[0], slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice, C.stream));
+        C.h.ommSlot = C.dSlot[0];
+        C.h.ommPosq = posq;
+        C.h.ommPosqCorr = static_cast<float*>(posqCorrection);
+        C.h.ommVelm = static_cast<double*>(velm);
+        C.h.ommForce = force;
+        C.h.ommStride = paddedNumAtoms;
+        C.adopted = true;
+        C.built = false;                 // the state is read from the arrays at the next state change
+        C.positionsDerived = false;
+        C.graphs.reset();
+        C.uploadSys();
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int rbhip_set_atom_order(rbhip_context* ctx, const int* atomOrder, int orderLength) {
+    return guard([&] {
+        require(ctx, "null context");
+        rbhip_context& C = *ctx;
+        require(C.adopted, "the context holds no adopted arrays", RBHIP_E_STATE);
+        require(orderLength <= C.h.ommStride, "atom order longer than the arrays");
+        std::vector<int> slot = slots_for_order(C, atomOrder, orderLength);
+        C.setDevice();
+        const int next = 1 - C.liveSlot;
+        HIPCHK(hipMemcpyAsync(C.dSlot[next], slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice, C.stream));
+        // the forces stay with their atoms (ReorderListener,
+        // CudaRigidBodyKernels.cpp:69-113): the arrays' owner permutes
+        // positions and velocities, the force array is moved here
+        HIPCHK(launch_adopted_move_forces(C.dSys, reinterpret_cast<long long*>(C.dTmp), C.dSlot[next], C.numAtoms,
+                                          C.stream));
+        C.liveSlot = next;
+        C.h.ommSlot = C.dSlot[next];
+        C.uploadSys();
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int rbhip_state_changed(rbhip_context* ctx, int what) {
+    return guard([&] {
+        require(ctx, "null context");
+        rbhip_context& C = *ctx;
+        require(C.adopted, "rbhip_state_changed reads adopted arrays; use rbhip_set_positions / rbhip_set_velocities",
+                RBHIP_E_STATE);
+        require(what == RBHIP_STATE_POSITIONS || what == RBHIP_STATE_VELOCITIES, "unknown state change");
+        require(what == RBHIP_STATE_POSITIONS || C.built, "positions must be set before velocities", RBHIP_E_STATE);
+        C.setDevice();
+        C.adoptedStateIn(what == RBHIP_STATE_POSITIONS, true);
+        const int geometry = what == RBHIP_STATE_POSITIONS ? 1 : 0;
+        C.h.useStageR = geometry;
+        C.runSetup(geometry);
+        C.h.useStageR = 0;
+        C.uploadSys();
+        HIPCHK(hipStreamSynchronize(C.stream));
     });
 }
 
@@ -1616,11 +1880,13 @@ int rbhip_set_positions(rbhip_context* ctx, const double* xyz) {
         C.setDevice();
         std::vector<double> r = C.toInternal(xyz);
         HIPCHK(hipMemcpyAsync(C.h.stageR, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice, C.stream));
-        HIPCHK(launch_stage_in(C.precision, C.dSys, C.h.stageR, C.numAtoms, 0, C.stream));
+        HIPCHK(launch_stage_in(C.precision, C.dSys, C.h.stageR, C.numAtoms, 0, C.stream, C.layout()));
         // stateChanged(Positions): forces, then geometry + dynamics (RigidBodyIntegrator.cpp:65-68)
         C.h.useStageR = 1;
         C.uploadSys();
         C.launchForces();
+        // (velocities set since the last step are staged in FP64 already)
+        C.adoptedStateIn(false, !C.stageVFresh);
         C.runSetup(1);
         C.positionsDerived = false;
         C.h.useStageR = 0;
@@ -1636,10 +1902,13 @@ int rbhip_set_velocities(rbhip_context* ctx, const double* xyz) {
         C.setDevice();
         std::vector<double> v = C.toInternal(xyz);
         HIPCHK(hipMemcpyAsync(C.h.stageV, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, C.stream));
-        HIPCHK(launch_stage_in(C.precision, C.dSys, C.h.stageV, C.numAtoms, 1, C.stream));
+        HIPCHK(launch_stage_in(C.precision, C.dSys, C.h.stageV, C.numAtoms, 1, C.stream, C.layout()));
         C.stageVFresh = true;
         // stateChanged(Velocities): dynamics rebuild on the live body geometry
-        if (C.built) C.runSetup(0);
+        if (C.built) {
+            C.adoptedForcesIn();
+            C.runSetup(0);
+        }
         HIPCHK(hipStreamSynchronize(C.stream));
     });
 }
@@ -1686,6 +1955,7 @@ int rbhip_part1(rbhip_context* ctx) {
         if (ctx->refined) ctx->launchRefineBegin(true);
         ctx->launchPart1();
         ctx->positionsDerived = true;
+        ctx->kickFolded = false;
     });
 }
 
@@ -1697,6 +1967,10 @@ int rbhip_part2(rbhip_context* ctx) {
         ctx->launchPart2();
         ctx->keSlots = ctx->partKeSlots();
         ctx->stageVFresh = false;
+        ctx->kickFolded = true;
+        // the step is complete (ReferenceRigidBodyKernels.cpp:106-107)
+        ctx->time += ctx->dt;
+        ctx->stepCount++;
         if (ctx->refined) {
             ctx->launchRefineEnd();
             ctx->refKeSlots = ctx->nPartTiles;
@@ -1799,6 +2073,7 @@ int rbhip_get_body_states(rbhip_context* ctx, double* out) {
         if (C.numBodies == 0) return;
         C.setDevice();
         C.fixKick();
+        C.adoptedForcesIn();
         HIPCHK(launch_body_states(C.precision, C.dSys, C.dBodyN, C.dBodyAtom0, C.numBodies, C.dTmp, C.stream));
         std::vector<double> tmp(36 * (size_t)C.numBodies);
         HIPCHK(hipMemcpyAsync(tmp.data(), C.dTmp, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, C.stream));
@@ -1906,8 +2181,14 @@ int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* tot
         time_phases(C.stream, steps, {[&] { C.launchFused(); }, [&] { C.launchForces(); }}, fusedMs, nullptr);
         C.launchPart2();
         avgMs[RBHIP_KERNEL_FUSED] = fusedMs[0];
+        // the state advanced 2 steps + 1 as rbhip_execute leaves it (the
+        // refined-energy accumulators do not describe it)
         C.keSlots = C.partKeSlots();
         C.stageVFresh = false;
+        C.positionsDerived = true;
+        C.kickFolded = true;
+        C.refinedValid = false;
+        C.refKeSlots = 0;
         C.time += C.dt * (2 * steps + 1);
         C.stepCount += 2 * steps + 1;
     });
@@ -2057,6 +2338,7 @@ int rbhip_batch_set_members(rbhip_batch* batch, int count, rbhip_context* const*
             require(m[i]->precision == m[0]->precision, "members must share the precision");
             require(m[i]->numConstraints == 0, "members with constraints cannot be batched", RBHIP_E_UNSUPPORTED);
             require(m[i]->numSites == 0, "members with virtual sites cannot be batched", RBHIP_E_UNSUPPORTED);
+            require(!m[i]->adopted, "members with adopted atom arrays cannot be batched", RBHIP_E_UNSUPPORTED);
             require((m[i]->rotationMode == 0) == (m[0]->rotationMode == 0), "members must share the rotation kind");
             for (auto& kv : m[i]->lanes)
                 require(batch->block % kv.second == 0, "member lanes per body must divide the batch block size");

@@ -86,8 +86,7 @@ struct DevSys {
     double* stageV;
     // force field
     void* tetherK;      // Real[numAtoms]
-    void* r0;           // Real[3][atomStride]
-    float* r0Lo;        // mixed precision low part of r0, float[3][atomStride]
+    void* r0;           // Real[3][atomStride], the state precision
     int* bondStart;     // [numAtoms + 1] CSR of bonds per atom (nullptr: none)
     int* bondOther;     // partner atom of each bond entry
     double* bondLen;
@@ -128,6 +127,23 @@ struct DevSys {
     int* parentStart;
     int* parentSite;
     int* parentRole;    // 0, 1, 2: which parent of the site
+    // Atom state adopted from a caller in OpenMM's GPU layout (ommSlot ==
+    // nullptr: the library's own arrays above hold it).  Per atom slot s:
+    //   ommPosq      Real4[s] = (x, y, z, charge)         charge left alone
+    //   ommPosqCorr  float4[s] = low parts of x, y, z     mixed precision only
+    //   ommVelm      double4[s] = (vx, vy, vz, 1/m)        1/m left alone
+    //   ommForce     long long[3][ommStride], fixed point with 2^32 per unit
+    // The step kernels reach internal atom i at slot ommSlot[i].
+    int* ommSlot;
+    void* ommPosq;
+    float* ommPosqCorr;
+    double* ommVelm;
+    long long* ommForce;
+    int ommStride;
+    int pad2;
 };
+
+// OpenMM's fixed-point force unit: 2^32 per kJ/mol/nm
+constexpr double kFixedPointScale = 4294967296.0;
 
 }  // namespace rbhip

@@ -7,14 +7,15 @@
 
 namespace rbhip {
 
-// hot path (rbhip_kernels.hip)
+// hot path (rbhip_kernels.hip); `adopted` = the atoms live in adopted
+// arrays in OpenMM's layout (DevSys::ommSlot), else in the library's own
 hipError_t launch_part1(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
-                        int block, int maxBlocks, hipStream_t stream);
+                        int block, int maxBlocks, hipStream_t stream, int adopted);
 hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
-                        int maxBlocks, hipStream_t stream);
+                        int maxBlocks, hipStream_t stream, int adopted);
 // part2 of step n and part1 of step n+1 in one pass (no atom velocities, no KE)
 hipError_t launch_fused(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
-                        int block, int maxBlocks, hipStream_t stream);
+                        int block, int maxBlocks, hipStream_t stream, int adopted);
 // refined energies of the systems whose refBody is set: the start terms of
 // the next step (`folded` = the stored momenta carry its first half-kick,
 // i.e. part1 follows; 0 = a fused boundary follows), the end terms after
@@ -26,7 +27,7 @@ hipError_t launch_refine_end(int precision, int rotationMode, const DevSys* syst
 hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
                             int maxBlocks, hipStream_t stream);
 hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
-                         int maxBlocks, hipStream_t stream);
+                         int maxBlocks, hipStream_t stream, int adopted);
 // virtual sites of one system: positions from the parents; after the forces,
 // the sites' forces moved to their parents and cleared
 hipError_t launch_site_positions(int precision, const DevSys* sys, int numSites, hipStream_t stream);
@@ -52,18 +53,29 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t str
 // first half-kick of the next step
 hipError_t launch_setup(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int geometry,
                         int block, hipStream_t stream);
-// FP64 array (internal order) -> device state: what = 0 positions, 1 velocities
+// FP64 array (internal order) -> device state: what = 0 positions, 1
+// velocities, 2 forces (the library's force array; not with `adopted`)
 hipError_t launch_stage_in(int precision, const DevSys* sys, const double* in, int numAtoms, int what,
-                           hipStream_t stream);
+                           hipStream_t stream, int adopted);
 // device state -> FP64 array (internal order): what = 0 positions, 1 velocities, 2 forces
 hipError_t launch_stage_out(int precision, const DevSys* sys, double* out, int numAtoms, int what,
-                            hipStream_t stream);
+                            hipStream_t stream, int adopted);
+// adopted fixed-point forces -> the library's force array (state precision)
+hipError_t launch_adopted_forces_in(int precision, const DevSys* sys, int numAtoms, hipStream_t stream);
+// moves the adopted forces of the actual atoms from the slots of DevSys::ommSlot
+// to newSlot (tmp: 3 * numAtoms long longs)
+hipError_t launch_adopted_move_forces(const DevSys* sys, long long* tmp, const int* newSlot, int numAtoms,
+                                      hipStream_t stream);
 // body states (internal body order) as 36 FP64 values each, half-kick removed
 hipError_t launch_body_states(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
                               int numBodies, double* out, hipStream_t stream);
 // FP64 body-atom positions rebuilt from the body state into `out` (internal order)
 hipError_t launch_body_atoms_out(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
                                  int numBodies, double* out, hipStream_t stream);
+// redoes the folded half-kick for the forces now in the force array, made
+// before with the forces `prev` (a copy of the force array)
+hipError_t launch_refold(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0, int numBodies,
+                         const void* prev, hipStream_t stream);
 // re-applies the folded half-kick for a new step size: ddt = dtNew - dtOld
 hipError_t launch_rekick(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0, int numBodies,
                          double ddt, hipStream_t stream);

@@ -80,7 +80,7 @@ template <class Real> __device__ __forceinline__ Body<Real> load_body(const DevS
 }
 
 // F = sum f_j and tau = sum delta_j x f_j in FP64, delta_j = A(q)^T d_j
-template <class Real, int NC>
+template <class Real, int NC, class IO>
 __device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, int atom0, int N, int sub, int lanes,
                                           V3<double>& F, V3<double>& tau) {
     const int dbase = atom0 - S.numFree;
@@ -89,7 +89,7 @@ __device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, i
 #pragma unroll
         for (int j = 0; j < NC; j++) {
             dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
-            fj[j] = ld3<Real>(S.force, S.atomStride, atom0 + j);
+            fj[j] = IO::template force<Real>(S, atom0 + j);
         }
 #pragma unroll
         for (int j = 0; j < NC; j++) {
@@ -101,14 +101,14 @@ __device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, i
     } else {
         for (int j = sub; j < N; j += lanes) {
             V3<double> dl = cvt<double>(apply(R, ld3<Real>(S.d, S.dStride, dbase + j)));
-            V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, atom0 + j));
+            V3<double> f = cvt<double>(IO::template force<Real>(S, atom0 + j));
             F = F + f;
             tau = tau + cross(dl, f);
         }
     }
 }
 
-template <class Real>
+template <class Real, class IO = SoaAtoms>
 __device__ __forceinline__ void body_sums_any(const DevSys& S, const Rot<Real>& R, int atom0, int N, int sub,
                                               int lanes, bool active, V3<double>& F, V3<double>& tau) {
     F = {0.0, 0.0, 0.0};
@@ -116,15 +116,15 @@ __device__ __forceinline__ void body_sums_any(const DevSys& S, const Rot<Real>& 
     if (lanes == 1) {
         if (active) {
             switch (N) {
-            case 2: body_sums<Real, 2>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 3: body_sums<Real, 3>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 4: body_sums<Real, 4>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 6: body_sums<Real, 6>(S, R, atom0, N, 0, 1, F, tau); break;
-            default: body_sums<Real, 0>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 2: body_sums<Real, 2, IO>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 3: body_sums<Real, 3, IO>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 4: body_sums<Real, 4, IO>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 6: body_sums<Real, 6, IO>(S, R, atom0, N, 0, 1, F, tau); break;
+            default: body_sums<Real, 0, IO>(S, R, atom0, N, 0, 1, F, tau); break;
             }
         }
     } else {
-        if (active) body_sums<Real, 0>(S, R, atom0, N, sub, lanes, F, tau);
+        if (active) body_sums<Real, 0, IO>(S, R, atom0, N, sub, lanes, F, tau);
         F = {xor_sum(F.x, lanes), xor_sum(F.y, lanes), xor_sum(F.z, lanes)};
         tau = {xor_sum(tau.x, lanes), xor_sum(tau.y, lanes), xor_sum(tau.z, lanes)};
     }
@@ -132,7 +132,7 @@ __device__ __forceinline__ void body_sums_any(const DevSys& S, const Rot<Real>& 
 
 // atom positions r_cm + A(q)^T d_j: fp32 in mixed precision (the force field's
 // view; the exact value is rebuilt from the body state on download)
-template <class Real, int NC>
+template <class Real, int NC, class IO>
 __device__ __forceinline__ void body_positions(const DevSys& S, const Rot<Real>& R, V3<double> rc, int atom0, int N,
                                                int sub, int lanes) {
     const int dbase = atom0 - S.numFree;
@@ -141,32 +141,32 @@ __device__ __forceinline__ void body_positions(const DevSys& S, const Rot<Real>&
 #pragma unroll
         for (int j = 0; j < NC; j++) dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
 #pragma unroll
-        for (int j = 0; j < NC; j++) st3<Real>(S.pos, S.atomStride, atom0 + j, cvt<Real>(rc + cvt<double>(apply(R, dj[j]))));
+        for (int j = 0; j < NC; j++) IO::template set_body_pos<Real>(S, atom0 + j, rc + cvt<double>(apply(R, dj[j])));
     } else {
         for (int j = sub; j < N; j += lanes) {
             V3<Real> dj = ld3<Real>(S.d, S.dStride, dbase + j);
-            st3<Real>(S.pos, S.atomStride, atom0 + j, cvt<Real>(rc + cvt<double>(apply(R, dj))));
+            IO::template set_body_pos<Real>(S, atom0 + j, rc + cvt<double>(apply(R, dj)));
         }
     }
 }
 
-template <class Real>
+template <class Real, class IO>
 __device__ __forceinline__ void body_positions_any(const DevSys& S, const Rot<Real>& R, V3<double> rc, int atom0,
                                                    int N, int sub, int lanes) {
     if (lanes == 1) {
         switch (N) {
-        case 2: body_positions<Real, 2>(S, R, rc, atom0, N, 0, 1); break;
-        case 3: body_positions<Real, 3>(S, R, rc, atom0, N, 0, 1); break;
-        case 4: body_positions<Real, 4>(S, R, rc, atom0, N, 0, 1); break;
-        case 6: body_positions<Real, 6>(S, R, rc, atom0, N, 0, 1); break;
-        default: body_positions<Real, 0>(S, R, rc, atom0, N, 0, 1); break;
+        case 2: body_positions<Real, 2, IO>(S, R, rc, atom0, N, 0, 1); break;
+        case 3: body_positions<Real, 3, IO>(S, R, rc, atom0, N, 0, 1); break;
+        case 4: body_positions<Real, 4, IO>(S, R, rc, atom0, N, 0, 1); break;
+        case 6: body_positions<Real, 6, IO>(S, R, rc, atom0, N, 0, 1); break;
+        default: body_positions<Real, 0, IO>(S, R, rc, atom0, N, 0, 1); break;
         }
     } else {
-        body_positions<Real, 0>(S, R, rc, atom0, N, sub, lanes);
+        body_positions<Real, 0, IO>(S, R, rc, atom0, N, sub, lanes);
     }
 }
 
-template <class Real, int NC>
+template <class Real, int NC, class IO>
 __device__ __forceinline__ void body_velocities(const DevSys& S, const Rot<Real>& R, V3<Real> vcm, V3<Real> ws,
                                                 int atom0, int N, int sub, int lanes) {
     const int dbase = atom0 - S.numFree;
@@ -174,12 +174,12 @@ __device__ __forceinline__ void body_velocities(const DevSys& S, const Rot<Real>
 #pragma unroll
         for (int j = 0; j < NC; j++) {
             V3<Real> dl = apply(R, ld3<Real>(S.d, S.dStride, dbase + j));
-            st3<Real>(S.vel, S.atomStride, atom0 + j, vcm + cross(ws, dl));
+            IO::template set_vel<Real>(S, atom0 + j, vcm + cross(ws, dl));
         }
     } else {
         for (int j = sub; j < N; j += lanes) {
             V3<Real> dl = apply(R, ld3<Real>(S.d, S.dStride, dbase + j));
-            st3<Real>(S.vel, S.atomStride, atom0 + j, vcm + cross(ws, dl));
+            IO::template set_vel<Real>(S, atom0 + j, vcm + cross(ws, dl));
         }
     }
 }
@@ -239,23 +239,23 @@ template <class Real> __device__ __forceinline__ V3<double> load_rcm(const DevSy
 // ---------------------------------------------------------------------------
 // free-atom tiles hold up to FREE_PER_THREAD atoms per thread (strided by
 // the workgroup size, so each pass over the tile is coalesced)
-template <class Real>
+template <class Real, class IO>
 __device__ __forceinline__ void part1_free(const DevSys& S, const Tile& tile) {
     Real halfDt = (Real)(0.5 * S.dt);
     for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
         int a = tile.first + i;
-        V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
-        V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
+        V3<Real> v = IO::template vel<Real>(S, a);
+        V3<Real> f = IO::template force<Real>(S, a);
         Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
         v = v + f * invm * halfDt;
-        V3<double> r = load_pos<Real>(S, a);
+        V3<double> r = IO::template pos<Real>(S, a);
         r = r + cvt<double>(v) * S.dt;
-        st3<Real>(S.vel, S.atomStride, a, v);
-        store_pos<Real>(S, a, r);
+        IO::template set_vel<Real>(S, a, v);
+        IO::template set_pos<Real>(S, a, r);
     }
 }
 
-template <class Real, bool EXACT>
+template <class Real, bool EXACT, class IO>
 __device__ __forceinline__ void part1_bodies(const DevSys& S, const Tile& tile) {
     const int shift = lane_shift(tile.lanes);
     const int b = threadIdx.x >> shift;
@@ -271,41 +271,41 @@ __device__ __forceinline__ void part1_bodies(const DevSys& S, const Tile& tile) 
         stp3<Real>(S.L, k, B.L);
         store_rcm<Real>(S, k, rc);
     }
-    body_positions_any<Real>(S, rotT(B.q), rc, atom0, tile.N, sub, tile.lanes);
+    body_positions_any<Real, IO>(S, rotT(B.q), rc, atom0, tile.N, sub, tile.lanes);
 }
 
-template <class Real, bool EXACT>
+template <class Real, bool EXACT, class IO>
 __global__ __launch_bounds__(EXACT ? 256 : 512) void k_part1(const DevSys* __restrict__ systems,
                                                               const Tile* __restrict__ tiles, int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const DevSys& S = systems[tile.sys];
         if (tile.kind == TILE_FREE)
-            part1_free<Real>(S, tile);
+            part1_free<Real, IO>(S, tile);
         else
-            part1_bodies<Real, EXACT>(S, tile);
+            part1_bodies<Real, EXACT, IO>(S, tile);
     }
 }
 
 // ---------------------------------------------------------------------------
 // part 2
 // ---------------------------------------------------------------------------
-template <class Real>
+template <class Real, class IO>
 __device__ __forceinline__ void part2_free(const DevSys& S, const Tile& tile, double (&ke)[2]) {
     Real halfDt = (Real)(0.5 * S.dt);
     for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
         int a = tile.first + i;
-        V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
-        V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
+        V3<Real> v = IO::template vel<Real>(S, a);
+        V3<Real> f = IO::template force<Real>(S, a);
         Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
         v = v + f * invm * halfDt;
-        st3<Real>(S.vel, S.atomStride, a, v);
+        IO::template set_vel<Real>(S, a, v);
         V3<double> vd = cvt<double>(v);
         ke[0] += dot(vd, vd) / (double)invm;
     }
 }
 
-template <class Real>
+template <class Real, class IO>
 __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, double (&ke)[2]) {
     const int shift = lane_shift(tile.lanes);
     const int b = threadIdx.x >> shift;
@@ -317,7 +317,7 @@ __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, 
     Body<Real> B = load_body<Real>(S, k);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real, IO>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
     if (!active) return;
     const double halfDt = 0.5 * S.dt;
     V3<double> tb = axial_torque(to_body(R, tau), B.invI);
@@ -327,11 +327,11 @@ __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, 
     V3<double> vcm = p * (double)B.invM;
     V3<Real> ws = apply(R, cvt<Real>(w));
     switch (tile.lanes == 1 ? N : 0) {
-    case 2: body_velocities<Real, 2>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-    case 3: body_velocities<Real, 3>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-    case 4: body_velocities<Real, 4>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-    case 6: body_velocities<Real, 6>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-    default: body_velocities<Real, 0>(S, R, cvt<Real>(vcm), ws, atom0, N, sub, tile.lanes); break;
+    case 2: body_velocities<Real, 2, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
+    case 3: body_velocities<Real, 3, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
+    case 4: body_velocities<Real, 4, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
+    case 6: body_velocities<Real, 6, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
+    default: body_velocities<Real, 0, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, sub, tile.lanes); break;
     }
     if (sub == 0) {
         ke[0] = dot(p, vcm);
@@ -342,7 +342,7 @@ __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, 
     }
 }
 
-template <class Real>
+template <class Real, class IO>
 __global__ __launch_bounds__(512) void k_part2(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
                                                 int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
@@ -350,9 +350,9 @@ __global__ __launch_bounds__(512) void k_part2(const DevSys* __restrict__ system
         const DevSys& S = systems[tile.sys];
         double ke[2] = {0.0, 0.0};
         if (tile.kind == TILE_FREE)
-            part2_free<Real>(S, tile, ke);
+            part2_free<Real, IO>(S, tile, ke);
         else
-            part2_bodies<Real>(S, tile, ke);
+            part2_bodies<Real, IO>(S, tile, ke);
         block_reduce_store<2>(ke, S.kePart, tile.slot);
     }
 }
@@ -365,20 +365,20 @@ __global__ __launch_bounds__(512) void k_part2(const DevSys* __restrict__ system
 // the last step of the call ends with a plain part2.  The body state is
 // rounded to storage precision at the same points as part2 + part1.
 // ---------------------------------------------------------------------------
-template <class Real>
+template <class Real, class IO>
 __device__ __forceinline__ void fused_free(const DevSys& S, const Tile& tile) {
     Real halfDt = (Real)(0.5 * S.dt);
     for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
         int a = tile.first + i;
-        V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
-        V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
+        V3<Real> v = IO::template vel<Real>(S, a);
+        V3<Real> f = IO::template force<Real>(S, a);
         Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
         v = v + f * invm * halfDt;     // end of step n
         v = v + f * invm * halfDt;     // start of step n+1
-        V3<double> r = load_pos<Real>(S, a);
+        V3<double> r = IO::template pos<Real>(S, a);
         r = r + cvt<double>(v) * S.dt;
-        st3<Real>(S.vel, S.atomStride, a, v);
-        store_pos<Real>(S, a, r);
+        IO::template set_vel<Real>(S, a, v);
+        IO::template set_pos<Real>(S, a, r);
     }
 }
 
@@ -404,7 +404,7 @@ template <class Real> __device__ __forceinline__ void store_body(const DevSys& S
 
 // one lane per body of NC atoms: the body-fixed coordinates stay in registers
 // from the torque sum to the position rebuild, so d is read once
-template <class Real, bool EXACT, int NC>
+template <class Real, bool EXACT, int NC, class IO>
 __device__ __forceinline__ void fused_body_regs(const DevSys& S, int k, int atom0) {
     const int dbase = atom0 - S.numFree;
     Body<Real> B = load_body<Real>(S, k);
@@ -413,7 +413,7 @@ __device__ __forceinline__ void fused_body_regs(const DevSys& S, int k, int atom
 #pragma unroll
     for (int j = 0; j < NC; j++) {
         dj[j] = ld3<Real>(S.d, S.dStride, dbase + j);
-        fj[j] = ld3<Real>(S.force, S.atomStride, atom0 + j);
+        fj[j] = IO::template force<Real>(S, atom0 + j);
     }
     Rot<Real> R = rotT(B.q);
     V3<double> F = {0.0, 0.0, 0.0}, tau = {0.0, 0.0, 0.0};
@@ -429,11 +429,10 @@ __device__ __forceinline__ void fused_body_regs(const DevSys& S, int k, int atom
     store_body<Real>(S, k, B, rc);
     Rot<Real> R1 = rotT(B.q);
 #pragma unroll
-    for (int j = 0; j < NC; j++)
-        st3<Real>(S.pos, S.atomStride, atom0 + j, cvt<Real>(rc + cvt<double>(apply(R1, dj[j]))));
+    for (int j = 0; j < NC; j++) IO::template set_body_pos<Real>(S, atom0 + j, rc + cvt<double>(apply(R1, dj[j])));
 }
 
-template <class Real, bool EXACT>
+template <class Real, bool EXACT, class IO>
 __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) {
     const int N = tile.N;
     // (the FP64 exact rotation leaves no registers to spare: it re-reads d)
@@ -441,9 +440,9 @@ __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) 
         const int b = threadIdx.x;
         if (b >= tile.count) return;
         const int k = tile.first + b, atom0 = tile.atomFirst + b * N;
-        if (N == 2) fused_body_regs<Real, EXACT, 2>(S, k, atom0);
-        else if (N == 3) fused_body_regs<Real, EXACT, 3>(S, k, atom0);
-        else fused_body_regs<Real, EXACT, 4>(S, k, atom0);
+        if (N == 2) fused_body_regs<Real, EXACT, 2, IO>(S, k, atom0);
+        else if (N == 3) fused_body_regs<Real, EXACT, 3, IO>(S, k, atom0);
+        else fused_body_regs<Real, EXACT, 4, IO>(S, k, atom0);
         return;
     }
     const int shift = lane_shift(tile.lanes);
@@ -456,24 +455,24 @@ __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) 
     V3<double> rc = load_rcm<Real>(S, k);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real, IO>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
     if (!active) return;
     fused_kicks<Real>(S, B, R, F, tau);
     drift_and_rotate<Real, EXACT>(S, B, rc);
     if (sub == 0) store_body<Real>(S, k, B, rc);
-    body_positions_any<Real>(S, rotT(B.q), rc, atom0, N, sub, tile.lanes);
+    body_positions_any<Real, IO>(S, rotT(B.q), rc, atom0, N, sub, tile.lanes);
 }
 
-template <class Real, bool EXACT>
+template <class Real, bool EXACT, class IO>
 __global__ __launch_bounds__(EXACT ? 256 : 512) void k_fused(const DevSys* __restrict__ systems,
                                                               const Tile* __restrict__ tiles, int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const DevSys& S = systems[tile.sys];
         if (tile.kind == TILE_FREE)
-            fused_free<Real>(S, tile);
+            fused_free<Real, IO>(S, tile);
         else
-            fused_bodies<Real, EXACT>(S, tile);
+            fused_bodies<Real, EXACT, IO>(S, tile);
     }
 }
 
@@ -1092,12 +1091,42 @@ __global__ __launch_bounds__(256) void k_clusters(const DevSys* __restrict__ sys
     }
 }
 
-// positions as the force field reads them: body atoms (and the massless
-// particles after them) at their fp32 positions in mixed precision, free
-// atoms at hi + lo
-template <class Real> __device__ __forceinline__ V3<double> force_pos(const DevSys& S, int i) {
-    if (sizeof(Real) == 4 && i >= S.numFree) return cvt<double>(ld3<Real>(S.pos, S.atomStride, i));
-    return load_pos<Real>(S, i);
+// positions as the force field reads them, as a hi part and a low part:
+// body atoms (and the massless particles after them) at their fp32
+// positions in mixed precision (lo = 0), free atoms at hi + lo
+template <class Real, class IO = SoaAtoms>
+__device__ __forceinline__ void field_pos_parts(const DevSys& S, int i, V3<Real>& hi, V3<Real>& lo) {
+    lo = {(Real)0, (Real)0, (Real)0};
+    if constexpr (IO::adopted) {
+        const int s = S.ommSlot[i];
+        Q4<Real> p = ld4<Real>(S.ommPosq, s);
+        hi = {p.a, p.b, p.c};
+        if (sizeof(Real) == 4 && i < S.numFree) {
+            const float* c = S.ommPosqCorr + 4 * (size_t)s;
+            lo = {(Real)c[0], (Real)c[1], (Real)c[2]};
+        }
+    } else {
+        hi = ld3<Real>(S.pos, S.atomStride, i);
+        if (sizeof(Real) == 4 && i < S.numFree) lo = cvt<Real>(ld3<float>(S.posLo, S.atomStride, i));
+    }
+}
+template <class Real, class IO = SoaAtoms> __device__ __forceinline__ V3<double> force_pos(const DevSys& S, int i) {
+    V3<Real> hi, lo;
+    field_pos_parts<Real, IO>(S, i, hi, lo);
+    return cvt<double>(hi) + cvt<double>(lo);
+}
+// a force into the library's array, or into the adopted fixed-point array
+// (OpenMM's conversion: truncation of f * 2^32)
+template <class Real, class IO> __device__ __forceinline__ void store_force(const DevSys& S, int i, V3<Real> f) {
+    if constexpr (IO::adopted) {
+        long long* o = S.ommForce + S.ommSlot[i];
+        const size_t st = (size_t)S.ommStride;
+        o[0] = (long long)((double)f.x * kFixedPointScale);
+        o[st] = (long long)((double)f.y * kFixedPointScale);
+        o[2 * st] = (long long)((double)f.z * kFixedPointScale);
+    } else {
+        st3<Real>(S.force, S.atomStride, i, f);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1199,10 +1228,13 @@ __global__ __launch_bounds__(256) void k_site_clear(const DevSys* __restrict__ s
 // ---------------------------------------------------------------------------
 // force field: harmonic tether + harmonic bonds, one lane per atom.  Body
 // atoms are read at their fp32 positions (as a mixed-precision force field
-// reads fp32 coordinates), free atoms at hi + lo.
+// reads fp32 coordinates), free atoms at hi + lo; the tether's anchor r0 is
+// held at the state precision (fp32 in mixed precision, as a mixed-precision
+// force field holds its parameters), so an atom moves 36 bytes: its
+// position, r0 and its force.
 // ---------------------------------------------------------------------------
 
-template <class Real>
+template <class Real, class IO>
 __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
                                                  int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
@@ -1214,16 +1246,9 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             V3<Real> f = {(Real)0, (Real)0, (Real)0};
             if (S.tetherScale != 0.0) {
                 Real k = S.tetherUniform ? (Real)S.tetherK0 : reinterpret_cast<const Real*>(S.tetherK)[a];
-                V3<Real> dx;
-                if (sizeof(Real) == 4) {
-                    V3<Real> ph = ld3<Real>(S.pos, S.atomStride, a), rh = ld3<Real>(S.r0, S.atomStride, a);
-                    V3<float> rl = ld3<float>(S.r0Lo, S.atomStride, a);
-                    V3<float> pl = a < S.numFree ? ld3<float>(S.posLo, S.atomStride, a) : V3<float>{0.f, 0.f, 0.f};
-                    dx = {(ph.x - rh.x) + (Real)(pl.x - rl.x), (ph.y - rh.y) + (Real)(pl.y - rl.y),
-                          (ph.z - rh.z) + (Real)(pl.z - rl.z)};
-                } else {
-                    dx = ld3<Real>(S.pos, S.atomStride, a) - ld3<Real>(S.r0, S.atomStride, a);
-                }
+                V3<Real> ph, pl;
+                field_pos_parts<Real, IO>(S, a, ph, pl);
+                V3<Real> dx = (ph - ld3<Real>(S.r0, S.atomStride, a)) + pl;
                 f = dx * (-k);
                 V3<double> dd = cvt<double>(dx);
                 e[0] += 0.5 * (double)k * dot(dd, dd);
@@ -1231,10 +1256,10 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             if (S.bondStart) {
                 int e0 = S.bondStart[a], e1 = S.bondStart[a + 1];
                 if (e0 < e1) {
-                    V3<double> pa = force_pos<Real>(S, a);
+                    V3<double> pa = force_pos<Real, IO>(S, a);
                     V3<double> fd = cvt<double>(f);
                     for (int b = e0; b < e1; b++) {
-                        V3<double> dr = force_pos<Real>(S, S.bondOther[b]) - pa;
+                        V3<double> dr = force_pos<Real, IO>(S, S.bondOther[b]) - pa;
                         double r = sqrt(dot(dr, dr));
                         double ext = r - S.bondLen[b];
                         fd = fd + dr * (S.bondK[b] * ext / r);
@@ -1243,7 +1268,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
                     f = cvt<Real>(fd);
                 }
             }
-            st3<Real>(S.force, S.atomStride, a, f);
+            store_force<Real, IO>(S, a, f);
         }
         block_reduce_store<1>(e, S.pePart, tile.slot);
     }
@@ -1283,40 +1308,60 @@ static int grid_for(int numTiles, int maxBlocks) {
     return maxBlocks > 0 && maxBlocks < numTiles ? maxBlocks : numTiles;
 }
 
+// the step kernels for the atom layout (library arrays or adopted arrays),
+// state precision and rotation kind
+template <class IO>
+static void part1_for(int precision, int rotationMode, int grid, int block, hipStream_t stream,
+                      const DevSys* systems, const Tile* tiles, int numTiles) {
+    if (precision == 0) {
+        if (rotationMode == 0) k_part1<float, true, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_part1<float, false, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    } else {
+        if (rotationMode == 0) k_part1<double, true, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_part1<double, false, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    }
+}
+template <class IO>
+static void fused_for(int precision, int rotationMode, int grid, int block, hipStream_t stream,
+                      const DevSys* systems, const Tile* tiles, int numTiles) {
+    if (precision == 0) {
+        if (rotationMode == 0) k_fused<float, true, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_fused<float, false, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    } else {
+        if (rotationMode == 0) k_fused<double, true, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_fused<double, false, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    }
+}
+
 hipError_t launch_part1(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
-                        int block, int maxBlocks, hipStream_t stream) {
+                        int block, int maxBlocks, hipStream_t stream, int adopted) {
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
-    if (precision == 0) {
-        if (rotationMode == 0) k_part1<float, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_part1<float, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-    } else {
-        if (rotationMode == 0) k_part1<double, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_part1<double, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-    }
+    if (adopted) part1_for<AdoptedAtoms>(precision, rotationMode, grid, block, stream, systems, tiles, numTiles);
+    else part1_for<SoaAtoms>(precision, rotationMode, grid, block, stream, systems, tiles, numTiles);
     return hipGetLastError();
 }
 
 hipError_t launch_fused(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
-                        int block, int maxBlocks, hipStream_t stream) {
+                        int block, int maxBlocks, hipStream_t stream, int adopted) {
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
-    if (precision == 0) {
-        if (rotationMode == 0) k_fused<float, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_fused<float, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-    } else {
-        if (rotationMode == 0) k_fused<double, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_fused<double, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-    }
+    if (adopted) fused_for<AdoptedAtoms>(precision, rotationMode, grid, block, stream, systems, tiles, numTiles);
+    else fused_for<SoaAtoms>(precision, rotationMode, grid, block, stream, systems, tiles, numTiles);
     return hipGetLastError();
 }
 
 hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
-                        int maxBlocks, hipStream_t stream) {
+                        int maxBlocks, hipStream_t stream, int adopted) {
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
-    if (precision == 0) k_part2<float><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-    else k_part2<double><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    if (precision == 0) {
+        if (adopted) k_part2<float, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_part2<float, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    } else {
+        if (adopted) k_part2<double, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_part2<double, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    }
     return hipGetLastError();
 }
 
@@ -1402,11 +1447,16 @@ hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* ti
 }
 
 hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
-                         int maxBlocks, hipStream_t stream) {
+                         int maxBlocks, hipStream_t stream, int adopted) {
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
-    if (precision == 0) k_forces<float><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-    else k_forces<double><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    if (precision == 0) {
+        if (adopted) k_forces<float, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_forces<float, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    } else {
+        if (adopted) k_forces<double, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_forces<double, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    }
     return hipGetLastError();
 }
 

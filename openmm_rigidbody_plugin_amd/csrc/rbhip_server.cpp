@@ -151,9 +151,11 @@ struct rbhip_server {
     // run_round thread only
     std::vector<std::unique_ptr<Job>> resident;     // batch members
     // two batches: one runs the queued round while the other takes a new
-    // member list; members[b] is the list batch b was last given
+    // member list; members[b] holds the ids of the jobs batch b was last
+    // given (ids, not context addresses: a finished job's context may be
+    // freed and a new job's context created at the same address)
     rbhip_batch* batches[2] = {nullptr, nullptr};
-    std::vector<rbhip_context*> members[2];
+    std::vector<long long> members[2];
     int cur = 0;
     hipStream_t batchStreams[2] = {nullptr, nullptr};
     hipStream_t memberStream = nullptr;
@@ -357,14 +359,18 @@ struct rbhip_server {
         }
         Clock::time_point t0 = Clock::now();
         std::vector<rbhip_context*> m;
-        for (auto& j : next) m.push_back(j->ctx);
+        std::vector<long long> ids;
+        for (auto& j : next) {
+            m.push_back(j->ctx);
+            ids.push_back(j->id);
+        }
         if (!m.empty() && (int)inFlight.size() >= kRoundsInFlight) retireOldestRound();
         Clock::time_point t1 = Clock::now();
-        if (!m.empty() && m != members[cur]) {
+        if (!m.empty() && ids != members[cur]) {
             cur = 1 - cur;
-            if (m != members[cur]) {
+            if (ids != members[cur]) {
                 call(rbhip_batch_set_members(batches[cur], (int)m.size(), m.data()));
-                members[cur] = m;
+                members[cur] = ids;
             }
         }
         Clock::time_point t2 = Clock::now();

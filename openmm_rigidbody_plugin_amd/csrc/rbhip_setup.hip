@@ -218,29 +218,85 @@ __global__ __launch_bounds__(256) void k_setup(const DevSys* __restrict__ system
     }
 }
 
-// FP64 staging -> device state
-template <class Real>
+// FP64 staging -> device state (the library's arrays or the adopted ones):
+// what = 0 positions, 1 velocities (rounded to the state precision), 2
+// forces (library arrays only)
+template <class Real, class IO>
 __global__ void k_stage_in(const DevSys* __restrict__ sys, const double* __restrict__ in, int n, int what) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     V3<double> x = {in[3 * (size_t)i], in[3 * (size_t)i + 1], in[3 * (size_t)i + 2]};
-    if (what == 0) store_pos<Real>(*sys, i, x);
-    else st3<Real>(sys->vel, sys->atomStride, i, cvt<Real>(x));
+    if (what == 0) IO::template set_pos<Real>(*sys, i, x);
+    else if (what == 1) IO::template set_vel<Real>(*sys, i, cvt<Real>(x));
+    else st3<Real>(sys->force, sys->atomStride, i, cvt<Real>(x));
 }
 
-// device state -> FP64 staging
-template <class Real>
+// device state -> FP64 staging: what = 0 positions, 1 velocities, 2 forces
+// (adopted forces at their fixed-point value)
+template <class Real, class IO>
 __global__ void k_stage_out(const DevSys* __restrict__ sys, double* __restrict__ out, int n, int what) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const DevSys& S = *sys;
     V3<double> x;
-    if (what == 0) x = load_pos<Real>(*sys, i);
-    else if (what == 1) x = cvt<double>(ld3<Real>(sys->vel, sys->atomStride, i));
-    else x = cvt<double>(ld3<Real>(sys->force, sys->atomStride, i));
+    if (what == 0) {
+        x = IO::template pos<Real>(S, i);
+    } else if (what == 1) {
+        if constexpr (IO::adopted) {   // velm holds FP64 values
+            const double* v = S.ommVelm + 4 * (size_t)S.ommSlot[i];
+            x = {v[0], v[1], v[2]};
+        } else {
+            x = cvt<double>(IO::template vel<Real>(S, i));
+        }
+    } else if constexpr (IO::adopted) {
+        const long long* f = S.ommForce + S.ommSlot[i];
+        const size_t st = (size_t)S.ommStride;
+        const double u = 1.0 / kFixedPointScale;
+        x = {(double)f[0] * u, (double)f[st] * u, (double)f[2 * st] * u};
+    } else {
+        x = cvt<double>(ld3<Real>(S.force, S.atomStride, i));
+    }
     out[3 * (size_t)i] = x.x;
     out[3 * (size_t)i + 1] = x.y;
     out[3 * (size_t)i + 2] = x.z;
 }
+
+// adopted fixed-point forces -> the library's force array at the state
+// precision (the body-state build, the re-kick and the body-state download
+// read it there)
+template <class Real>
+__global__ void k_adopted_forces_in(const DevSys* __restrict__ sys, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    st3<Real>(sys->force, sys->atomStride, i, AdoptedAtoms::force<Real>(*sys, i));
+}
+
+// moves the adopted force array's entries of the n internal atoms from
+// their old slots to new ones (the atom order of the arrays changed): a
+// gather into `tmp` (3 per atom) with the old map, then a scatter with the
+// new one
+__global__ void k_adopted_gather_forces(const DevSys* __restrict__ sys, long long* __restrict__ tmp, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevSys& S = *sys;
+    const long long* f = S.ommForce + S.ommSlot[i];
+    const size_t st = (size_t)S.ommStride;
+    tmp[3 * (size_t)i] = f[0];
+    tmp[3 * (size_t)i + 1] = f[st];
+    tmp[3 * (size_t)i + 2] = f[2 * st];
+}
+__global__ void k_adopted_scatter_forces(const DevSys* __restrict__ sys, const long long* __restrict__ tmp,
+                                         const int* __restrict__ newSlot, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevSys& S = *sys;
+    long long* f = S.ommForce + newSlot[i];
+    const size_t st = (size_t)S.ommStride;
+    f[0] = tmp[3 * (size_t)i];
+    f[st] = tmp[3 * (size_t)i + 1];
+    f[2 * st] = tmp[3 * (size_t)i + 2];
+}
+
 
 // body force and space-frame torque from the current atom forces
 template <class Real>
@@ -316,6 +372,33 @@ __global__ void k_rekick(const DevSys* __restrict__ sys, const int* __restrict__
     stp3<Real>(S.L, k, cvt<Real>(L));
 }
 
+// the next step's first half-kick, folded into the momenta with the forces
+// `prev` (the layout of the force array), redone with the forces now in the
+// force array: p += (F - F_prev) dt/2, L += A(q)(tau - tau_prev) dt/2; bit
+// for bit unchanged where the forces are
+template <class Real>
+__global__ void k_refold(const DevSys* __restrict__ sys, const int* __restrict__ bodyN,
+                         const int* __restrict__ bodyAtom0, int numBodies, const void* __restrict__ prev) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= numBodies) return;
+    const DevSys& S = *sys;
+    const int N = bodyN[k], atom0 = bodyAtom0[k], dbase = atom0 - S.numFree;
+    Q4<double> q = cvt<double>(ld4<Real>(S.q, k));
+    V3<double> dF = {0.0, 0.0, 0.0}, dtau = {0.0, 0.0, 0.0};
+    for (int j = 0; j < N; j++) {
+        V3<double> dl = Atmul(q, cvt<double>(ld3<Real>(S.d, S.dStride, dbase + j)));
+        V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, atom0 + j)) -
+                       cvt<double>(ld3<Real>(prev, S.atomStride, atom0 + j));
+        dF = dF + f;
+        dtau = dtau + cross(dl, f);
+    }
+    const double halfDt = 0.5 * S.dt;
+    V3<double> pcm = cvt<double>(ldp3<Real>(S.pcm, k)) + dF * halfDt;
+    V3<double> L = cvt<double>(ldp3<Real>(S.L, k)) + body_torque(q, dtau, (double)ld4<Real>(S.iI, k).c) * halfDt;
+    stp3<Real>(S.pcm, k, cvt<Real>(pcm));
+    stp3<Real>(S.L, k, cvt<Real>(L));
+}
+
 // body-atom positions r_cm + A(q)^T d_j in FP64 into the internal-order
 // staging buffer (the rotation is the one part1 applies)
 template <class Real>
@@ -347,20 +430,48 @@ hipError_t launch_setup(int precision, const DevSys* systems, const Tile* tiles,
 }
 
 hipError_t launch_stage_in(int precision, const DevSys* sys, const double* in, int numAtoms, int what,
-                           hipStream_t stream) {
+                           hipStream_t stream, int adopted) {
     if (numAtoms <= 0) return hipSuccess;
+    if (adopted && what == 2) return hipErrorInvalidValue;
     int grid = (numAtoms + 255) / 256;
-    if (precision == 0) k_stage_in<float><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
-    else k_stage_in<double><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
+    if (precision == 0) {
+        if (adopted) k_stage_in<float, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
+        else k_stage_in<float, SoaAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
+    } else {
+        if (adopted) k_stage_in<double, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
+        else k_stage_in<double, SoaAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_stage_out(int precision, const DevSys* sys, double* out, int numAtoms, int what,
-                            hipStream_t stream) {
+                            hipStream_t stream, int adopted) {
     if (numAtoms <= 0) return hipSuccess;
     int grid = (numAtoms + 255) / 256;
-    if (precision == 0) k_stage_out<float><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
-    else k_stage_out<double><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
+    if (precision == 0) {
+        if (adopted) k_stage_out<float, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
+        else k_stage_out<float, SoaAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
+    } else {
+        if (adopted) k_stage_out<double, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
+        else k_stage_out<double, SoaAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_adopted_forces_in(int precision, const DevSys* sys, int numAtoms, hipStream_t stream) {
+    if (numAtoms <= 0) return hipSuccess;
+    int grid = (numAtoms + 255) / 256;
+    if (precision == 0) k_adopted_forces_in<float><<<grid, 256, 0, stream>>>(sys, numAtoms);
+    else k_adopted_forces_in<double><<<grid, 256, 0, stream>>>(sys, numAtoms);
+    return hipGetLastError();
+}
+
+hipError_t launch_adopted_move_forces(const DevSys* sys, long long* tmp, const int* newSlot, int numAtoms,
+                                      hipStream_t stream) {
+    if (numAtoms <= 0) return hipSuccess;
+    int grid = (numAtoms + 255) / 256;
+    k_adopted_gather_forces<<<grid, 256, 0, stream>>>(sys, tmp, numAtoms);
+    k_adopted_scatter_forces<<<grid, 256, 0, stream>>>(sys, tmp, newSlot, numAtoms);
     return hipGetLastError();
 }
 
@@ -379,6 +490,15 @@ hipError_t launch_body_atoms_out(int precision, const DevSys* sys, const int* bo
     int grid = (numBodies + 255) / 256;
     if (precision == 0) k_body_atoms_out<float><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
     else k_body_atoms_out<double><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_refold(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0, int numBodies,
+                         const void* prev, hipStream_t stream) {
+    if (numBodies <= 0) return hipSuccess;
+    int grid = (numBodies + 255) / 256;
+    if (precision == 0) k_refold<float><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, prev);
+    else k_refold<double><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, prev);
     return hipGetLastError();
 }
 

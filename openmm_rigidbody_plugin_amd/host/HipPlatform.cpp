@@ -68,8 +68,8 @@ void HipPlatform::PlatformData::createSimulation(const System& system, const std
                     for (int c = 0; c < 3; c++) r0[3 * (size_t)atom + c] = r[c];
                 }
                 rbhip_check(rbhip_set_tether(s, k.data(), r0.data()));
-            } else {
-                throw Exception("The HIP platform supports HarmonicBondForce and TetherForce only");
+            } else if (dynamic_cast<const HostForce*>(&force) == nullptr) {
+                throw Exception("The HIP platform supports HarmonicBondForce, TetherForce and HostForce only");
             }
         }
         std::vector<int> types, siteAtoms, parents;
@@ -92,6 +92,12 @@ void HipPlatform::PlatformData::createSimulation(const System& system, const std
         throw;
     }
     sim = s;
+}
+
+bool HipPlatform::PlatformData::hasHostForces(const System& system) const {
+    for (int f = 0; f < system.getNumForces(); f++)
+        if (dynamic_cast<const HostForce*>(&system.getForce(f))) return true;
+    return false;
 }
 
 HipPlatform::HipPlatform() = default;
@@ -159,9 +165,31 @@ void HipPlatform::getForces(ContextImpl& context, std::vector<Vec3>& forces) con
     unflatten(xyz, forces);
 }
 
+// the device force field, then each HostForce at the current positions,
+// added to the device forces and handed back through rbhip_set_forces
 double HipPlatform::calcForcesAndEnergy(ContextImpl& context, bool includeEnergy) const {
+    PlatformData& d = data(context);
+    rbhip_context* sim = d.require();
+    const System& system = context.getSystem();
     double e = 0.0;
-    rbhip_check(rbhip_compute_forces(data(context).require(), includeEnergy ? &e : nullptr));
+    const bool host = d.hasHostForces(system);
+    rbhip_check(rbhip_compute_forces(sim, (includeEnergy || host) ? &e : nullptr));
+    if (!host) return e;
+    const size_t n = (size_t)system.getNumParticles();
+    std::vector<double> xyz(3 * n), f(3 * n);
+    rbhip_check(rbhip_get_positions(sim, xyz.data()));
+    rbhip_check(rbhip_get_forces(sim, f.data()));
+    std::vector<Vec3> positions, forces;
+    unflatten(xyz, positions);
+    unflatten(f, forces);
+    for (int k = 0; k < system.getNumForces(); k++)
+        if (auto* hf = dynamic_cast<const HostForce*>(&system.getForce(k))) {
+            std::vector<Vec3> fk(n);
+            e += hf->evaluate(positions, fk);
+            for (size_t i = 0; i < n; i++) forces[i] = forces[i] + fk[i];
+        }
+    f = flatten(forces);
+    rbhip_check(rbhip_set_forces(sim, f.data()));
     return e;
 }
 

@@ -22,11 +22,13 @@ public:
         int precision = RBHIP_PRECISION_MIXED;
         rbhip_context* sim = nullptr;   // created by the integrator's kernel
         ~PlatformData();
-        // creates the device context for `system` and installs its forces
-        // (HarmonicBondForce, TetherForce; anything else is rejected)
+        // creates the device context for `system` and installs its device
+        // forces (HarmonicBondForce, TetherForce); HostForce is evaluated on
+        // the host at each force evaluation; anything else is rejected
         void createSimulation(const System& system, const std::vector<int>& bodyIndices, double stepSize,
                               int rotationMode);
         rbhip_context* require() const;
+        bool hasHostForces(const System& system) const;
     };
 
     HipPlatform();

@@ -27,7 +27,12 @@ void HipIntegrateRigidBodyStepKernel::syncStepSize(const RigidBodyIntegrator& in
 }
 
 void HipIntegrateRigidBodyStepKernel::execute(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator) {
-    execute(context, integrator, 1);
+    syncStepSize(integrator);
+    syncTolerance(integrator);
+    rbhip_context* sim = data.require();
+    rbhip_check(rbhip_part1(sim));
+    context.calcForcesAndEnergy(true, false);
+    rbhip_check(rbhip_part2(sim));
 }
 
 void HipIntegrateRigidBodyStepKernel::syncTolerance(const RigidBodyIntegrator& integrator) {
@@ -38,7 +43,12 @@ void HipIntegrateRigidBodyStepKernel::syncTolerance(const RigidBodyIntegrator& i
     }
 }
 
-void HipIntegrateRigidBodyStepKernel::execute(rbhost::ContextImpl&, const RigidBodyIntegrator& integrator, int steps) {
+bool HipIntegrateRigidBodyStepKernel::canExecuteSteps(rbhost::ContextImpl& context) const {
+    return !data.hasHostForces(context.getSystem());
+}
+
+void HipIntegrateRigidBodyStepKernel::executeSteps(rbhost::ContextImpl&, const RigidBodyIntegrator& integrator,
+                                                   int steps) {
     syncStepSize(integrator);
     syncTolerance(integrator);
     rbhip_check(rbhip_execute(data.require(), steps));

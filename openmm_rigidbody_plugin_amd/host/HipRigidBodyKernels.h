@@ -10,7 +10,7 @@
 
 namespace RigidBodyPlugin {
 
-class HipIntegrateRigidBodyStepKernel : public IntegrateRigidBodyStepKernel {
+class HipIntegrateRigidBodyStepKernel : public IntegrateRigidBodyStepKernel, public MultiStepKernel {
 public:
     HipIntegrateRigidBodyStepKernel(std::string name, const rbhost::Platform& platform,
                                     rbhost::HipPlatform::PlatformData& data)
@@ -21,9 +21,13 @@ public:
     // the body state is built on the device when positions or velocities are
     // set; this binds the host view to it
     void uploadBodySystem(RigidBodySystem& bodySystem) override;
+    // part 1 on the device, the context's force evaluation, part 2 on the
+    // device (ReferenceRigidBodyKernels.cpp:82-108)
     void execute(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator) override;
-    // all steps in one call: fused step boundaries, replayed from HIP graphs
-    void execute(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator, int steps) override;
+    // all forces on the device (no HostForce): the steps run in one call,
+    // with fused step boundaries replayed from HIP graphs
+    bool canExecuteSteps(rbhost::ContextImpl& context) const override;
+    void executeSteps(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator, int steps) override;
     double computeKineticEnergy(rbhost::ContextImpl& context, const RigidBodyIntegrator& integrator) override;
     std::vector<double> getKineticEnergies(const RigidBodyIntegrator& integrator) override;
     // with setComputeRefinedEnergies(true), the CUDA platform's refined values

@@ -87,6 +87,17 @@ extern "C" int rbplugin_create_ex(int numAtoms, const double* masses, const int*
                                   int rotationMode, int computeRefinedEnergies, int precision, int device,
                                   int numBonds, const int* bondPairs, const double* bondLength, const double* bondK,
                                   const double* tetherK, const double* tetherR0, rbplugin_sim** out) {
+    return rbplugin_create_with_force(numAtoms, masses, bodyIndices, stepSize, rotationMode, computeRefinedEnergies,
+                                      precision, device, numBonds, bondPairs, bondLength, bondK, tetherK, tetherR0,
+                                      nullptr, nullptr, out);
+}
+
+extern "C" int rbplugin_create_with_force(int numAtoms, const double* masses, const int* bodyIndices,
+                                          double stepSize, int rotationMode, int computeRefinedEnergies,
+                                          int precision, int device, int numBonds, const int* bondPairs,
+                                          const double* bondLength, const double* bondK, const double* tetherK,
+                                          const double* tetherR0, rbplugin_force_fn hostForce, void* user,
+                                          rbplugin_sim** out) {
     return guard([&] {
         need(out != nullptr, "null output handle");
         *out = nullptr;
@@ -109,6 +120,17 @@ extern "C" int rbplugin_create_ex(int numAtoms, const double* masses, const int*
                 if (tetherK[i] != 0.0)
                     tether->addParticle(i, tetherK[i], Vec3(tetherR0[3 * i], tetherR0[3 * i + 1], tetherR0[3 * i + 2]));
             s->system.addForce(tether);
+        }
+        if (hostForce) {
+            s->system.addForce(new rbhost::HostForce(
+                [hostForce, user](const std::vector<Vec3>& x, std::vector<Vec3>& f) {
+                    const int n = (int)x.size();
+                    std::vector<double> xyz(3 * (size_t)n), fxyz(3 * (size_t)n, 0.0);
+                    flat(x, xyz.data());
+                    double e = hostForce(user, n, xyz.data(), fxyz.data());
+                    for (int i = 0; i < n; i++) f[i] = f[i] + Vec3(fxyz[3 * i], fxyz[3 * i + 1], fxyz[3 * i + 2]);
+                    return e;
+                }));
         }
         s->integrator.reset(new RigidBodyIntegrator(stepSize, std::vector<int>(bodyIndices, bodyIndices + numAtoms)));
         s->integrator->setRotationMode(rotationMode);

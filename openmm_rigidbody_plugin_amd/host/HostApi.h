@@ -7,12 +7,14 @@
 // the reference's (tests/TestRigidBodyIntegrator.h).  Everything here is
 // host C++; device work goes through the rbhip C ABI (include/rbhip.h).
 //
-// Force fields: this host evaluates its forces on the device inside the rbhip
-// context, so only the force types the library implements are accepted:
-// HarmonicBondForce and TetherForce (a per-particle harmonic restraint,
-// E = k/2 |r - r0|^2, the benchmark's stand-in force field).
+// Force fields: HarmonicBondForce and TetherForce (a per-particle harmonic
+// restraint, E = k/2 |r - r0|^2, the benchmark's stand-in force field) are
+// evaluated on the device inside the rbhip context; HostForce is any other
+// force field, evaluated on the host by a callback and handed to the
+// integrator through the same force slot.
 #pragma once
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -58,6 +60,22 @@ public:
 private:
     std::vector<int> p1, p2;
     std::vector<double> len, kb;
+};
+
+// A force field evaluated on the host: the callback receives the positions
+// of all particles (FP64) and adds its forces to `forces` (same length,
+// zero-initialised by the caller) and returns its potential energy.  It
+// stands for any force field the platform does not evaluate itself.
+class HostForce : public Force {
+public:
+    using Callback = std::function<double(const std::vector<Vec3>& positions, std::vector<Vec3>& forces)>;
+    explicit HostForce(Callback callback) : callback(std::move(callback)) {}
+    double evaluate(const std::vector<Vec3>& positions, std::vector<Vec3>& forces) const {
+        return callback(positions, forces);
+    }
+
+private:
+    Callback callback;
 };
 
 class TetherForce : public Force {

@@ -43,9 +43,12 @@ std::vector<std::string> RigidBodyIntegrator::getKernelNames() { return {Integra
 
 // positions: forces, then geometry + dynamics; velocities: dynamics
 // (RigidBodyIntegrator.cpp:63-74).  The platform has already rebuilt the body
-// state from the new atom state; the host view refreshes its counts.
+// state from the new atom state with its device forces; the context's full
+// force evaluation (host forces included) then redoes the first half-kick,
+// and the host view refreshes its counts.
 void RigidBodyIntegrator::stateChanged(rbhost::State::DataType changed) {
     if (changed == rbhost::State::Positions || changed == rbhost::State::Velocities) {
+        if (changed == rbhost::State::Positions) context->calcForcesAndEnergy(true, false);
         auto& k = kernel.getAs<IntegrateRigidBodyStepKernel>();
         k.uploadBodySystem(bodySystem);
         bodySystem.update(*context, changed == rbhost::State::Positions, true);
@@ -71,10 +74,18 @@ double RigidBodyIntegrator::getPotentialEnergyRefinement() {
     return kernel.getAs<IntegrateRigidBodyStepKernel>().getPotentialEnergyRefinement(*this);
 }
 
+// RigidBodyIntegrator.cpp:96-101: one execute() per step, or all of them in
+// one call of a kernel that can run them without the host in between
 void RigidBodyIntegrator::step(int steps) {
     if (context == nullptr) throw Exception("This Integrator is not bound to a context!");
     if (steps <= 0) return;
-    kernel.getAs<IntegrateRigidBodyStepKernel>().execute(*context, *this, steps);
+    auto& k = kernel.getAs<IntegrateRigidBodyStepKernel>();
+    auto* multi = dynamic_cast<MultiStepKernel*>(&k);
+    if (multi != nullptr && multi->canExecuteSteps(*context)) {
+        multi->executeSteps(*context, *this, steps);
+        return;
+    }
+    for (int i = 0; i < steps; i++) k.execute(*context, *this);
 }
 
 }  // namespace RigidBodyPlugin

@@ -108,16 +108,19 @@ class RigidBodyIntegrator:
 
 class Context:
     """Binds a RigidBodyIntegrator to a system on the HIP platform: per-atom
-    masses and the force field (harmonic bonds as (i, j, length, k), and
-    harmonic tethers of constant k to points r0)."""
+    masses and the force field: harmonic bonds as (i, j, length, k) and
+    harmonic tethers of constant k to points r0, evaluated on the device, and
+    `force`, any force field written in Python, f(positions) -> (forces,
+    energy) with (n, 3) arrays, evaluated between part 1 and part 2 of every
+    step as the reference's execute() evaluates the context's forces."""
 
     def __init__(self, masses, integrator, precision="mixed", device=0, bonds=None, tether_k=None,
-                 tether_r0=None):
+                 tether_r0=None, force=None):
         if integrator._sim is not None:
             raise RuntimeError("the integrator is already bound to a context")
         self._sim = plugin.Simulation(masses, integrator._bodies, integrator._dt, integrator._mode, precision,
                                       device, bonds=bonds, tether_k=tether_k, tether_r0=tether_r0,
-                                      refined_energies=integrator._refined)
+                                      refined_energies=integrator._refined, host_force=force)
         integrator._sim = self._sim
         self._integrator = integrator
 

@@ -28,6 +28,8 @@ _SIG = {
     "rbplugin_create": (_I, [_I, _DP, _IP, _D, _I, _I, _I, _I, _IP, _DP, _DP, _DP, _DP, ctypes.POINTER(_P)]),
     "rbplugin_create_ex": (_I, [_I, _DP, _IP, _D, _I, _I, _I, _I, _I, _IP, _DP, _DP, _DP, _DP,
                                 ctypes.POINTER(_P)]),
+    "rbplugin_create_with_force": (_I, [_I, _DP, _IP, _D, _I, _I, _I, _I, _I, _IP, _DP, _DP, _DP, _DP, _P, _P,
+                                        ctypes.POINTER(_P)]),
     "rbplugin_destroy": (_I, [_P]),
     "rbplugin_set_positions": (_I, [_P, _DP]),
     "rbplugin_set_velocities": (_I, [_P, _DP]),
@@ -72,6 +74,10 @@ def _check(status):
         raise PluginError(lib().rbplugin_last_error().decode())
 
 
+# rbplugin_force_fn: double (*)(void* user, int numAtoms, const double* positions, double* forces)
+_FORCE_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_int, _DP, _DP)
+
+
 def _dp(a):
     return a.ctypes.data_as(_DP)
 
@@ -89,8 +95,13 @@ class Simulation:
     """System + RigidBodyIntegrator + Context (HIP platform) of the C++ host layer."""
 
     def __init__(self, masses, body_indices, step_size, rotation_mode=0, precision="mixed", device=0,
-                 bonds=None, tether_k=None, tether_r0=None, refined_energies=False):
+                 bonds=None, tether_k=None, tether_r0=None, refined_energies=False, host_force=None):
+        """host_force: a force field evaluated on the host, f(positions) ->
+        (forces, energy) with (n, 3) arrays in original atom order; it runs
+        at every force evaluation (one execute() per step)."""
         self._h = None
+        self._callback = None
+        self._callback_error = None
         m = np.ascontiguousarray(masses, dtype=np.float64)
         bi = np.ascontiguousarray(body_indices, dtype=np.int32)
         n = m.size
@@ -111,11 +122,32 @@ class Simulation:
             tkp, trp = _dp(tk), _dp(tr)
         else:
             tkp, trp = None, None
+        fn = None
+        if host_force is not None:
+            def trampoline(_user, count, xp, fp):
+                # a Python error cannot cross the C++ layer: it is kept, the
+                # forces left at zero, and raised by the call that ran it
+                try:
+                    x = np.ctypeslib.as_array(xp, shape=(count, 3)).copy()
+                    f, e = host_force(x)
+                    np.ctypeslib.as_array(fp, shape=(count, 3))[:] += np.asarray(f, dtype=np.float64)
+                    return float(e)
+                except BaseException as err:   # noqa: BLE001
+                    self._callback_error = err
+                    return 0.0
+            fn = _FORCE_FN(trampoline)
+            self._callback = fn
         h = _P()
-        _check(lib().rbplugin_create_ex(n, _dp(m), _ip(bi), float(step_size), int(rotation_mode),
-                                        int(bool(refined_energies)), prec, int(device), nb, pp, lp, kp, tkp, trp,
-                                        ctypes.byref(h)))
+        self._raise_after(lib().rbplugin_create_with_force(
+            n, _dp(m), _ip(bi), float(step_size), int(rotation_mode), int(bool(refined_energies)), prec,
+            int(device), nb, pp, lp, kp, tkp, trp, fn, None, ctypes.byref(h)))
         self._h = h
+
+    def _raise_after(self, status):
+        err, self._callback_error = self._callback_error, None
+        if err is not None:
+            raise PluginError(f"host force failed: {err!r}") from err
+        _check(status)
 
     @classmethod
     def from_spec(cls, spec, precision="mixed", device=0, refined_energies=False):
@@ -141,14 +173,14 @@ class Simulation:
 
     def set_positions(self, x):
         a = np.ascontiguousarray(x, dtype=np.float64).reshape(self.num_atoms, 3)
-        _check(lib().rbplugin_set_positions(self._h, _dp(a)))
+        self._raise_after(lib().rbplugin_set_positions(self._h, _dp(a)))
 
     def set_velocities(self, v):
         a = np.ascontiguousarray(v, dtype=np.float64).reshape(self.num_atoms, 3)
-        _check(lib().rbplugin_set_velocities(self._h, _dp(a)))
+        self._raise_after(lib().rbplugin_set_velocities(self._h, _dp(a)))
 
     def step(self, steps):
-        _check(lib().rbplugin_step(self._h, int(steps)))
+        self._raise_after(lib().rbplugin_step(self._h, int(steps)))
 
     def set_step_size(self, dt):
         _check(lib().rbplugin_set_step_size(self._h, float(dt)))
@@ -161,7 +193,7 @@ class Simulation:
         v = np.empty((n, 3)) if velocities else None
         f = np.empty((n, 3)) if forces else None
         e = np.empty(3) if energy else None
-        _check(lib().rbplugin_get_state(self._h, *(None if a is None else _dp(a) for a in (x, v, f, e))))
+        self._raise_after(lib().rbplugin_get_state(self._h, *(None if a is None else _dp(a) for a in (x, v, f, e))))
         out = {}
         if positions:
             out["positions"] = x

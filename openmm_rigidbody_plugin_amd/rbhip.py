@@ -62,6 +62,10 @@ def lib() -> ctypes.CDLL:
             "rbhip_set_tether": (_I, [_P, _DP, _DP]),
             "rbhip_add_bonds": (_I, [_P, _I, _IP, _DP, _DP]),
             "rbhip_compute_forces": (_I, [_P, _DP]),
+            "rbhip_set_forces": (_I, [_P, _DP]),
+            "rbhip_adopt_arrays": (_I, [_P, _P, _P, _P, _P, _I, _IP, _I]),
+            "rbhip_set_atom_order": (_I, [_P, _IP, _I]),
+            "rbhip_state_changed": (_I, [_P, _I]),
             "rbhip_set_positions": (_I, [_P, _DP]),
             "rbhip_set_velocities": (_I, [_P, _DP]),
             "rbhip_get_positions": (_I, [_P, _DP]),
@@ -100,6 +104,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_batch_autotune": (_I, [_P, _I]),
             "rbhip_batch_describe": (_I, [_P, ctypes.c_char_p, _I]),
             "rbhip_estimate_bytes": (_I, [_I, _IP, _I, _SIZEP, _IP]),
+            "rbhip_body_bookkeeping": (_I, [_I, _P, _P, _P, _P, _P, _IP, _I, _P]),
             "rbhip_server_last_error": (ctypes.c_char_p, []),
             "rbhip_server_create": (_I, [_I, ctypes.c_size_t, _I, _I, _I, ctypes.POINTER(_P)]),
             "rbhip_server_destroy": (_I, [_P]),
@@ -275,6 +280,29 @@ class Context:
         _check(lib().rbhip_compute_forces(self._h, ctypes.byref(e) if energy else None))
         return e.value if energy else None
 
+    def set_forces(self, f):
+        """Forces from outside the library (the force slot): (n, 3) FP64 in
+        original atom order, for the next part 2 (see rbhip_set_forces)."""
+        _check(lib().rbhip_set_forces(self._h, _dp(_vec_in(f, self.n))))
+
+    # ---- adopted arrays in OpenMM's GPU layout ----
+    STATE_POSITIONS, STATE_VELOCITIES = 1, 2
+
+    def adopt_arrays(self, posq, posq_correction, velm, force, padded_num_atoms, atom_order):
+        """Device pointers (ints) of posq, posqCorrection (None in double
+        precision), velm and the fixed-point force array, their padded
+        length, and the atom order (slot -> original atom)."""
+        order = np.ascontiguousarray(atom_order, dtype=np.int32)
+        _check(lib().rbhip_adopt_arrays(self._h, posq, posq_correction, velm, force, int(padded_num_atoms),
+                                        _ip(order), order.size))
+
+    def set_atom_order(self, atom_order):
+        order = np.ascontiguousarray(atom_order, dtype=np.int32)
+        _check(lib().rbhip_set_atom_order(self._h, _ip(order), order.size))
+
+    def state_changed(self, what):
+        _check(lib().rbhip_state_changed(self._h, int(what)))
+
     # ---- state ----
     def set_positions(self, x):
         _check(lib().rbhip_set_positions(self._h, _dp(_vec_in(x, self.n))))
@@ -444,6 +472,16 @@ class Batch:
         tot = ctypes.c_double()
         _check(lib().rbhip_batch_time_kernels(self._h, int(steps), _dp(avg), ctypes.byref(tot)))
         return dict(zip(KERNELS, avg)), tot.value
+
+
+def body_bookkeeping(num_atoms, body_indices_ptr, masses_ptr, body_index_out, atom_index_out, body_start_out,
+                     device=0, stream=None):
+    """Device bookkeeping (rbhip_body_bookkeeping) on device arrays given by
+    their addresses; returns (numFree, numBodies, numMassless, numBodyAtoms)."""
+    c = np.zeros(4, dtype=np.int32)
+    _check(lib().rbhip_body_bookkeeping(int(num_atoms), body_indices_ptr, masses_ptr, body_index_out, atom_index_out,
+                                        body_start_out, _ip(c), int(device), stream))
+    return tuple(int(x) for x in c)
 
 
 def estimate_bytes(body_indices, precision="mixed"):

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librbref.so")
+# RBREF_LIB selects another build of the same source (the CPU baseline times
+# one built for its host)
+LIB_PATH = os.environ.get("RBREF_LIB") or os.path.join(HERE, "librbref.so")
 REF_ELLIPTIC_PATH = os.path.join(HERE, "_ref", "libref_elliptic.so")
 
 FIX_PCM_RESET = 1

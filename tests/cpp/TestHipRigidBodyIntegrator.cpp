@@ -7,7 +7,11 @@
 //                        atoms on a harmonic bond are an analytic oscillator
 //   testRigidDimerSpin   a free rigid dimer keeps its bond length, its centre
 //                        of mass moves uniformly, its angular momentum and
-//                        kinetic energy are conserved, it rotates at |L|/I
+//                        kinetic energy are conserved, it rotates at |L|/I,
+//                        and each atom moves at v_cm + w e_z x (r_i - c)
+//   testHostForce        the force slot: a tether evaluated on the host
+//                        (HostForce) drives the same trajectory as the
+//                        device tether, alone and added to device forces
 //   testApiErrors        the reference's exceptions (negative rotation mode,
 //                        mode change after binding, body-index count, body
 //                        constraints) and this platform's documented limits
@@ -150,6 +154,8 @@ void testRigidDimerSpin(Platform& platform, const std::string& precision, int mo
     CHECK_VEC(Vec3(d * std::cos(w * t), d * std::sin(w * t), 0), bond, 2e-4);
     Vec3 vc = (V[0] * m1 + V[1] * m2) * (1.0 / M);
     CHECK_VEC(vcm, vc, 1e-5);
+    // each atom: v_i = v_cm + w e_z x (r_i - c) (RigidBody.cpp:159-168)
+    for (int i = 0; i < 2; i++) CHECK_VEC(vcm + Vec3(0, 0, w).cross(R[i] - c), V[i], 2e-5);
     CHECK_TOL(ke0, st.getKineticEnergy(), 1e-5);
     std::vector<double> ke = integrator.getKineticEnergies();
     CHECK_TOL(0.5 * I * w * w, ke[1], 1e-5);
@@ -307,6 +313,83 @@ void testVirtualSites(Platform& platform, const std::string& precision) {
     }
 }
 
+// tethered rigid waters and free atoms; the tether is split as asked
+// between the device TetherForce (fraction kDevice of k) and a HostForce
+// evaluating the rest on the host
+static State runTethered(Platform& platform, const std::string& precision, int mode, double kDevice, int steps,
+                         bool& multiStep) {
+    const double k = 800.0;
+    const Vec3 tmpl[3] = {Vec3(0, 0, 0.0066), Vec3(0, 0.0757, -0.0520), Vec3(0, -0.0757, -0.0520)};
+    const double wm[3] = {15.9994, 1.008, 1.008};
+    System system;
+    std::vector<int> bodies;
+    std::vector<Vec3> pos, vel;
+    for (int b = 0; b < 27; b++) {
+        Vec3 c(0.4 * (b % 3), 0.4 * ((b / 3) % 3), 0.4 * (b / 9));
+        double a = 0.7 * b, ca = std::cos(a), sa = std::sin(a);
+        for (int j = 0; j < 3; j++) {
+            system.addParticle(wm[j]);
+            bodies.push_back(b + 1);
+            Vec3 t = tmpl[j];
+            pos.push_back(c + Vec3(ca * t[0] - sa * t[1], sa * t[0] + ca * t[1], t[2]));
+            vel.push_back(Vec3(0.3 * std::sin(1.3 * b + j), -0.2 * std::cos(0.7 * b - j), 0.25 * std::sin(b * j + 0.5)));
+        }
+    }
+    for (int f = 0; f < 10; f++) {
+        system.addParticle(39.948);
+        bodies.push_back(0);
+        pos.push_back(Vec3(1.3 + 0.1 * f, 0.2 * f, -0.3));
+        vel.push_back(Vec3(0.1 * f - 0.4, 0.05 * f, 0.2));
+    }
+    const int n = system.getNumParticles();
+    std::vector<Vec3> r0(n);
+    for (int i = 0; i < n; i++) r0[i] = pos[i] + Vec3(0.01 * std::sin(i), -0.01 * std::cos(2.0 * i), 0.005);
+    if (kDevice > 0.0) {
+        TetherForce* tf = new TetherForce();
+        for (int i = 0; i < n; i++) tf->addParticle(i, kDevice * k, r0[i]);
+        system.addForce(tf);
+    }
+    if (kDevice < 1.0) {
+        const double kh = (1.0 - kDevice) * k;
+        system.addForce(new HostForce([r0, kh](const std::vector<Vec3>& x, std::vector<Vec3>& f) {
+            double e = 0.0;
+            for (size_t i = 0; i < x.size(); i++) {
+                Vec3 d = x[i] - r0[i];
+                f[i] = f[i] + d * (-kh);
+                e += 0.5 * kh * d.dot(d);
+            }
+            return e;
+        }));
+    }
+    RigidBodyIntegrator integrator(0.002, bodies);
+    integrator.setRotationMode(mode);
+    Context context(system, integrator, platform, props(precision));
+    context.setPositions(pos);
+    context.setVelocities(vel);
+    // device forces only: one multi-step call; with a HostForce, one
+    // execute() per step with the host force evaluation in between
+    multiStep = kDevice >= 1.0;
+    integrator.step(steps);
+    return context.getState(State::Positions | State::Velocities | State::Energy);
+}
+
+void testHostForce(Platform& platform, const std::string& precision, int mode) {
+    const double tol = precision == "mixed" ? 1e-6 : 1e-10;
+    bool m0, m1, m2;
+    State dev = runTethered(platform, precision, mode, 1.0, 20, m0);
+    State host = runTethered(platform, precision, mode, 0.0, 20, m1);
+    State split = runTethered(platform, precision, mode, 0.5, 20, m2);
+    for (const State* s : {&host, &split}) {
+        for (size_t i = 0; i < dev.getPositions().size(); i++) {
+            CHECK_VEC(dev.getPositions()[i], s->getPositions()[i], tol);
+            CHECK_VEC(dev.getVelocities()[i], s->getVelocities()[i], 100 * tol);
+        }
+        CHECK_TOL(dev.getKineticEnergy(), s->getKineticEnergy(), 100 * tol);
+        CHECK_TOL(dev.getPotentialEnergy(), s->getPotentialEnergy(), 100 * tol);
+        CHECK_TOL(dev.getTime(), s->getTime(), 1e-12);
+    }
+}
+
 void testApiErrors(Platform& platform) {
     RigidBodyIntegrator a(0.001, std::vector<int>{1, 1});
     CHECK_THROWS(a.setRotationMode(-1));
@@ -416,6 +499,9 @@ int main() {
         {"testConstrainedClusters double", [&] { testConstrainedClusters(platform, "double"); }},
         {"testVirtualSites mixed", [&] { testVirtualSites(platform, "mixed"); }},
         {"testVirtualSites double", [&] { testVirtualSites(platform, "double"); }},
+        {"testHostForce mixed NO-SQUISH", [&] { testHostForce(platform, "mixed", 10); }},
+        {"testHostForce mixed exact", [&] { testHostForce(platform, "mixed", 0); }},
+        {"testHostForce double exact", [&] { testHostForce(platform, "double", 0); }},
     };
     for (auto& t : tests) {
         std::printf("%s ...\n", t.first.c_str());

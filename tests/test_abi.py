@@ -88,3 +88,15 @@ def test_estimate_counts_bodies_whatever_the_labels():
     # every atom free, and one body holding every atom
     assert estimate(np.zeros(10, int))[1] == 0
     assert estimate(np.full(10, 123456789))[1] == 1
+
+
+def test_graph_cache_logic():
+    """The keyed graph cache (csrc/rbhip_graph_cache.h) on the CPU with fake
+    handles: replay only under the capture key, destroy only after the
+    graph's stream was waited for (tests/cpp/TestGraphCache.cpp)."""
+    import subprocess
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    subprocess.run(["make", "-C", cpp, "_bin/TestGraphCache"], check=True, capture_output=True)
+    r = subprocess.run([os.path.join(cpp, "_bin", "TestGraphCache")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("Done")

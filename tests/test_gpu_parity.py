@@ -1,35 +1,19 @@
 """HIP path vs the FP64 oracle on the same seeded inputs, through the C ABI.
 
-Tolerances (after 10 steps):
-  mixed precision (fp32 state, FP64 accumulation): positions <= 1e-6 nm;
-    each body's momentum and space-frame angular momentum within 1e-5 of its
-    own magnitude, above an absolute floor of 1e-6 of the RMS over the
-    bodies (the error fp32 positions put into the forces over 10 steps,
-    k ulp(x) 10 dt, is about 1e-6 of a 300 K water's momentum, so the
-    relative bound applies to every body down to a tenth of the RMS);
-    free-atom velocities likewise per atom; kinetic energies <= 1e-5 relative
-  double precision: positions <= 1e-10 nm; momenta <= 1e-9 relative above a
-    floor of 1e-10 of the RMS; KE <= 1e-11
+Tolerances (after 10 steps) are those of tests/tolerances.py: in mixed
+precision positions <= 1e-6 nm, each body's momentum and angular momentum
+within 1e-5 of its own magnitude above the floor the force field's fp32
+rounding sets, free-atom velocities likewise, kinetic energies <= 1e-5; in
+double precision 1e-10 nm, 1e-9 relative (floor 1e-10 of the RMS), 1e-11.
 Integer bookkeeping (body numbering, atom order, counts, DOF) is bit-exact.
 """
 import numpy as np
 import pytest
 
 from openmm_rigidbody_plugin_amd import systems
-from subsample import body_frames
+from tolerances import TOL, check_state
 
 pytestmark = pytest.mark.gpu
-
-TOL = {"mixed": dict(x=1e-6, p=1e-5, floor=1e-6, ke=1e-5), "double": dict(x=1e-10, p=1e-9, floor=1e-10, ke=1e-11)}
-
-
-def per_item_within(ref, got, rel, floor):
-    """|got_i - ref_i| <= rel |ref_i| + floor * RMS_i |ref_i| for every row i;
-    returns the largest ratio of the error to its bound."""
-    mag = np.linalg.norm(ref, axis=1)
-    bound = rel * mag + floor * np.sqrt((mag ** 2).mean())
-    ratio = np.linalg.norm(got - ref, axis=1) / bound
-    return ratio.max()
 
 
 def configs():
@@ -41,26 +25,10 @@ def configs():
     }
 
 
-def compare_states(spec, o, g, tol):
-    xo, xg = o.positions(), g.positions()
-    vo, vg = o.velocities(), g.velocities()
-    assert np.all(np.isfinite(xg)) and np.all(np.isfinite(vg))
-    assert np.abs(xo - xg).max() <= tol["x"], np.abs(xo - xg).max()
-    fo = body_frames(spec.masses, spec.body_indices, xo, vo)
-    fg = body_frames(spec.masses, spec.body_indices, xg, vg)
-    P = np.array([fo[b][1] for b in fo]); Pg = np.array([fg[b][1] for b in fo])
-    L = np.array([fo[b][2] for b in fo]); Lg = np.array([fg[b][2] for b in fo])
-    rp = per_item_within(P, Pg, tol["p"], tol["floor"])
-    rl = per_item_within(L, Lg, tol["p"], tol["floor"])
-    assert rp <= 1.0, f"momentum error {rp:.3g} x its per-body bound"
-    assert rl <= 1.0, f"angular momentum error {rl:.3g} x its per-body bound"
-    free = spec.body_indices <= 0
-    if free.any():
-        rv = per_item_within(vo[free], vg[free], tol["p"], tol["floor"])
-        assert rv <= 1.0, f"free-atom velocity error {rv:.3g} x its per-atom bound"
-    print(f"per-body bound ratios: p {rp:.3g}, L {rl:.3g}")
+def compare_states(spec, o, g, precision, steps=10):
+    check_state(spec, o.positions(), o.velocities(), g.positions(), g.velocities(), precision, steps)
     ko, kg = o.kinetic_energies(), g.kinetic_energies()
-    assert np.allclose(kg, ko, rtol=tol["ke"], atol=0)
+    assert np.allclose(kg, ko, rtol=TOL[precision]["ke"], atol=0)
 
 
 @pytest.mark.parametrize("precision", ["mixed", "double"])
@@ -75,7 +43,7 @@ def test_ten_steps_match_oracle(hip, oracle, name, precision):
     assert np.allclose(kg, ko, rtol=tol["ke"], atol=0)
     o.step(10)
     g.step(10)
-    compare_states(spec, o, g, tol)
+    compare_states(spec, o, g, precision)
     t, nsteps = g.time()
     assert t == pytest.approx(o.time(), rel=1e-15) and nsteps == 10
 
@@ -195,7 +163,7 @@ def test_set_velocities_after_steps_and_step_size_change(hip, oracle):
     g.set_step_size(0.0015)
     o.step(5)
     g.step(5)
-    compare_states(spec, o, g, TOL["double"])
+    compare_states(spec, o, g, "double")
 
 
 def test_autotune_keeps_state(hip, oracle):
@@ -210,7 +178,7 @@ def test_autotune_keeps_state(hip, oracle):
     assert "L" in g.describe()
     g.step(7)
     o.step(7)
-    compare_states(spec, o, g, TOL["mixed"])
+    compare_states(spec, o, g, "mixed")
 
 
 @pytest.mark.parametrize("lanes", [2, 4, 8])
@@ -222,7 +190,7 @@ def test_lanes_per_body_tiles(hip, oracle, lanes):
         g.set_lanes(n, lanes)
     g.step(10)
     o.step(10)
-    compare_states(spec, o, g, TOL["double"])
+    compare_states(spec, o, g, "double")
 
 
 def test_errors_are_reported(hip):

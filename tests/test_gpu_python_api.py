@@ -72,3 +72,47 @@ def test_python_api_refined_energies_and_step_size(hip):
     assert it.getPotentialEnergyRefinement() == e["potential_refinement"]
     assert it.getRefinedKineticEnergies() != it.getKineticEnergies()
     np.testing.assert_array_equal(ctx.getState(getPositions=True)["positions"], sim.state()["positions"])
+
+
+@pytest.mark.parametrize("mode", [10, 0])
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_python_force_field_through_the_slot(hip, oracle, mode, precision):
+    """A force field written in Python (the tether, in numpy) passed as the
+    Context's `force` runs between part 1 and part 2 of every step and gives
+    the oracle's trajectory (C1 tolerances, tests/tolerances.py)."""
+    from tolerances import TOL, check_state
+    spec = systems.water_c1(rotation_mode=mode)
+    calls = []
+
+    def tether(x):
+        calls.append(1)
+        d = x - spec.tether_r0
+        return -spec.tether_k[:, None] * d, 0.5 * float((spec.tether_k[:, None] * d * d).sum())
+
+    it = RigidBodyIntegrator(spec.dt, spec.body_indices)
+    it.setRotationMode(mode)
+    ctx = Context(spec.masses, it, precision=precision, force=tether)
+    ctx.setPositions(spec.positions)
+    ctx.setVelocities(spec.velocities)
+    n0 = len(calls)
+    it.step(10)
+    assert len(calls) - n0 == 10          # one force evaluation per step
+    ref = oracle.OracleSim.from_spec(spec)
+    ref.step(10)
+    st = ctx.getState(getPositions=True, getVelocities=True, getEnergy=True)
+    check_state(spec, ref.positions(), ref.velocities(), st["positions"], st["velocities"], precision, 10)
+    np.testing.assert_allclose(it.getKineticEnergies(), ref.kinetic_energies(), rtol=TOL[precision]["ke"])
+    assert st["potential"] == pytest.approx(ref.potential_energy(), rel=1e-6)
+
+
+def test_python_force_field_errors_are_raised(hip):
+    from openmm_rigidbody_plugin_amd import plugin
+    spec = systems.water_box(8, 0.3125, seed=3)
+
+    def broken(x):
+        raise ValueError("no forces today")
+
+    it = RigidBodyIntegrator(spec.dt, spec.body_indices)
+    ctx = Context(spec.masses, it, force=broken)
+    with pytest.raises(plugin.PluginError, match="no forces today"):
+        ctx.setPositions(spec.positions)
