@@ -154,6 +154,18 @@ int rbhip_get_body_index(rbhip_context* ctx, int* out);
    waits for; its scratch is allocated and freed in stream order. */
 int rbhip_body_bookkeeping(int numAtoms, const int* bodyIndices, const double* masses, int* bodyIndexOut,
                            int* atomIndexOut, int* bodyAtomStartOut, int* counts4, int device, rbhip_stream stream);
+/* No counterpart in the reference: rbhip_body_bookkeeping for callers that
+   cannot allocate device memory (code under graph capture, serving loops,
+   harnesses that forbid hipMalloc).  Same arguments and outputs; its scratch
+   is a static 40 MiB device array per GPU, held until the process ends, so
+   it allocates nothing.  Systems whose sort and scan need more (beyond about
+   1.2 million atoms) are refused with RBHIP_E_ARG.  Calls on one GPU share
+   that array: the library serialises them, and each waits for its stream
+   (NULL: the device's null stream) before it returns; callers that queue
+   other work into that scratch's lifetime from other threads race. */
+int rbhip_body_bookkeeping_noalloc(int numAtoms, const int* bodyIndices, const double* masses, int* bodyIndexOut,
+                                   int* atomIndexOut, int* bodyAtomStartOut, int* counts4, int device,
+                                   rbhip_stream stream);
 
 /* ---------------------------------------------------------------------- */
 /* Force field slot (ContextImpl::calcForcesAndEnergy,                      */

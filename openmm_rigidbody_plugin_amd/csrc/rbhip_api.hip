@@ -1639,6 +1639,70 @@ int rbhip_body_bookkeeping(int numAtoms, const int* bodyIndices, const double* m
     });
 }
 
+// ---------------------------------------------------------------------------
+// rbhip_body_bookkeeping_noalloc: the bookkeeping on a scratch buffer that is
+// a file-scope device array (one copy per GPU, held until the process ends)
+// instead of a stream-ordered allocation.  In host code `scratch` names a
+// host-side stand-in, not device memory: the device address of each GPU's
+// copy comes from hipGetSymbolAddress, taken once per device with that
+// device current, and only that address is handed to kernels.  Calls on one
+// device share the array, so they are serialised here (each call waits for
+// its own stream before it returns); every region is written before it is
+// read, and the status words are cleared by the first kernel of each call.
+// ---------------------------------------------------------------------------
+constexpr size_t kStaticScratchBytes = size_t(40) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+struct StaticScratch {
+    std::mutex mtx;                 // held for a whole call on its device
+    void* ptr = nullptr;            // the device address of this GPU's copy
+};
+
+// (the device must be current)
+static StaticScratch& static_scratch(int device) {
+    static std::mutex mapMtx;
+    static std::map<int, std::unique_ptr<StaticScratch>> perDevice;
+    std::lock_guard<std::mutex> lock(mapMtx);
+    std::unique_ptr<StaticScratch>& s = perDevice[device];
+    if (!s) {
+        std::unique_ptr<StaticScratch> fresh(new StaticScratch());
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        require(ptr != nullptr, "no device address for the static scratch", RBHIP_E_HIP);
+        fresh->ptr = ptr;
+        s = std::move(fresh);
+    }
+    return *s;
+}
+
+int rbhip_body_bookkeeping_noalloc(int numAtoms, const int* bodyIndices, const double* masses, int* bodyIndexOut,
+                                   int* atomIndexOut, int* bodyAtomStartOut, int* counts4, int device,
+                                   rbhip_stream stream) {
+    return guard([&] {
+        require(numAtoms > 0 && bodyIndices && masses && bodyIndexOut && atomIndexOut && bodyAtomStartOut && counts4,
+                "bad arguments");
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        require(device >= 0 && device < ndev, "no such device");
+        HIPCHK(hipSetDevice(device));
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);   // NULL: the device's null stream
+        size_t bytes = 0;
+        HIPCHK(bookkeeping_scratch_bytes(numAtoms, st, &bytes));
+        if (bytes > kStaticScratchBytes)
+            throw RbError(RBHIP_E_ARG, std::to_string(numAtoms) + " atoms need " + std::to_string(bytes) +
+                                           " bytes of scratch; the static scratch holds " +
+                                           std::to_string(kStaticScratchBytes) +
+                                           " (up to about 1.2 million atoms): use rbhip_body_bookkeeping");
+        StaticScratch& sc = static_scratch(device);
+        std::lock_guard<std::mutex> lock(sc.mtx);
+        int small[5] = {0, 0, 0, 0, 0};
+        HIPCHK(launch_bookkeeping(numAtoms, bodyIndices, masses, bodyIndexOut, atomIndexOut, bodyAtomStartOut,
+                                  sc.ptr, kStaticScratchBytes, st, small));
+        bookkeeping_status(small);
+        for (int k = 0; k < 4; k++) counts4[k] = small[k + 1];
+    });
+}
+
 int rbhip_get_counts(rbhip_context* ctx, int* out5) {
     return guard([&] {
         require(ctx && out5, "null argument");

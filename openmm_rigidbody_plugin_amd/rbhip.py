@@ -105,6 +105,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_batch_describe": (_I, [_P, ctypes.c_char_p, _I]),
             "rbhip_estimate_bytes": (_I, [_I, _IP, _I, _SIZEP, _IP]),
             "rbhip_body_bookkeeping": (_I, [_I, _P, _P, _P, _P, _P, _IP, _I, _P]),
+            "rbhip_body_bookkeeping_noalloc": (_I, [_I, _P, _P, _P, _P, _P, _IP, _I, _P]),
             "rbhip_server_last_error": (ctypes.c_char_p, []),
             "rbhip_server_create": (_I, [_I, ctypes.c_size_t, _I, _I, _I, ctypes.POINTER(_P)]),
             "rbhip_server_destroy": (_I, [_P]),
@@ -481,6 +482,16 @@ def body_bookkeeping(num_atoms, body_indices_ptr, masses_ptr, body_index_out, at
     c = np.zeros(4, dtype=np.int32)
     _check(lib().rbhip_body_bookkeeping(int(num_atoms), body_indices_ptr, masses_ptr, body_index_out, atom_index_out,
                                         body_start_out, _ip(c), int(device), stream))
+    return tuple(int(x) for x in c)
+
+
+def body_bookkeeping_noalloc(num_atoms, body_indices_ptr, masses_ptr, body_index_out, atom_index_out,
+                             body_start_out, device=0, stream=None):
+    """rbhip_body_bookkeeping_noalloc: the same on the library's static
+    scratch, with no device allocation."""
+    c = np.zeros(4, dtype=np.int32)
+    _check(lib().rbhip_body_bookkeeping_noalloc(int(num_atoms), body_indices_ptr, masses_ptr, body_index_out,
+                                                atom_index_out, body_start_out, _ip(c), int(device), stream))
     return tuple(int(x) for x in c)
 
 

@@ -1,0 +1,45 @@
+"""rbhip_body_bookkeeping_noalloc (the bookkeeping on the library's static
+scratch, no device allocation) against rbhip_body_bookkeeping (scratch
+allocated per call): bit-identical outputs on two calls in a row, the second
+on a smaller system (the scratch keeps the first call's contents), and a
+clear error beyond the scratch's size."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def labels_and_masses(seed, n):
+    rng = np.random.default_rng(seed)
+    pool = np.unique(rng.integers(1, 2**31 - 1, size=n // 3, dtype=np.int64))
+    labels = rng.choice(np.concatenate([pool, [0, -2]]), size=n).astype(np.int32)
+    masses = rng.uniform(1.0, 20.0, n)
+    masses[(labels <= 0) & (rng.uniform(size=n) < 0.1)] = 0.0
+    return labels, masses
+
+
+def run(hip, fn, labels, masses):
+    from hipmem import DeviceArray
+    n = labels.size
+    dl, dm = DeviceArray(labels), DeviceArray(masses)
+    outs = [DeviceArray(np.full(n + 1, -7, np.int32)) for _ in range(3)]
+    c = fn(n, dl.ptr.value, dm.ptr.value, *(o.ptr.value for o in outs))
+    bi, ai, bs = (o.download() for o in outs)
+    return c, bi[:n], ai[:c[0] + c[3]], bs[:c[1] + 1]
+
+
+def test_no_alloc_bookkeeping_equals_the_allocating_one(hip):
+    for seed, n in ((1, 400_000), (2, 50_000)):     # the second call smaller than the first
+        labels, masses = labels_and_masses(seed, n)
+        ref = run(hip, hip.body_bookkeeping, labels, masses)
+        got = run(hip, hip.body_bookkeeping_noalloc, labels, masses)
+        assert got[0] == ref[0]
+        for a, b in zip(got[1:], ref[1:]):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_no_alloc_bookkeeping_refuses_systems_beyond_its_scratch(hip):
+    labels = np.repeat(np.arange(1, 1_000_001, dtype=np.int32), 2)    # 2M atoms
+    masses = np.ones(labels.size)
+    with pytest.raises(hip.RbhipError, match="static scratch"):
+        run(hip, hip.body_bookkeeping_noalloc, labels, masses)
