@@ -366,6 +366,7 @@ static size_t context_arena_bytes(size_t na, int numBodies, int numFree, int num
     b += a(4 * sizeof(double) * maxKe) + a(4 * sizeof(double) * maxPe);            // partials
     b += a(sizeof(DevSys)) + a(4 * sizeof(int)) + a(sizeof(int) * nb) * 2;
     b += a(sizeof(double) * std::max(3 * na, 36 * nb)) + a(4 * sizeof(double));  // staging, reduction
+    b += a(sizeof(int) * na) * 3;                      // body index, atom index, internal order
     return b;
 }
 
@@ -395,11 +396,31 @@ struct rbhip_context {
     DeviceArena siteArena;
     std::vector<int> clusterN;                   // atoms per cluster (small classes first)
     std::vector<int> clusterK;                   // constraints per cluster
-    std::vector<int> bodyIndex, atomIndex;
+    // built on the device: the reference's cleanBodyIndices result and
+    // atomIndex, and the internal order (internal atom -> original atom)
+    int* dRefBodyIndex = nullptr;
+    int* dRefAtomIndex = nullptr;
+    int* dAtomOrig = nullptr;
     std::vector<int> refBodyN, refBodyLoc;       // per reference body
     std::vector<int> internalBody;               // internal body -> reference body
     std::vector<int> bodyN, bodyAtom0;           // per internal body
-    std::vector<int> atomOrig;                   // internal atom -> original atom
+    // host copies of the device arrays above, read back when first needed
+    std::vector<int> hostBodyIndex, hostAtomIndex, hostAtomOrig;
+    const std::vector<int>& hostCopy(std::vector<int>& copy, const int* src, size_t count) {
+        if (copy.size() != count) {
+            std::vector<int> v(count);
+            if (count > 0) {
+                setDevice();
+                HIPCHK(hipMemcpyAsync(v.data(), src, count * sizeof(int), hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipStreamSynchronize(stream));
+            }
+            copy.swap(v);
+        }
+        return copy;
+    }
+    const std::vector<int>& bodyIndex() { return hostCopy(hostBodyIndex, dRefBodyIndex, numAtoms); }
+    const std::vector<int>& atomIndex() { return hostCopy(hostAtomIndex, dRefAtomIndex, numFree + numBodyAtoms); }
+    const std::vector<int>& atomOrig() { return hostCopy(hostAtomOrig, dAtomOrig, numAtoms); }
     struct Class { int N, firstBody, count, firstAtom; };
     std::vector<Class> classes;
     std::map<int, int> lanes;                    // atoms per body -> lanes per body
@@ -783,31 +804,20 @@ struct rbhip_context {
         if (geometry) numDOF = numFree - numConstraints + (int)std::lround(r[2]);
     }
 
-    // original-order FP64 xyz -> internal order
-    std::vector<double> toInternal(const double* xyz) const {
-        std::vector<double> out(3 * (size_t)numAtoms);
-        for (int i = 0; i < numAtoms; i++) {
-            int o = atomOrig[i];
-            out[3 * (size_t)i] = xyz[3 * (size_t)o];
-            out[3 * (size_t)i + 1] = xyz[3 * (size_t)o + 1];
-            out[3 * (size_t)i + 2] = xyz[3 * (size_t)o + 2];
-        }
-        return out;
+    // original-order FP64 xyz rows (host) -> the staging buffer dTmp, still
+    // in original order: the kernels that read it gather through dAtomOrig
+    void uploadRows(const double* xyz, size_t rowLength) {
+        HIPCHK(hipMemcpyAsync(dTmp, xyz, rowLength * (size_t)numAtoms * sizeof(double), hipMemcpyHostToDevice,
+                              stream));
     }
 
+    // the state in original order: the kernels scatter through dAtomOrig
     void download(int what, double* xyz) {
-        HIPCHK(launch_stage_out(precision, dSys, dTmp, numAtoms, what, stream, layout()));
+        HIPCHK(launch_stage_out(precision, dSys, dTmp, numAtoms, what, stream, layout(), dAtomOrig));
         if (what == 0 && positionsDerived && precision == RBHIP_PRECISION_MIXED && !adopted)
-            HIPCHK(launch_body_atoms_out(precision, dSys, dBodyN, dBodyAtom0, numBodies, dTmp, stream));
-        std::vector<double> tmp(3 * (size_t)numAtoms);
-        HIPCHK(hipMemcpyAsync(tmp.data(), dTmp, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+            HIPCHK(launch_body_atoms_out(precision, dSys, dBodyN, dBodyAtom0, numBodies, dTmp, stream, dAtomOrig));
+        HIPCHK(hipMemcpyAsync(xyz, dTmp, 3 * (size_t)numAtoms * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        for (int i = 0; i < numAtoms; i++) {
-            int o = atomOrig[i];
-            xyz[3 * (size_t)o] = tmp[3 * (size_t)i];
-            xyz[3 * (size_t)o + 1] = tmp[3 * (size_t)i + 1];
-            xyz[3 * (size_t)o + 2] = tmp[3 * (size_t)i + 2];
-        }
     }
 
     // Forces replaced outside a step: momenta that carry the next step's
@@ -1109,6 +1119,12 @@ int rbhip_device_synchronize(void) {
     return guard([&] { HIPCHK(hipDeviceSynchronize()); });
 }
 
+// the checks of rbhip_create on the device bookkeeping's status flags
+static void bookkeeping_status(const int* small) {
+    if (small[0] & 2) throw RbError(RBHIP_E_ARG, "negative or invalid mass");
+    if (small[0] & 1) throw RbError(RBHIP_E_ARG, "massless particles cannot belong to rigid bodies");
+}
+
 static int create_context(int numAtoms, const double* masses, const int* bodyIndices, int numConstraints,
                           const int* constraintAtoms, const double* constraintDistances, double stepSize,
                           int rotationMode, int precision, int device, rbhip_context** out) {
@@ -1135,51 +1151,79 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
         C.numAtoms = numAtoms;
         C.setDevice();
 
-        // ---- reference bookkeeping (RigidBodySystem.cpp:55-105) ----
-        C.bodyIndex = clean_body_indices(bodyIndices, numAtoms);
-        for (int v : C.bodyIndex) C.numBodies = std::max(C.numBodies, v);
-        C.refBodyN.assign(C.numBodies, 0);
-        C.atomIndex.assign(numAtoms, 0);
         // massless particles (virtual sites among them) are neither free nor
         // body atoms (RigidBodySystem.cpp:66-76); they must be labelled free,
         // because the reference's body-atom loop (:99-104) does not skip them
         std::vector<int> massless;
-        for (int i = 0; i < numAtoms; i++) {
+        for (int i = 0; i < numAtoms; i++)
             if (masses[i] == 0.0) {
-                if (C.bodyIndex[i] != 0)
+                if (bodyIndices[i] > 0)
                     throw RbError(RBHIP_E_ARG, "massless particles cannot belong to rigid bodies (particle " +
                                                    std::to_string(i) + ")");
                 massless.push_back(i);
-                continue;
             }
-            if (C.bodyIndex[i] == 0) C.atomIndex[C.numFree++] = i;
-            else C.refBodyN[C.bodyIndex[i] - 1]++;
-        }
-        C.numMassless = (int)massless.size();
-        C.numBodyAtoms = numAtoms - C.numFree - C.numMassless;
+
+        // ---- reference bookkeeping on the device (RigidBodySystem.cpp:28-105):
+        // cleanBodyIndices, atomIndex and each body's first atom, kept on the
+        // device; the host reads back one int per body
+        C.ownStream = pooled_stream(device);
+        C.stream = C.ownStream;
+        C.arena.stream = C.tileArena.stream = C.stream;
+        hipStream_t st = C.stream;
+        const size_t na = (size_t)numAtoms;
+        C.dRefBodyIndex = C.arena.allocT<int>(na);
+        C.dRefAtomIndex = C.arena.allocT<int>(na);
+        C.dAtomOrig = C.arena.allocT<int>(na);
+        DeviceArena setupArena;   // inputs and scratch of the set-up, freed at its end
+        setupArena.stream = st;
+        struct ReleaseAtExit {
+            DeviceArena& a;
+            ~ReleaseAtExit() { a.release(); }
+        } releaseSetup{setupArena};
+        int* dLabels = setupArena.allocT<int>(na);
+        double* dMassIn = setupArena.allocT<double>(na);
+        int* dRefStart = setupArena.allocT<int>(na + 1);
+        size_t bkBytes = 0;
+        HIPCHK(bookkeeping_scratch_bytes(numAtoms, st, &bkBytes));
+        void* bkScratch = setupArena.alloc(bkBytes);
+        HIPCHK(hipMemcpyAsync(dLabels, bodyIndices, na * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dMassIn, masses, na * sizeof(double), hipMemcpyHostToDevice, st));
+        int small[5] = {0, 0, 0, 0, 0};
+        HIPCHK(launch_bookkeeping(numAtoms, dLabels, dMassIn, C.dRefBodyIndex, C.dRefAtomIndex, dRefStart, bkScratch,
+                                  bkBytes, st, small));
+        bookkeeping_status(small);
+        C.numFree = small[1];
+        C.numBodies = small[2];
+        C.numMassless = small[3];
+        C.numBodyAtoms = small[4];
+        require(C.numMassless == (int)massless.size(), "internal: massless particle count", RBHIP_E_STATE);
+        std::vector<int> refStart(C.numBodies + 1);
+        HIPCHK(hipMemcpyAsync(refStart.data(), dRefStart, refStart.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        C.refBodyN.assign(C.numBodies, 0);
         C.refBodyLoc.assign(C.numBodies, 0);
-        int loc = 0;
+        int maxN = 0;
         for (int b = 0; b < C.numBodies; b++) {
+            C.refBodyLoc[b] = refStart[b];
+            C.refBodyN[b] = refStart[b + 1] - refStart[b];
             if (C.refBodyN[b] < 2)
                 throw RbError(RBHIP_E_UNSUPPORTED, "rigid body " + std::to_string(b + 1) +
                                                        " has a single atom; single-atom bodies are not supported");
-            C.refBodyLoc[b] = loc;
-            loc += C.refBodyN[b];
-        }
-        std::vector<int> fill(C.numBodies, 0);
-        for (int i = 0; i < numAtoms; i++) {
-            int b = C.bodyIndex[i];
-            if (b > 0) C.atomIndex[C.numFree + C.refBodyLoc[b - 1] + fill[b - 1]++] = i;
+            maxN = std::max(maxN, C.refBodyN[b]);
         }
 
         // ---- constraints between free atoms (RigidBodySystem.cpp:107-112):
         // connected clusters, in the order of their first constraint, go
         // after the unconstrained free atoms, each cluster's atoms together
-        std::vector<int> freeOrder(C.atomIndex.begin(), C.atomIndex.begin() + C.numFree);
+        std::vector<int> freeOrder;   // (with constraints only; else the reference's free atoms)
         std::vector<std::vector<int>> clusterAtoms, clusterCons;
         require(numConstraints >= 0 && (numConstraints == 0 || (constraintAtoms && constraintDistances)),
                 "bad constraint arrays");
         if (numConstraints > 0) {
+            freeOrder.reserve(C.numFree);
+            for (int i = 0; i < numAtoms; i++)
+                if (bodyIndices[i] <= 0 && masses[i] != 0.0) freeOrder.push_back(i);
+            require((int)freeOrder.size() == C.numFree, "internal: free atom count", RBHIP_E_STATE);
             std::vector<int> root(numAtoms);
             for (int i = 0; i < numAtoms; i++) root[i] = i;
             auto find = [&](int x) {
@@ -1192,7 +1236,7 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
                 require(a >= 0 && a < numAtoms && b >= 0 && b < numAtoms && a != b,
                         "constraint " + std::to_string(k) + ": bad atom indices");
                 require(d > 0.0 && std::isfinite(d), "constraint " + std::to_string(k) + ": bad distance");
-                if (C.bodyIndex[a] != 0 || C.bodyIndex[b] != 0)
+                if (bodyIndices[a] > 0 || bodyIndices[b] > 0)
                     throw RbError(RBHIP_E_ARG, "Constraints involving rigid-body atoms are not allowed");
                 if (masses[a] == 0.0 || masses[b] == 0.0)
                     throw RbError(RBHIP_E_ARG, "A constraint cannot involve a massless particle");
@@ -1246,29 +1290,28 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
             C.numClusters = (int)clusterAtoms.size();
         }
 
-        // ---- internal order: bodies grouped by size class ----
+        // ---- internal order: bodies grouped by size class (a stable
+        // counting sort by atom count), their atoms placed on the device ----
         std::vector<int> order(C.numBodies);
-        for (int b = 0; b < C.numBodies; b++) order[b] = b;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int x, int y) { return C.refBodyN[x] < C.refBodyN[y]; });
+        {
+            std::vector<int> next(maxN + 2, 0);
+            for (int b = 0; b < C.numBodies; b++) next[C.refBodyN[b] + 1]++;
+            for (int v = 0; v <= maxN; v++) next[v + 1] += next[v];
+            for (int b = 0; b < C.numBodies; b++) order[next[C.refBodyN[b]]++] = b;
+        }
         C.internalBody = order;
         C.bodyN.resize(C.numBodies);
         C.bodyAtom0.resize(C.numBodies);
-        C.atomOrig.resize(numAtoms);
-        for (int k = 0; k < C.numFree; k++) C.atomOrig[k] = freeOrder[k];
         int a = C.numFree;
         for (int ib = 0; ib < C.numBodies; ib++) {
-            int r = order[ib];
-            int N = C.refBodyN[r];
+            int N = C.refBodyN[order[ib]];
             C.bodyN[ib] = N;
             C.bodyAtom0[ib] = a;
-            for (int j = 0; j < N; j++) C.atomOrig[a + j] = C.atomIndex[C.numFree + C.refBodyLoc[r] + j];
             if (C.classes.empty() || C.classes.back().N != N)
                 C.classes.push_back(rbhip_context::Class{N, ib, 0, a});
             C.classes.back().count++;
             a += N;
         }
-        for (int i : massless) C.atomOrig[a++] = i;
 
         // lanes per body: one lane per body up to 7 atoms, then a power of
         // two that leaves each lane 4-7 atoms (shorter serial loops)
@@ -1280,11 +1323,8 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
 
         // ---- device allocation ----
         if (rotationMode == 0) C.block = 64;   // the FP64 exact rotation is register heavy
-        C.ownStream = pooled_stream(device);
-        C.stream = C.ownStream;
-        C.arena.stream = C.tileArena.stream = C.stream;
         const size_t R = C.realSize();
-        const size_t nb = std::max(C.numBodies, 1), na = (size_t)numAtoms;
+        const size_t nb = std::max(C.numBodies, 1);
         DevSys& h = C.h;
         h.numFree = C.numFree;
         h.numBodies = C.numBodies;
@@ -1339,7 +1379,6 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
 
         // zero state, constants (queued on the context's stream; the host
         // buffers live until the synchronisation below)
-        hipStream_t st = C.stream;
         HIPCHK(hipMemsetAsync(h.pos, 0, 3 * R * as, st));
         if (h.posLo) HIPCHK(hipMemsetAsync(h.posLo, 0, 3 * sizeof(float) * as, st));
         HIPCHK(hipMemsetAsync(h.vel, 0, 3 * R * as, st));
@@ -1349,31 +1388,33 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
         HIPCHK(hipMemsetAsync(h.r0, 0, 3 * R * as, st));
         HIPCHK(hipMemsetAsync(h.kePart, 0, 4 * sizeof(double) * C.maxKeSlots, st));
         HIPCHK(hipMemsetAsync(h.pePart, 0, 4 * sizeof(double) * C.maxPeSlots, st));
-        std::vector<double> m(na);
-        for (size_t i = 0; i < na; i++) m[i] = masses[C.atomOrig[i]];
-        HIPCHK(hipMemcpyAsync(h.mass, m.data(), na * sizeof(double), hipMemcpyHostToDevice, st));
-        std::vector<float> invf;
-        std::vector<double> invd;
-        if (C.numFree > 0) {
-            if (R == 4) {
-                invf.resize(C.numFree);
-                for (int k = 0; k < C.numFree; k++) invf[k] = (float)(1.0 / m[k]);
-                HIPCHK(hipMemcpyAsync(h.freeInvM, invf.data(), invf.size() * 4, hipMemcpyHostToDevice, st));
-            } else {
-                invd.resize(C.numFree);
-                for (int k = 0; k < C.numFree; k++) invd[k] = 1.0 / m[k];
-                HIPCHK(hipMemcpyAsync(h.freeInvM, invd.data(), invd.size() * 8, hipMemcpyHostToDevice, st));
-            }
-        }
         if (C.numBodies > 0) {
             HIPCHK(hipMemcpyAsync(C.dBodyN, C.bodyN.data(), C.numBodies * sizeof(int), hipMemcpyHostToDevice, st));
             HIPCHK(hipMemcpyAsync(C.dBodyAtom0, C.bodyAtom0.data(), C.numBodies * sizeof(int), hipMemcpyHostToDevice,
                                   st));
         }
+        // the internal atom order, then the masses and free-atom inverse
+        // masses gathered through it
+        int* dOrder = setupArena.allocT<int>(C.numBodies);
+        int* dFreeOrder = freeOrder.empty() ? nullptr : setupArena.allocT<int>(freeOrder.size());
+        int* dMassless = setupArena.allocT<int>(massless.size());
+        if (C.numBodies > 0)
+            HIPCHK(hipMemcpyAsync(dOrder, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        if (dFreeOrder)
+            HIPCHK(hipMemcpyAsync(dFreeOrder, freeOrder.data(), freeOrder.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, st));
+        if (!massless.empty())
+            HIPCHK(hipMemcpyAsync(dMassless, massless.data(), massless.size() * sizeof(int), hipMemcpyHostToDevice,
+                                  st));
+        HIPCHK(launch_internal_order(C.numFree, C.numBodies, C.numMassless, C.numFree + C.numBodyAtoms,
+                                     C.dRefAtomIndex, dRefStart, dFreeOrder, dOrder, C.dBodyN, C.dBodyAtom0,
+                                     dMassless, C.dAtomOrig, st));
+        HIPCHK(launch_masses_in(precision, numAtoms, C.numFree, C.dAtomOrig, dMassIn, h.mass, h.freeInvM, st));
         if (C.numClusters > 0) {
-            // cluster tables in internal atom numbering
+            // cluster tables in internal atom numbering (the free atoms'
+            // internal order is freeOrder)
             std::vector<int> internalOf(numAtoms, -1);
-            for (int k = 0; k < C.numFree; k++) internalOf[C.atomOrig[k]] = k;
+            for (int k = 0; k < C.numFree; k++) internalOf[freeOrder[k]] = k;
             std::vector<int> cs(1, C.numFree - C.numClustered), ks(1, 0), ca, cb;
             std::vector<double> cd;
             for (int c2 = 0; c2 < C.numClusters; c2++) {
@@ -1430,7 +1471,8 @@ int rbhip_set_virtual_sites(rbhip_context* ctx, int count, const int* types, con
         require(!C.refined, "refined energies are not supported with virtual sites", RBHIP_E_UNSUPPORTED);
         if (count == 0) return;
         std::vector<int> internalOf(C.numAtoms);
-        for (int k = 0; k < C.numAtoms; k++) internalOf[C.atomOrig[k]] = k;
+        const std::vector<int>& atomOrig = C.atomOrig();
+        for (int k = 0; k < C.numAtoms; k++) internalOf[atomOrig[k]] = k;
         const int firstMassless = C.numAtoms - C.numMassless;
         std::vector<char> isSite(C.numAtoms, 0);
         std::vector<int> sa(count), st(count), sp(3 * (size_t)count, 0);
@@ -1608,12 +1650,6 @@ int rbhip_estimate_bytes(int numAtoms, const int* bodyIndices, int precision, si
     });
 }
 
-// the checks of rbhip_create on the device bookkeeping's status flags
-static void bookkeeping_status(const int* small) {
-    if (small[0] & 2) throw RbError(RBHIP_E_ARG, "negative or invalid mass");
-    if (small[0] & 1) throw RbError(RBHIP_E_ARG, "massless particles cannot belong to rigid bodies");
-}
-
 int rbhip_body_bookkeeping(int numAtoms, const int* bodyIndices, const double* masses, int* bodyIndexOut,
                            int* atomIndexOut, int* bodyAtomStartOut, int* counts4, int device, rbhip_stream stream) {
     return guard([&] {
@@ -1721,14 +1757,21 @@ int rbhip_get_counts(rbhip_context* ctx, int* out5) {
 int rbhip_get_atom_index(rbhip_context* ctx, int* out) {
     return guard([&] {
         require(ctx && out, "null argument");
-        std::copy(ctx->atomIndex.begin(), ctx->atomIndex.begin() + (ctx->numAtoms - ctx->numSites), out);
+        ctx->setDevice();
+        // free and body atoms, then zeros up to numActualAtoms
+        const std::vector<int>& ai = ctx->atomIndex();
+        const size_t n = (size_t)(ctx->numAtoms - ctx->numSites);
+        std::copy(ai.begin(), ai.begin() + std::min(n, ai.size()), out);
+        std::fill(out + std::min(n, ai.size()), out + n, 0);
     });
 }
 
 int rbhip_get_body_index(rbhip_context* ctx, int* out) {
     return guard([&] {
         require(ctx && out, "null argument");
-        std::copy(ctx->bodyIndex.begin(), ctx->bodyIndex.end(), out);
+        ctx->setDevice();
+        const std::vector<int>& bi = ctx->bodyIndex();
+        std::copy(bi.begin(), bi.end(), out);
     });
 }
 
@@ -1737,26 +1780,14 @@ int rbhip_set_tether(rbhip_context* ctx, const double* k, const double* r0) {
         require(ctx && k && r0, "null argument");
         rbhip_context& C = *ctx;
         C.setDevice();
-        size_t na = C.numAtoms, as = C.h.atomStride;
-        std::vector<double> r = C.toInternal(r0);
-        if (C.precision == RBHIP_PRECISION_MIXED) {
-            // spring constants and anchors at the state precision
-            std::vector<float> kk(na), rr(3 * as, 0.f);
-            for (size_t i = 0; i < na; i++) kk[i] = (float)k[C.atomOrig[i]];
-            for (size_t i = 0; i < na; i++)
-                for (int c = 0; c < 3; c++) rr[c * as + i] = (float)r[3 * i + c];
-            HIPCHK(hipMemcpyAsync(C.h.tetherK, kk.data(), na * 4, hipMemcpyHostToDevice, C.stream));
-            HIPCHK(hipMemcpyAsync(C.h.r0, rr.data(), 3 * as * 4, hipMemcpyHostToDevice, C.stream));
-            HIPCHK(hipStreamSynchronize(C.stream));
-        } else {
-            std::vector<double> kk(na), rr(3 * as, 0.0);
-            for (size_t i = 0; i < na; i++) kk[i] = k[C.atomOrig[i]];
-            for (size_t i = 0; i < na; i++)
-                for (int c = 0; c < 3; c++) rr[c * as + i] = r[3 * i + c];
-            HIPCHK(hipMemcpyAsync(C.h.tetherK, kk.data(), na * 8, hipMemcpyHostToDevice, C.stream));
-            HIPCHK(hipMemcpyAsync(C.h.r0, rr.data(), 3 * as * 8, hipMemcpyHostToDevice, C.stream));
-            HIPCHK(hipStreamSynchronize(C.stream));
-        }
+        size_t na = C.numAtoms;
+        // anchors, then spring constants, at the state precision, each
+        // through the staging buffer (the second upload waits in stream
+        // order for the first gather)
+        C.uploadRows(r0, 3);
+        HIPCHK(launch_tether_in(C.precision, C.dSys, C.dTmp, C.numAtoms, 0, C.dAtomOrig, C.stream));
+        C.uploadRows(k, 1);
+        HIPCHK(launch_tether_in(C.precision, C.dSys, C.dTmp, C.numAtoms, 1, C.dAtomOrig, C.stream));
         C.h.tetherScale = 1.0;
         C.h.tetherUniform = 1;
         C.h.tetherK0 = k[0];
@@ -1782,7 +1813,8 @@ int rbhip_add_bonds(rbhip_context* ctx, int count, const int* atomPairs, const d
         }
         // CSR per internal atom: each bond listed for both of its atoms
         std::vector<int> origToInt(C.numAtoms);
-        for (int i = 0; i < C.numAtoms; i++) origToInt[C.atomOrig[i]] = i;
+        const std::vector<int>& atomOrig = C.atomOrig();
+        for (int i = 0; i < C.numAtoms; i++) origToInt[atomOrig[i]] = i;
         std::vector<int> start(C.numAtoms + 1, 0);
         for (size_t b = 0; b < C.bondA.size(); b++) {
             start[origToInt[C.bondA[b]] + 1]++;
@@ -1831,10 +1863,9 @@ int rbhip_set_forces(rbhip_context* ctx, const double* xyz) {
         rbhip_context& C = *ctx;
         require(!C.adopted, "the forces of this context live in its adopted force array", RBHIP_E_STATE);
         C.setDevice();
-        std::vector<double> f = C.toInternal(xyz);
-        HIPCHK(hipMemcpyAsync(C.dTmp, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice, C.stream));
+        C.uploadRows(xyz, 3);
         const bool refold = C.beginForceChange();
-        HIPCHK(launch_stage_in(C.precision, C.dSys, C.dTmp, C.numAtoms, 2, C.stream, 0));
+        HIPCHK(launch_stage_in(C.precision, C.dSys, C.dTmp, C.numAtoms, 2, C.stream, 0, C.dAtomOrig));
         // forces on virtual sites move to their parents, as a force
         // evaluation leaves them
         if (C.numSites > 0) HIPCHK(launch_site_forces(C.precision, C.dSys, C.numSites, C.numSiteParents, C.stream));
@@ -1845,7 +1876,7 @@ int rbhip_set_forces(rbhip_context* ctx, const double* xyz) {
 
 // internal atom -> slot in arrays of OpenMM order `order` (order[s] = the
 // original atom held at slot s; a permutation of 0 .. length - 1)
-static std::vector<int> slots_for_order(const rbhip_context& C, const int* order, int length) {
+static std::vector<int> slots_for_order(rbhip_context& C, const int* order, int length) {
     require(order != nullptr && length >= C.numAtoms, "atom order shorter than the system");
     std::vector<int> inv(length, -1);
     for (int s = 0; s < length; s++) {
@@ -1854,7 +1885,8 @@ static std::vector<int> slots_for_order(const rbhip_context& C, const int* order
         inv[o] = s;
     }
     std::vector<int> slot(C.numAtoms);
-    for (int i = 0; i < C.numAtoms; i++) slot[i] = inv[C.atomOrig[i]];
+    const std::vector<int>& atomOrig = C.atomOrig();
+    for (int i = 0; i < C.numAtoms; i++) slot[i] = inv[atomOrig[i]];
     return slot;
 }
 
@@ -1942,9 +1974,9 @@ int rbhip_set_positions(rbhip_context* ctx, const double* xyz) {
         require(ctx && xyz, "null argument");
         rbhip_context& C = *ctx;
         C.setDevice();
-        std::vector<double> r = C.toInternal(xyz);
-        HIPCHK(hipMemcpyAsync(C.h.stageR, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice, C.stream));
-        HIPCHK(launch_stage_in(C.precision, C.dSys, C.h.stageR, C.numAtoms, 0, C.stream, C.layout()));
+        C.uploadRows(xyz, 3);
+        HIPCHK(launch_stage_in(C.precision, C.dSys, C.dTmp, C.numAtoms, 0, C.stream, C.layout(), C.dAtomOrig,
+                               C.h.stageR));
         // stateChanged(Positions): forces, then geometry + dynamics (RigidBodyIntegrator.cpp:65-68)
         C.h.useStageR = 1;
         C.uploadSys();
@@ -1964,9 +1996,9 @@ int rbhip_set_velocities(rbhip_context* ctx, const double* xyz) {
         require(ctx && xyz, "null argument");
         rbhip_context& C = *ctx;
         C.setDevice();
-        std::vector<double> v = C.toInternal(xyz);
-        HIPCHK(hipMemcpyAsync(C.h.stageV, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, C.stream));
-        HIPCHK(launch_stage_in(C.precision, C.dSys, C.h.stageV, C.numAtoms, 1, C.stream, C.layout()));
+        C.uploadRows(xyz, 3);
+        HIPCHK(launch_stage_in(C.precision, C.dSys, C.dTmp, C.numAtoms, 1, C.stream, C.layout(), C.dAtomOrig,
+                               C.h.stageV));
         C.stageVFresh = true;
         // stateChanged(Velocities): dynamics rebuild on the live body geometry
         if (C.built) {

@@ -22,4 +22,21 @@ hipError_t launch_bookkeeping(int n, const int* labels, const double* masses, in
                               int* bodyAtomStart, void* scratch, size_t scratchBytes, hipStream_t stream,
                               int* smallOut);
 
+// The internal atom order of a context, atomOrig[internal atom] = original
+// atom: the numFree free atoms first (freeOrder, or the reference's free
+// atoms atomIndex[0, numFree) when freeOrder is null), then the bodies in
+// internal order (internal body ib is reference body order[ib]; its bodyN[ib]
+// atoms, from bodyAtom0[ib] on, are the reference body's atoms in the
+// reference's order, atomIndex[numFree + bodyAtomStart[order[ib]] + j]),
+// then the numMassless massless particles at massStart (massless[k]).
+// Device arrays; queued on `stream`.
+hipError_t launch_internal_order(int numFree, int numBodies, int numMassless, int massStart, const int* atomIndex,
+                                 const int* bodyAtomStart, const int* freeOrder, const int* order, const int* bodyN,
+                                 const int* bodyAtom0, const int* massless, int* atomOrig, hipStream_t stream);
+
+// masses in internal order, mass[i] = raw[atomOrig[i]], and the free atoms'
+// inverse masses at the state precision (precision 0: fp32, else FP64)
+hipError_t launch_masses_in(int precision, int n, int numFree, const int* atomOrig, const double* raw, double* mass,
+                            void* freeInvM, hipStream_t stream);
+
 }  // namespace rbhip

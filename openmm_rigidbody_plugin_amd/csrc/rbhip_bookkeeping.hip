@@ -177,4 +177,59 @@ hipError_t launch_bookkeeping(int n, const int* labels, const double* masses, in
     return hipStreamSynchronize(stream);
 }
 
+namespace {
+
+__global__ void k_order_free(int numFree, int numMassless, int massStart, const int* __restrict__ atomIndex,
+                             const int* __restrict__ freeOrder, const int* __restrict__ massless,
+                             int* __restrict__ atomOrig) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < numFree) atomOrig[i] = freeOrder ? freeOrder[i] : atomIndex[i];
+    else if (i - numFree < numMassless) atomOrig[massStart + i - numFree] = massless[i - numFree];
+}
+
+__global__ void k_order_bodies(int numBodies, int numFree, const int* __restrict__ atomIndex,
+                               const int* __restrict__ bodyAtomStart, const int* __restrict__ order,
+                               const int* __restrict__ bodyN, const int* __restrict__ bodyAtom0,
+                               int* __restrict__ atomOrig) {
+    int ib = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ib >= numBodies) return;
+    const int* src = atomIndex + numFree + bodyAtomStart[order[ib]];
+    int* dst = atomOrig + bodyAtom0[ib];
+    for (int j = 0, N = bodyN[ib]; j < N; j++) dst[j] = src[j];
+}
+
+template <class Real>
+__global__ void k_masses_in(int n, int numFree, const int* __restrict__ atomOrig, const double* __restrict__ raw,
+                            double* __restrict__ mass, Real* __restrict__ freeInvM) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = raw[atomOrig[i]];
+    mass[i] = m;
+    if (i < numFree) freeInvM[i] = (Real)(1.0 / m);
+}
+
+}  // namespace
+
+hipError_t launch_internal_order(int numFree, int numBodies, int numMassless, int massStart, const int* atomIndex,
+                                 const int* bodyAtomStart, const int* freeOrder, const int* order, const int* bodyN,
+                                 const int* bodyAtom0, const int* massless, int* atomOrig, hipStream_t stream) {
+    const int nf = numFree + numMassless;
+    if (nf > 0) k_order_free<<<(nf + 255) / 256, 256, 0, stream>>>(numFree, numMassless, massStart, atomIndex,
+                                                                     freeOrder, massless, atomOrig);
+    if (numBodies > 0)
+        k_order_bodies<<<(numBodies + 255) / 256, 256, 0, stream>>>(numBodies, numFree, atomIndex, bodyAtomStart,
+                                                                     order, bodyN, bodyAtom0, atomOrig);
+    return hipGetLastError();
+}
+
+hipError_t launch_masses_in(int precision, int n, int numFree, const int* atomOrig, const double* raw, double* mass,
+                            void* freeInvM, hipStream_t stream) {
+    const int grid = (n + 255) / 256;
+    if (precision == 0)
+        k_masses_in<float><<<grid, 256, 0, stream>>>(n, numFree, atomOrig, raw, mass, static_cast<float*>(freeInvM));
+    else
+        k_masses_in<double><<<grid, 256, 0, stream>>>(n, numFree, atomOrig, raw, mass, static_cast<double*>(freeInvM));
+    return hipGetLastError();
+}
+
 }  // namespace rbhip

@@ -53,13 +53,20 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t str
 // first half-kick of the next step
 hipError_t launch_setup(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int geometry,
                         int block, hipStream_t stream);
-// FP64 array (internal order) -> device state: what = 0 positions, 1
-// velocities, 2 forces (the library's force array; not with `adopted`)
+// FP64 xyz rows -> device state: what = 0 positions, 1 velocities, 2 forces
+// (the library's force array; not with `adopted`).  Internal atom i reads row
+// orig[i] (row i when orig is null); with `keep`, its FP64 row is also left
+// at row i of keep (internal order).
 hipError_t launch_stage_in(int precision, const DevSys* sys, const double* in, int numAtoms, int what,
-                           hipStream_t stream, int adopted);
-// device state -> FP64 array (internal order): what = 0 positions, 1 velocities, 2 forces
+                           hipStream_t stream, int adopted, const int* orig = nullptr, double* keep = nullptr);
+// device state -> FP64 xyz rows: what = 0 positions, 1 velocities, 2 forces;
+// internal atom i goes to row orig[i] (row i when orig is null)
 hipError_t launch_stage_out(int precision, const DevSys* sys, double* out, int numAtoms, int what,
-                            hipStream_t stream, int adopted);
+                            hipStream_t stream, int adopted, const int* orig = nullptr);
+// FP64 tether input in original order -> the state-precision arrays: what = 0
+// anchors (xyz rows), 1 spring constants; internal atom i reads row orig[i]
+hipError_t launch_tether_in(int precision, const DevSys* sys, const double* in, int numAtoms, int what,
+                            const int* orig, hipStream_t stream);
 // adopted fixed-point forces -> the library's force array (state precision)
 hipError_t launch_adopted_forces_in(int precision, const DevSys* sys, int numAtoms, hipStream_t stream);
 // moves the adopted forces of the actual atoms from the slots of DevSys::ommSlot
@@ -69,9 +76,10 @@ hipError_t launch_adopted_move_forces(const DevSys* sys, long long* tmp, const i
 // body states (internal body order) as 36 FP64 values each, half-kick removed
 hipError_t launch_body_states(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
                               int numBodies, double* out, hipStream_t stream);
-// FP64 body-atom positions rebuilt from the body state into `out` (internal order)
+// FP64 body-atom positions rebuilt from the body state into `out`, internal
+// atom a at row orig[a] (row a when orig is null)
 hipError_t launch_body_atoms_out(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
-                                 int numBodies, double* out, hipStream_t stream);
+                                 int numBodies, double* out, hipStream_t stream, const int* orig = nullptr);
 // redoes the folded half-kick for the forces now in the force array, made
 // before with the forces `prev` (a copy of the force array)
 hipError_t launch_refold(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0, int numBodies,

@@ -220,21 +220,31 @@ __global__ __launch_bounds__(256) void k_setup(const DevSys* __restrict__ system
 
 // FP64 staging -> device state (the library's arrays or the adopted ones):
 // what = 0 positions, 1 velocities (rounded to the state precision), 2
-// forces (library arrays only)
+// forces (library arrays only).  Internal atom i reads row orig[i] of `in`
+// (row i when orig is null) and, with `keep`, leaves its FP64 row there.
 template <class Real, class IO>
-__global__ void k_stage_in(const DevSys* __restrict__ sys, const double* __restrict__ in, int n, int what) {
+__global__ void k_stage_in(const DevSys* __restrict__ sys, const double* __restrict__ in, int n, int what,
+                           const int* __restrict__ orig, double* __restrict__ keep) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    V3<double> x = {in[3 * (size_t)i], in[3 * (size_t)i + 1], in[3 * (size_t)i + 2]};
+    const size_t r = orig ? (size_t)orig[i] : (size_t)i;
+    V3<double> x = {in[3 * r], in[3 * r + 1], in[3 * r + 2]};
+    if (keep) {
+        keep[3 * (size_t)i] = x.x;
+        keep[3 * (size_t)i + 1] = x.y;
+        keep[3 * (size_t)i + 2] = x.z;
+    }
     if (what == 0) IO::template set_pos<Real>(*sys, i, x);
     else if (what == 1) IO::template set_vel<Real>(*sys, i, cvt<Real>(x));
     else st3<Real>(sys->force, sys->atomStride, i, cvt<Real>(x));
 }
 
 // device state -> FP64 staging: what = 0 positions, 1 velocities, 2 forces
-// (adopted forces at their fixed-point value)
+// (adopted forces at their fixed-point value); internal atom i goes to row
+// orig[i] of `out` (row i when orig is null)
 template <class Real, class IO>
-__global__ void k_stage_out(const DevSys* __restrict__ sys, double* __restrict__ out, int n, int what) {
+__global__ void k_stage_out(const DevSys* __restrict__ sys, double* __restrict__ out, int n, int what,
+                            const int* __restrict__ orig) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const DevSys& S = *sys;
@@ -256,9 +266,31 @@ __global__ void k_stage_out(const DevSys* __restrict__ sys, double* __restrict__
     } else {
         x = cvt<double>(ld3<Real>(S.force, S.atomStride, i));
     }
-    out[3 * (size_t)i] = x.x;
-    out[3 * (size_t)i + 1] = x.y;
-    out[3 * (size_t)i + 2] = x.z;
+    const size_t r = orig ? (size_t)orig[i] : (size_t)i;
+    out[3 * r] = x.x;
+    out[3 * r + 1] = x.y;
+    out[3 * r + 2] = x.z;
+}
+
+// tether anchors (what = 0: FP64 xyz rows) or spring constants (what = 1:
+// one FP64 value per atom), row orig[i] for internal atom i, at the state
+// precision
+template <class Real>
+__global__ void k_tether_in(const DevSys* __restrict__ sys, const double* __restrict__ in, int n, int what,
+                            const int* __restrict__ orig) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevSys& S = *sys;
+    const size_t r = (size_t)orig[i];
+    if (what == 0) {
+        Real* r0 = static_cast<Real*>(S.r0);
+        const size_t as = (size_t)S.atomStride;
+        r0[i] = (Real)in[3 * r];
+        r0[i + as] = (Real)in[3 * r + 1];
+        r0[i + 2 * as] = (Real)in[3 * r + 2];
+    } else {
+        static_cast<Real*>(S.tetherK)[i] = (Real)in[r];
+    }
 }
 
 // adopted fixed-point forces -> the library's force array at the state
@@ -399,11 +431,13 @@ __global__ void k_refold(const DevSys* __restrict__ sys, const int* __restrict__
     stp3<Real>(S.L, k, cvt<Real>(L));
 }
 
-// body-atom positions r_cm + A(q)^T d_j in FP64 into the internal-order
-// staging buffer (the rotation is the one part1 applies)
+// body-atom positions r_cm + A(q)^T d_j in FP64 into the staging buffer,
+// internal atom a at row orig[a] (row a when orig is null); the rotation is
+// the one part1 applies
 template <class Real>
 __global__ void k_body_atoms_out(const DevSys* __restrict__ sys, const int* __restrict__ bodyN,
-                                 const int* __restrict__ bodyAtom0, int numBodies, double* __restrict__ out) {
+                                 const int* __restrict__ bodyAtom0, int numBodies, double* __restrict__ out,
+                                 const int* __restrict__ orig) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= numBodies) return;
     const DevSys& S = *sys;
@@ -413,7 +447,7 @@ __global__ void k_body_atoms_out(const DevSys* __restrict__ sys, const int* __re
     int N = bodyN[k], atom0 = bodyAtom0[k];
     for (int j = 0; j < N; j++) {
         V3<double> r = rc + cvt<double>(apply(R, ld3<Real>(S.d, S.dStride, atom0 + j - S.numFree)));
-        size_t a = atom0 + j;
+        size_t a = orig ? (size_t)orig[atom0 + j] : (size_t)(atom0 + j);
         out[3 * a] = r.x;
         out[3 * a + 1] = r.y;
         out[3 * a + 2] = r.z;
@@ -430,31 +464,40 @@ hipError_t launch_setup(int precision, const DevSys* systems, const Tile* tiles,
 }
 
 hipError_t launch_stage_in(int precision, const DevSys* sys, const double* in, int numAtoms, int what,
-                           hipStream_t stream, int adopted) {
+                           hipStream_t stream, int adopted, const int* orig, double* keep) {
     if (numAtoms <= 0) return hipSuccess;
     if (adopted && what == 2) return hipErrorInvalidValue;
     int grid = (numAtoms + 255) / 256;
     if (precision == 0) {
-        if (adopted) k_stage_in<float, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
-        else k_stage_in<float, SoaAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
+        if (adopted) k_stage_in<float, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what, orig, keep);
+        else k_stage_in<float, SoaAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what, orig, keep);
     } else {
-        if (adopted) k_stage_in<double, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
-        else k_stage_in<double, SoaAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what);
+        if (adopted) k_stage_in<double, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what, orig, keep);
+        else k_stage_in<double, SoaAtoms><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what, orig, keep);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_stage_out(int precision, const DevSys* sys, double* out, int numAtoms, int what,
-                            hipStream_t stream, int adopted) {
+                            hipStream_t stream, int adopted, const int* orig) {
     if (numAtoms <= 0) return hipSuccess;
     int grid = (numAtoms + 255) / 256;
     if (precision == 0) {
-        if (adopted) k_stage_out<float, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
-        else k_stage_out<float, SoaAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
+        if (adopted) k_stage_out<float, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what, orig);
+        else k_stage_out<float, SoaAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what, orig);
     } else {
-        if (adopted) k_stage_out<double, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
-        else k_stage_out<double, SoaAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what);
+        if (adopted) k_stage_out<double, AdoptedAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what, orig);
+        else k_stage_out<double, SoaAtoms><<<grid, 256, 0, stream>>>(sys, out, numAtoms, what, orig);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_tether_in(int precision, const DevSys* sys, const double* in, int numAtoms, int what,
+                            const int* orig, hipStream_t stream) {
+    if (numAtoms <= 0) return hipSuccess;
+    int grid = (numAtoms + 255) / 256;
+    if (precision == 0) k_tether_in<float><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what, orig);
+    else k_tether_in<double><<<grid, 256, 0, stream>>>(sys, in, numAtoms, what, orig);
     return hipGetLastError();
 }
 
@@ -485,11 +528,12 @@ hipError_t launch_body_states(int precision, const DevSys* sys, const int* bodyN
 }
 
 hipError_t launch_body_atoms_out(int precision, const DevSys* sys, const int* bodyN, const int* bodyAtom0,
-                                int numBodies, double* out, hipStream_t stream) {
+                                int numBodies, double* out, hipStream_t stream, const int* orig) {
     if (numBodies <= 0) return hipSuccess;
     int grid = (numBodies + 255) / 256;
-    if (precision == 0) k_body_atoms_out<float><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
-    else k_body_atoms_out<double><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out);
+    if (precision == 0)
+        k_body_atoms_out<float><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out, orig);
+    else k_body_atoms_out<double><<<grid, 256, 0, stream>>>(sys, bodyN, bodyAtom0, numBodies, out, orig);
     return hipGetLastError();
 }
 
