@@ -116,3 +116,40 @@ def test_python_force_field_errors_are_raised(hip):
     ctx = Context(spec.masses, it, force=broken)
     with pytest.raises(plugin.PluginError, match="no forces today"):
         ctx.setPositions(spec.positions)
+
+
+def test_state_data_reporter_on_the_gpu(hip, oracle):
+    """The reference's StateDataReporter columns (python/statedatareporter.py)
+    from a bound integrator stepped by app.Simulation: the step and time, the
+    kinetic energy split, the temperature over the rigid DOF, and the refined
+    energies, each against the oracle at the report steps."""
+    import io
+    from openmm_rigidbody_plugin_amd import app
+    spec = systems.water_c1(rotation_mode=10)
+    it, ctx = bind(spec, refined=True)
+    ref = oracle.OracleSim(spec.masses, spec.body_indices, spec.dt, spec.rotation_mode)
+    ref.set_tether(spec.tether_k, spec.tether_r0)
+    ref.set_refined(True)
+    ref.set_positions(spec.positions)
+    ref.set_velocities(spec.velocities)
+    sim = app.Simulation(it, ctx)
+    out = io.StringIO()
+    sim.reporters.append(app.StateDataReporter(out, 5, step=True, time=True, kineticEnergy=True, temperature=True,
+                                               translationalEnergy=True, rotationalEnergy=True,
+                                               refinedKineticEnergy=True, refinedTemperature=True))
+    sim.step(10)
+    lines = out.getvalue().splitlines()
+    assert lines[0].startswith('#"Step","Time (ps)","Kinetic Energy (kJ/mole)","Temperature (K)"')
+    assert len(lines) == 3
+    dof = ref.counts()["numDOF"]
+    for row, steps in zip(lines[1:], (5, 10)):
+        ref.step(5)
+        v = [float(x) for x in row.split(",")]
+        kt, kr = ref.kinetic_energies()
+        rkt, rkr = ref.refined_kinetic_energies()
+        R = app.MOLAR_GAS_CONSTANT_R
+        assert v[0] == steps and v[1] == pytest.approx(steps * spec.dt, rel=1e-12)
+        # mixed precision: kinetic energies to 1e-5, refined ones (differences
+        # of fp32 quaternions) to 2e-5
+        np.testing.assert_allclose(v[2:6], [kt + kr, 2 * (kt + kr) / (dof * R), kt, kr], rtol=1e-5)
+        np.testing.assert_allclose(v[6:], [rkt + rkr, 2 * (rkt + rkr) / (dof * R)], rtol=2e-5)
