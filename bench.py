@@ -401,7 +401,8 @@ def served_fp64_check(check, bodies=64, free=64, tol_nm=1e-5):
 
 def kernel_roofline(kms, natoms_body, nb, nf, config):
     """The HBM roofline of the dominant kernel of step(K) (the fused pass)
-    from its HIP-event duration and its algorithmic bytes; the two-pass step
+    from its duration (HIP events on its own dispatch) and its algorithmic
+    bytes; the two-pass step
     (part1 + part2, what one execute() with forces computed outside the
     library runs) and the full step with the tether beside it."""
     alg = {k: algorithmic_bytes(k, natoms_body, nb, nf) for k in ("part1", "part2", "fused", "forces")}
@@ -481,6 +482,9 @@ def main():
     ap.add_argument("--config", default="water1M")
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch", default=None, metavar="BLOCK[,MAXBLOCKS]",
+                    help="a fixed launch shape for the main workload instead of the autotuner's")
+    ap.add_argument("--serve-jobs", type=int, default=96, help="jobs in the served stream")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica batched measurement")
     ap.add_argument("--batch-steps", type=int, default=1000)
@@ -511,7 +515,11 @@ def main():
     ctx = rbhip.Context.from_spec(spec, precision=cfg["precision"])
     counts = ctx.counts()
     nb, nf = counts["numBodies"], counts["numFree"]
-    if not args.no_autotune:
+    if args.launch:
+        shape = [int(v) for v in args.launch.split(",")]
+        ctx.set_launch(shape[0], shape[1] if len(shape) > 1 else 0)
+        log(f"launch {ctx.describe()}")
+    elif not args.no_autotune:
         t = time.perf_counter()
         ctx.autotune(reps=3)
         log(f"autotune {time.perf_counter() - t:.1f}s: {ctx.describe()}")
@@ -526,7 +534,9 @@ def main():
     value = nb * args.steps / elapsed
     log(f"{args.steps} steps in {elapsed:.4f}s: {ms_per_step:.4f} ms/step, {value:.4e} body-steps/s")
 
-    # per-kernel HIP-event timing on the context's stream
+    # per-kernel timing: HIP events attached to each step kernel's dispatch
+    # (hipExtLaunchKernelGGL) on the context's stream, so each mean is the
+    # kernel's own duration, as rocprofv3's kernel trace reports it
     kms, span = ctx.time_kernels(args.kernel_steps)
     log("kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in kms.items()))
     natoms_body = counts["numBodyAtoms"] / max(nb, 1)
@@ -562,7 +572,7 @@ def main():
         result["batched"] = batched(args.batch_steps, args.warmup)
     if not args.no_serve and args.config == "water1M":
         log("served job stream ...")
-        result["served"] = served()
+        result["served"] = served(num_jobs=args.serve_jobs)
     check = result.get("served", {}).pop("_fp64_check_jobs", None)
     if not args.no_cpu_baseline:
         # the CPU leg: the oracle timed on this host, and the served jobs'

@@ -311,30 +311,45 @@ struct EventSet {
     }
 };
 
-// times `steps` repetitions of the phases (each a launch sequence) with HIP
-// events around each phase on `stream`; avgMs[i] = mean duration of phase i
+// times `steps` repetitions of the phases (each a launch sequence) on
+// `stream`: avgMs[i] = the mean duration of the step kernel of phase i (its
+// first k_part1 / k_part2 / k_fused / k_forces launch), from events attached
+// to that kernel's dispatch, so launch overheads and gaps are not counted;
+// totalMs = the whole run, gaps and any other kernels included
 void time_phases(hipStream_t stream, int steps, const std::vector<std::function<void()>>& phases, double* avgMs,
                  double* totalMs) {
     const size_t np = phases.size();
-    EventSet es((np + 1) * (size_t)steps);
+    EventSet starts(np * (size_t)steps), stops(np * (size_t)steps), span(2);
+    HIPCHK(hipEventRecord(span.ev[0], stream));
     for (int s = 0; s < steps; s++)
-        for (size_t k = 0; k <= np; k++) {
-            HIPCHK(hipEventRecord(es.ev[(np + 1) * s + k], stream));
-            if (k < np) phases[k]();
+        for (size_t k = 0; k < np; k++) {
+            const size_t e = np * s + k;
+            set_step_kernel_events(starts.ev[e], stops.ev[e]);
+            try {
+                phases[k]();
+            } catch (...) {
+                (void)clear_step_kernel_events();
+                throw;
+            }
+            if (clear_step_kernel_events()) {   // no step kernel in this phase
+                HIPCHK(hipEventRecord(starts.ev[e], stream));
+                HIPCHK(hipEventRecord(stops.ev[e], stream));
+            }
         }
-    HIPCHK(hipEventSynchronize(es.ev[(np + 1) * steps - 1]));
+    HIPCHK(hipEventRecord(span.ev[1], stream));
+    HIPCHK(hipEventSynchronize(span.ev[1]));
     std::vector<double> acc(np, 0.0);
     for (int s = 0; s < steps; s++)
         for (size_t k = 0; k < np; k++) {
             float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, es.ev[(np + 1) * s + k], es.ev[(np + 1) * s + k + 1]));
+            HIPCHK(hipEventElapsedTime(&ms, starts.ev[np * s + k], stops.ev[np * s + k]));
             acc[k] += ms;
         }
     for (size_t k = 0; k < np; k++) avgMs[k] = acc[k] / steps;
     if (totalMs) {
-        float span = 0.f;
-        HIPCHK(hipEventElapsedTime(&span, es.ev[0], es.ev[(np + 1) * steps - 1]));
-        *totalMs = span;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, span.ev[0], span.ev[1]));
+        *totalMs = ms;
     }
 }
 

@@ -7,6 +7,13 @@
 
 namespace rbhip {
 
+// events for the next step kernel (k_part1, k_part2, k_fused or k_forces)
+// launched on this thread: attached to its dispatch, they record its own
+// start and end; cleared by that launch
+void set_step_kernel_events(hipEvent_t start, hipEvent_t stop);
+// clears them; true if no step kernel took them
+bool clear_step_kernel_events();
+
 // hot path (rbhip_kernels.hip); `adopted` = the atoms live in adopted
 // arrays in OpenMM's layout (DevSys::ommSlot), else in the library's own
 hipError_t launch_part1(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,

@@ -20,6 +20,7 @@
 // pi = 2 B(q) L, its torque kick pi += C(q) tau dt is L += A(q) tau dt / 2.
 // Every workgroup walks the tile list with a grid stride, so one launch
 // covers any number of systems (the batch path) and any grid cap.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <utility>
@@ -1308,28 +1309,57 @@ static int grid_for(int numTiles, int maxBlocks) {
     return maxBlocks > 0 && maxBlocks < numTiles ? maxBlocks : numTiles;
 }
 
+// Timing of the step kernels: with events set by set_step_kernel_events
+// (per thread), the next step kernel is launched with them attached to its
+// dispatch (hipExtLaunchKernelGGL), so they record its own start and end;
+// both are then cleared.  Without them, a plain launch.
+namespace {
+thread_local hipEvent_t t_kernelStart = nullptr, t_kernelStop = nullptr;
+
+template <class... KArgs, class... Args>
+void step_kernel(void (*kernel)(KArgs...), int grid, int block, hipStream_t stream, Args... args) {
+    if (t_kernelStart) {
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, t_kernelStart, t_kernelStop, 0, args...);
+        t_kernelStart = t_kernelStop = nullptr;
+    } else {
+        kernel<<<grid, block, 0, stream>>>(args...);
+    }
+}
+}  // namespace
+
+void set_step_kernel_events(hipEvent_t start, hipEvent_t stop) {
+    t_kernelStart = start;
+    t_kernelStop = stop;
+}
+
+bool clear_step_kernel_events() {
+    const bool pending = t_kernelStart != nullptr;
+    t_kernelStart = t_kernelStop = nullptr;
+    return pending;
+}
+
 // the step kernels for the atom layout (library arrays or adopted arrays),
 // state precision and rotation kind
 template <class IO>
 static void part1_for(int precision, int rotationMode, int grid, int block, hipStream_t stream,
                       const DevSys* systems, const Tile* tiles, int numTiles) {
     if (precision == 0) {
-        if (rotationMode == 0) k_part1<float, true, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_part1<float, false, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (rotationMode == 0) step_kernel(k_part1<float, true, IO>, grid, block, stream, systems, tiles, numTiles);
+        else step_kernel(k_part1<float, false, IO>, grid, block, stream, systems, tiles, numTiles);
     } else {
-        if (rotationMode == 0) k_part1<double, true, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_part1<double, false, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (rotationMode == 0) step_kernel(k_part1<double, true, IO>, grid, block, stream, systems, tiles, numTiles);
+        else step_kernel(k_part1<double, false, IO>, grid, block, stream, systems, tiles, numTiles);
     }
 }
 template <class IO>
 static void fused_for(int precision, int rotationMode, int grid, int block, hipStream_t stream,
                       const DevSys* systems, const Tile* tiles, int numTiles) {
     if (precision == 0) {
-        if (rotationMode == 0) k_fused<float, true, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_fused<float, false, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (rotationMode == 0) step_kernel(k_fused<float, true, IO>, grid, block, stream, systems, tiles, numTiles);
+        else step_kernel(k_fused<float, false, IO>, grid, block, stream, systems, tiles, numTiles);
     } else {
-        if (rotationMode == 0) k_fused<double, true, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_fused<double, false, IO><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (rotationMode == 0) step_kernel(k_fused<double, true, IO>, grid, block, stream, systems, tiles, numTiles);
+        else step_kernel(k_fused<double, false, IO>, grid, block, stream, systems, tiles, numTiles);
     }
 }
 
@@ -1356,11 +1386,11 @@ hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles,
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
     if (precision == 0) {
-        if (adopted) k_part2<float, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_part2<float, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (adopted) step_kernel(k_part2<float, AdoptedAtoms>, grid, block, stream, systems, tiles, numTiles);
+        else step_kernel(k_part2<float, SoaAtoms>, grid, block, stream, systems, tiles, numTiles);
     } else {
-        if (adopted) k_part2<double, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_part2<double, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (adopted) step_kernel(k_part2<double, AdoptedAtoms>, grid, block, stream, systems, tiles, numTiles);
+        else step_kernel(k_part2<double, SoaAtoms>, grid, block, stream, systems, tiles, numTiles);
     }
     return hipGetLastError();
 }
@@ -1451,11 +1481,11 @@ hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
     if (precision == 0) {
-        if (adopted) k_forces<float, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_forces<float, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (adopted) step_kernel(k_forces<float, AdoptedAtoms>, grid, block, stream, systems, tiles, numTiles);
+        else step_kernel(k_forces<float, SoaAtoms>, grid, block, stream, systems, tiles, numTiles);
     } else {
-        if (adopted) k_forces<double, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_forces<double, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (adopted) step_kernel(k_forces<double, AdoptedAtoms>, grid, block, stream, systems, tiles, numTiles);
+        else step_kernel(k_forces<double, SoaAtoms>, grid, block, stream, systems, tiles, numTiles);
     }
     return hipGetLastError();
 }
