@@ -383,7 +383,10 @@ def served_fp64_check(check, bodies=64, free=64, tol_nm=1e-5):
     (part of the CPU leg): for each (spec, steps, served positions, job) in
     `check`, `bodies` bodies and `free` free atoms stepped alone by the
     oracle (every force is a per-atom tether, so a sample evolves as it
-    does in the whole job), compared atom by atom with the served result."""
+    does in the whole job), compared atom by atom with the served result.
+    The checked jobs run 100-120 steps, and mixed-precision rounding grows
+    with the step count, so the bound is 1e-5 nm rather than the 1e-6 nm
+    the parity tests apply after 10 steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import pyoracle
