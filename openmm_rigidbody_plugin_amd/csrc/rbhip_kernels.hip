@@ -100,11 +100,28 @@ __device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, i
             tau = tau + cross(dl, f);
         }
     } else {
-        for (int j = sub; j < N; j += lanes) {
-            V3<double> dl = cvt<double>(apply(R, ld3<Real>(S.d, S.dStride, dbase + j)));
-            V3<double> f = cvt<double>(IO::template force<Real>(S, atom0 + j));
-            F = F + f;
-            tau = tau + cross(dl, f);
+        // atoms sub, sub + lanes, ...: four per lane loaded before any is
+        // summed (four reads in flight), summed in the same order
+        constexpr int U = 4;
+        for (int j0 = sub; j0 < N; j0 += U * lanes) {
+            V3<Real> dj[U], fj[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int j = j0 + u * lanes;
+                if (j < N) {
+                    dj[u] = ld3<Real>(S.d, S.dStride, dbase + j);
+                    fj[u] = IO::template force<Real>(S, atom0 + j);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (j0 + u * lanes < N) {
+                    V3<double> dl = cvt<double>(apply(R, dj[u]));
+                    V3<double> f = cvt<double>(fj[u]);
+                    F = F + f;
+                    tau = tau + cross(dl, f);
+                }
+            }
         }
     }
 }
