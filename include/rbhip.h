@@ -318,10 +318,11 @@ int rbhip_set_lanes(rbhip_context* ctx, int atomsPerBody, int lanes);
 int rbhip_autotune(rbhip_context* ctx, int reps);
 /* describes the chosen tiling as text */
 int rbhip_describe(rbhip_context* ctx, char* buf, int len);
-/* advances the state 2*steps+1 steps with HIP events around each kernel on
-   the context's stream: `steps` two-pass steps (part1, forces, part2), then
-   the fused path of step(steps+1); avgMs[RBHIP_NUM_KERNELS] receives the mean
-   duration of each kernel and totalMs the span of the two-pass steps */
+/* advances the state 2*steps+1 steps on the context's stream: `steps`
+   two-pass steps (part1, forces, part2), then the fused path of
+   step(steps+1); avgMs[RBHIP_NUM_KERNELS] receives the mean duration of each
+   step kernel, from HIP events attached to its own dispatch (launch overhead
+   not counted), and totalMs the span of the two-pass steps, gaps included */
 int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* totalMs);
 /* streams `bytes` through a plain device copy kernel `reps` times (profiler
    calibration of HBM traffic counters); avgMs receives its mean duration */
