@@ -80,6 +80,10 @@ template <class Real> __device__ __forceinline__ Body<Real> load_body(const DevS
     return Body<Real>{ld4<Real>(S.q, k), ldp3<Real>(S.L, k), ldp3<Real>(S.pcm, k), V3<Real>{iI.a, iI.b, iI.c}, iI.d};
 }
 
+// atoms per lane a strided per-body loop loads before it uses any (bodies of
+// more than 6 atoms, several lanes each)
+constexpr int STRIDED_BATCH = 8;
+
 // F = sum f_j and tau = sum delta_j x f_j in FP64, delta_j = A(q)^T d_j
 template <class Real, int NC, class IO>
 __device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, int atom0, int N, int sub, int lanes,
@@ -100,9 +104,10 @@ __device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, i
             tau = tau + cross(dl, f);
         }
     } else {
-        // atoms sub, sub + lanes, ...: four per lane loaded before any is
-        // summed (four reads in flight), summed in the same order
-        constexpr int U = 4;
+        // atoms sub, sub + lanes, ...: up to STRIDED_BATCH per lane loaded
+        // before any is summed (one round trip for the bodies the default
+        // lanes give), summed in the same order
+        constexpr int U = STRIDED_BATCH;
         for (int j0 = sub; j0 < N; j0 += U * lanes) {
             V3<Real> dj[U], fj[U];
 #pragma unroll
@@ -161,9 +166,17 @@ __device__ __forceinline__ void body_positions(const DevSys& S, const Rot<Real>&
 #pragma unroll
         for (int j = 0; j < NC; j++) IO::template set_body_pos<Real>(S, atom0 + j, rc + cvt<double>(apply(R, dj[j])));
     } else {
-        for (int j = sub; j < N; j += lanes) {
-            V3<Real> dj = ld3<Real>(S.d, S.dStride, dbase + j);
-            IO::template set_body_pos<Real>(S, atom0 + j, rc + cvt<double>(apply(R, dj)));
+        // loads of a batch first, then its stores (one round trip per batch)
+        constexpr int U = STRIDED_BATCH;
+        for (int j0 = sub; j0 < N; j0 += U * lanes) {
+            V3<Real> dj[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (j0 + u * lanes < N) dj[u] = ld3<Real>(S.d, S.dStride, dbase + j0 + u * lanes);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (j0 + u * lanes < N)
+                    IO::template set_body_pos<Real>(S, atom0 + j0 + u * lanes, rc + cvt<double>(apply(R, dj[u])));
         }
     }
 }
@@ -195,9 +208,16 @@ __device__ __forceinline__ void body_velocities(const DevSys& S, const Rot<Real>
             IO::template set_vel<Real>(S, atom0 + j, vcm + cross(ws, dl));
         }
     } else {
-        for (int j = sub; j < N; j += lanes) {
-            V3<Real> dl = apply(R, ld3<Real>(S.d, S.dStride, dbase + j));
-            IO::template set_vel<Real>(S, atom0 + j, vcm + cross(ws, dl));
+        constexpr int U = STRIDED_BATCH;
+        for (int j0 = sub; j0 < N; j0 += U * lanes) {
+            V3<Real> dj[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (j0 + u * lanes < N) dj[u] = ld3<Real>(S.d, S.dStride, dbase + j0 + u * lanes);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (j0 + u * lanes < N)
+                    IO::template set_vel<Real>(S, atom0 + j0 + u * lanes, vcm + cross(ws, apply(R, dj[u])));
         }
     }
 }
