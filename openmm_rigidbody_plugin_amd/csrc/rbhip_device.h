@@ -133,9 +133,29 @@ __device__ __forceinline__ void sin_cosm1_small(double phi, double& s, double& c
     s = phi * fma(x2, fma(x2, fma(x2, fma(x2, 1.0 / 362880.0, -1.0 / 5040.0), 1.0 / 120.0), -1.0 / 6.0), 1.0);
     cm1 = x2 * fma(x2, fma(x2, fma(x2, fma(x2, -1.0 / 3628800.0, 1.0 / 40320.0), -1.0 / 720.0), 1.0 / 24.0), -0.5);
 }
+// fp32 on |phi| <= 0.03: through phi^3 / phi^4 (truncation < 7e-9 relative,
+// below fp32 rounding)
+__device__ __forceinline__ void sin_cosm1_tiny(float phi, float& s, float& cm1) {
+    float x2 = phi * phi;
+    s = fmaf(x2 * phi, -1.0f / 6.0f, phi);
+    cm1 = x2 * fmaf(x2, 1.0f / 24.0f, -0.5f);
+}
+__device__ __forceinline__ void sin_cosm1_tiny(double phi, double& s, double& cm1) { sin_cosm1_small(phi, s, cm1); }
 template <class T> struct SmallAngle;
 template <> struct SmallAngle<float> { static constexpr float value = 0.1f; };
 template <> struct SmallAngle<double> { static constexpr double value = 0.0625; };
+// (FP64 has no shorter tier)
+template <class T> struct TinyAngle;
+template <> struct TinyAngle<float> { static constexpr float value = 0.03f; };
+template <> struct TinyAngle<double> { static constexpr double value = -1.0; };
+
+// the sin / cos - 1 evaluation a rotation uses: TIER 0 general, 1 the
+// small-angle polynomials, 2 the tiny-angle ones
+template <int TIER, class T> __device__ __forceinline__ void sin_cosm1_tier(T phi, T& s, T& cm1) {
+    if (TIER == 2) sin_cosm1_tiny(phi, s, cm1);
+    else if (TIER == 1) sin_cosm1_small(phi, s, cm1);
+    else sin_cosm1(phi, s, cm1);
+}
 
 // One uniaxial rotation of NO-SQUISH in body-frame variables.  For
 // pi = 2 B(q) L and |q| = 1 the reference's angle (pi . P_K q) tau / (4 I_K)
@@ -150,12 +170,11 @@ template <> struct SmallAngle<double> { static constexpr double value = 0.0625; 
 // and |L|.  In fp32 this keeps the sequential updates of a 10-substep step
 // from random-walking the orientation (~1e-6 rad per 10 steps when q itself is
 // updated in fp32) at the cost of fp32 arithmetic.
-template <int K, bool SMALL, class T>
+template <int K, int TIER, class T>
 __device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho, T coef) {
     T LK = K == 1 ? L0.x + lam.x : (K == 2 ? L0.y + lam.y : L0.z + lam.z);
     T s, cm1;
-    if (SMALL) sin_cosm1_small(LK * coef, s, cm1);
-    else sin_cosm1(LK * coef, s, cm1);
+    sin_cosm1_tier<TIER>(LK * coef, s, cm1);
     // rho <- rho + rho cm1 + P_K(rho) s + (cm1, s e_K)   [r (x) (1 + cm1, s e_K)]
     Q4<T> Pr = perm<K>(rho);
     Q4<T> nr = {fma(rho.a, cm1, fma(Pr.a, s, rho.a)) + cm1, fma(rho.b, cm1, fma(Pr.b, s, rho.b)),
@@ -181,24 +200,24 @@ __device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho,
 // axis leave that component of L unchanged, so the back-to-back R3(h/2) R3(h/2)
 // of neighbouring substeps are one R3(h) (R2 likewise for dof-5 bodies): 4n+1
 // rotations instead of 5n (2n+1 instead of 3n).
-template <bool SMALL, class T>
+template <int TIER, class T>
 __device__ __forceinline__ void no_squish_turns(V3<T> L0, V3<T>& lam, Q4<T>& rho, V3<T> invI, T h, int n) {
     T c1 = (T)0.5 * h * invI.x;
     T c2h = (T)0.25 * h * invI.y, c2 = (T)0.5 * h * invI.y;
     T c3h = (T)0.25 * h * invI.z, c3 = (T)0.5 * h * invI.z;
     if (invI.z != (T)0) {
-        body_axis_turn<3, SMALL>(L0, lam, rho, c3h);
+        body_axis_turn<3, TIER>(L0, lam, rho, c3h);
         for (int i = 0; i < n; i++) {
-            body_axis_turn<2, SMALL>(L0, lam, rho, c2h);
-            body_axis_turn<1, SMALL>(L0, lam, rho, c1);
-            body_axis_turn<2, SMALL>(L0, lam, rho, c2h);
-            body_axis_turn<3, SMALL>(L0, lam, rho, i + 1 < n ? c3 : c3h);
+            body_axis_turn<2, TIER>(L0, lam, rho, c2h);
+            body_axis_turn<1, TIER>(L0, lam, rho, c1);
+            body_axis_turn<2, TIER>(L0, lam, rho, c2h);
+            body_axis_turn<3, TIER>(L0, lam, rho, i + 1 < n ? c3 : c3h);
         }
     } else {
-        body_axis_turn<2, SMALL>(L0, lam, rho, c2h);
+        body_axis_turn<2, TIER>(L0, lam, rho, c2h);
         for (int i = 0; i < n; i++) {
-            body_axis_turn<1, SMALL>(L0, lam, rho, c1);
-            body_axis_turn<2, SMALL>(L0, lam, rho, i + 1 < n ? c2 : c2h);
+            body_axis_turn<1, TIER>(L0, lam, rho, c1);
+            body_axis_turn<2, TIER>(L0, lam, rho, i + 1 < n ? c2 : c2h);
         }
     }
 }
@@ -207,7 +226,7 @@ __device__ __forceinline__ void no_squish_turns(V3<T> L0, V3<T>& lam, Q4<T>& rho
 // are marked by invI.z == 0.  On return q = q (x) r renormalised in FP64 and
 // L = L0 + lam formed in FP64.  |L| is conserved by every turn, so
 // |phi| <= |L| max(coef) bounds all angles of the step and selects the
-// branch-free small-angle path for the whole body.
+// branch-free small- or tiny-angle path for the whole body.
 template <class T>
 __device__ __forceinline__ void no_squish(Q4<T>& q, V3<T>& L, V3<T> invI, double dt, int n) {
     V3<T> L0 = L;
@@ -217,10 +236,12 @@ __device__ __forceinline__ void no_squish(Q4<T>& q, V3<T>& L, V3<T> invI, double
     double cmax = 0.5 * (double)h * fmax((double)invI.x, fmax((double)invI.y, (double)invI.z));
     V3<double> Ld = cvt<double>(L0);
     double phimax = 1.001 * sqrt(dot(Ld, Ld)) * cmax;
-    if (phimax <= (double)SmallAngle<T>::value)
-        no_squish_turns<true>(L0, lam, rho, invI, h, n);
+    if (phimax <= (double)TinyAngle<T>::value)
+        no_squish_turns<2>(L0, lam, rho, invI, h, n);
+    else if (phimax <= (double)SmallAngle<T>::value)
+        no_squish_turns<1>(L0, lam, rho, invI, h, n);
     else
-        no_squish_turns<false>(L0, lam, rho, invI, h, n);
+        no_squish_turns<0>(L0, lam, rho, invI, h, n);
     // q (x) r, renormalised in FP64
     Q4<double> qd = cvt<double>(q);
     Q4<double> r = {1.0 + (double)rho.a, (double)rho.b, (double)rho.c, (double)rho.d};
