@@ -313,6 +313,15 @@ int rbhip_get_body_fixed_positions(rbhip_context* ctx, double* xyz);
 int rbhip_set_launch(rbhip_context* ctx, int blockSize, int maxBlocks);
 /* lanes per body for the bodies of N atoms (power of two, 1..64) */
 int rbhip_set_lanes(rbhip_context* ctx, int atomsPerBody, int lanes);
+/* bodies of one shape (mixed precision, on by default): at every build of
+   the body frames from positions, the bodies of a size class whose
+   body-fixed coordinates agree to within four fp32 ulps of the body's size
+   (one molecule type; up to 8 shapes per class, the frame's axis signs) take
+   one set of values and read shared rows in the step passes, 12 B per atom
+   less HBM traffic per pass.  Takes effect at the next position upload or
+   state change.  (A storage choice of this platform: the reference keeps
+   every body's coordinates, RigidBodySystem.cpp:94-95.) */
+int rbhip_set_shared_frames(rbhip_context* ctx, int enable);
 /* times candidate (lanes per body, workgroup size) tiles on a scratch copy of
    the state and keeps the fastest; the context's state is not advanced */
 int rbhip_autotune(rbhip_context* ctx, int reps);

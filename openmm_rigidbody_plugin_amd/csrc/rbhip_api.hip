@@ -382,6 +382,8 @@ static size_t context_arena_bytes(size_t na, int numBodies, int numFree, int num
     b += a(sizeof(DevSys)) + a(4 * sizeof(int)) + a(sizeof(int) * nb) * 2;
     b += a(sizeof(double) * std::max(3 * na, 36 * nb)) + a(4 * sizeof(double));  // staging, reduction
     b += a(sizeof(int) * na) * 3;                      // body index, atom index, internal order
+    const size_t nc = std::max(numClasses, 1);
+    b += a(sizeof(int) * nb) + a(sizeof(int) * 8 * nc) + a(sizeof(int) * nc);   // dOff, shape rows, matches
     return b;
 }
 
@@ -436,10 +438,18 @@ struct rbhip_context {
     const std::vector<int>& bodyIndex() { return hostCopy(hostBodyIndex, dRefBodyIndex, numAtoms); }
     const std::vector<int>& atomIndex() { return hostCopy(hostAtomIndex, dRefAtomIndex, numFree + numBodyAtoms); }
     const std::vector<int>& atomOrig() { return hostCopy(hostAtomOrig, dAtomOrig, numAtoms); }
-    struct Class { int N, firstBody, count, firstAtom; };
+    // shapes: how many shapes of body-fixed coordinates the class's bodies
+    // share (0: each body reads its own rows)
+    struct Class { int N, firstBody, count, firstAtom, shapes = 0; };
     std::vector<Class> classes;
     std::map<int, int> lanes;                    // atoms per body -> lanes per body
     int block = 256, maxBlocks = 0;
+    // bodies of one shape share body-fixed coordinate rows (mixed precision)
+    static constexpr int kMaxShapes = 8, kShapeSample = 32, kShareMinBodies = 64;
+    bool shareFramesOn = true;                   // rbhip_set_shared_frames
+    int* dShapeRows = nullptr;                   // candidate rows, kMaxShapes per class
+    int* dShapeMatched = nullptr;                // bodies that matched, per class
+    int shapesVersion = 0;                       // bumped when a class's shapes change
 
     DeviceArena arena;
     DeviceArena tileArena;
@@ -572,7 +582,7 @@ struct rbhip_context {
             int per = blk / L;
             for (int b = 0; b < c.count; b += per)
                 t.push_back(Tile{0, TILE_BODIES, c.firstBody + b, std::min(per, c.count - b), c.N,
-                                 c.firstAtom + b * c.N, L, 0});
+                                 c.firstAtom + b * c.N, L, 0, c.shapes > 0 ? 1 : 0});
         }
         for (size_t i = 0; i < t.size(); i++) t[i].slot = (int)i;
         return launchOrder(t);
@@ -663,18 +673,19 @@ struct rbhip_context {
     // body, kept for reuse (a batch concatenates them on every change of its
     // member list)
     struct TileCache {
-        int block = -1;
+        int block = -1, shapesVersion = -1;
         std::map<int, int> lanes;
         std::vector<Tile> part, force;
     };
     TileCache tileCache;
     const TileCache& tilesFor(int blk) {
-        if (tileCache.block != blk || tileCache.lanes != lanes) {
+        if (tileCache.block != blk || tileCache.lanes != lanes || tileCache.shapesVersion != shapesVersion) {
             tileCache.part = partTiles(blk, lanes);
             tileCache.force = forceTiles(blk);
             validateTiles(tileCache.part, blk, maxKeSlots);
             validateTiles(tileCache.force, blk, maxPeSlots);
             tileCache.block = blk;
+            tileCache.shapesVersion = shapesVersion;
             tileCache.lanes = lanes;
         }
         return tileCache;
@@ -806,6 +817,7 @@ struct rbhip_context {
         h.useStageV = stageVFresh ? 1 : 0;
         uploadSys();
         HIPCHK(launch_setup(precision, dSys, dSetupTiles, nSetupTiles, geometry, setupBlock(), stream));
+        if (geometry) shareFrames();
         if (nClusterTiles > 0)
             HIPCHK(launch_clusters(precision, dSys, dClusterSetupTiles, clusterClassTiles, CLUSTER_KE, kClusterBlock,
                                    stream));
@@ -818,6 +830,93 @@ struct rbhip_context {
         reduce(0, r);
         // RigidBodySystem.cpp:130: each constraint removes a degree of freedom
         if (geometry) numDOF = numFree - numConstraints + (int)std::lround(r[2]);
+    }
+
+    // Bodies of one shape (mixed precision).  Rigid bodies of one molecule
+    // type have the same body-fixed coordinates d up to the frame's axis
+    // signs and rounding far below fp32's, so most bodies of such a class
+    // can read a few shared rows (cached) instead of 12 B per atom of their
+    // own in every step pass.  For each class of at least kShareMinBodies
+    // bodies, its first kShapeSample bodies give the candidate shapes
+    // (greedily: a body within tol of no earlier candidate is a new one; more
+    // than kMaxShapes, and the class keeps its own rows); k_match_frames then
+    // gives every body of the class that lies within tol of a candidate that
+    // candidate's values and rows.  tol is 2^-21 of the largest coordinate of
+    // the class's first body: four fp32 ulps at its scale, so a body moves by
+    // at most that (1e-8 nm for water).  A class whose bodies matched for at
+    // least half of them reads the shared rows (Tile::sharedD).  Runs after
+    // every build of the body frames.
+    void shareFrames() {
+        std::vector<int> before;
+        for (Class& c : classes) {
+            before.push_back(c.shapes);
+            c.shapes = 0;
+        }
+        if (shareFramesOn && precision == RBHIP_PRECISION_MIXED && numBodies > 0) {
+            const size_t nc = classes.size();
+            std::vector<int> rows(kMaxShapes * nc, 0), ncand(nc, 0);
+            std::vector<double> tol(nc, 0.0);
+            std::vector<float> sample;
+            for (size_t ci = 0; ci < nc; ci++) {
+                const Class& c = classes[ci];
+                if (c.count < kShareMinBodies) continue;
+                const int ns = std::min(c.count, kShapeSample), row0 = c.firstAtom - numFree, len = ns * c.N;
+                sample.assign(3 * (size_t)len, 0.0f);
+                for (int comp = 0; comp < 3; comp++)
+                    HIPCHK(hipMemcpyAsync(sample.data() + (size_t)comp * len,
+                                          reinterpret_cast<const float*>(h.d) + (size_t)comp * h.dStride + row0,
+                                          sizeof(float) * len, hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipStreamSynchronize(stream));
+                auto at = [&](int body, int j, int comp) { return (double)sample[(size_t)comp * len + body * c.N + j]; };
+                double scale = 0.0;
+                for (int j = 0; j < c.N; j++)
+                    for (int comp = 0; comp < 3; comp++) scale = std::max(scale, std::fabs(at(0, j, comp)));
+                if (!(scale > 0.0)) continue;
+                tol[ci] = std::ldexp(scale, -21);
+                std::vector<int> cands;
+                bool tooMany = false;
+                for (int b = 0; b < ns && !tooMany; b++) {
+                    bool found = false;
+                    for (int cb : cands) {
+                        bool same = true;
+                        for (int j = 0; same && j < c.N; j++)
+                            for (int comp = 0; same && comp < 3; comp++)
+                                same = std::fabs(at(b, j, comp) - at(cb, j, comp)) <= tol[ci];
+                        if (same) { found = true; break; }
+                    }
+                    if (found) continue;
+                    if ((int)cands.size() == kMaxShapes) tooMany = true;
+                    else cands.push_back(b);
+                }
+                if (tooMany) continue;
+                ncand[ci] = (int)cands.size();
+                for (size_t i = 0; i < cands.size(); i++) rows[kMaxShapes * ci + i] = row0 + cands[i] * c.N;
+            }
+            bool any = false;
+            for (int n : ncand) any = any || n > 0;
+            if (any) {
+                std::vector<int> matched(nc, 0);
+                HIPCHK(hipMemcpyAsync(dShapeRows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, stream));
+                HIPCHK(hipMemcpyAsync(dShapeMatched, matched.data(), sizeof(int) * nc, hipMemcpyHostToDevice, stream));
+                for (size_t ci = 0; ci < nc; ci++) {
+                    const Class& c = classes[ci];
+                    if (ncand[ci] > 0)
+                        HIPCHK(launch_match_frames(precision, dSys, c.firstBody, c.count, c.N, c.firstAtom - numFree,
+                                                   dShapeRows + kMaxShapes * ci, ncand[ci], tol[ci],
+                                                   dShapeMatched + ci, stream));
+                }
+                HIPCHK(hipMemcpyAsync(matched.data(), dShapeMatched, sizeof(int) * nc, hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipStreamSynchronize(stream));
+                for (size_t ci = 0; ci < nc; ci++)
+                    if (ncand[ci] > 0 && 2 * matched[ci] >= classes[ci].count) classes[ci].shapes = ncand[ci];
+            }
+        }
+        bool changed = false;
+        for (size_t ci = 0; ci < classes.size(); ci++) changed = changed || before[ci] != classes[ci].shapes;
+        if (changed) {
+            shapesVersion++;
+            rebuildTiles();
+        }
     }
 
     // original-order FP64 xyz rows (host) -> the staging buffer dTmp, still
@@ -1362,6 +1461,7 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
         h.iI = C.arena.alloc(4 * R * nb);
         h.rcm = C.arena.allocT<double>(3 * (size_t)h.bodyStride);
         h.d = C.arena.alloc(3 * R * (size_t)h.dStride);
+        h.dOff = C.arena.allocT<int>(nb);
         h.pos = C.arena.alloc(3 * R * as);
         h.posLo = precision == RBHIP_PRECISION_MIXED ? C.arena.allocT<float>(3 * as) : nullptr;
         h.vel = C.arena.alloc(3 * R * as);
@@ -1392,6 +1492,8 @@ static int create_context(int numAtoms, const double* masses, const int* bodyInd
         C.tmpCount = std::max(3 * na, 36 * nb);
         C.dTmp = C.arena.allocT<double>(C.tmpCount);
         C.dRed = C.arena.allocT<double>(4);
+        C.dShapeRows = C.arena.allocT<int>((size_t)rbhip_context::kMaxShapes * std::max<size_t>(C.classes.size(), 1));
+        C.dShapeMatched = C.arena.allocT<int>(std::max<size_t>(C.classes.size(), 1));
         require(C.arena.bytes == context_arena_bytes(na, C.numBodies, C.numFree, C.numBodyAtoms,
                                                      (int)C.classes.size(), R, clusterSlots),
                 "internal: device size estimate out of step with the allocation", RBHIP_E_STATE);
@@ -2264,6 +2366,13 @@ int rbhip_set_lanes(rbhip_context* ctx, int atomsPerBody, int lanes) {
     });
 }
 
+int rbhip_set_shared_frames(rbhip_context* ctx, int enable) {
+    return guard([&] {
+        require(ctx, "null context");
+        ctx->shareFramesOn = enable != 0;
+    });
+}
+
 int rbhip_describe(rbhip_context* ctx, char* buf, int len) {
     return guard([&] {
         require(ctx && buf && len > 0, "null argument");
@@ -2275,6 +2384,10 @@ int rbhip_describe(rbhip_context* ctx, char* buf, int len) {
                  std::to_string(it == ctx->lanes.end() ? 1 : it->second) + " ";
         }
         s += "] free=" + std::to_string(ctx->numFree);
+        std::string shapes;
+        for (auto& c : ctx->classes)
+            if (c.shapes > 0) shapes += " N" + std::to_string(c.N) + ":" + std::to_string(c.shapes);
+        if (!shapes.empty()) s += " sharedShapes=[" + shapes.substr(1) + "]";
         std::snprintf(buf, len, "%s", s.c_str());
     });
 }

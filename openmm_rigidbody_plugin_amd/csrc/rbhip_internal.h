@@ -52,6 +52,9 @@ struct Tile {
     int atomFirst;  // internal index of the tile's first body atom (TILE_BODIES)
     int lanes;      // lanes per body, a power of two (TILE_BODIES)
     int slot;       // reduction slot of this tile in the system's partials
+    // 1: the tile's bodies read their body-fixed coordinates at the rows
+    // DevSys::dOff names (bodies of one shape share rows), 0: at their own
+    int sharedD = 0;
 };
 
 // One system as the kernels see it.  Pointers to Real data are void* and
@@ -75,6 +78,10 @@ struct DevSys {
     void* iI;
     double* rcm;
     void* d;
+    // per body, the first row of d it reads when its tile shares rows: its
+    // own rows, or those of a body of the same shape whose coordinates it
+    // equals to within a few fp32 ulps (its own rows then hold the same values)
+    int* dOff;
     // atoms
     void* pos;
     float* posLo;

@@ -85,10 +85,15 @@ template <class Real> __device__ __forceinline__ Body<Real> load_body(const DevS
 constexpr int STRIDED_BATCH = 8;
 
 // F = sum f_j and tau = sum delta_j x f_j in FP64, delta_j = A(q)^T d_j
+// first row of body k's body-fixed coordinates: its own, or, in a tile of
+// bodies that share rows, the rows DevSys::dOff names for it
+__device__ __forceinline__ int d_row(const DevSys& S, const Tile& tile, int k, int atom0) {
+    return tile.sharedD ? S.dOff[k] : atom0 - S.numFree;
+}
+
 template <class Real, int NC, class IO>
-__device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, int atom0, int N, int sub, int lanes,
-                                          V3<double>& F, V3<double>& tau) {
-    const int dbase = atom0 - S.numFree;
+__device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, int atom0, int dbase, int N, int sub,
+                                          int lanes, V3<double>& F, V3<double>& tau) {
     if constexpr (NC > 0) {
         V3<Real> dj[NC], fj[NC];
 #pragma unroll
@@ -132,22 +137,22 @@ __device__ __forceinline__ void body_sums(const DevSys& S, const Rot<Real>& R, i
 }
 
 template <class Real, class IO = SoaAtoms>
-__device__ __forceinline__ void body_sums_any(const DevSys& S, const Rot<Real>& R, int atom0, int N, int sub,
-                                              int lanes, bool active, V3<double>& F, V3<double>& tau) {
+__device__ __forceinline__ void body_sums_any(const DevSys& S, const Rot<Real>& R, int atom0, int dbase, int N,
+                                              int sub, int lanes, bool active, V3<double>& F, V3<double>& tau) {
     F = {0.0, 0.0, 0.0};
     tau = {0.0, 0.0, 0.0};
     if (lanes == 1) {
         if (active) {
             switch (N) {
-            case 2: body_sums<Real, 2, IO>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 3: body_sums<Real, 3, IO>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 4: body_sums<Real, 4, IO>(S, R, atom0, N, 0, 1, F, tau); break;
-            case 6: body_sums<Real, 6, IO>(S, R, atom0, N, 0, 1, F, tau); break;
-            default: body_sums<Real, 0, IO>(S, R, atom0, N, 0, 1, F, tau); break;
+            case 2: body_sums<Real, 2, IO>(S, R, atom0, dbase, N, 0, 1, F, tau); break;
+            case 3: body_sums<Real, 3, IO>(S, R, atom0, dbase, N, 0, 1, F, tau); break;
+            case 4: body_sums<Real, 4, IO>(S, R, atom0, dbase, N, 0, 1, F, tau); break;
+            case 6: body_sums<Real, 6, IO>(S, R, atom0, dbase, N, 0, 1, F, tau); break;
+            default: body_sums<Real, 0, IO>(S, R, atom0, dbase, N, 0, 1, F, tau); break;
             }
         }
     } else {
-        if (active) body_sums<Real, 0, IO>(S, R, atom0, N, sub, lanes, F, tau);
+        if (active) body_sums<Real, 0, IO>(S, R, atom0, dbase, N, sub, lanes, F, tau);
         F = {xor_sum(F.x, lanes), xor_sum(F.y, lanes), xor_sum(F.z, lanes)};
         tau = {xor_sum(tau.x, lanes), xor_sum(tau.y, lanes), xor_sum(tau.z, lanes)};
     }
@@ -156,9 +161,8 @@ __device__ __forceinline__ void body_sums_any(const DevSys& S, const Rot<Real>& 
 // atom positions r_cm + A(q)^T d_j: fp32 in mixed precision (the force field's
 // view; the exact value is rebuilt from the body state on download)
 template <class Real, int NC, class IO>
-__device__ __forceinline__ void body_positions(const DevSys& S, const Rot<Real>& R, V3<double> rc, int atom0, int N,
-                                               int sub, int lanes) {
-    const int dbase = atom0 - S.numFree;
+__device__ __forceinline__ void body_positions(const DevSys& S, const Rot<Real>& R, V3<double> rc, int atom0,
+                                               int dbase, int N, int sub, int lanes) {
     if constexpr (NC > 0) {
         V3<Real> dj[NC];
 #pragma unroll
@@ -183,24 +187,23 @@ __device__ __forceinline__ void body_positions(const DevSys& S, const Rot<Real>&
 
 template <class Real, class IO>
 __device__ __forceinline__ void body_positions_any(const DevSys& S, const Rot<Real>& R, V3<double> rc, int atom0,
-                                                   int N, int sub, int lanes) {
+                                                   int dbase, int N, int sub, int lanes) {
     if (lanes == 1) {
         switch (N) {
-        case 2: body_positions<Real, 2, IO>(S, R, rc, atom0, N, 0, 1); break;
-        case 3: body_positions<Real, 3, IO>(S, R, rc, atom0, N, 0, 1); break;
-        case 4: body_positions<Real, 4, IO>(S, R, rc, atom0, N, 0, 1); break;
-        case 6: body_positions<Real, 6, IO>(S, R, rc, atom0, N, 0, 1); break;
-        default: body_positions<Real, 0, IO>(S, R, rc, atom0, N, 0, 1); break;
+        case 2: body_positions<Real, 2, IO>(S, R, rc, atom0, dbase, N, 0, 1); break;
+        case 3: body_positions<Real, 3, IO>(S, R, rc, atom0, dbase, N, 0, 1); break;
+        case 4: body_positions<Real, 4, IO>(S, R, rc, atom0, dbase, N, 0, 1); break;
+        case 6: body_positions<Real, 6, IO>(S, R, rc, atom0, dbase, N, 0, 1); break;
+        default: body_positions<Real, 0, IO>(S, R, rc, atom0, dbase, N, 0, 1); break;
         }
     } else {
-        body_positions<Real, 0, IO>(S, R, rc, atom0, N, sub, lanes);
+        body_positions<Real, 0, IO>(S, R, rc, atom0, dbase, N, sub, lanes);
     }
 }
 
 template <class Real, int NC, class IO>
 __device__ __forceinline__ void body_velocities(const DevSys& S, const Rot<Real>& R, V3<Real> vcm, V3<Real> ws,
-                                                int atom0, int N, int sub, int lanes) {
-    const int dbase = atom0 - S.numFree;
+                                                int atom0, int dbase, int N, int sub, int lanes) {
     if constexpr (NC > 0) {
 #pragma unroll
         for (int j = 0; j < NC; j++) {
@@ -301,6 +304,7 @@ __device__ __forceinline__ void part1_bodies(const DevSys& S, const Tile& tile) 
     if (b >= tile.count) return;
     const int k = tile.first + b;
     const int atom0 = tile.atomFirst + b * tile.N;
+    const int dbase = d_row(S, tile, k, atom0);
     Body<Real> B = load_body<Real>(S, k);
     V3<double> rc = load_rcm<Real>(S, k);
     drift_and_rotate<Real, EXACT>(S, B, rc);
@@ -309,7 +313,7 @@ __device__ __forceinline__ void part1_bodies(const DevSys& S, const Tile& tile) 
         stp3<Real>(S.L, k, B.L);
         store_rcm<Real>(S, k, rc);
     }
-    body_positions_any<Real, IO>(S, rotT(B.q), rc, atom0, tile.N, sub, tile.lanes);
+    body_positions_any<Real, IO>(S, rotT(B.q), rc, atom0, dbase, tile.N, sub, tile.lanes);
 }
 
 template <class Real, bool EXACT, class IO>
@@ -352,10 +356,11 @@ __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, 
     const int k = tile.first + (active ? b : 0);
     const int N = tile.N;
     const int atom0 = tile.atomFirst + (active ? b : 0) * N;
+    const int dbase = d_row(S, tile, k, atom0);
     Body<Real> B = load_body<Real>(S, k);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real, IO>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real, IO>(S, R, atom0, dbase, N, sub, tile.lanes, active, F, tau);
     if (!active) return;
     const double halfDt = 0.5 * S.dt;
     V3<double> tb = axial_torque(to_body(R, tau), B.invI);
@@ -365,11 +370,11 @@ __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, 
     V3<double> vcm = p * (double)B.invM;
     V3<Real> ws = apply(R, cvt<Real>(w));
     switch (tile.lanes == 1 ? N : 0) {
-    case 2: body_velocities<Real, 2, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-    case 3: body_velocities<Real, 3, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-    case 4: body_velocities<Real, 4, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-    case 6: body_velocities<Real, 6, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, 0, 1); break;
-    default: body_velocities<Real, 0, IO>(S, R, cvt<Real>(vcm), ws, atom0, N, sub, tile.lanes); break;
+    case 2: body_velocities<Real, 2, IO>(S, R, cvt<Real>(vcm), ws, atom0, dbase, N, 0, 1); break;
+    case 3: body_velocities<Real, 3, IO>(S, R, cvt<Real>(vcm), ws, atom0, dbase, N, 0, 1); break;
+    case 4: body_velocities<Real, 4, IO>(S, R, cvt<Real>(vcm), ws, atom0, dbase, N, 0, 1); break;
+    case 6: body_velocities<Real, 6, IO>(S, R, cvt<Real>(vcm), ws, atom0, dbase, N, 0, 1); break;
+    default: body_velocities<Real, 0, IO>(S, R, cvt<Real>(vcm), ws, atom0, dbase, N, sub, tile.lanes); break;
     }
     if (sub == 0) {
         ke[0] = dot(p, vcm);
@@ -443,8 +448,7 @@ template <class Real> __device__ __forceinline__ void store_body(const DevSys& S
 // one lane per body of NC atoms: the body-fixed coordinates stay in registers
 // from the torque sum to the position rebuild, so d is read once
 template <class Real, bool EXACT, int NC, class IO>
-__device__ __forceinline__ void fused_body_regs(const DevSys& S, int k, int atom0) {
-    const int dbase = atom0 - S.numFree;
+__device__ __forceinline__ void fused_body_regs(const DevSys& S, int k, int atom0, int dbase) {
     Body<Real> B = load_body<Real>(S, k);
     V3<double> rc = load_rcm<Real>(S, k);
     V3<Real> dj[NC], fj[NC];
@@ -478,9 +482,10 @@ __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) 
         const int b = threadIdx.x;
         if (b >= tile.count) return;
         const int k = tile.first + b, atom0 = tile.atomFirst + b * N;
-        if (N == 2) fused_body_regs<Real, EXACT, 2, IO>(S, k, atom0);
-        else if (N == 3) fused_body_regs<Real, EXACT, 3, IO>(S, k, atom0);
-        else fused_body_regs<Real, EXACT, 4, IO>(S, k, atom0);
+        const int dbase = d_row(S, tile, k, atom0);
+        if (N == 2) fused_body_regs<Real, EXACT, 2, IO>(S, k, atom0, dbase);
+        else if (N == 3) fused_body_regs<Real, EXACT, 3, IO>(S, k, atom0, dbase);
+        else fused_body_regs<Real, EXACT, 4, IO>(S, k, atom0, dbase);
         return;
     }
     const int shift = lane_shift(tile.lanes);
@@ -489,16 +494,17 @@ __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) 
     const bool active = b < tile.count;
     const int k = tile.first + (active ? b : 0);
     const int atom0 = tile.atomFirst + (active ? b : 0) * N;
+    const int dbase = d_row(S, tile, k, atom0);
     Body<Real> B = load_body<Real>(S, k);
     V3<double> rc = load_rcm<Real>(S, k);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real, IO>(S, R, atom0, N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real, IO>(S, R, atom0, dbase, N, sub, tile.lanes, active, F, tau);
     if (!active) return;
     fused_kicks<Real>(S, B, R, F, tau);
     drift_and_rotate<Real, EXACT>(S, B, rc);
     if (sub == 0) store_body<Real>(S, k, B, rc);
-    body_positions_any<Real, IO>(S, rotT(B.q), rc, atom0, N, sub, tile.lanes);
+    body_positions_any<Real, IO>(S, rotT(B.q), rc, atom0, dbase, N, sub, tile.lanes);
 }
 
 template <class Real, bool EXACT, class IO>
@@ -577,7 +583,7 @@ __device__ __forceinline__ void refine_begin_bodies(const DevSys& S, const Tile&
     Body<Real> B = load_body<Real>(S, k);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real>(S, R, atom0, tile.N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real>(S, R, atom0, d_row(S, tile, k, atom0), tile.N, sub, tile.lanes, active, F, tau);
     if (!active || sub != 0) return;
     V3<double> tb = axial_torque(to_body(R, tau), B.invI);
     V3<double> ar;
@@ -642,7 +648,7 @@ __device__ __forceinline__ void refine_end_bodies(const DevSys& S, const Tile& t
     Body<Real> B = load_body<Real>(S, k);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real>(S, R, atom0, tile.N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real>(S, R, atom0, d_row(S, tile, k, atom0), tile.N, sub, tile.lanes, active, F, tau);
     if (!active || sub != 0) return;
     const double halfDt = 0.5 * S.dt;
     V3<double> tb = axial_torque(to_body(R, tau), B.invI);
@@ -704,7 +710,7 @@ __global__ __launch_bounds__(512) void k_refine_pe(const DevSys* __restrict__ sy
             Body<Real> B = load_body<Real>(S, k);
             Rot<Real> R = rotT(B.q);
             V3<double> F, tau;
-            body_sums_any<Real>(S, R, atom0, tile.N, sub, tile.lanes, active, F, tau);
+            body_sums_any<Real>(S, R, atom0, d_row(S, tile, k, atom0), tile.N, sub, tile.lanes, active, F, tau);
             if (active && sub == 0) {
                 V3<double> tb = axial_torque(to_body(R, tau), B.invI);
                 u[0] = dot(F, F) * (double)B.invM + dot(hadamard(tb, cvt<double>(B.invI)), tb);

@@ -12,6 +12,7 @@
 //   k_stage_in/out FP64 host-order staging <-> device precision state
 //   k_body_atoms_out  FP64 body-atom positions r_cm + A(q)^T d_j rebuilt from
 //                  the body state (the device keeps them in fp32 only)
+//   k_match_frames bodies of one shape: shared body-fixed coordinates
 #include <hip/hip_runtime.h>
 
 #include "rbhip_access.h"
@@ -124,6 +125,7 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
             A = eigenvectors3(inertia, I);
         }
         q = quat_from_matrix(A);
+        S.dOff[k] = dbase;   // its own rows until k_match_frames finds a shared shape
         for (int j = 0; j < N; j++) {
             V3<double> dl = setup_pos<Real>(S, atom0 + j) - rcm;
             st3<Real>(S.d, S.dStride, dbase + j, cvt<Real>(m3_apply(A, dl)));
@@ -454,7 +456,52 @@ __global__ void k_body_atoms_out(const DevSys* __restrict__ sys, const int* __re
     }
 }
 
+// Bodies of one shape.  The bodies of a size class (count bodies of N atoms
+// from firstBody, their d rows from row0) are compared with candidate shapes:
+// the d rows of a few bodies of the class, given by their first row.  A body
+// whose coordinates all lie within tol of a candidate's takes that
+// candidate's values into its own rows and its rows as DevSys::dOff, so the
+// step kernels of a sharing tile read the shared rows (few, cached) and
+// every other reader sees the same values in the body's own rows.  The
+// bodies that matched are counted in *matched.  A candidate matches itself
+// and copies nothing; a body that is no candidate writes only its own rows.
+template <class Real>
+__global__ void k_match_frames(const DevSys* __restrict__ sys, int firstBody, int count, int N, int row0,
+                               const int* __restrict__ candRows, int numCand, double tol, int* matched) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= count) return;
+    const DevSys& S = *sys;
+    const int own = row0 + b * N;
+    for (int c = 0; c < numCand; c++) {
+        const int cr = candRows[c];
+        bool same = true;
+        for (int j = 0; same && j < N; j++) {
+            V3<Real> x = ld3<Real>(S.d, S.dStride, own + j), y = ld3<Real>(S.d, S.dStride, cr + j);
+            same = fabs((double)x.x - (double)y.x) <= tol && fabs((double)x.y - (double)y.y) <= tol &&
+                   fabs((double)x.z - (double)y.z) <= tol;
+        }
+        if (same) {
+            if (cr != own)
+                for (int j = 0; j < N; j++) st3<Real>(S.d, S.dStride, own + j, ld3<Real>(S.d, S.dStride, cr + j));
+            S.dOff[firstBody + b] = cr;
+            atomicAdd(matched, 1);
+            return;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
+hipError_t launch_match_frames(int precision, const DevSys* sys, int firstBody, int count, int N, int row0,
+                               const int* candRows, int numCand, double tol, int* matched, hipStream_t stream) {
+    if (count <= 0 || numCand <= 0) return hipSuccess;
+    const int grid = (count + 255) / 256;
+    if (precision == 0)
+        k_match_frames<float><<<grid, 256, 0, stream>>>(sys, firstBody, count, N, row0, candRows, numCand, tol, matched);
+    else
+        k_match_frames<double><<<grid, 256, 0, stream>>>(sys, firstBody, count, N, row0, candRows, numCand, tol, matched);
+    return hipGetLastError();
+}
+
 hipError_t launch_setup(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int geometry,
                         int block, hipStream_t stream) {
     if (numTiles <= 0) return hipSuccess;

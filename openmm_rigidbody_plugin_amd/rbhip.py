@@ -87,6 +87,7 @@ def lib() -> ctypes.CDLL:
             "rbhip_get_body_fixed_positions": (_I, [_P, _DP]),
             "rbhip_set_launch": (_I, [_P, _I, _I]),
             "rbhip_set_lanes": (_I, [_P, _I, _I]),
+            "rbhip_set_shared_frames": (_I, [_P, _I]),
             "rbhip_autotune": (_I, [_P, _I]),
             "rbhip_describe": (_I, [_P, ctypes.c_char_p, _I]),
             "rbhip_time_kernels": (_I, [_P, _I, _DP, _DP]),
@@ -192,9 +193,11 @@ class Context:
 
     @classmethod
     def from_spec(cls, spec, precision="mixed", rotation_mode=None, dt=None, device=0, set_state=True,
-                  constraints=None):
+                  constraints=None, shared_frames=True):
         c = cls(spec.masses, spec.body_indices, spec.dt if dt is None else dt,
                 spec.rotation_mode if rotation_mode is None else rotation_mode, precision, device, constraints)
+        if not shared_frames:
+            c.set_shared_frames(False)
         c.set_tether(spec.tether_k, spec.tether_r0)
         if set_state:
             c.set_positions(spec.positions)
@@ -400,6 +403,11 @@ class Context:
 
     def set_lanes(self, atoms_per_body, lanes):
         _check(lib().rbhip_set_lanes(self._h, int(atoms_per_body), int(lanes)))
+
+    def set_shared_frames(self, enable=True):
+        """bodies of one shape read shared body-fixed coordinates (mixed
+        precision; applies from the next position upload)"""
+        _check(lib().rbhip_set_shared_frames(self._h, 1 if enable else 0))
 
     def autotune(self, reps=5):
         _check(lib().rbhip_autotune(self._h, int(reps)))
