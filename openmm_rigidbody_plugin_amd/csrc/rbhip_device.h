@@ -170,11 +170,12 @@ template <int TIER, class T> __device__ __forceinline__ void sin_cosm1_tier(T ph
 // and |L|.  In fp32 this keeps the sequential updates of a 10-substep step
 // from random-walking the orientation (~1e-6 rad per 10 steps when q itself is
 // updated in fp32) at the cost of fp32 arithmetic.
+// (coef and L0coef = L0_K coef: the angle is phi = L0coef + lam_K coef)
 template <int K, int TIER, class T>
-__device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho, T coef) {
-    T LK = K == 1 ? L0.x + lam.x : (K == 2 ? L0.y + lam.y : L0.z + lam.z);
+__device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho, T coef, T L0coef) {
+    T lamK = K == 1 ? lam.x : (K == 2 ? lam.y : lam.z);
     T s, cm1;
-    sin_cosm1_tier<TIER>(LK * coef, s, cm1);
+    sin_cosm1_tier<TIER>(fma(lamK, coef, L0coef), s, cm1);
     // rho <- rho + rho cm1 + P_K(rho) s + (cm1, s e_K)   [r (x) (1 + cm1, s e_K)]
     Q4<T> Pr = perm<K>(rho);
     Q4<T> nr = {fma(rho.a, cm1, fma(Pr.a, s, rho.a)) + cm1, fma(rho.b, cm1, fma(Pr.b, s, rho.b)),
@@ -205,19 +206,24 @@ __device__ __forceinline__ void no_squish_turns(V3<T> L0, V3<T>& lam, Q4<T>& rho
     T c1 = (T)0.5 * h * invI.x;
     T c2h = (T)0.25 * h * invI.y, c2 = (T)0.5 * h * invI.y;
     T c3h = (T)0.25 * h * invI.z, c3 = (T)0.5 * h * invI.z;
+    const T p1 = L0.x * c1, p2h = L0.y * c2h;
     if (invI.z != (T)0) {
-        body_axis_turn<3, TIER>(L0, lam, rho, c3h);
+        const T p3h = L0.z * c3h, p3 = L0.z * c3;
+        body_axis_turn<3, TIER>(L0, lam, rho, c3h, p3h);
         for (int i = 0; i < n; i++) {
-            body_axis_turn<2, TIER>(L0, lam, rho, c2h);
-            body_axis_turn<1, TIER>(L0, lam, rho, c1);
-            body_axis_turn<2, TIER>(L0, lam, rho, c2h);
-            body_axis_turn<3, TIER>(L0, lam, rho, i + 1 < n ? c3 : c3h);
+            body_axis_turn<2, TIER>(L0, lam, rho, c2h, p2h);
+            body_axis_turn<1, TIER>(L0, lam, rho, c1, p1);
+            body_axis_turn<2, TIER>(L0, lam, rho, c2h, p2h);
+            const bool last = i + 1 == n;
+            body_axis_turn<3, TIER>(L0, lam, rho, last ? c3h : c3, last ? p3h : p3);
         }
     } else {
-        body_axis_turn<2, TIER>(L0, lam, rho, c2h);
+        const T p2 = L0.y * c2;
+        body_axis_turn<2, TIER>(L0, lam, rho, c2h, p2h);
         for (int i = 0; i < n; i++) {
-            body_axis_turn<1, TIER>(L0, lam, rho, c1);
-            body_axis_turn<2, TIER>(L0, lam, rho, i + 1 < n ? c2 : c2h);
+            body_axis_turn<1, TIER>(L0, lam, rho, c1, p1);
+            const bool last = i + 1 == n;
+            body_axis_turn<2, TIER>(L0, lam, rho, last ? c2h : c2, last ? p2h : p2);
         }
     }
 }
@@ -233,9 +239,11 @@ __device__ __forceinline__ void no_squish(Q4<T>& q, V3<T>& L, V3<T> invI, double
     V3<T> lam = {(T)0, (T)0, (T)0};
     Q4<T> rho = {(T)0, (T)0, (T)0, (T)0};
     T h = (T)(dt / (double)n);
-    double cmax = 0.5 * (double)h * fmax((double)invI.x, fmax((double)invI.y, (double)invI.z));
+    // (a bound, so fp32 is plenty: the 0.1% margin covers its rounding)
+    float cmax = 0.5f * (float)h * fmaxf((float)invI.x, fmaxf((float)invI.y, (float)invI.z));
+    V3<float> Lf = cvt<float>(L0);
+    double phimax = 1.001f * sqrtf(dot(Lf, Lf)) * cmax;
     V3<double> Ld = cvt<double>(L0);
-    double phimax = 1.001 * sqrt(dot(Ld, Ld)) * cmax;
     if (phimax <= (double)TinyAngle<T>::value)
         no_squish_turns<2>(L0, lam, rho, invI, h, n);
     else if (phimax <= (double)SmallAngle<T>::value)
