@@ -56,7 +56,10 @@ def algorithmic_bytes(kernel, n_atoms_per_body, num_bodies, num_free, indexed=Fa
     Atoms not stored body by body (`indexed`) add an int32 index read per
     atom and pass (+4 B).  What the implementation moves beyond these bytes
     (the FP64 centre of mass, the free atoms' low position parts) is
-    precision overhead and is never counted here."""
+    precision overhead and is never counted here; what it moves below them
+    (bodies of one shape reading shared body-fixed coordinates) is not
+    subtracted either: these are the bytes of the canonical step, and the
+    measured traffic is reported beside them."""
     N = n_atoms_per_body
     idx = 4 if indexed else 0
     atoms = num_bodies * N + num_free
@@ -73,21 +76,25 @@ def algorithmic_bytes(kernel, n_atoms_per_body, num_bodies, num_free, indexed=Fa
     return b + idx * atoms
 
 
-def implementation_bytes(kernel, n_atoms_per_body, num_bodies, num_free):
+def implementation_bytes(kernel, n_atoms_per_body, num_bodies, num_free, shared_frames=False):
     """Bytes the mixed-precision kernels move by design: body state as q (16),
     body-frame L and p (12 each), 1/I and 1/M (16), r_cm in FP64 (24); free
     atoms' positions as fp32 hi + lo (24); the tether's anchor in fp32.
       part1: 132 + 24N per body, 88 per free atom
       part2: 80 + 36N per body, 40 per free atom
       fused: 144 + 36N per body, 88 per free atom
-      forces: 36 per atom, + 12 per free atom (low position part)"""
+      forces: 36 per atom, + 12 per free atom (low position part)
+    With shared frames (bodies of one shape read a few cached rows of
+    body-fixed coordinates) each pass reads a 4-byte row index per body
+    instead of its 12N bytes of coordinates."""
     N = n_atoms_per_body
+    d = (4 - 12 * N) if shared_frames else 0
     if kernel == "part1":
-        return num_bodies * (132 + 24 * N) + num_free * 88
+        return num_bodies * (132 + 24 * N + d) + num_free * 88
     if kernel == "part2":
-        return num_bodies * (80 + 36 * N) + num_free * 40
+        return num_bodies * (80 + 36 * N + d) + num_free * 40
     if kernel == "fused":
-        return num_bodies * (144 + 36 * N) + num_free * 88
+        return num_bodies * (144 + 36 * N + d) + num_free * 88
     if kernel == "forces":
         return (num_bodies * N + num_free) * 36 + num_free * 12
     raise ValueError(kernel)
@@ -402,14 +409,14 @@ def served_fp64_check(check, bodies=64, free=64, tol_nm=1e-5):
     return {"tolerance_nm": tol_nm, "jobs": out, "ok": all(j["max_dx_nm"] <= tol_nm for j in out)}
 
 
-def kernel_roofline(kms, natoms_body, nb, nf, config):
+def kernel_roofline(kms, natoms_body, nb, nf, config, shared_frames=False):
     """The HBM roofline of the dominant kernel of step(K) (the fused pass)
     from its duration (HIP events on its own dispatch) and its algorithmic
     bytes; the two-pass step
     (part1 + part2, what one execute() with forces computed outside the
     library runs) and the full step with the tether beside it."""
     alg = {k: algorithmic_bytes(k, natoms_body, nb, nf) for k in ("part1", "part2", "fused", "forces")}
-    impl = {k: implementation_bytes(k, natoms_body, nb, nf) for k in ("part1", "part2", "fused", "forces")}
+    impl = {k: implementation_bytes(k, natoms_body, nb, nf, shared_frames) for k in ("part1", "part2", "fused", "forces")}
     achieved = {k: alg[k] / (kms[k] * 1e-3) / 1e9 for k in alg}
     dominant = "fused"
     two_ms = kms["part1"] + kms["part2"]
@@ -426,7 +433,8 @@ def kernel_roofline(kms, natoms_body, nb, nf, config):
             "traffic_source": "recorded: profiles/traffic_fused_" + config + ".json (rocprofv3 --pmc, not this run)",
             "algorithmic_bytes_per_launch": alg[dominant],
             "implementation_bytes_per_launch": impl[dominant],
-            "precision_overhead": impl[dominant] / alg[dominant],
+            "implementation_over_algorithmic": impl[dominant] / alg[dominant],
+            "shared_frames": shared_frames,
             "avg_launch_ms": kms[dominant],
             "valu_issue_frac_recorded": valu_from_profiles(config, dominant),
         },
@@ -466,7 +474,8 @@ def hetero_leg(steps, warmup):
     rbhip.synchronize()
     el = time.perf_counter() - t
     kms, _ = ctx.time_kernels(100)
-    roof = kernel_roofline(kms, counts["numBodyAtoms"] / max(nb, 1), nb, nf, "hetero200k")
+    roof = kernel_roofline(kms, counts["numBodyAtoms"] / max(nb, 1), nb, nf, "hetero200k",
+                           "sharedShapes=" in ctx.describe())
     ke = ctx.kinetic_energies()
     assert np.all(np.isfinite(ke)), "non-finite kinetic energy"
     log(f"  hetero200k: {el / steps * 1e3:.4f} ms/step, fused {kms['fused']:.4f} ms, "
@@ -543,7 +552,8 @@ def main():
     kms, span = ctx.time_kernels(args.kernel_steps)
     log("kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in kms.items()))
     natoms_body = counts["numBodyAtoms"] / max(nb, 1)
-    roof = kernel_roofline(kms, natoms_body, nb, nf, args.config)
+    # (one body class in these workloads: shared frames cover all bodies or none)
+    roof = kernel_roofline(kms, natoms_body, nb, nf, args.config, "sharedShapes=" in ctx.describe())
     ke = ctx.kinetic_energies()
     assert np.all(np.isfinite(ke)), "non-finite kinetic energy"
 
