@@ -184,10 +184,10 @@ __device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho,
     if (K == 2) nr.c += s;
     if (K == 3) nr.d += s;
     rho = nr;
-    // L turns by -2 phi about axis K: sin 2phi = 2s + 2s cm1, cos 2phi - 1 = 4 cm1 + 2 cm1^2
-    T s1 = s + s, c1 = cm1 + cm1;
+    // L turns by -2 phi about axis K: sin 2phi = 2s (1 + cm1), cos 2phi - 1 = -2 s^2
+    T s1 = s + s;
     T s2 = fma(s1, cm1, s1);
-    T c2m1 = fma(c1, cm1, c1 + c1);
+    T c2m1 = -(s1 * s);
     T& la = K == 1 ? lam.y : (K == 2 ? lam.z : lam.x);
     T& lb = K == 1 ? lam.z : (K == 2 ? lam.x : lam.y);
     T La = (K == 1 ? L0.y : (K == 2 ? L0.z : L0.x)) + la;
