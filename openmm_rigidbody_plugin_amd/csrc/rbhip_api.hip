@@ -27,6 +27,7 @@
 #include "rbhip.h"
 #include "rbhip_bookkeeping.h"
 #include "rbhip_graph_cache.h"
+#include "rbhip_shapes.h"
 #include "rbhip_internal.h"
 #include "rbhip_kernels.h"
 
@@ -838,14 +839,12 @@ struct rbhip_context {
     // can read a few shared rows (cached) instead of 12 B per atom of their
     // own in every step pass.  For each class of at least kShareMinBodies
     // bodies, its first kShapeSample bodies give the candidate shapes
-    // (greedily: a body within tol of no earlier candidate is a new one; more
-    // than kMaxShapes, and the class keeps its own rows); k_match_frames then
-    // gives every body of the class that lies within tol of a candidate that
-    // candidate's values and rows.  tol is 2^-21 of the largest coordinate of
-    // the class's first body: four fp32 ulps at its scale, so a body moves by
-    // at most that (1e-8 nm for water).  A class whose bodies matched for at
-    // least half of them reads the shared rows (Tile::sharedD).  Runs after
-    // every build of the body frames.
+    // (pick_shapes, rbhip_shapes.h); k_match_frames then gives every body of
+    // the class that lies within tol of a candidate that candidate's values
+    // and rows, so a body moves by at most four fp32 ulps of its size (1e-8
+    // nm for water).  A class whose bodies matched for at least half of them
+    // reads the shared rows (Tile::sharedD).  Runs after every build of the
+    // body frames.
     void shareFrames() {
         std::vector<int> before;
         for (Class& c : classes) {
@@ -867,28 +866,8 @@ struct rbhip_context {
                                           reinterpret_cast<const float*>(h.d) + (size_t)comp * h.dStride + row0,
                                           sizeof(float) * len, hipMemcpyDeviceToHost, stream));
                 HIPCHK(hipStreamSynchronize(stream));
-                auto at = [&](int body, int j, int comp) { return (double)sample[(size_t)comp * len + body * c.N + j]; };
-                double scale = 0.0;
-                for (int j = 0; j < c.N; j++)
-                    for (int comp = 0; comp < 3; comp++) scale = std::max(scale, std::fabs(at(0, j, comp)));
-                if (!(scale > 0.0)) continue;
-                tol[ci] = std::ldexp(scale, -21);
                 std::vector<int> cands;
-                bool tooMany = false;
-                for (int b = 0; b < ns && !tooMany; b++) {
-                    bool found = false;
-                    for (int cb : cands) {
-                        bool same = true;
-                        for (int j = 0; same && j < c.N; j++)
-                            for (int comp = 0; same && comp < 3; comp++)
-                                same = std::fabs(at(b, j, comp) - at(cb, j, comp)) <= tol[ci];
-                        if (same) { found = true; break; }
-                    }
-                    if (found) continue;
-                    if ((int)cands.size() == kMaxShapes) tooMany = true;
-                    else cands.push_back(b);
-                }
-                if (tooMany) continue;
+                if (!pick_shapes(sample.data(), ns, c.N, kMaxShapes, cands, tol[ci])) continue;
                 ncand[ci] = (int)cands.size();
                 for (size_t i = 0; i < cands.size(); i++) rows[kMaxShapes * ci + i] = row0 + cands[i] * c.N;
             }

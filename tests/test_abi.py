@@ -100,3 +100,16 @@ def test_graph_cache_logic():
     r = subprocess.run([os.path.join(cpp, "_bin", "TestGraphCache")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("Done")
+
+
+def test_shape_choice_logic():
+    """The choice of shapes that bodies of one size class share
+    (csrc/rbhip_shapes.h) on the CPU: sign variants with rounding noise
+    share, noise above the tolerance or too many shapes do not
+    (tests/cpp/TestShapes.cpp)."""
+    import subprocess
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    subprocess.run(["make", "-C", cpp, "_bin/TestShapes"], check=True, capture_output=True)
+    r = subprocess.run([os.path.join(cpp, "_bin", "TestShapes")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("Done")
