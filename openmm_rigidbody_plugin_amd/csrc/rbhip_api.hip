@@ -2720,7 +2720,7 @@ int rbhip_batch_autotune(rbhip_batch* batch, int reps) {
             // makes each workgroup loop over tiles, a persistent grid)
             int bestBlock = B.block, bestGrid = B.maxBlocks;
             double best = 1e300;
-            for (int blk : {128, 256, 512}) {
+            for (int blk : {64, 128, 256, 512}) {
                 if (blk > cap || !lanesFit(blk)) continue;
                 for (int grid : {0, 2048, 4096, 8192, 16384, 32768}) {
                     B.block = blk;
