@@ -211,8 +211,10 @@ int rbhip_set_forces(rbhip_context* ctx, const double* xyz);
    the arrays at rbhip_state_changed; rbhip_part1 / rbhip_part2 step them,
    with the caller's force evaluation in between, and rbhip_execute steps
    them with the library's force field.  The get/set calls reach the arrays.
-   Not with constraints, virtual sites, refined energies or batches.  The
-   arrays must outlive the context or a later rbhip_destroy. */
+   Not with the library's own constraints or virtual sites (the caller's:
+   rbhip_part1_constrained below; sites are the caller's to place after
+   part 1), refined energies or batches.  The arrays must outlive the
+   context or a later rbhip_destroy. */
 int rbhip_adopt_arrays(rbhip_context* ctx, void* posq, void* posqCorrection, void* velm, long long* force,
                        int paddedNumAtoms, const int* atomOrder, int orderLength);
 /* the arrays' atom order changed (ReorderListener::execute,
@@ -226,6 +228,27 @@ int rbhip_set_atom_order(rbhip_context* ctx, const int* atomOrder, int orderLeng
    arrays (RigidBodySystem::update, RigidBodySystem.cpp:120-142) */
 enum { RBHIP_STATE_POSITIONS = 1, RBHIP_STATE_VELOCITIES = 2 };
 int rbhip_state_changed(rbhip_context* ctx, int what);
+/* Constraints between free atoms applied by the caller to adopted arrays,
+   the CUDA platform's order (CudaRigidBodyKernels.cpp:414-418, 429):
+     rbhip_free_delta         the free atoms' drift displacements
+                              (v + F (1/m) dt/2) dt into posDelta (OpenMM's
+                              integration posDelta: double4 per slot, w = 0;
+                              rigidbodyintegrator.cu:270-279);
+     (the caller's applyConstraints adjusts posDelta)
+     rbhip_part1_constrained  part 1 with the free atoms moved by the
+                              constrained posDelta; the difference from the
+                              unconstrained drift is kept, and the next
+                              rbhip_part2 adds it / dt to their velocities,
+                              as the reference's (R - saved) / dt
+                              (RigidBodySystem.cpp:196-197,
+                              ReferenceRigidBodyKernels.cpp:92-104);
+     (forces, rbhip_part2, then the caller's applyVelocityConstraints)
+   After such steps the kinetic energies re-read the free atoms' velocities
+   from the arrays, so they include the caller's velocity constraints.  Not
+   between rbhip_part1_constrained and rbhip_part2: another part 1,
+   rbhip_free_delta or rbhip_execute. */
+int rbhip_free_delta(rbhip_context* ctx, double* posDelta);
+int rbhip_part1_constrained(rbhip_context* ctx, const double* posDelta);
 
 /* ---------------------------------------------------------------------- */
 /* Context state                                                           */

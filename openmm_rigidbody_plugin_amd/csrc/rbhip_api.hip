@@ -501,6 +501,9 @@ struct rbhip_context {
     int* dSlot[2] = {nullptr, nullptr};
     int liveSlot = 0;
     DeviceArena adoptArena;
+    DeviceArena corrArena;                        // freeCorr, at the first constrained part 1
+    void* dFreeCorr = nullptr;
+    bool externalConstraints = false;             // a caller's constraints moved the free atoms
     bool useGraph = true;
     bool useFusion = true;
     StepGraphs graphs;
@@ -1024,6 +1027,7 @@ struct rbhip_context {
     ~rbhip_context() {
         graphs.reset();
         adoptArena.release();
+        corrArena.release();
         prevArena.release();
         tileArena.release();
         refArena.release();
@@ -1693,7 +1697,7 @@ int rbhip_set_stream(rbhip_context* ctx, rbhip_stream stream) {
         }
         ctx->stream = next;
         ctx->arena.stream = ctx->tileArena.stream = ctx->refArena.stream = ctx->clusterArena.stream =
-            ctx->siteArena.stream = ctx->adoptArena.stream = ctx->prevArena.stream = ctx->stream;
+            ctx->siteArena.stream = ctx->adoptArena.stream = ctx->prevArena.stream = ctx->corrArena.stream = ctx->stream;
     });
 }
 
@@ -2146,6 +2150,7 @@ int rbhip_part1(rbhip_context* ctx) {
     return guard([&] {
         require(ctx, "null context");
         require(ctx->built, "positions must be set before stepping", RBHIP_E_STATE);
+        require(!ctx->h.corrPending, "a constrained part 1 must be completed by rbhip_part2 first", RBHIP_E_STATE);
         ctx->setDevice();
         ctx->fixKick();
         if (ctx->refined) ctx->launchRefineBegin(true);
@@ -2155,12 +2160,58 @@ int rbhip_part1(rbhip_context* ctx) {
     });
 }
 
+int rbhip_free_delta(rbhip_context* ctx, double* posDelta) {
+    return guard([&] {
+        require(ctx && posDelta, "null argument");
+        rbhip_context& C = *ctx;
+        require(C.adopted, "free-atom displacements are for adopted arrays", RBHIP_E_STATE);
+        require(C.built, "positions must be set before stepping", RBHIP_E_STATE);
+        require(!C.h.corrPending, "a constrained part 1 must be completed by rbhip_part2 first", RBHIP_E_STATE);
+        require(((uintptr_t)posDelta & 15) == 0, "posDelta must be 16-byte aligned");
+        C.setDevice();
+        C.fixKick();
+        HIPCHK(launch_free_delta(C.precision, C.dSys, C.numFree, posDelta, C.stream));
+    });
+}
+
+int rbhip_part1_constrained(rbhip_context* ctx, const double* posDelta) {
+    return guard([&] {
+        require(ctx && posDelta, "null argument");
+        rbhip_context& C = *ctx;
+        require(C.adopted, "constrained displacements are for adopted arrays", RBHIP_E_STATE);
+        require(C.built, "positions must be set before stepping", RBHIP_E_STATE);
+        require(!C.h.corrPending, "a constrained part 1 must be completed by rbhip_part2 first", RBHIP_E_STATE);
+        require(((uintptr_t)posDelta & 15) == 0, "posDelta must be 16-byte aligned");
+        C.setDevice();
+        C.fixKick();
+        if (!C.dFreeCorr) {
+            C.corrArena.stream = C.stream;
+            C.dFreeCorr = C.corrArena.alloc(3 * C.realSize() * (size_t)std::max(C.numFree, 1));
+        }
+        C.h.posDelta = const_cast<double*>(posDelta);
+        C.h.freeCorr = C.dFreeCorr;
+        C.uploadSys();
+        C.launchPart1();
+        // part 1 read posDelta; the correction waits for part 2
+        C.h.posDelta = nullptr;
+        C.h.corrPending = 1;
+        C.uploadSys();
+        C.positionsDerived = true;
+        C.kickFolded = false;
+        C.externalConstraints = true;
+    });
+}
+
 int rbhip_part2(rbhip_context* ctx) {
     return guard([&] {
         require(ctx, "null context");
         require(ctx->built, "positions must be set before stepping", RBHIP_E_STATE);
         ctx->setDevice();
         ctx->launchPart2();
+        if (ctx->h.corrPending) {
+            ctx->h.corrPending = 0;
+            ctx->uploadSys();
+        }
         ctx->keSlots = ctx->partKeSlots();
         ctx->stageVFresh = false;
         ctx->kickFolded = true;
@@ -2178,6 +2229,7 @@ int rbhip_part2(rbhip_context* ctx) {
 int rbhip_execute(rbhip_context* ctx, int steps) {
     return guard([&] {
         require(ctx, "null context");
+        require(!ctx->h.corrPending, "a constrained part 1 must be completed by rbhip_part2 first", RBHIP_E_STATE);
         ctx->setDevice();
         ctx->executeSteps(steps);
     });
@@ -2203,6 +2255,9 @@ int rbhip_kinetic_energies(rbhip_context* ctx, double* out2) {
         require(ctx && out2, "null argument");
         ctx->setDevice();
         double r[4] = {0, 0, 0, 0};
+        // after a caller's velocity constraints, the free atoms' partials again
+        if (ctx->adopted && ctx->externalConstraints && ctx->keSlots == ctx->partKeSlots())
+            HIPCHK(launch_free_ke(ctx->precision, ctx->dSys, ctx->dPartTiles, ctx->nPartTiles, ctx->block, ctx->stream));
         if (ctx->keSlots > 0) ctx->reduce(0, r);
         out2[0] = 0.5 * r[0];
         out2[1] = 0.5 * r[1];

@@ -147,7 +147,17 @@ struct DevSys {
     double* ommVelm;
     long long* ommForce;
     int ommStride;
-    int pad2;
+    // Constraints applied by the caller to adopted arrays (OpenMM's
+    // applyConstraints between the free atoms' drift and the rest of part 1):
+    // posDelta (double4 per slot) holds the constrained displacements during
+    // a constrained part 1, which keeps each free atom's correction (the
+    // constrained minus the unconstrained drift, Real[3][numFree]) in
+    // freeCorr; corrPending = 1 makes the next part 2 add correction / dt to
+    // the free atoms' velocities, as the reference's (R - saved) / dt does
+    // (RigidBodySystem.cpp:196-197).
+    int corrPending;
+    double* posDelta;
+    void* freeCorr;
 };
 
 // OpenMM's fixed-point force unit: 2^32 per kJ/mol/nm

@@ -20,6 +20,13 @@ hipError_t launch_part1(int precision, int rotationMode, const DevSys* systems, 
                         int block, int maxBlocks, hipStream_t stream, int adopted);
 hipError_t launch_part2(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
                         int maxBlocks, hipStream_t stream, int adopted);
+// adopted arrays: the free atoms' drift displacements into the caller's
+// posDelta (double4 per slot) for its constraint solver
+hipError_t launch_free_delta(int precision, const DevSys* sys, int numFree, double* posDelta, hipStream_t stream);
+// adopted arrays: the free-atom tiles' KE partials from the velocities as
+// they are now (after the caller's velocity constraints)
+hipError_t launch_free_ke(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
+                          hipStream_t stream);
 // part2 of step n and part1 of step n+1 in one pass (no atom velocities, no KE)
 hipError_t launch_fused(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
                         int block, int maxBlocks, hipStream_t stream, int adopted);

@@ -290,10 +290,45 @@ __device__ __forceinline__ void part1_free(const DevSys& S, const Tile& tile) {
         Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
         v = v + f * invm * halfDt;
         V3<double> r = IO::template pos<Real>(S, a);
+        if constexpr (IO::adopted) {
+            if (S.posDelta) {
+                // the caller's constrained displacement; the correction to
+                // the unconstrained drift waits for part 2
+                const double* pd = S.posDelta + 4 * (size_t)S.ommSlot[a];
+                V3<double> ru = r + cvt<double>(v) * S.dt;
+                r = r + V3<double>{pd[0], pd[1], pd[2]};
+                st3<Real>(S.freeCorr, S.numFree, a, cvt<Real>(r - ru));
+                IO::template set_vel<Real>(S, a, v);
+                IO::template set_pos<Real>(S, a, r);
+                continue;
+            }
+        }
         r = r + cvt<double>(v) * S.dt;
         IO::template set_vel<Real>(S, a, v);
         IO::template set_pos<Real>(S, a, r);
     }
+}
+
+// The free atoms' drift displacements (v + F (1/m) dt/2) dt, as part 1
+// computes them, into a caller's posDelta (double4 per slot, w = 0) for its
+// constraint solver (the CUDA platform's freeAtomsDelta,
+// rigidbodyintegrator.cu:270-279); adopted arrays only.
+template <class Real>
+__global__ void k_free_delta(const DevSys* __restrict__ sys, double* __restrict__ posDelta) {
+    const DevSys& S = *sys;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= S.numFree) return;
+    const Real halfDt = (Real)(0.5 * S.dt);
+    V3<Real> v = AdoptedAtoms::vel<Real>(S, a);
+    V3<Real> f = AdoptedAtoms::force<Real>(S, a);
+    const Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
+    v = v + f * invm * halfDt;
+    const V3<double> d = cvt<double>(v) * S.dt;
+    double* o = posDelta + 4 * (size_t)S.ommSlot[a];
+    o[0] = d.x;
+    o[1] = d.y;
+    o[2] = d.z;
+    o[3] = 0.0;
 }
 
 template <class Real, bool EXACT, class IO>
@@ -341,6 +376,8 @@ __device__ __forceinline__ void part2_free(const DevSys& S, const Tile& tile, do
         V3<Real> f = IO::template force<Real>(S, a);
         Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
         v = v + f * invm * halfDt;
+        // (R - saved) / dt of a constrained part 1
+        if (IO::adopted && S.corrPending) v = v + ld3<Real>(S.freeCorr, S.numFree, a) * (Real)(1.0 / S.dt);
         IO::template set_vel<Real>(S, a, v);
         V3<double> vd = cvt<double>(v);
         ke[0] += dot(vd, vd) / (double)invm;
@@ -1421,6 +1458,42 @@ hipError_t launch_fused(int precision, int rotationMode, const DevSys* systems, 
     if (grid == 0) return hipSuccess;
     if (adopted) fused_for<AdoptedAtoms>(precision, rotationMode, grid, block, stream, systems, tiles, numTiles);
     else fused_for<SoaAtoms>(precision, rotationMode, grid, block, stream, systems, tiles, numTiles);
+    return hipGetLastError();
+}
+
+// The free-atom tiles' kinetic-energy partials again, from the adopted
+// velocities as they are now (after the caller's velocity constraints), into
+// the slots part 2 filled; body tiles keep theirs.
+template <class Real>
+__global__ __launch_bounds__(512) void k_free_ke(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
+                                                  int numTiles) {
+    for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        if (tile.kind != TILE_FREE) continue;
+        const DevSys& S = systems[tile.sys];
+        double ke[2] = {0.0, 0.0};
+        for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
+            const int a = tile.first + i;
+            V3<double> vd = cvt<double>(AdoptedAtoms::vel<Real>(S, a));
+            ke[0] += dot(vd, vd) / (double)reinterpret_cast<const Real*>(S.freeInvM)[a];
+        }
+        block_reduce_store<2>(ke, S.kePart, tile.slot);
+    }
+}
+
+hipError_t launch_free_ke(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
+                          hipStream_t stream) {
+    if (numTiles <= 0) return hipSuccess;
+    if (precision == 0) k_free_ke<float><<<numTiles, block, 0, stream>>>(systems, tiles, numTiles);
+    else k_free_ke<double><<<numTiles, block, 0, stream>>>(systems, tiles, numTiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_free_delta(int precision, const DevSys* sys, int numFree, double* posDelta, hipStream_t stream) {
+    if (numFree <= 0) return hipSuccess;
+    const int grid = (numFree + 255) / 256;
+    if (precision == 0) k_free_delta<float><<<grid, 256, 0, stream>>>(sys, posDelta);
+    else k_free_delta<double><<<grid, 256, 0, stream>>>(sys, posDelta);
     return hipGetLastError();
 }
 

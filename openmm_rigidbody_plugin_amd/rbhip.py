@@ -66,6 +66,8 @@ def lib() -> ctypes.CDLL:
             "rbhip_adopt_arrays": (_I, [_P, _P, _P, _P, _P, _I, _IP, _I]),
             "rbhip_set_atom_order": (_I, [_P, _IP, _I]),
             "rbhip_state_changed": (_I, [_P, _I]),
+            "rbhip_free_delta": (_I, [_P, _P]),
+            "rbhip_part1_constrained": (_I, [_P, _P]),
             "rbhip_set_positions": (_I, [_P, _DP]),
             "rbhip_set_velocities": (_I, [_P, _DP]),
             "rbhip_get_positions": (_I, [_P, _DP]),
@@ -342,6 +344,17 @@ class Context:
 
     def part2(self):
         _check(lib().rbhip_part2(self._h))
+
+    def free_delta(self, pos_delta):
+        """adopted arrays: the free atoms' drift displacements into the
+        device array pos_delta (double4 per slot) for the caller's
+        constraint solver"""
+        _check(lib().rbhip_free_delta(self._h, pos_delta))
+
+    def part1_constrained(self, pos_delta):
+        """adopted arrays: part 1 with the free atoms moved by the caller's
+        constrained displacements (device pointer, double4 per slot)"""
+        _check(lib().rbhip_part1_constrained(self._h, pos_delta))
 
     def step(self, steps=1):
         _check(lib().rbhip_execute(self._h, int(steps)))
