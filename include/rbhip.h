@@ -246,7 +246,8 @@ int rbhip_state_changed(rbhip_context* ctx, int what);
    After such steps the kinetic energies re-read the free atoms' velocities
    from the arrays, so they include the caller's velocity constraints.  Not
    between rbhip_part1_constrained and rbhip_part2: another part 1,
-   rbhip_free_delta or rbhip_execute. */
+   rbhip_free_delta, rbhip_execute, rbhip_time_kernels or rbhip_autotune;
+   rbhip_state_changed abandons the pending correction. */
 int rbhip_free_delta(rbhip_context* ctx, double* posDelta);
 int rbhip_part1_constrained(rbhip_context* ctx, const double* posDelta);
 

@@ -503,7 +503,9 @@ struct rbhip_context {
     DeviceArena adoptArena;
     DeviceArena corrArena;                        // freeCorr, at the first constrained part 1
     void* dFreeCorr = nullptr;
-    bool externalConstraints = false;             // a caller's constraints moved the free atoms
+    // the free atoms' KE partials predate the caller's velocity constraints
+    // (set by a part 2 that completes a constrained part 1)
+    bool freeKeStale = false;
     bool useGraph = true;
     bool useFusion = true;
     StepGraphs graphs;
@@ -531,7 +533,7 @@ struct rbhip_context {
         std::vector<void*> live, copy;
         std::vector<size_t> bytes;
         int keSlots = 0, refKeSlots = 0;
-        bool positionsDerived = false, refinedValid = false;
+        bool positionsDerived = false, refinedValid = false, freeKeStale = false;
         ~Snapshot() { arena.release(); }
     };
     void takeSnapshot(Snapshot& s) {
@@ -563,6 +565,7 @@ struct rbhip_context {
         s.refKeSlots = refKeSlots;
         s.positionsDerived = positionsDerived;
         s.refinedValid = refinedValid;
+        s.freeKeStale = freeKeStale;
     }
     // enqueues the copies back on `on` (the caller orders and synchronises)
     void restoreSnapshot(const Snapshot& s, hipStream_t on) {
@@ -572,6 +575,7 @@ struct rbhip_context {
         refKeSlots = s.refKeSlots;
         positionsDerived = s.positionsDerived;
         refinedValid = s.refinedValid;
+        freeKeStale = s.freeKeStale;
     }
 
     std::vector<Tile> partTiles(int blk, const std::map<int, int>& lanesByN) const {
@@ -826,6 +830,7 @@ struct rbhip_context {
             HIPCHK(launch_clusters(precision, dSys, dClusterSetupTiles, clusterClassTiles, CLUSTER_KE, kClusterBlock,
                                    stream));
         keSlots = nSetupTiles + nClusterTiles;
+        freeKeStale = false;
         kickDt = dt;
         built = true;
         kickFolded = true;
@@ -989,6 +994,7 @@ struct rbhip_context {
             refinedValid = true;
         }
         keSlots = partKeSlots();
+        freeKeStale = false;
         stageVFresh = false;
         positionsDerived = true;
         kickFolded = true;
@@ -1168,6 +1174,7 @@ struct rbhip_batch {
                 c->refinedValid = true;
             }
             c->keSlots = memberPartTiles[m];
+            c->freeKeStale = false;
             c->stageVFresh = false;
             if (steps > 0) {
                 c->positionsDerived = true;
@@ -2068,6 +2075,7 @@ int rbhip_state_changed(rbhip_context* ctx, int what) {
         C.h.useStageR = geometry;
         C.runSetup(geometry);
         C.h.useStageR = 0;
+        C.h.corrPending = 0;      // (a pending constrained step is abandoned)
         C.uploadSys();
         HIPCHK(hipStreamSynchronize(C.stream));
     });
@@ -2146,11 +2154,17 @@ int rbhip_get_time(rbhip_context* ctx, double* time, long long* stepCount) {
     });
 }
 
+// nothing steps between rbhip_part1_constrained and the rbhip_part2 that adds
+// its correction to the velocities
+static void require_step_complete(const rbhip_context& C) {
+    require(!C.h.corrPending, "a constrained part 1 must be completed by rbhip_part2 first", RBHIP_E_STATE);
+}
+
 int rbhip_part1(rbhip_context* ctx) {
     return guard([&] {
         require(ctx, "null context");
         require(ctx->built, "positions must be set before stepping", RBHIP_E_STATE);
-        require(!ctx->h.corrPending, "a constrained part 1 must be completed by rbhip_part2 first", RBHIP_E_STATE);
+        require_step_complete(*ctx);
         ctx->setDevice();
         ctx->fixKick();
         if (ctx->refined) ctx->launchRefineBegin(true);
@@ -2166,7 +2180,7 @@ int rbhip_free_delta(rbhip_context* ctx, double* posDelta) {
         rbhip_context& C = *ctx;
         require(C.adopted, "free-atom displacements are for adopted arrays", RBHIP_E_STATE);
         require(C.built, "positions must be set before stepping", RBHIP_E_STATE);
-        require(!C.h.corrPending, "a constrained part 1 must be completed by rbhip_part2 first", RBHIP_E_STATE);
+        require_step_complete(C);
         require(((uintptr_t)posDelta & 15) == 0, "posDelta must be 16-byte aligned");
         C.setDevice();
         C.fixKick();
@@ -2180,7 +2194,7 @@ int rbhip_part1_constrained(rbhip_context* ctx, const double* posDelta) {
         rbhip_context& C = *ctx;
         require(C.adopted, "constrained displacements are for adopted arrays", RBHIP_E_STATE);
         require(C.built, "positions must be set before stepping", RBHIP_E_STATE);
-        require(!C.h.corrPending, "a constrained part 1 must be completed by rbhip_part2 first", RBHIP_E_STATE);
+        require_step_complete(C);
         require(((uintptr_t)posDelta & 15) == 0, "posDelta must be 16-byte aligned");
         C.setDevice();
         C.fixKick();
@@ -2198,7 +2212,6 @@ int rbhip_part1_constrained(rbhip_context* ctx, const double* posDelta) {
         C.uploadSys();
         C.positionsDerived = true;
         C.kickFolded = false;
-        C.externalConstraints = true;
     });
 }
 
@@ -2208,11 +2221,12 @@ int rbhip_part2(rbhip_context* ctx) {
         require(ctx->built, "positions must be set before stepping", RBHIP_E_STATE);
         ctx->setDevice();
         ctx->launchPart2();
+        ctx->keSlots = ctx->partKeSlots();
+        ctx->freeKeStale = ctx->h.corrPending != 0;
         if (ctx->h.corrPending) {
             ctx->h.corrPending = 0;
             ctx->uploadSys();
         }
-        ctx->keSlots = ctx->partKeSlots();
         ctx->stageVFresh = false;
         ctx->kickFolded = true;
         // the step is complete (ReferenceRigidBodyKernels.cpp:106-107)
@@ -2229,7 +2243,7 @@ int rbhip_part2(rbhip_context* ctx) {
 int rbhip_execute(rbhip_context* ctx, int steps) {
     return guard([&] {
         require(ctx, "null context");
-        require(!ctx->h.corrPending, "a constrained part 1 must be completed by rbhip_part2 first", RBHIP_E_STATE);
+        require_step_complete(*ctx);
         ctx->setDevice();
         ctx->executeSteps(steps);
     });
@@ -2256,8 +2270,9 @@ int rbhip_kinetic_energies(rbhip_context* ctx, double* out2) {
         ctx->setDevice();
         double r[4] = {0, 0, 0, 0};
         // after a caller's velocity constraints, the free atoms' partials again
-        if (ctx->adopted && ctx->externalConstraints && ctx->keSlots == ctx->partKeSlots())
+        if (ctx->freeKeStale) {
             HIPCHK(launch_free_ke(ctx->precision, ctx->dSys, ctx->dPartTiles, ctx->nPartTiles, ctx->block, ctx->stream));
+        }
         if (ctx->keSlots > 0) ctx->reduce(0, r);
         out2[0] = 0.5 * r[0];
         out2[1] = 0.5 * r[1];
@@ -2432,6 +2447,7 @@ int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* tot
         require(ctx && avgMs && steps > 0, "bad arguments");
         rbhip_context& C = *ctx;
         require(C.built, "positions must be set before stepping", RBHIP_E_STATE);
+        require_step_complete(C);
         C.setDevice();
         C.fixKick();
         time_steps(C.stream, steps, [&] { C.launchPart1(); }, [&] { C.launchForces(); }, [&] { C.launchPart2(); },
@@ -2446,6 +2462,7 @@ int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* tot
         // the state advanced 2 steps + 1 as rbhip_execute leaves it (the
         // refined-energy accumulators do not describe it)
         C.keSlots = C.partKeSlots();
+        C.freeKeStale = false;
         C.stageVFresh = false;
         C.positionsDerived = true;
         C.kickFolded = true;
@@ -2461,6 +2478,7 @@ int rbhip_autotune(rbhip_context* ctx, int reps) {
         require(ctx && reps > 0, "bad arguments");
         rbhip_context& C = *ctx;
         require(C.built, "positions must be set before tuning", RBHIP_E_STATE);
+        require_step_complete(C);
         C.setDevice();
         C.fixKick();
         HIPCHK(hipStreamSynchronize(C.stream));

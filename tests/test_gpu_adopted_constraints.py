@@ -135,8 +135,13 @@ def test_constrained_part1_refusals(hip):
     g.part1_constrained(pd.ptr.value)
     # the constrained part 1 must be completed by part 2
     for call in (lambda: g.part1(), lambda: g.step(1), lambda: g.free_delta(pd.ptr.value),
-                 lambda: g.part1_constrained(pd.ptr.value)):
+                 lambda: g.part1_constrained(pd.ptr.value), lambda: g.time_kernels(1), lambda: g.autotune()):
         with pytest.raises(hip.RbhipError):
             call()
     g.part2()
     g.free_delta(pd.ptr.value)
+    # a state change abandons a pending constrained step
+    g.part1_constrained(pd.ptr.value)
+    g.state_changed(g.STATE_POSITIONS)
+    g.part1()
+    g.part2()
