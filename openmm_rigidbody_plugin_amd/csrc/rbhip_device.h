@@ -177,13 +177,16 @@ __device__ __forceinline__ void body_axis_turn(V3<T> L0, V3<T>& lam, Q4<T>& rho,
     T s, cm1;
     sin_cosm1_tier<TIER>(fma(lamK, coef, L0coef), s, cm1);
     // rho <- rho + rho cm1 + P_K(rho) s + (cm1, s e_K)   [r (x) (1 + cm1, s e_K)]
+    // The e_K component of P_K(rho) is rho.a, so (cm1, s e_K) joins the terms
+    // in rho.a as (1 + rho.a) cm1 and (1 + rho.a) s.  Rounding 1 + rho.a costs
+    // at most 2^-24 |s|, below the rounding of rho itself over the step.
+    const T a1 = rho.a + (T)1;
     Q4<T> Pr = perm<K>(rho);
-    Q4<T> nr = {fma(rho.a, cm1, fma(Pr.a, s, rho.a)) + cm1, fma(rho.b, cm1, fma(Pr.b, s, rho.b)),
-                fma(rho.c, cm1, fma(Pr.c, s, rho.c)), fma(rho.d, cm1, fma(Pr.d, s, rho.d))};
-    if (K == 1) nr.b += s;
-    if (K == 2) nr.c += s;
-    if (K == 3) nr.d += s;
-    rho = nr;
+    if (K == 1) Pr.b = a1;
+    if (K == 2) Pr.c = a1;
+    if (K == 3) Pr.d = a1;
+    rho = {fma(a1, cm1, fma(Pr.a, s, rho.a)), fma(rho.b, cm1, fma(Pr.b, s, rho.b)),
+           fma(rho.c, cm1, fma(Pr.c, s, rho.c)), fma(rho.d, cm1, fma(Pr.d, s, rho.d))};
     // L turns by -2 phi about axis K: sin 2phi = 2s (1 + cm1), cos 2phi - 1 = -2 s^2
     T s1 = s + s;
     T s2 = fma(s1, cm1, s1);
