@@ -214,41 +214,49 @@ template <class F> hipGraphExec_t ensure_graph(GraphCache& g, const GraphKey& ke
 
 // The repeated launch sequences of a call, as HIP graphs: the two-pass step
 // (part1, forces, part2), one fused step boundary (fused pass, forces), and
-// kFusedChunk of those in one graph.  A graph launch costs the host tens of
-// microseconds, as much as a small system's whole step, so long calls replay
-// the chunk.
+// kFusedChunk of those in one graph; the *Last graphs are the same steps with
+// the potential energy, which the last step of a call takes.  A graph launch
+// costs the host tens of microseconds, as much as a small system's whole
+// step, so long calls replay the chunk.
 struct StepGraphs {
     static constexpr int kFusedChunk = 16;
-    enum Kind { TWO_PASS = 1, FUSED = 2, CHUNK = 3 };
-    GraphCache twoPass, fused, chunk;
+    enum Kind { TWO_PASS = 1, FUSED = 2, CHUNK = 3, TWO_PASS_LAST = 4, FUSED_LAST = 5 };
+    GraphCache twoPass, fused, chunk, twoPassLast, fusedLast;
     // drops every graph (each after its stream has drained)
     void reset() {
         HipGraphOps ops;
         twoPass.drop(ops);
         fused.drop(ops);
         chunk.drop(ops);
+        twoPassLast.drop(ops);
+        fusedLast.drop(ops);
     }
-    bool any() const { return twoPass.valid() || fused.valid() || chunk.valid(); }
+    bool any() const {
+        return twoPass.valid() || fused.valid() || chunk.valid() || twoPassLast.valid() || fusedLast.valid();
+    }
 };
 
 // RigidBodyIntegrator::step(steps) on a context or a batch (`o` provides the
 // launches, its stream, graphs and switches).  With fusion the steps'
 // interior boundaries run as one fused pass each: part1, forces,
 // (steps - 1) x [fused, forces], part2.  With `refine` the last step is
-// bracketed by the refined-energy kernels.
+// bracketed by the refined-energy kernels.  Only the last force evaluation
+// takes the potential energy (a step asks for forces alone); the graphs hold
+// the other steps, and the last one is launched on its own.
 template <class O> void run_steps(O& o, int steps, bool refine) {
     if (steps <= 0) return;
     const int tail = refine ? 1 : 0;
     if (o.useFusion && steps >= 2) {
-        o.launchPart1();
-        o.launchForces();
         int interior = steps - 1 - tail;
+        o.launchPart1();
+        o.launchForces(interior == 0 && !tail);
+        if (interior > 0) interior--;                 // the last one below, with the energy
         if (o.useGraph) {
             if (interior >= StepGraphs::kFusedChunk) {
                 hipGraphExec_t g = ensure_graph(o.graphs.chunk, o.graphKey(StepGraphs::CHUNK), [&] {
                     for (int i = 0; i < StepGraphs::kFusedChunk; i++) {
                         o.launchFused();
-                        o.launchForces();
+                        o.launchForces(false);
                     }
                 });
                 for (; interior >= StepGraphs::kFusedChunk; interior -= StepGraphs::kFusedChunk)
@@ -257,41 +265,68 @@ template <class O> void run_steps(O& o, int steps, bool refine) {
             if (interior > 0) {
                 hipGraphExec_t g = ensure_graph(o.graphs.fused, o.graphKey(StepGraphs::FUSED), [&] {
                     o.launchFused();
-                    o.launchForces();
+                    o.launchForces(false);
                 });
                 for (; interior > 0; interior--) HIPCHK(hipGraphLaunch(g, o.stream));
             }
         } else {
             for (; interior > 0; interior--) {
                 o.launchFused();
-                o.launchForces();
+                o.launchForces(false);
+            }
+        }
+        if (steps - 1 - tail > 0) {
+            if (o.useGraph && !tail) {
+                HIPCHK(hipGraphLaunch(ensure_graph(o.graphs.fusedLast, o.graphKey(StepGraphs::FUSED_LAST), [&] {
+                                          o.launchFused();
+                                          o.launchForces(true);
+                                      }),
+                                      o.stream));
+            } else {
+                o.launchFused();
+                o.launchForces(!tail);
             }
         }
         if (tail) {
             o.launchRefineBegin(false);
             o.launchFused();
-            o.launchForces();
+            o.launchForces(true);
         }
         o.launchPart2();
     } else {
-        if (o.useGraph) {
+        const int plain = steps - tail;
+        if (o.useGraph && plain > 1) {
             hipGraphExec_t g = ensure_graph(o.graphs.twoPass, o.graphKey(StepGraphs::TWO_PASS), [&] {
                 o.launchPart1();
-                o.launchForces();
+                o.launchForces(false);
                 o.launchPart2();
             });
-            for (int s = 0; s < steps - tail; s++) HIPCHK(hipGraphLaunch(g, o.stream));
+            for (int s = 0; s < plain - 1; s++) HIPCHK(hipGraphLaunch(g, o.stream));
         } else {
-            for (int s = 0; s < steps - tail; s++) {
+            for (int s = 0; s < plain - 1; s++) {
                 o.launchPart1();
-                o.launchForces();
+                o.launchForces(false);
+                o.launchPart2();
+            }
+        }
+        if (plain > 0) {
+            if (o.useGraph && !tail) {
+                HIPCHK(hipGraphLaunch(ensure_graph(o.graphs.twoPassLast, o.graphKey(StepGraphs::TWO_PASS_LAST), [&] {
+                                          o.launchPart1();
+                                          o.launchForces(true);
+                                          o.launchPart2();
+                                      }),
+                                      o.stream));
+            } else {
+                o.launchPart1();
+                o.launchForces(!tail);
                 o.launchPart2();
             }
         }
         if (tail) {
             o.launchRefineBegin(true);
             o.launchPart1();
-            o.launchForces();
+            o.launchForces(true);
             o.launchPart2();
         }
     }
@@ -776,8 +811,9 @@ struct rbhip_context {
         k.kind = kind;
         return k;
     }
-    void launchForces() {
-        HIPCHK(launch_forces(precision, dSys, dForceTiles, nForceTiles, block, maxBlocks, stream, layout()));
+    // (energy: the potential-energy partials too, for the last evaluation of a call)
+    void launchForces(bool energy = true) {
+        HIPCHK(launch_forces(precision, dSys, dForceTiles, nForceTiles, block, maxBlocks, stream, layout(), energy));
         if (numSites > 0) HIPCHK(launch_site_forces(precision, dSys, numSites, numSiteParents, stream));
     }
     void launchClusters(int phase) {
@@ -1142,8 +1178,8 @@ struct rbhip_batch {
     void launchPart1() {
         HIPCHK(launch_part1(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream, 0));
     }
-    void launchForces() {
-        HIPCHK(launch_forces(precision, dSystems, dForceTiles, nForce, block, maxBlocks, stream, 0));
+    void launchForces(bool energy = true) {
+        HIPCHK(launch_forces(precision, dSystems, dForceTiles, nForce, block, maxBlocks, stream, 0, energy));
     }
     void launchPart2() {
         HIPCHK(launch_part2(precision, dSystems, dPartTiles, nPart, block, maxBlocks, stream, 0));
@@ -2450,13 +2486,14 @@ int rbhip_time_kernels(rbhip_context* ctx, int steps, double* avgMs, double* tot
         require_step_complete(C);
         C.setDevice();
         C.fixKick();
-        time_steps(C.stream, steps, [&] { C.launchPart1(); }, [&] { C.launchForces(); }, [&] { C.launchPart2(); },
+        time_steps(C.stream, steps, [&] { C.launchPart1(); }, [&] { C.launchForces(false); }, [&] { C.launchPart2(); },
                    avgMs, totalMs);
         // the fused path of step(steps + 1): part1, forces, steps x [fused, forces], part2
         C.launchPart1();
         C.launchForces();
         double fusedMs[2];
-        time_phases(C.stream, steps, {[&] { C.launchFused(); }, [&] { C.launchForces(); }}, fusedMs, nullptr);
+        time_phases(C.stream, steps, {[&] { C.launchFused(); }, [&] { C.launchForces(false); }}, fusedMs, nullptr);
+        C.launchForces(true);   // (the same forces again, with the energy)
         C.launchPart2();
         avgMs[RBHIP_KERNEL_FUSED] = fusedMs[0];
         // the state advanced 2 steps + 1 as rbhip_execute leaves it (the
@@ -2496,12 +2533,12 @@ int rbhip_autotune(rbhip_context* ctx, int reps) {
             if (C.useFusion) {
                 C.launchPart1(); C.launchForces();
                 C.launchFused(); C.launchForces();
-                time_phases(C.stream, reps, {[&] { C.launchFused(); }, [&] { C.launchForces(); }}, avg, &total);
+                time_phases(C.stream, reps, {[&] { C.launchFused(); }, [&] { C.launchForces(false); }}, avg, &total);
                 C.launchPart2();
                 t = avg[0] + avg[1];
             } else {
                 C.launchPart1(); C.launchForces(); C.launchPart2();
-                time_steps(C.stream, reps, [&] { C.launchPart1(); }, [&] { C.launchForces(); },
+                time_steps(C.stream, reps, [&] { C.launchPart1(); }, [&] { C.launchForces(false); },
                            [&] { C.launchPart2(); }, avg, &total);
                 t = avg[0] + avg[1] + avg[2];
             }
@@ -2728,12 +2765,13 @@ int rbhip_batch_time_kernels(rbhip_batch* batch, int steps, double* avgMs, doubl
             require(c->kickDt == c->dt, "member step size changed since it joined the batch", RBHIP_E_STATE);
         B.joinMembers();
         B.refreshSystems();
-        time_steps(B.stream, steps, [&] { B.launchPart1(); }, [&] { B.launchForces(); }, [&] { B.launchPart2(); },
+        time_steps(B.stream, steps, [&] { B.launchPart1(); }, [&] { B.launchForces(false); }, [&] { B.launchPart2(); },
                    avgMs, totalMs);
         B.launchPart1();
         B.launchForces();
         double fusedMs[2];
-        time_phases(B.stream, steps, {[&] { B.launchFused(); }, [&] { B.launchForces(); }}, fusedMs, nullptr);
+        time_phases(B.stream, steps, {[&] { B.launchFused(); }, [&] { B.launchForces(false); }}, fusedMs, nullptr);
+        B.launchForces(true);   // (the same forces again, with the energy)
         B.launchPart2();
         avgMs[RBHIP_KERNEL_FUSED] = fusedMs[0];
         B.releaseMembers();
@@ -2769,12 +2807,12 @@ int rbhip_batch_autotune(rbhip_batch* batch, int reps) {
             if (B.useFusion) {
                 B.launchPart1(); B.launchForces();
                 B.launchFused(); B.launchForces();
-                time_phases(B.stream, reps, {[&] { B.launchFused(); }, [&] { B.launchForces(); }}, avg, &total);
+                time_phases(B.stream, reps, {[&] { B.launchFused(); }, [&] { B.launchForces(false); }}, avg, &total);
                 B.launchPart2();
                 t = avg[0] + avg[1];
             } else {
                 B.launchPart1(); B.launchForces(); B.launchPart2();
-                time_steps(B.stream, reps, [&] { B.launchPart1(); }, [&] { B.launchForces(); },
+                time_steps(B.stream, reps, [&] { B.launchPart1(); }, [&] { B.launchForces(false); },
                            [&] { B.launchPart2(); }, avg, &total);
                 t = avg[0] + avg[1] + avg[2];
             }

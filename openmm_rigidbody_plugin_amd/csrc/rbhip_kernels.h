@@ -40,8 +40,9 @@ hipError_t launch_refine_end(int precision, int rotationMode, const DevSys* syst
                              int block, int maxBlocks, hipStream_t stream);
 hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
                             int maxBlocks, hipStream_t stream);
+// (energy: also the tiles' potential-energy partials)
 hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
-                         int maxBlocks, hipStream_t stream, int adopted);
+                         int maxBlocks, hipStream_t stream, int adopted, int energy);
 // virtual sites of one system: positions from the parents; after the forces,
 // the sites' forces moved to their parents and cleared
 hipError_t launch_site_positions(int precision, const DevSys* sys, int numSites, hipStream_t stream);

@@ -1312,10 +1312,13 @@ __global__ __launch_bounds__(256) void k_site_clear(const DevSys* __restrict__ s
 // reads fp32 coordinates), free atoms at hi + lo; the tether's anchor r0 is
 // held at the state precision (fp32 in mixed precision, as a mixed-precision
 // force field holds its parameters), so an atom moves 36 bytes: its
-// position, r0 and its force.
+// position, r0 and its force.  With ENERGY the tiles' potential-energy
+// partials too; a step asks for forces only (calcForcesAndEnergy(true,
+// false), ReferenceRigidBodyKernels.cpp:101), so a sequence of steps takes
+// the energy at its last evaluation alone.
 // ---------------------------------------------------------------------------
 
-template <class Real, class IO>
+template <class Real, class IO, bool ENERGY>
 __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ systems, const Tile* __restrict__ tiles,
                                                  int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
@@ -1331,8 +1334,10 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
                 field_pos_parts<Real, IO>(S, a, ph, pl);
                 V3<Real> dx = (ph - ld3<Real>(S.r0, S.atomStride, a)) + pl;
                 f = dx * (-k);
-                V3<double> dd = cvt<double>(dx);
-                e[0] += 0.5 * (double)k * dot(dd, dd);
+                if (ENERGY) {
+                    V3<double> dd = cvt<double>(dx);
+                    e[0] += 0.5 * (double)k * dot(dd, dd);
+                }
             }
             if (S.bondStart) {
                 int e0 = S.bondStart[a], e1 = S.bondStart[a + 1];
@@ -1344,14 +1349,14 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
                         double r = sqrt(dot(dr, dr));
                         double ext = r - S.bondLen[b];
                         fd = fd + dr * (S.bondK[b] * ext / r);
-                        e[0] += 0.25 * S.bondK[b] * ext * ext;
+                        if (ENERGY) e[0] += 0.25 * S.bondK[b] * ext * ext;
                     }
                     f = cvt<Real>(fd);
                 }
             }
             store_force<Real, IO>(S, a, f);
         }
-        block_reduce_store<1>(e, S.pePart, tile.slot);
+        if (ENERGY) block_reduce_store<1>(e, S.pePart, tile.slot);
     }
 }
 
@@ -1592,16 +1597,23 @@ hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* ti
     return hipGetLastError();
 }
 
+template <class Real, class IO>
+static void forces_for(bool energy, int grid, int block, hipStream_t stream, const DevSys* systems, const Tile* tiles,
+                       int numTiles) {
+    if (energy) step_kernel(k_forces<Real, IO, true>, grid, block, stream, systems, tiles, numTiles);
+    else step_kernel(k_forces<Real, IO, false>, grid, block, stream, systems, tiles, numTiles);
+}
+
 hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
-                         int maxBlocks, hipStream_t stream, int adopted) {
+                         int maxBlocks, hipStream_t stream, int adopted, int energy) {
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
     if (precision == 0) {
-        if (adopted) step_kernel(k_forces<float, AdoptedAtoms>, grid, block, stream, systems, tiles, numTiles);
-        else step_kernel(k_forces<float, SoaAtoms>, grid, block, stream, systems, tiles, numTiles);
+        if (adopted) forces_for<float, AdoptedAtoms>(energy, grid, block, stream, systems, tiles, numTiles);
+        else forces_for<float, SoaAtoms>(energy, grid, block, stream, systems, tiles, numTiles);
     } else {
-        if (adopted) step_kernel(k_forces<double, AdoptedAtoms>, grid, block, stream, systems, tiles, numTiles);
-        else step_kernel(k_forces<double, SoaAtoms>, grid, block, stream, systems, tiles, numTiles);
+        if (adopted) forces_for<double, AdoptedAtoms>(energy, grid, block, stream, systems, tiles, numTiles);
+        else forces_for<double, SoaAtoms>(energy, grid, block, stream, systems, tiles, numTiles);
     }
     return hipGetLastError();
 }

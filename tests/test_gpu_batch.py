@@ -53,6 +53,7 @@ def test_batch_members_equal_alone(hip, mode, graph, fusion):
         np.testing.assert_array_equal(m.velocities(), a.velocities())
         np.testing.assert_array_equal(ke[i], a.kinetic_energies())
         np.testing.assert_array_equal(m.kinetic_energies(), a.kinetic_energies())
+        assert m.potential_energy() == a.potential_energy()
         assert m.time() == a.time()
 
 

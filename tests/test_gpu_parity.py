@@ -234,3 +234,32 @@ def test_fused_steps_equal_two_pass(hip, precision, mode):
         assert np.abs(a.positions() - b.positions()).max() < 1e-14
         assert np.abs(a.velocities() - b.velocities()).max() < 1e-13
         assert np.allclose(a.kinetic_energies(), b.kinetic_energies(), rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("graph,fusion", [(True, True), (False, True), (True, False), (False, False)])
+def test_potential_energy_is_the_last_evaluations(hip, oracle, graph, fusion):
+    """A step asks for forces alone; the library takes the potential energy
+    at the last force evaluation of each call only.  After calls of every
+    length (1, 2, one fused chunk and more, with and without the
+    refined-energy tail) the reported energy is that of the final positions:
+    the oracle's within the mixed tolerance, and the same bits on every
+    launch path."""
+    spec = systems.heterogeneous(700, 900, seed=19, rotation_mode=10)
+    # (anchors off the start positions, so the energy is not zero)
+    spec.tether_r0 = spec.tether_r0 + 0.03 * np.random.default_rng(19).normal(size=spec.tether_r0.shape)
+    o = oracle.OracleSim.from_spec(spec)
+    g = hip.Context.from_spec(spec, precision="mixed")
+    ref = hip.Context.from_spec(spec, precision="mixed")
+    ref.use_graph(False)
+    ref.use_fusion(False)
+    g.use_graph(graph)
+    g.use_fusion(fusion)
+    assert g.potential_energy() == pytest.approx(o.potential_energy(), rel=1e-5)
+    for k, refined in ((1, False), (2, False), (3, True), (17, False), (40, False), (2, True), (1, True)):
+        for c in (g, ref):
+            c.set_refined(refined)
+        o.step(k)
+        g.step(k)
+        ref.step(k)
+        assert g.potential_energy() == pytest.approx(o.potential_energy(), rel=1e-5), k
+        assert g.potential_energy() == ref.potential_energy(), k
