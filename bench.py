@@ -456,6 +456,59 @@ def kernel_roofline(kms, natoms_body, nb, nf, config, shared_frames=False):
     }
 
 
+def adopted_leg(spec, precision, kernel_steps):
+    """The drop-in path on the same system: the atom arrays in OpenMM's HIP
+    layout, owned by the caller (posq + posqCorrection float4, velm double4
+    with 1/m in w, the fixed-point force array; slot order = atom order) and
+    handed to rbhip_adopt_arrays, the library's tether writing the
+    fixed-point forces.  part1 and part2 are the kernels of rbhip_part1 /
+    rbhip_part2, the per-step calls of an OpenMM binding's execute()."""
+    from openmm_rigidbody_plugin_amd import rbhip
+    from openmm_rigidbody_plugin_amd.hipmem import DeviceArray
+    n = spec.num_atoms
+    padded = (n + 31) // 32 * 32
+    m = spec.masses
+    rt = np.float32 if precision == "mixed" else np.float64
+    velm = np.zeros((padded, 4))
+    velm[:n, 3] = np.where(m > 0, 1.0 / np.where(m > 0, m, 1.0), 0.0)
+    posq = DeviceArray(np.zeros((padded, 4), rt))
+    corr = DeviceArray(np.zeros((padded, 4), np.float32)) if precision == "mixed" else None
+    velm = DeviceArray(velm)
+    force = DeviceArray(np.zeros((3, padded), np.int64))
+    ctx = rbhip.Context(spec.masses, spec.body_indices, spec.dt, spec.rotation_mode, precision)
+    ctx.set_tether(spec.tether_k, spec.tether_r0)
+    ctx.adopt_arrays(posq.ptr.value, corr.ptr.value if corr else None, velm.ptr.value, force.ptr.value, padded,
+                     np.arange(n, dtype=np.int32))
+    ctx.set_positions(spec.positions)
+    ctx.set_velocities(spec.velocities)
+    ctx.autotune(reps=3)
+    kms, _ = ctx.time_kernels(kernel_steps)
+    counts = ctx.counts()
+    nb, nf = counts["numBodies"], counts["numFree"]
+    N = counts["numBodyAtoms"] / max(nb, 1)
+    alg = {k: algorithmic_bytes(k, N, nb, nf, indexed=True) for k in ("part1", "part2", "forces")}
+    two_ms = kms["part1"] + kms["part2"]
+    ke = ctx.kinetic_energies()
+    assert np.all(np.isfinite(ke)), "non-finite kinetic energy"
+    launch = ctx.describe()
+    ctx.close()
+    for a in (posq, corr, velm, force):
+        if a:
+            a.free()
+    log(f"  adopted arrays: part1 {kms['part1']:.4f} ms, part2 {kms['part2']:.4f} ms, forces {kms['forces']:.4f} ms")
+    return {
+        "note": "the same system on OpenMM-layout arrays owned by the caller (rbhip_adopt_arrays); "
+                "algorithmic bytes add the int32 slot read per atom and pass",
+        "kernel_ms": {k: kms[k] for k in ("part1", "forces", "part2")},
+        "two_pass_ms_per_step": two_ms,
+        "two_pass_body_steps_per_s": nb / (two_ms * 1e-3),
+        "two_pass_frac_hbm": (alg["part1"] + alg["part2"]) / (two_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "with_forces_ms_per_step": two_ms + kms["forces"],
+        "kernel_algorithmic_bytes": alg,
+        "launch": launch,
+    }
+
+
 def hetero_leg(steps, warmup):
     """The heterogeneous configuration timed by the same clock: 200k bodies
     (30% 2-atom rotors, 40% 3-site, 20% 6-rings, 10% 20-atom clusters, random
@@ -504,6 +557,8 @@ def main():
     ap.add_argument("--no-hetero", action="store_true",
                     help="skip the heterogeneous 200k-body exact-rotation measurement")
     ap.add_argument("--hetero-steps", type=int, default=1000)
+    ap.add_argument("--no-adopted", action="store_true",
+                    help="skip the timing of the step kernels on OpenMM-layout (adopted) arrays")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only stream MIB MiB through the plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -575,6 +630,11 @@ def main():
         "kernel_ms": kms,
         "integrator": roof["integrator"],
     }
+    if args.config == "water1M" and not args.no_adopted:
+        del ctx
+        ctx = None
+        log("the same system on adopted OpenMM-layout arrays ...")
+        result["integrator"]["adopted"] = adopted_leg(spec, cfg["precision"], args.kernel_steps)
     if args.config == "water1M" and not args.no_hetero:
         del ctx
         log("heterogeneous 200k bodies + 500k free atoms, exact rotation ...")

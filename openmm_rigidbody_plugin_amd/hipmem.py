@@ -1,5 +1,6 @@
-"""Device buffers owned by a test, standing in for the arrays an OpenMM HIP
-context owns (posq, posqCorrection, velm, the fixed-point force array).
+"""Device buffers owned by the caller (the tests, the benchmark), standing in
+for the arrays an OpenMM HIP context owns (posq, posqCorrection, velm, the
+fixed-point force array) when they are handed to rbhip_adopt_arrays.
 
 They come from the HIP runtime the library itself links (the same loaded
 libamdhip64), so the library's kernels read and write them as a caller's
@@ -16,7 +17,7 @@ def runtime():
     """The HIP runtime already loaded by librbhip.so."""
     global _hip
     if _hip is None:
-        from openmm_rigidbody_plugin_amd import rbhip
+        from . import rbhip
         rbhip.lib()
         path = None
         with open("/proc/self/maps") as f:

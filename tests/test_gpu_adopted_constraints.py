@@ -64,7 +64,7 @@ def rattle(r, v, cons, inv_mass, tol=CTOL):
 @pytest.mark.parametrize("mode", [10, 0])
 @pytest.mark.parametrize("precision", ["mixed", "double"])
 def test_caller_constraints_on_adopted_arrays_match_oracle(hip, oracle, precision, mode):
-    from hipmem import DeviceArray
+    from openmm_rigidbody_plugin_amd.hipmem import DeviceArray
     spec, cons = constrained_spec(mode)
     inv_mass = 1.0 / spec.masses
     o = oracle.OracleSim.from_spec(spec)
@@ -117,7 +117,7 @@ def test_caller_constraints_on_adopted_arrays_match_oracle(hip, oracle, precisio
 
 
 def test_constrained_part1_refusals(hip):
-    from hipmem import DeviceArray
+    from openmm_rigidbody_plugin_amd.hipmem import DeviceArray
     spec, _ = constrained_spec(10)
     lib = hip.Context.from_spec(spec, precision="mixed")
     pos_delta = DeviceArray(np.zeros((spec.num_atoms + 64, 4)))

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 def run_device(hip, labels, masses):
-    from hipmem import DeviceArray
+    from openmm_rigidbody_plugin_amd.hipmem import DeviceArray
     n = labels.size
     dl, dm = DeviceArray(labels.astype(np.int32)), DeviceArray(masses.astype(np.float64))
     bi = DeviceArray(np.zeros(n, np.int32))

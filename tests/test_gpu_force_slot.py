@@ -121,7 +121,7 @@ class OpenMMArrays:
     slot s holds original atom order[s]; padded slots beyond the atoms."""
 
     def __init__(self, spec, precision, order, padded):
-        from hipmem import DeviceArray
+        from openmm_rigidbody_plugin_amd.hipmem import DeviceArray
         n = spec.num_atoms
         self.n, self.padded, self.mixed = n, padded, precision == "mixed"
         self.order = np.asarray(order, dtype=np.int32)

@@ -19,7 +19,7 @@ def labels_and_masses(seed, n):
 
 
 def run(hip, fn, labels, masses):
-    from hipmem import DeviceArray
+    from openmm_rigidbody_plugin_amd.hipmem import DeviceArray
     n = labels.size
     dl, dm = DeviceArray(labels), DeviceArray(masses)
     outs = [DeviceArray(np.full(n + 1, -7, np.int32)) for _ in range(3)]
