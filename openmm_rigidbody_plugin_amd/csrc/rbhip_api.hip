@@ -2241,7 +2241,14 @@ int rbhip_part1_constrained(rbhip_context* ctx, const double* posDelta) {
         C.h.posDelta = const_cast<double*>(posDelta);
         C.h.freeCorr = C.dFreeCorr;
         C.uploadSys();
-        C.launchPart1();
+        try {
+            C.launchPart1();
+        } catch (...) {
+            // the caller's buffer is not kept past this call
+            C.h.posDelta = nullptr;
+            C.uploadSys();
+            throw;
+        }
         // part 1 read posDelta; the correction waits for part 2
         C.h.posDelta = nullptr;
         C.h.corrPending = 1;
