@@ -230,12 +230,13 @@ def plain_run(ctxs, steps, per_step):
     return dt
 
 
-def batched(steps, warmup, count=256, log_every=64):
+def batched(steps, warmup, count=256, log_every=64, plain=True):
     """256 independent replicas (units ~ LogUniform(1e3, 2e5), 80% water bodies
     / 20% argon, k ~ U[500, 2000]) stepped `steps` times: one system at a time
-    (each context on its own stream) with per-step launches and with step(K),
-    then all of them as one batch (one launch per phase over every replica's
-    tiles, HIP-graph replay, launch shape from the batch autotuner)."""
+    (each context on its own stream) with per-step launches and with step(K)
+    (unless `plain` is false), then all of them as one batch (one launch per
+    phase over every replica's tiles, HIP-graph replay, launch shape from the
+    batch autotuner), and the batch's kernels timed on their own dispatches."""
     from openmm_rigidbody_plugin_amd import rbhip, systems
     t = time.perf_counter()
     specs = systems.replica_set(count)
@@ -251,10 +252,12 @@ def batched(steps, warmup, count=256, log_every=64):
     for c in ctxs:
         c.step(warmup)
     rbhip.synchronize()
-    per_step_s = plain_run(ctxs, steps, per_step=True)
-    log(f"  plain, per-step launches: {per_step_s:.3f}s")
-    plain_s = plain_run(ctxs, steps, per_step=False)
-    log(f"  plain, one step(K) call per system: {plain_s:.3f}s")
+    per_step_s = plain_s = None
+    if plain:
+        per_step_s = plain_run(ctxs, steps, per_step=True)
+        log(f"  plain, per-step launches: {per_step_s:.3f}s")
+        plain_s = plain_run(ctxs, steps, per_step=False)
+        log(f"  plain, one step(K) call per system: {plain_s:.3f}s")
     batch = rbhip.Batch()
     batch.set_members(ctxs)
     t = time.perf_counter()
@@ -269,21 +272,32 @@ def batched(steps, warmup, count=256, log_every=64):
     ke = batch.kinetic_energies()
     assert np.all(np.isfinite(ke)), "non-finite kinetic energy in the batch"
     log(f"  batched: {batch_s:.3f}s")
+    kms, _ = batch.time_kernels(50)
+    alg = {k: algorithmic_bytes(k, 3, nb, nf) for k in ("fused", "forces")}
+    log("  batch kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in kms.items()))
     units = nb + nf
-    return {
+    out = {
         "workload": f"{count} replicas, units LogUniform(1e3, 2e5), 80% 3-site bodies / 20% free atoms, "
                     "k ~ U[500, 2000], NO-SQUISH n=10, dt 2 fs, mixed",
         "replicas": count, "bodies": nb, "free_atoms": nf, "steps": steps,
         "value": nb * steps / batch_s, "unit": "body-steps/s",
         "units_per_s": units * steps / batch_s,
-        "plain_per_step_value": nb * steps / per_step_s,
-        "plain_step_k_value": nb * steps / plain_s,
-        "speedup_vs_plain_per_step": per_step_s / batch_s,
-        "speedup_vs_plain_step_k": plain_s / batch_s,
-        "ms_per_step": batch_s / steps * 1e3, "plain_per_step_ms_per_step": per_step_s / steps * 1e3,
-        "plain_step_k_ms_per_step": plain_s / steps * 1e3,
+        "ms_per_step": batch_s / steps * 1e3,
         "launch": batch.describe(),
+        "kernel_ms": kms,
+        "kernel_algorithmic_bytes": alg,
+        "kernel_frac_hbm": {k: alg[k] / (kms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS for k in alg},
     }
+    if plain:
+        out.update({
+            "plain_per_step_value": nb * steps / per_step_s,
+            "plain_step_k_value": nb * steps / plain_s,
+            "speedup_vs_plain_per_step": per_step_s / batch_s,
+            "speedup_vs_plain_step_k": plain_s / batch_s,
+            "plain_per_step_ms_per_step": per_step_s / steps * 1e3,
+            "plain_step_k_ms_per_step": plain_s / steps * 1e3,
+        })
+    return out
 
 
 def served(num_jobs=96, rate=25.0, budget=1 << 30, steps_per_round=250, seed=11):
@@ -557,6 +571,8 @@ def main():
     ap.add_argument("--no-hetero", action="store_true",
                     help="skip the heterogeneous 200k-body exact-rotation measurement")
     ap.add_argument("--hetero-steps", type=int, default=1000)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched replicas, without the plain baselines (for profiling the batch)")
     ap.add_argument("--no-adopted", action="store_true",
                     help="skip the timing of the step kernels on OpenMM-layout (adopted) arrays")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -575,6 +591,9 @@ def main():
         ms = rbhip.copy_calibration(nbytes, reps=20)
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "avg_ms": ms,
                           "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batched": batched(args.batch_steps, args.warmup, plain=False)}), flush=True)
         return
     t0 = time.perf_counter()
     spec, cfg = make_workload(args.config)
