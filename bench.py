@@ -85,10 +85,11 @@ def implementation_bytes(kernel, n_atoms_per_body, num_bodies, num_free, shared_
       fused: 144 + 36N per body, 88 per free atom
       forces: 36 per atom, + 12 per free atom (low position part)
     With shared frames (bodies of one shape read a few cached rows of
-    body-fixed coordinates) each pass reads a 4-byte row index per body
-    instead of its 12N bytes of coordinates."""
+    body-fixed coordinates and one cached 1/I, 1/M) each pass reads a
+    4-byte body index per body instead of its 12N bytes of coordinates and
+    16 bytes of inertia."""
     N = n_atoms_per_body
-    d = (4 - 12 * N) if shared_frames else 0
+    d = (4 - 12 * N - 16) if shared_frames else 0
     if kernel == "part1":
         return num_bodies * (132 + 24 * N + d) + num_free * 88
     if kernel == "part2":

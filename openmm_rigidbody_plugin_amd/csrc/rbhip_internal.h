@@ -52,8 +52,8 @@ struct Tile {
     int atomFirst;  // internal index of the tile's first body atom (TILE_BODIES)
     int lanes;      // lanes per body, a power of two (TILE_BODIES)
     int slot;       // reduction slot of this tile in the system's partials
-    // 1: the tile's bodies read their body-fixed coordinates at the rows
-    // DevSys::dOff names (bodies of one shape share rows), 0: at their own
+    // 1: the tile's bodies read their body-fixed coordinates and inertia at
+    // the body DevSys::dOff names (bodies of one shape share them), 0: their own
     int sharedD = 0;
 };
 
@@ -78,9 +78,10 @@ struct DevSys {
     void* iI;
     double* rcm;
     void* d;
-    // per body, the first row of d it reads when its tile shares rows: its
-    // own rows, or those of a body of the same shape whose coordinates it
-    // equals to within a few fp32 ulps (its own rows then hold the same values)
+    // per body, the body whose d rows and inertia (1/I, 1/M) it reads when
+    // its tile shares them: itself, or a body of the same size class whose
+    // coordinates and inertia it equals to within a few fp32 ulps (its own
+    // rows and inertia then hold the same values)
     int* dOff;
     // atoms
     void* pos;

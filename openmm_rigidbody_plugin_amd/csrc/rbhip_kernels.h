@@ -69,8 +69,9 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t str
 hipError_t launch_setup(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int geometry,
                         int block, hipStream_t stream);
 // bodies of one size class whose body-fixed coordinates equal a candidate's
-// to within tol take its values and share its d rows (DevSys::dOff); the
-// count of such bodies is added to *matched (rbhip_setup.hip)
+// to within tol, and whose inertia equals it to within 2^-21, take its values
+// and share its d rows and inertia (DevSys::dOff); the count of such bodies
+// is added to *matched (rbhip_setup.hip)
 hipError_t launch_match_frames(int precision, const DevSys* sys, int firstBody, int count, int N, int row0,
                                const int* candRows, int numCand, double tol, int* matched, hipStream_t stream);
 // FP64 xyz rows -> device state: what = 0 positions, 1 velocities, 2 forces

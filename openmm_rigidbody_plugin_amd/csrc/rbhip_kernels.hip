@@ -75,8 +75,10 @@ template <class Real> struct Body {
     Real invM;
 };
 
-template <class Real> __device__ __forceinline__ Body<Real> load_body(const DevSys& S, int k) {
-    Q4<Real> iI = ld4<Real>(S.iI, k);
+// body k's state, with 1/I and 1/M read at body ks (the body whose inertia it
+// shares, or itself)
+template <class Real> __device__ __forceinline__ Body<Real> load_body(const DevSys& S, int k, int ks) {
+    Q4<Real> iI = ld4<Real>(S.iI, ks);
     return Body<Real>{ld4<Real>(S.q, k), ldp3<Real>(S.L, k), ldp3<Real>(S.pcm, k), V3<Real>{iI.a, iI.b, iI.c}, iI.d};
 }
 
@@ -85,10 +87,17 @@ template <class Real> __device__ __forceinline__ Body<Real> load_body(const DevS
 constexpr int STRIDED_BATCH = 8;
 
 // F = sum f_j and tau = sum delta_j x f_j in FP64, delta_j = A(q)^T d_j
-// first row of body k's body-fixed coordinates: its own, or, in a tile of
-// bodies that share rows, the rows DevSys::dOff names for it
-__device__ __forceinline__ int d_row(const DevSys& S, const Tile& tile, int k, int atom0) {
-    return tile.sharedD ? S.dOff[k] : atom0 - S.numFree;
+// Where body k (first atom atom0, N atoms) reads its body-fixed coordinates
+// and inertia: its own, or, in a tile of bodies that share them, those of
+// body DevSys::dOff[k] of the same size class, whose rows lie (k - ks) N rows
+// before its own.  ks: the body whose 1/I and 1/M it reads; row: its first
+// row of d.
+struct Shape { int ks, row; };
+__device__ __forceinline__ Shape body_shape(const DevSys& S, const Tile& tile, int k, int atom0, int N) {
+    const int own = atom0 - S.numFree;
+    if (!tile.sharedD) return {k, own};
+    const int ks = S.dOff[k];
+    return {ks, own - (k - ks) * N};
 }
 
 template <class Real, int NC, class IO>
@@ -339,8 +348,9 @@ __device__ __forceinline__ void part1_bodies(const DevSys& S, const Tile& tile) 
     if (b >= tile.count) return;
     const int k = tile.first + b;
     const int atom0 = tile.atomFirst + b * tile.N;
-    const int dbase = d_row(S, tile, k, atom0);
-    Body<Real> B = load_body<Real>(S, k);
+    const Shape sh = body_shape(S, tile, k, atom0, tile.N);
+    const int dbase = sh.row;
+    Body<Real> B = load_body<Real>(S, k, sh.ks);
     V3<double> rc = load_rcm<Real>(S, k);
     drift_and_rotate<Real, EXACT>(S, B, rc);
     if (sub == 0) {
@@ -393,8 +403,9 @@ __device__ __forceinline__ void part2_bodies(const DevSys& S, const Tile& tile, 
     const int k = tile.first + (active ? b : 0);
     const int N = tile.N;
     const int atom0 = tile.atomFirst + (active ? b : 0) * N;
-    const int dbase = d_row(S, tile, k, atom0);
-    Body<Real> B = load_body<Real>(S, k);
+    const Shape sh = body_shape(S, tile, k, atom0, N);
+    const int dbase = sh.row;
+    Body<Real> B = load_body<Real>(S, k, sh.ks);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
     body_sums_any<Real, IO>(S, R, atom0, dbase, N, sub, tile.lanes, active, F, tau);
@@ -485,8 +496,8 @@ template <class Real> __device__ __forceinline__ void store_body(const DevSys& S
 // one lane per body of NC atoms: the body-fixed coordinates stay in registers
 // from the torque sum to the position rebuild, so d is read once
 template <class Real, bool EXACT, int NC, class IO>
-__device__ __forceinline__ void fused_body_regs(const DevSys& S, int k, int atom0, int dbase) {
-    Body<Real> B = load_body<Real>(S, k);
+__device__ __forceinline__ void fused_body_regs(const DevSys& S, int k, int atom0, int ks, int dbase) {
+    Body<Real> B = load_body<Real>(S, k, ks);
     V3<double> rc = load_rcm<Real>(S, k);
     V3<Real> dj[NC], fj[NC];
 #pragma unroll
@@ -519,10 +530,10 @@ __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) 
         const int b = threadIdx.x;
         if (b >= tile.count) return;
         const int k = tile.first + b, atom0 = tile.atomFirst + b * N;
-        const int dbase = d_row(S, tile, k, atom0);
-        if (N == 2) fused_body_regs<Real, EXACT, 2, IO>(S, k, atom0, dbase);
-        else if (N == 3) fused_body_regs<Real, EXACT, 3, IO>(S, k, atom0, dbase);
-        else fused_body_regs<Real, EXACT, 4, IO>(S, k, atom0, dbase);
+        const Shape sh = body_shape(S, tile, k, atom0, N);
+        if (N == 2) fused_body_regs<Real, EXACT, 2, IO>(S, k, atom0, sh.ks, sh.row);
+        else if (N == 3) fused_body_regs<Real, EXACT, 3, IO>(S, k, atom0, sh.ks, sh.row);
+        else fused_body_regs<Real, EXACT, 4, IO>(S, k, atom0, sh.ks, sh.row);
         return;
     }
     const int shift = lane_shift(tile.lanes);
@@ -531,8 +542,9 @@ __device__ __forceinline__ void fused_bodies(const DevSys& S, const Tile& tile) 
     const bool active = b < tile.count;
     const int k = tile.first + (active ? b : 0);
     const int atom0 = tile.atomFirst + (active ? b : 0) * N;
-    const int dbase = d_row(S, tile, k, atom0);
-    Body<Real> B = load_body<Real>(S, k);
+    const Shape sh = body_shape(S, tile, k, atom0, N);
+    const int dbase = sh.row;
+    Body<Real> B = load_body<Real>(S, k, sh.ks);
     V3<double> rc = load_rcm<Real>(S, k);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
@@ -617,10 +629,11 @@ __device__ __forceinline__ void refine_begin_bodies(const DevSys& S, const Tile&
     const bool active = b < tile.count;
     const int k = tile.first + (active ? b : 0);
     const int atom0 = tile.atomFirst + (active ? b : 0) * tile.N;
-    Body<Real> B = load_body<Real>(S, k);
+    const Shape sh = body_shape(S, tile, k, atom0, tile.N);
+    Body<Real> B = load_body<Real>(S, k, sh.ks);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real>(S, R, atom0, d_row(S, tile, k, atom0), tile.N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real>(S, R, atom0, sh.row, tile.N, sub, tile.lanes, active, F, tau);
     if (!active || sub != 0) return;
     V3<double> tb = axial_torque(to_body(R, tau), B.invI);
     V3<double> ar;
@@ -682,10 +695,11 @@ __device__ __forceinline__ void refine_end_bodies(const DevSys& S, const Tile& t
     const int k = tile.first + (active ? b : 0);
     const int atom0 = tile.atomFirst + (active ? b : 0) * tile.N;
     // stored after part2: q(t+dt), p and L with the next half-kick folded in
-    Body<Real> B = load_body<Real>(S, k);
+    const Shape sh = body_shape(S, tile, k, atom0, tile.N);
+    Body<Real> B = load_body<Real>(S, k, sh.ks);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real>(S, R, atom0, d_row(S, tile, k, atom0), tile.N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real>(S, R, atom0, sh.row, tile.N, sub, tile.lanes, active, F, tau);
     if (!active || sub != 0) return;
     const double halfDt = 0.5 * S.dt;
     V3<double> tb = axial_torque(to_body(R, tau), B.invI);
@@ -744,10 +758,11 @@ __global__ __launch_bounds__(512) void k_refine_pe(const DevSys* __restrict__ sy
             const bool active = b < tile.count;
             const int k = tile.first + (active ? b : 0);
             const int atom0 = tile.atomFirst + (active ? b : 0) * tile.N;
-            Body<Real> B = load_body<Real>(S, k);
+            const Shape sh = body_shape(S, tile, k, atom0, tile.N);
+            Body<Real> B = load_body<Real>(S, k, sh.ks);
             Rot<Real> R = rotT(B.q);
             V3<double> F, tau;
-            body_sums_any<Real>(S, R, atom0, d_row(S, tile, k, atom0), tile.N, sub, tile.lanes, active, F, tau);
+            body_sums_any<Real>(S, R, atom0, sh.row, tile.N, sub, tile.lanes, active, F, tau);
             if (active && sub == 0) {
                 V3<double> tb = axial_torque(to_body(R, tau), B.invI);
                 u[0] = dot(F, F) * (double)B.invM + dot(hadamard(tb, cvt<double>(B.invI)), tb);

@@ -125,7 +125,7 @@ __device__ void setup_body(const DevSys& S, int k, int N, int atom0, int geometr
             A = eigenvectors3(inertia, I);
         }
         q = quat_from_matrix(A);
-        S.dOff[k] = dbase;   // its own rows until k_match_frames finds a shared shape
+        S.dOff[k] = k;   // its own rows and inertia until k_match_frames finds a shared shape
         for (int j = 0; j < N; j++) {
             V3<double> dl = setup_pos<Real>(S, atom0 + j) - rcm;
             st3<Real>(S.d, S.dStride, dbase + j, cvt<Real>(m3_apply(A, dl)));
@@ -459,31 +459,41 @@ __global__ void k_body_atoms_out(const DevSys* __restrict__ sys, const int* __re
 // Bodies of one shape.  The bodies of a size class (count bodies of N atoms
 // from firstBody, their d rows from row0) are compared with candidate shapes:
 // the d rows of a few bodies of the class, given by their first row.  A body
-// whose coordinates all lie within tol of a candidate's takes that
-// candidate's values into its own rows and its rows as DevSys::dOff, so the
-// step kernels of a sharing tile read the shared rows (few, cached) and
-// every other reader sees the same values in the body's own rows.  The
-// bodies that matched are counted in *matched.  A candidate matches itself
-// and copies nothing; a body that is no candidate writes only its own rows.
+// whose coordinates all lie within tol of a candidate's, and whose 1/I and
+// 1/M lie within 2^-21 of the candidate's relative to them, takes that
+// candidate's values into its own rows and inertia and the candidate's body
+// index as DevSys::dOff, so the step kernels of a sharing tile read the
+// shared rows and inertia (few, cached) and every other reader sees the same
+// values in the body's own.  The bodies that matched are counted in
+// *matched.  A candidate matches itself and copies nothing; a body that is
+// no candidate writes only its own rows.
 template <class Real>
 __global__ void k_match_frames(const DevSys* __restrict__ sys, int firstBody, int count, int N, int row0,
                                const int* __restrict__ candRows, int numCand, double tol, int* matched) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= count) return;
     const DevSys& S = *sys;
-    const int own = row0 + b * N;
+    const int own = row0 + b * N, k = firstBody + b;
+    const Q4<Real> ownI = ld4<Real>(S.iI, k);
+    auto near = [](Real x, Real y) {
+        const double a = (double)x, c = (double)y;
+        return fabs(a - c) <= 0x1p-21 * fmax(fabs(a), fabs(c));
+    };
     for (int c = 0; c < numCand; c++) {
-        const int cr = candRows[c];
-        bool same = true;
+        const int cr = candRows[c], kc = firstBody + (cr - row0) / N;
+        const Q4<Real> candI = ld4<Real>(S.iI, kc);
+        bool same = near(ownI.a, candI.a) && near(ownI.b, candI.b) && near(ownI.c, candI.c) && near(ownI.d, candI.d);
         for (int j = 0; same && j < N; j++) {
             V3<Real> x = ld3<Real>(S.d, S.dStride, own + j), y = ld3<Real>(S.d, S.dStride, cr + j);
             same = fabs((double)x.x - (double)y.x) <= tol && fabs((double)x.y - (double)y.y) <= tol &&
                    fabs((double)x.z - (double)y.z) <= tol;
         }
         if (same) {
-            if (cr != own)
+            if (cr != own) {
                 for (int j = 0; j < N; j++) st3<Real>(S.d, S.dStride, own + j, ld3<Real>(S.d, S.dStride, cr + j));
-            S.dOff[firstBody + b] = cr;
+                st4<Real>(S.iI, k, candI);
+            }
+            S.dOff[k] = kc;
             atomicAdd(matched, 1);
             return;
         }

@@ -87,3 +87,30 @@ def test_minority_shape_keeps_own_rows_with_the_same_trajectory(hip):
     a.step(10)
     b.step(10)
     assert np.abs(a.positions() - b.positions()).max() <= 1e-7
+
+
+@pytest.mark.parametrize("heavy", ["alternate", "last"])
+def test_equal_geometry_other_masses_keep_their_inertia(hip, oracle, heavy):
+    """Bodies of one geometry whose masses are all scaled (their body-fixed
+    coordinates are unchanged, their 1/I and 1/M are not) never take another
+    body's inertia: the step passes of a sharing tile read 1/I and 1/M at the
+    body they share rows with, so matching compares the inertia too.  With
+    every second water twice as heavy, or the last 112 of 512, the
+    trajectory stays with the unshared one and the FP64 oracle's; in the
+    second case the light majority (whose bodies give the candidate shapes)
+    shares, and the heavy bodies sit in sharing tiles."""
+    spec = systems.water_c1(rotation_mode=10)
+    bi = spec.body_indices
+    sel = (bi % 2 == 0) if heavy == "alternate" else (bi > 400)
+    spec.masses = np.where((bi > 0) & sel, 2.0 * spec.masses, spec.masses)
+    a = hip.Context.from_spec(spec, precision="mixed")
+    b = hip.Context.from_spec(spec, precision="mixed", shared_frames=False)
+    if heavy == "last":
+        assert shapes(a).startswith("N3:")
+    for c in (a, b):
+        c.step(20)
+    o = oracle.OracleSim.from_spec(spec)
+    o.step(20)
+    assert np.abs(a.positions() - b.positions()).max() <= 1e-7
+    assert np.abs(a.positions() - o.positions()).max() <= 1e-6
+    np.testing.assert_allclose(a.kinetic_energies(), o.kinetic_energies(), rtol=1e-5)
