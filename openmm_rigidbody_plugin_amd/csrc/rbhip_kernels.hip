@@ -1338,7 +1338,10 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
                                                  int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
         const Tile tile = tiles[t];
-        const DevSys& S = systems[tile.sys];
+        // (a copy: read through the table, the descriptor's fields would be
+        // re-read after every store, which may alias it, and each re-read
+        // waits for the atom loads in flight)
+        const DevSys S = systems[tile.sys];
         double e[1] = {0.0};
         for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
             int a = tile.first + i;
