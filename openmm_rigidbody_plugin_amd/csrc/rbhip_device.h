@@ -141,6 +141,18 @@ __device__ __forceinline__ void sin_cosm1_tiny(float phi, float& s, float& cm1) 
     cm1 = x2 * fmaf(x2, 1.0f / 24.0f, -0.5f);
 }
 __device__ __forceinline__ void sin_cosm1_tiny(double phi, double& s, double& cm1) { sin_cosm1_small(phi, s, cm1); }
+// sin and cos of an FP64 angle that is usually small (one step's turn): the
+// Taylor polynomials above for |x| <= 1/16 (truncation < 2e-20 relative), the
+// library's sincos otherwise
+__device__ __forceinline__ void sincos_turn(double x, double& s, double& c) {
+    if (fabs(x) <= 0.0625) {
+        double cm1;
+        sin_cosm1_small(x, s, cm1);
+        c = 1.0 + cm1;
+    } else {
+        sincos(x, &s, &c);
+    }
+}
 template <class T> struct SmallAngle;
 template <> struct SmallAngle<float> { static constexpr float value = 0.1f; };
 template <> struct SmallAngle<double> { static constexpr double value = 0.0625; };
@@ -445,7 +457,7 @@ __device__ inline void linear_rotor_exact(Q4<double>& q, V3<double> L, V3<double
     double wn = sqrt(dot(w, w));
     if (wn == 0.0) return;
     double sh, ch;
-    sincos(0.5 * wn * dt, &sh, &ch);
+    sincos_turn(0.5 * wn * dt, sh, ch);
     V3<double> axis = w * (1.0 / wn);
     q = q * ch + Bmul(q, axis) * sh;
     double r = 1.0 / sqrt(dot(q, q));
@@ -711,7 +723,7 @@ __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> 
     V3<double> Iw1 = body_L(delta);
     Q4<double> z0 = {Iw.z, Iw.y, L - Iw.x, 0.0};
     double st, ct;
-    sincos(theta, &st, &ct);
+    sincos_turn(theta, st, ct);
     Q4<double> z = Q4<double>{Iw1.z, Iw1.y, L - Iw1.x, 0.0} * ct + Q4<double>{-Iw1.y, Iw1.z, 0.0, L - Iw1.x} * st;
     Q4<double> qf = flip ? perm<3>(q) : q;
     qf = z * dot(z0, qf) + Cmul(z, Ctmul(z0, qf));
