@@ -153,6 +153,13 @@ __device__ __forceinline__ void sincos_turn(double x, double& s, double& c) {
         sincos(x, &s, &c);
     }
 }
+// 1 / sqrt(n2) for the squared norm of a quaternion being renormalised:
+// 1 - e/2 + 3e^2/8 for |e| = |n2 - 1| < 1e-5 (truncation < 4e-16), the square
+// root otherwise
+__device__ __forceinline__ double inv_norm(double n2) {
+    const double e = n2 - 1.0;
+    return fabs(e) < 1e-5 ? 1.0 + e * (-0.5 + 0.375 * e) : 1.0 / sqrt(n2);
+}
 template <class T> struct SmallAngle;
 template <> struct SmallAngle<float> { static constexpr float value = 0.1f; };
 template <> struct SmallAngle<double> { static constexpr double value = 0.0625; };
@@ -460,8 +467,7 @@ __device__ inline void linear_rotor_exact(Q4<double>& q, V3<double> L, V3<double
     sincos_turn(0.5 * wn * dt, sh, ch);
     V3<double> axis = w * (1.0 / wn);
     q = q * ch + Bmul(q, axis) * sh;
-    double r = 1.0 / sqrt(dot(q, q));
-    q = q * r;
+    q = q * inv_norm(dot(q, q));
 }
 
 // exact free-rotor rotation of (q, L) over dt by the reference's closed form
@@ -691,20 +697,32 @@ __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> 
     JacobiSeries J;
     jacobi_series_at<TERMS>(m, J);
     const double ms0 = m * s0, c0d0 = c0 * d0;
-    // body-frame L at u0 + x
-    auto body_L = [&](double x) {
+    // body-frame L at u0 + x as numerators over the addition theorem's
+    // common denominator: L = Ln / D
+    auto body_Ln = [&](double x, double& D) {
         double sx, cx, dx;
         jacobi_series<TERMS>(x, J, sx, cx, dx);
-        double inv = 1.0 / (1.0 - ms0 * s0 * sx * sx);
-        double sn = (s0 * cx * dx + sx * c0d0) * inv;
-        double cn = (c0 * cx - s0 * sx * d0 * dx) * inv;
-        double dn = (d0 * dx - ms0 * sx * c0 * cx) * inv;
+        D = 1.0 - ms0 * s0 * sx * sx;
+        double sn = s0 * cx * dx + sx * c0d0;
+        double cn = c0 * cx - s0 * sx * d0 * dx;
+        double dn = d0 * dx - ms0 * sx * c0 * cx;
         return low ? V3<double>{Ia.x * cn, Ia.y * sn, Ia.z * dn} : V3<double>{Ia.x * dn, Ia.y * sn, Ia.z * cn};
     };
-    // 2 dtheta/dt at body-frame L
-    auto rate = [&](V3<double> Lt) {
-        double y2 = Lt.y * Lt.y, z2 = Lt.z * Lt.z;
-        return (L + Lt.x) * (y2 * invI.y + z2 * invI.z) / (y2 + z2) - Lt.x * invI.x;
+    auto body_L = [&](double x) {
+        double D;
+        V3<double> Ln = body_Ln(x, D);
+        return Ln * (1.0 / D);
+    };
+    // 2 dtheta/dt at u0 + x, (|L| + L1) X - L1 / I1: X is the same for Ln as
+    // for L, so one reciprocal serves X and L1 = Ln1 / D
+    auto rate = [&](double x) {
+        double D;
+        V3<double> Ln = body_Ln(x, D);
+        double y2 = Ln.y * Ln.y, z2 = Ln.z * Ln.z, s2 = y2 + z2;
+        double p = 1.0 / (D * s2);
+        double X = (y2 * invI.y + z2 * invI.z) * D * p;
+        double L1 = Ln.x * s2 * p;
+        return (L + L1) * X - L1 * invI.x;
     };
     // Gauss-Legendre nodes and weights on [-1, 1]: 5 points, or 3 for fp32 state
     const double h = 0.5 * delta;
@@ -712,12 +730,12 @@ __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> 
     if constexpr (FP32_STATE) {
         const double gx1 = 0.7745966692414834;
         const double gw0 = 0.8888888888888888, gw1 = 0.5555555555555556;
-        sum = gw0 * rate(body_L(h)) + gw1 * (rate(body_L(h * (1.0 - gx1))) + rate(body_L(h * (1.0 + gx1))));
+        sum = gw0 * rate(h) + gw1 * (rate(h * (1.0 - gx1)) + rate(h * (1.0 + gx1)));
     } else {
         const double gx1 = 0.5384693101056831, gx2 = 0.9061798459386640;
         const double gw0 = 0.5688888888888889, gw1 = 0.4786286704993665, gw2 = 0.2369268850561891;
-        sum = gw0 * rate(body_L(h)) + gw1 * (rate(body_L(h * (1.0 - gx1))) + rate(body_L(h * (1.0 + gx1)))) +
-              gw2 * (rate(body_L(h * (1.0 - gx2))) + rate(body_L(h * (1.0 + gx2))));
+        sum = gw0 * rate(h) + gw1 * (rate(h * (1.0 - gx1)) + rate(h * (1.0 + gx1))) +
+              gw2 * (rate(h * (1.0 - gx2)) + rate(h * (1.0 + gx2)));
     }
     double theta = 0.25 * dt * sum;
     V3<double> Iw1 = body_L(delta);
@@ -727,7 +745,7 @@ __device__ inline void exact_rotation(Q4<double>& q, V3<double>& Lb, V3<double> 
     Q4<double> z = Q4<double>{Iw1.z, Iw1.y, L - Iw1.x, 0.0} * ct + Q4<double>{-Iw1.y, Iw1.z, 0.0, L - Iw1.x} * st;
     Q4<double> qf = flip ? perm<3>(q) : q;
     qf = z * dot(z0, qf) + Cmul(z, Ctmul(z0, qf));
-    double r = 1.0 / sqrt(dot(qf, qf));
+    double r = inv_norm(dot(qf, qf));
     // P3 P3 q = -q: the turn back is -P3
     q = flip ? perm<3>(qf) * -r : qf * r;
     Lb = flip ? V3<double>{-Iw1.x, -Iw1.y, Iw1.z} : Iw1;
