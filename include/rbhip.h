@@ -247,7 +247,10 @@ int rbhip_state_changed(rbhip_context* ctx, int what);
    from the arrays, so they include the caller's velocity constraints.  Not
    between rbhip_part1_constrained and rbhip_part2: another part 1,
    rbhip_free_delta, rbhip_execute, rbhip_time_kernels or rbhip_autotune;
-   rbhip_state_changed abandons the pending correction. */
+   rbhip_state_changed abandons the pending correction.  posDelta is indexed
+   by slot, so the atom order must not change between rbhip_free_delta and
+   rbhip_part1_constrained (OpenMM reorders at the end of a step,
+   CudaRigidBodyKernels.cpp:443). */
 int rbhip_free_delta(rbhip_context* ctx, double* posDelta);
 int rbhip_part1_constrained(rbhip_context* ctx, const double* posDelta);
 
