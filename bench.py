@@ -262,7 +262,7 @@ def batched(steps, warmup, count=256, log_every=64, plain=True):
     batch = rbhip.Batch()
     batch.set_members(ctxs)
     t = time.perf_counter()
-    batch.autotune(reps=2)
+    batch.autotune(reps=5)
     log(f"  batch autotune {time.perf_counter() - t:.1f}s: {batch.describe()}")
     batch.step(warmup)
     rbhip.synchronize()
