@@ -213,8 +213,9 @@ int rbhip_set_forces(rbhip_context* ctx, const double* xyz);
    them with the library's force field.  The get/set calls reach the arrays.
    Not with the library's own constraints or virtual sites (the caller's:
    rbhip_part1_constrained below; sites are the caller's to place after
-   part 1), refined energies or batches.  The arrays must outlive the
-   context or a later rbhip_destroy. */
+   part 1) or batches; refined energies read the arrays too, but not with
+   the caller's constraints.  The arrays must outlive the context or a
+   later rbhip_destroy. */
 int rbhip_adopt_arrays(rbhip_context* ctx, void* posq, void* posqCorrection, void* velm, long long* force,
                        int paddedNumAtoms, const int* atomOrder, int orderLength);
 /* the arrays' atom order changed (ReorderListener::execute,

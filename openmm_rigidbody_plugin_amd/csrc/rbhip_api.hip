@@ -840,10 +840,11 @@ struct rbhip_context {
     int partKeSlots() const { return nPartTiles + nClusterTiles; }
     void launchRefineBegin(bool folded) {
         HIPCHK(launch_refine_begin(precision, rotationMode, dSys, dPartTiles, nPartTiles, folded ? 1 : 0, block,
-                                   maxBlocks, stream));
+                                   maxBlocks, stream, layout()));
     }
     void launchRefineEnd() {
-        HIPCHK(launch_refine_end(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream));
+        HIPCHK(launch_refine_end(precision, rotationMode, dSys, dPartTiles, nPartTiles, block, maxBlocks, stream,
+                                 layout()));
     }
 
     // 4 doubles reduced from the KE (which = 0), PE (1), refined KE (2) or
@@ -1046,8 +1047,8 @@ struct rbhip_context {
             throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with constraints");
         if (on && numSites > 0)
             throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with virtual sites");
-        if (on && adopted)
-            throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with adopted atom arrays");
+        if (on && dFreeCorr)
+            throw RbError(RBHIP_E_UNSUPPORTED, "refined energies are not supported with the caller's constraints");
         if (on && !refBuf[0]) {
             const size_t R = realSize();
             refArena.stream = stream;
@@ -1189,10 +1190,10 @@ struct rbhip_batch {
     }
     void launchRefineBegin(bool folded) {
         HIPCHK(launch_refine_begin(precision, rotationMode, dSystems, dPartTiles, nPart, folded ? 1 : 0, block,
-                                   maxBlocks, stream));
+                                   maxBlocks, stream, 0));
     }
     void launchRefineEnd() {
-        HIPCHK(launch_refine_end(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream));
+        HIPCHK(launch_refine_end(precision, rotationMode, dSystems, dPartTiles, nPart, block, maxBlocks, stream, 0));
     }
     bool anyRefined() const {
         for (const rbhip_context* c : members)
@@ -2049,7 +2050,6 @@ int rbhip_adopt_arrays(rbhip_context* ctx, void* posq, void* posqCorrection, voi
         require(paddedNumAtoms >= C.numAtoms && orderLength <= paddedNumAtoms, "arrays shorter than the system");
         require(C.numConstraints == 0, "adopted arrays are not supported with constraints", RBHIP_E_UNSUPPORTED);
         require(C.numSites == 0, "adopted arrays are not supported with virtual sites", RBHIP_E_UNSUPPORTED);
-        require(!C.refined, "adopted arrays are not supported with refined energies", RBHIP_E_UNSUPPORTED);
         for (void* p : {posq, posqCorrection, velm, (void*)force})
             require(((uintptr_t)p & 15) == 0, "adopted arrays must be 16-byte aligned");
         std::vector<int> slot = slots_for_order(C, atomOrder, orderLength);
@@ -2215,6 +2215,7 @@ int rbhip_free_delta(rbhip_context* ctx, double* posDelta) {
         require(ctx && posDelta, "null argument");
         rbhip_context& C = *ctx;
         require(C.adopted, "free-atom displacements are for adopted arrays", RBHIP_E_STATE);
+        require(!C.refined, "refined energies are not supported with the caller's constraints", RBHIP_E_UNSUPPORTED);
         require(C.built, "positions must be set before stepping", RBHIP_E_STATE);
         require_step_complete(C);
         require(((uintptr_t)posDelta & 15) == 0, "posDelta must be 16-byte aligned");
@@ -2229,6 +2230,7 @@ int rbhip_part1_constrained(rbhip_context* ctx, const double* posDelta) {
         require(ctx && posDelta, "null argument");
         rbhip_context& C = *ctx;
         require(C.adopted, "constrained displacements are for adopted arrays", RBHIP_E_STATE);
+        require(!C.refined, "refined energies are not supported with the caller's constraints", RBHIP_E_UNSUPPORTED);
         require(C.built, "positions must be set before stepping", RBHIP_E_STATE);
         require_step_complete(C);
         require(((uintptr_t)posDelta & 15) == 0, "posDelta must be 16-byte aligned");
@@ -2357,7 +2359,8 @@ int rbhip_potential_energy_refinement(rbhip_context* ctx, double* energy) {
         *energy = 0.0;
         if (!C.refined || !C.built || C.nPartTiles == 0) return;
         C.setDevice();
-        HIPCHK(launch_refine_pe(C.precision, C.dSys, C.dPartTiles, C.nPartTiles, C.block, C.maxBlocks, C.stream));
+        HIPCHK(launch_refine_pe(C.precision, C.dSys, C.dPartTiles, C.nPartTiles, C.block, C.maxBlocks, C.stream,
+                                C.layout()));
         double r[4];
         C.reduce(3, r);
         *energy = -r[0] * C.dt * C.dt / 24.0;

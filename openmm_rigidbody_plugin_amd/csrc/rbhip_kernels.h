@@ -35,11 +35,11 @@ hipError_t launch_fused(int precision, int rotationMode, const DevSys* systems, 
 // i.e. part1 follows; 0 = a fused boundary follows), the end terms after
 // its part2, and the potential refinement of the current forces
 hipError_t launch_refine_begin(int precision, int rotationMode, const DevSys* systems, const Tile* tiles,
-                               int numTiles, int folded, int block, int maxBlocks, hipStream_t stream);
+                               int numTiles, int folded, int block, int maxBlocks, hipStream_t stream, int adopted);
 hipError_t launch_refine_end(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
-                             int block, int maxBlocks, hipStream_t stream);
+                             int block, int maxBlocks, hipStream_t stream, int adopted);
 hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
-                            int maxBlocks, hipStream_t stream);
+                            int maxBlocks, hipStream_t stream, int adopted);
 // (energy: also the tiles' potential-energy partials)
 hipError_t launch_forces(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
                          int maxBlocks, hipStream_t stream, int adopted, int energy);

@@ -609,13 +609,13 @@ template <class Real> __device__ __forceinline__ void store_refine_body(const De
     st4<Real>(S.refBody, 2 * k + 1, cvt<Real>(aq));
 }
 
-template <class Real>
+template <class Real, class IO>
 __device__ __forceinline__ void refine_begin_free(const DevSys& S, const Tile& tile, bool folded) {
     Real halfDt = (Real)(0.5 * S.dt);
     for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
         int a = tile.first + i;
-        V3<Real> v = ld3<Real>(S.vel, S.atomStride, a);
-        V3<Real> f = ld3<Real>(S.force, S.atomStride, a);
+        V3<Real> v = IO::template vel<Real>(S, a);
+        V3<Real> f = IO::template force<Real>(S, a);
         Real invm = reinterpret_cast<const Real*>(S.freeInvM)[a];
         V3<Real> dv = f * invm * halfDt;
         V3<double> av;
@@ -630,7 +630,7 @@ __device__ __forceinline__ void refine_begin_free(const DevSys& S, const Tile& t
     }
 }
 
-template <class Real, bool EXACT>
+template <class Real, bool EXACT, class IO>
 __device__ __forceinline__ void refine_begin_bodies(const DevSys& S, const Tile& tile, bool folded) {
     const int shift = lane_shift(tile.lanes);
     const int b = threadIdx.x >> shift;
@@ -642,7 +642,7 @@ __device__ __forceinline__ void refine_begin_bodies(const DevSys& S, const Tile&
     Body<Real> B = load_body<Real>(S, k, sh.ks);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real>(S, R, atom0, sh.row, tile.N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real, IO>(S, R, atom0, sh.row, tile.N, sub, tile.lanes, active, F, tau);
     if (!active || sub != 0) return;
     V3<double> tb = axial_torque(to_body(R, tau), B.invI);
     V3<double> ar;
@@ -666,7 +666,7 @@ __device__ __forceinline__ void refine_begin_bodies(const DevSys& S, const Tile&
     store_refine_body<Real>(S, k, ar, aq);
 }
 
-template <class Real, bool EXACT>
+template <class Real, bool EXACT, class IO>
 __global__ __launch_bounds__(EXACT ? 256 : 512) void k_refine_begin(const DevSys* __restrict__ systems,
                                                                      const Tile* __restrict__ tiles, int numTiles,
                                                                      int folded) {
@@ -675,19 +675,19 @@ __global__ __launch_bounds__(EXACT ? 256 : 512) void k_refine_begin(const DevSys
         const DevSys& S = systems[tile.sys];
         if (!S.refBody) continue;
         if (tile.kind == TILE_FREE)
-            refine_begin_free<Real>(S, tile, folded != 0);
+            refine_begin_free<Real, IO>(S, tile, folded != 0);
         else
-            refine_begin_bodies<Real, EXACT>(S, tile, folded != 0);
+            refine_begin_bodies<Real, EXACT, IO>(S, tile, folded != 0);
     }
 }
 
-template <class Real>
+template <class Real, class IO>
 __device__ __forceinline__ void refine_end_free(const DevSys& S, const Tile& tile, double (&rk)[2]) {
     const double halfDt = 0.5 * S.dt;
     for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
         int a = tile.first + i;
-        V3<double> v = cvt<double>(ld3<Real>(S.vel, S.atomStride, a));
-        V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, a));
+        V3<double> v = cvt<double>(IO::template vel<Real>(S, a));
+        V3<double> f = cvt<double>(IO::template force<Real>(S, a));
         double invm = (double)reinterpret_cast<const Real*>(S.freeInvM)[a];
         // posDot / dt = a_v + 2 v(t+3dt/2)
         V3<double> pd = cvt<double>(ldp3<Real>(S.refFree, a)) + (v + f * (invm * halfDt)) * 2.0;
@@ -695,7 +695,7 @@ __device__ __forceinline__ void refine_end_free(const DevSys& S, const Tile& til
     }
 }
 
-template <class Real, bool EXACT>
+template <class Real, bool EXACT, class IO>
 __device__ __forceinline__ void refine_end_bodies(const DevSys& S, const Tile& tile, double (&rk)[2]) {
     const int shift = lane_shift(tile.lanes);
     const int b = threadIdx.x >> shift;
@@ -708,7 +708,7 @@ __device__ __forceinline__ void refine_end_bodies(const DevSys& S, const Tile& t
     Body<Real> B = load_body<Real>(S, k, sh.ks);
     Rot<Real> R = rotT(B.q);
     V3<double> F, tau;
-    body_sums_any<Real>(S, R, atom0, sh.row, tile.N, sub, tile.lanes, active, F, tau);
+    body_sums_any<Real, IO>(S, R, atom0, sh.row, tile.N, sub, tile.lanes, active, F, tau);
     if (!active || sub != 0) return;
     const double halfDt = 0.5 * S.dt;
     V3<double> tb = axial_torque(to_body(R, tau), B.invI);
@@ -727,7 +727,7 @@ __device__ __forceinline__ void refine_end_bodies(const DevSys& S, const Tile& t
     rk[1] = 2.0 * dot(Btmul(q1, qdot), L1);
 }
 
-template <class Real, bool EXACT>
+template <class Real, bool EXACT, class IO>
 __global__ __launch_bounds__(EXACT ? 256 : 512) void k_refine_end(const DevSys* __restrict__ systems,
                                                                    const Tile* __restrict__ tiles, int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
@@ -736,9 +736,9 @@ __global__ __launch_bounds__(EXACT ? 256 : 512) void k_refine_end(const DevSys* 
         if (!S.refBody) continue;
         double rk[2] = {0.0, 0.0};
         if (tile.kind == TILE_FREE)
-            refine_end_free<Real>(S, tile, rk);
+            refine_end_free<Real, IO>(S, tile, rk);
         else
-            refine_end_bodies<Real, EXACT>(S, tile, rk);
+            refine_end_bodies<Real, EXACT, IO>(S, tile, rk);
         block_reduce_store<2>(rk, S.refKePart, tile.slot);
     }
 }
@@ -746,7 +746,7 @@ __global__ __launch_bounds__(EXACT ? 256 : 512) void k_refine_end(const DevSys* 
 // potential-energy refinement partials (rigidbodyintegrator.cu:449-469):
 // sum |F|^2/m over free atoms and |F|^2/M + tau.(I^-1 tau) over bodies, with
 // the body-frame torque tau = B(q)^T C(q) tau_space of the current forces
-template <class Real>
+template <class Real, class IO>
 __global__ __launch_bounds__(512) void k_refine_pe(const DevSys* __restrict__ systems,
                                                     const Tile* __restrict__ tiles, int numTiles) {
     for (int t = blockIdx.x; t < numTiles; t += gridDim.x) {
@@ -757,7 +757,7 @@ __global__ __launch_bounds__(512) void k_refine_pe(const DevSys* __restrict__ sy
         if (tile.kind == TILE_FREE) {
             for (int i = threadIdx.x; i < tile.count; i += blockDim.x) {
                 int a = tile.first + i;
-                V3<double> f = cvt<double>(ld3<Real>(S.force, S.atomStride, a));
+                V3<double> f = cvt<double>(IO::template force<Real>(S, a));
                 u[0] += dot(f, f) * (double)reinterpret_cast<const Real*>(S.freeInvM)[a];
             }
         } else {
@@ -771,7 +771,7 @@ __global__ __launch_bounds__(512) void k_refine_pe(const DevSys* __restrict__ sy
             Body<Real> B = load_body<Real>(S, k, sh.ks);
             Rot<Real> R = rotT(B.q);
             V3<double> F, tau;
-            body_sums_any<Real>(S, R, atom0, sh.row, tile.N, sub, tile.lanes, active, F, tau);
+            body_sums_any<Real, IO>(S, R, atom0, sh.row, tile.N, sub, tile.lanes, active, F, tau);
             if (active && sub == 0) {
                 V3<double> tb = axial_torque(to_body(R, tau), B.invI);
                 u[0] = dot(F, F) * (double)B.invM + dot(hadamard(tb, cvt<double>(B.invI)), tb);
@@ -1587,40 +1587,59 @@ hipError_t launch_site_forces(int precision, const DevSys* sys, int numSites, in
     return hipGetLastError();
 }
 
+template <class Real, bool EXACT>
+static void refine_begin_for(int adopted, int grid, int block, hipStream_t stream, const DevSys* systems,
+                             const Tile* tiles, int numTiles, int folded) {
+    if (adopted) k_refine_begin<Real, EXACT, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
+    else k_refine_begin<Real, EXACT, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
+}
+
 hipError_t launch_refine_begin(int precision, int rotationMode, const DevSys* systems, const Tile* tiles,
-                               int numTiles, int folded, int block, int maxBlocks, hipStream_t stream) {
+                               int numTiles, int folded, int block, int maxBlocks, hipStream_t stream, int adopted) {
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
     if (precision == 0) {
-        if (rotationMode == 0) k_refine_begin<float, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
-        else k_refine_begin<float, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
+        if (rotationMode == 0) refine_begin_for<float, true>(adopted, grid, block, stream, systems, tiles, numTiles, folded);
+        else refine_begin_for<float, false>(adopted, grid, block, stream, systems, tiles, numTiles, folded);
     } else {
-        if (rotationMode == 0) k_refine_begin<double, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
-        else k_refine_begin<double, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles, folded);
+        if (rotationMode == 0) refine_begin_for<double, true>(adopted, grid, block, stream, systems, tiles, numTiles, folded);
+        else refine_begin_for<double, false>(adopted, grid, block, stream, systems, tiles, numTiles, folded);
     }
     return hipGetLastError();
 }
 
+template <class Real, bool EXACT>
+static void refine_end_for(int adopted, int grid, int block, hipStream_t stream, const DevSys* systems,
+                           const Tile* tiles, int numTiles) {
+    if (adopted) k_refine_end<Real, EXACT, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    else k_refine_end<Real, EXACT, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+}
+
 hipError_t launch_refine_end(int precision, int rotationMode, const DevSys* systems, const Tile* tiles, int numTiles,
-                             int block, int maxBlocks, hipStream_t stream) {
+                             int block, int maxBlocks, hipStream_t stream, int adopted) {
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
     if (precision == 0) {
-        if (rotationMode == 0) k_refine_end<float, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_refine_end<float, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (rotationMode == 0) refine_end_for<float, true>(adopted, grid, block, stream, systems, tiles, numTiles);
+        else refine_end_for<float, false>(adopted, grid, block, stream, systems, tiles, numTiles);
     } else {
-        if (rotationMode == 0) k_refine_end<double, true><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-        else k_refine_end<double, false><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        if (rotationMode == 0) refine_end_for<double, true>(adopted, grid, block, stream, systems, tiles, numTiles);
+        else refine_end_for<double, false>(adopted, grid, block, stream, systems, tiles, numTiles);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_refine_pe(int precision, const DevSys* systems, const Tile* tiles, int numTiles, int block,
-                            int maxBlocks, hipStream_t stream) {
+                            int maxBlocks, hipStream_t stream, int adopted) {
     int grid = grid_for(numTiles, maxBlocks);
     if (grid == 0) return hipSuccess;
-    if (precision == 0) k_refine_pe<float><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
-    else k_refine_pe<double><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    if (precision == 0) {
+        if (adopted) k_refine_pe<float, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_refine_pe<float, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    } else {
+        if (adopted) k_refine_pe<double, AdoptedAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+        else k_refine_pe<double, SoaAtoms><<<grid, block, 0, stream>>>(systems, tiles, numTiles);
+    }
     return hipGetLastError();
 }
 

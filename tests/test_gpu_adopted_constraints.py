@@ -131,6 +131,13 @@ def test_constrained_part1_refusals(hip):
     arr.write_forces(tether(spec, spec.positions))
     g.state_changed(g.STATE_POSITIONS)
     pd = DeviceArray(np.zeros((arr.padded, 4)))
+    # refined energies do not combine with the caller's constraints
+    g.set_refined(True)
+    with pytest.raises(hip.RbhipError):
+        g.free_delta(pd.ptr.value)
+    with pytest.raises(hip.RbhipError):
+        g.part1_constrained(pd.ptr.value)
+    g.set_refined(False)
     g.free_delta(pd.ptr.value)
     g.part1_constrained(pd.ptr.value)
     # the constrained part 1 must be completed by part 2
@@ -145,3 +152,5 @@ def test_constrained_part1_refusals(hip):
     g.state_changed(g.STATE_POSITIONS)
     g.part1()
     g.part2()
+    with pytest.raises(hip.RbhipError):
+        g.set_refined(True)           # after constrained steps

@@ -251,11 +251,14 @@ def test_adopted_arrays_match_oracle(hip, oracle, precision, mode):
     arr.check_w_untouched()
 
 
+@pytest.mark.parametrize("refined", [False, True])
 @pytest.mark.parametrize("mode", [10, 0])
 @pytest.mark.parametrize("precision", ["mixed", "double"])
-def test_adopted_layout_equals_library_layout(hip, precision, mode):
+def test_adopted_layout_equals_library_layout(hip, precision, mode, refined):
     """The same forces through the fixed-point array and through
-    rbhip_set_forces: the two atom layouts give one trajectory."""
+    rbhip_set_forces: the two atom layouts give one trajectory, and with
+    refined energies on (each part1/part2 pair brackets its step) the same
+    refined kinetic energies and potential-energy refinement."""
     spec = systems.heterogeneous(500, 300, seed=23, rotation_mode=mode)
     g, arr, _ = adopted_context(hip, spec, precision, mode, seed=7)
     b = hip.Context(spec.masses, spec.body_indices, spec.dt, mode, precision)
@@ -270,6 +273,8 @@ def test_adopted_layout_equals_library_layout(hip, precision, mode):
     b.set_positions(arr.positions())
     b.set_velocities(arr.velocities())
     b.set_forces(f0)
+    for c in (g, b):
+        c.set_refined(refined)
     for _ in range(8):
         g.part1()
         b.part1()
@@ -280,6 +285,9 @@ def test_adopted_layout_equals_library_layout(hip, precision, mode):
         b.part2()
     np.testing.assert_array_equal(g.velocities(), b.velocities())
     np.testing.assert_array_equal(g.kinetic_energies(), b.kinetic_energies())
+    if refined:
+        np.testing.assert_array_equal(g.refined_kinetic_energies(), b.refined_kinetic_energies())
+        assert g.potential_energy_refinement() == b.potential_energy_refinement() != 0.0
     free = spec.body_indices <= 0
     np.testing.assert_array_equal(g.positions()[free], b.positions()[free])
     # body atoms: the adopted arrays hold the step kernel's r_cm + A(q)^T d at
@@ -325,8 +333,6 @@ def test_adopted_arrays_refusals(hip):
         g.set_forces(np.zeros((spec.num_atoms, 3)))      # the forces live in the array
     with pytest.raises(hip.RbhipError):
         g.adopt_arrays(*arr.pointers())                   # adopted once
-    with pytest.raises(hip.RbhipError):
-        g.set_refined(True)
     with pytest.raises(hip.RbhipError):
         g.set_atom_order(np.arange(spec.num_atoms - 1))   # too short
     with pytest.raises(hip.RbhipError):
